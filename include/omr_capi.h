@@ -1,0 +1,232 @@
+/*
+ * omr_capi.h -- C ABI of the MI355X path-tracing library (libomr.so).
+ *
+ * The reference has no FFI: its operator API is the C++ virtual interface of
+ * Renderer / PathTracer / Scene (src/renderer/Renderer.h:10-23,
+ * src/renderer/PathTracer.h:18-88, src/scenes/Scene.h:18-124).  The C++
+ * drop-in (omochi-renderer3_amd/dropin/) keeps that interface and calls these
+ * entry points; tests and benchmarks bind them with ctypes.
+ *
+ * Conventions: 0 = OK, negative = error code (no exceptions, no exit()).  The
+ * caller owns host buffers; the library owns device memory and handles.  A
+ * context is driven by one host thread, except omr_job_submit / omr_job_poll /
+ * omr_job_wait / omr_job_cancel / omr_server_stats, which are thread-safe.
+ */
+#ifndef OMR_CAPI_H
+#define OMR_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct omr_ctx omr_ctx;
+typedef int32_t omr_scene_id;
+typedef int64_t omr_job_id;
+
+enum {
+  OMR_OK = 0,
+  OMR_ERR_INVALID = -1,   /* bad argument or scene data */
+  OMR_ERR_NOMEM = -2,     /* device or host allocation failed */
+  OMR_ERR_HIP = -3,       /* HIP runtime error */
+  OMR_ERR_BUDGET = -4,    /* request does not fit the HBM budget */
+  OMR_ERR_NOTFOUND = -5,  /* unknown scene or job id */
+  OMR_ERR_STATE = -6,     /* wrong call order (e.g. server not started) */
+  OMR_ERR_NODEVICE = -7,  /* no usable GPU */
+  OMR_ERR_TIMEOUT = -8,   /* omr_job_wait timed out */
+  OMR_ERR_CANCELLED = -9  /* the job was cancelled */
+};
+
+/* Material::REFLECTION_TYPE (src/renderer/Material.h:13-17) */
+enum { OMR_MAT_LAMBERT = 0, OMR_MAT_SPECULAR = 1, OMR_MAT_REFRACTION = 2 };
+
+/* Material (src/renderer/Material.h:11-36) + multi-material rate
+ * (SceneObject::MultiMaterialElem, src/renderer/SceneObject.h:17-30).  48 bytes. */
+typedef struct {
+  uint32_t type;
+  float color[3];
+  float emission[3];
+  float ior;         /* refraction_rate */
+  float rate;        /* roulette rate among the primitive's materials (normalized) */
+  uint32_t pad[3];
+} omr_material;
+
+/* Scene contents handed to omr_scene_upload; replaces the object lists of
+ * Scene (src/scenes/Scene.h:112-116) built by AddObject / AddModel.
+ * Primitive ids: triangles 0..n_triangles-1, then spheres. */
+typedef struct {
+  size_t n_triangles;
+  const float *triangles;        /* n x 9: v0.xyz, e1.xyz = v1-v0, e2.xyz = v2-v0 (Polygon.h:103-105,112) */
+  const float *tri_normals;      /* n x 9: n0.xyz, n1-n0, n2-n0 (Polygon.h:109-111); NULL = face normal */
+  size_t n_spheres;
+  const float *spheres;          /* n x 4: center.xyz, radius (Sphere.h:13-19) */
+  const uint32_t *prim_material; /* per primitive: index of its first material */
+  const uint32_t *prim_material_count; /* per primitive: number of materials, NULL = 1 */
+  size_t n_materials;
+  const omr_material *materials;
+  uint32_t env_width, env_height;  /* latlong environment (IBL, src/renderer/IBL.h:14-45) */
+  const float *env_rgb;            /* env_height rows x env_width x 3, row 0 = +y; NULL = black */
+} omr_scene_desc;
+
+/* Camera (src/renderer/Camera.h:12-43) in its derived form; the drop-in
+ * Camera class computes it from position / direction / up exactly as the
+ * reference constructor does. */
+typedef struct {
+  uint32_t width, height;
+  float position[3];
+  float screen_center[3];
+  float screen_x[3];        /* full screen width vector */
+  float screen_y[3];        /* full screen height vector */
+  float aperture_radius;    /* CircleAperture radius (Aperture.h:23-35); 0 = pinhole */
+  float focal_distance;
+} omr_camera;
+
+/* Render parameters; the reference's compile-time MinDepth/MaxDepth
+ * (src/renderer/PathTracer.cpp:177-178) become rr_depth / max_depth. */
+typedef struct {
+  uint32_t supersamples;   /* ss: ss x ss sub-pixel grid (PathTracer.cpp:156-161) */
+  uint32_t spp_begin;      /* samples already accumulated ("previous_samples") */
+  uint32_t spp_end;        /* samples after this call ("next_samples") */
+  uint32_t max_depth;      /* segments per path branch; 0 = unlimited (reference mode) */
+  uint32_t rr_depth;       /* Russian roulette when depth > rr_depth (reference: 5) */
+  uint32_t env_is;         /* 1 = environment importance sampling with shadow rays */
+  uint64_t seed;           /* per-path seeds = hash(seed, pixel, sub-pixel, sample, branch) */
+} omr_render_params;
+
+typedef struct {
+  uint64_t segments;       /* closest-hit queries (Scene::CheckIntersection calls) */
+  uint64_t shadow_rays;    /* environment shadow rays */
+  uint64_t bounce_steps;   /* extend/shade steps executed */
+  double gpu_ms;           /* device time of the render (events) */
+  double wall_ms;          /* host wall time of the call */
+  /* filled when profiling is on (omr_set_profiling): per-kernel device time
+   * from events around every launch, and traversal work counted by extend */
+  double extend_ms, shade_ms, shadow_ms;
+  uint64_t extend_launches;
+  uint64_t node_visits, prim_tests;
+} omr_render_stats;
+
+typedef struct {
+  uint32_t n_nodes;
+  uint32_t depth;
+  uint32_t stack_bound;
+  uint32_t n_prims;
+  uint64_t device_bytes;
+  double build_ms;
+} omr_scene_info;
+
+/* Launch tuning (chosen by omr_autotune or set by hand). */
+typedef struct {
+  uint32_t node_width;        /* 4 (128-byte nodes) */
+  uint32_t wg_size;           /* 64, 128 or 256 */
+  uint32_t blocks_per_cu;     /* persistent grid = CUs x blocks_per_cu */
+  uint32_t pool_paths;        /* path slots in flight (max) */
+  uint32_t steps_per_sync;    /* bounce steps per host check (graph length) */
+  uint32_t use_graph;         /* capture bounce steps in a hipGraph */
+} omr_tuning;
+
+typedef struct {
+  uint64_t pool_paths;        /* path slots (0 = default) */
+  uint64_t hbm_budget_bytes;  /* admission budget for per-job buffers (0 = 16 GiB) */
+  uint32_t max_live_jobs;     /* 0 = 512 */
+} omr_server_config;
+
+typedef struct {
+  uint64_t jobs_submitted, jobs_done, jobs_rejected, jobs_cancelled;
+  uint64_t segments, shadow_rays;
+  double busy_ms;             /* wall time with work on the device */
+  double lat_p50_ms, lat_p99_ms, lat_max_ms;  /* submit -> result on host */
+} omr_server_stats;
+
+/* ---- context ---- */
+int omr_create(omr_ctx **ctx, int device);
+void omr_destroy(omr_ctx *ctx);
+const char *omr_last_error(omr_ctx *ctx);
+int omr_device_info(omr_ctx *ctx, int *n_cu, uint64_t *hbm_bytes, char *name, size_t name_len);
+
+/* ---- scenes: Scene::ConstructQBVH (src/scenes/Scene.cpp:36-49) becomes an
+ * upload that builds the wide BVH on the host and keeps it resident ---- */
+int omr_scene_upload(omr_ctx *ctx, const omr_scene_desc *desc, omr_scene_id *id);
+int omr_scene_release(omr_ctx *ctx, omr_scene_id id);
+int omr_scene_info_get(omr_ctx *ctx, omr_scene_id id, omr_scene_info *info);
+/* Host copy of the built BVH (128-byte nodes; leaf slot -> primitive id). */
+int omr_scene_bvh(omr_ctx *ctx, omr_scene_id id, const void **nodes, size_t *n_nodes, const uint32_t **prim_order,
+                  size_t *n_order);
+/* Host-only BVH build (no device needed): the same builder omr_scene_upload
+ * runs.  The handle owns the arrays until omr_bvh_free. */
+typedef struct omr_bvh omr_bvh;
+int omr_bvh_build(const omr_scene_desc *desc, omr_bvh **out);
+int omr_bvh_data(const omr_bvh *b, const void **nodes, size_t *n_nodes, const uint32_t **prim_order, size_t *n_order,
+                 uint32_t *depth, uint32_t *stack_bound);
+void omr_bvh_free(omr_bvh *b);
+/* Environment sampling tables as built on the device (uint64 prefix sums). */
+int omr_scene_env_tables(omr_ctx *ctx, omr_scene_id id, uint64_t *row /* h x (w+1) */, uint64_t *marg /* h+1 */);
+
+/* ---- closest hit: Scene::CheckIntersection (src/scenes/Scene.cpp:51-85) ----
+ * rays: n x 8 floats (o.xyz, tmax, d.xyz, pad).  prim_id = -1 on a miss,
+ * t = INF then.  _host: host buffers; _device: device pointers (no copies). */
+int omr_trace_closest(omr_ctx *ctx, omr_scene_id scene, const float *rays, size_t n, int32_t *prim_id, float *t);
+int omr_trace_closest_device(omr_ctx *ctx, omr_scene_id scene, const float *d_rays, size_t n, int32_t *d_prim_id,
+                             float *d_t);
+/* Node visits / primitive tests for a ray set (device pointers; stats pass). */
+int omr_trace_stats_device(omr_ctx *ctx, omr_scene_id scene, const float *d_rays, size_t n, int32_t *d_prim_id,
+                           float *d_t, uint64_t *node_visits, uint64_t *prim_tests);
+
+/* ---- rendering: PathTracer::RenderScene for one sample batch
+ * (src/renderer/PathTracer.cpp:55-175).  out_rgb: width x height x 3 floats,
+ * top-down rows, mean radiance over (spp_begin, spp_end] x ss^2 samples. ---- */
+int omr_render(omr_ctx *ctx, omr_scene_id scene, const omr_camera *cam, const omr_render_params *params,
+               float *out_rgb);
+/* Fixed-point per-pixel sums (x 2^-24), width x height x 3, for bit-level checks. */
+int omr_render_sums(omr_ctx *ctx, omr_scene_id scene, const omr_camera *cam, const omr_render_params *params,
+                    int64_t *out_sums);
+int omr_last_render_stats(omr_ctx *ctx, omr_render_stats *stats);
+
+/* Batched: n jobs packed into one persistent wavefront run (shared queues). */
+int omr_render_batch(omr_ctx *ctx, size_t n, const omr_scene_id *scenes, const omr_camera *cams,
+                     const omr_render_params *params, float *const *outs);
+
+/* Profiling: launches bounce steps without a graph, times each kernel with
+ * HIP events on the library's stream and counts traversal work. */
+int omr_set_profiling(omr_ctx *ctx, int enable);
+
+/* ---- tuning ---- */
+int omr_get_tuning(omr_ctx *ctx, omr_tuning *t);
+int omr_set_tuning(omr_ctx *ctx, const omr_tuning *t);
+
+/* ---- served mode: continuous batching with HBM-budget admission ---- */
+int omr_server_start(omr_ctx *ctx, const omr_server_config *cfg);
+int omr_server_stop(omr_ctx *ctx);
+int omr_job_submit(omr_ctx *ctx, omr_scene_id scene, const omr_camera *cam, const omr_render_params *params,
+                   float *out_rgb, omr_job_id *job);
+/* state: 0 queued, 1 running, 2 done, 3 cancelled, 4 failed */
+int omr_job_poll(omr_ctx *ctx, omr_job_id job, int *state);
+int omr_job_wait(omr_ctx *ctx, omr_job_id job, int64_t timeout_ms);
+int omr_job_cancel(omr_ctx *ctx, omr_job_id job);
+int omr_server_stats_get(omr_ctx *ctx, omr_server_stats *stats);
+
+/* ---- procedural benchmark scenes (no reference counterpart: the inputs the
+ * measurements use).  kind: 0 Cornell box, 1 Cornell box + displaced
+ * icosphere (level 6), 2 heightfield terrain + 200 spheres + HDR sky,
+ * 3 displaced torus knot.  The library owns the arrays until
+ * omr_procedural_free. ---- */
+typedef struct {
+  omr_scene_desc desc;
+  omr_camera camera;
+  void *internal;
+} omr_procedural;
+int omr_procedural_scene(int kind, uint64_t seed, uint32_t width, uint32_t height, omr_procedural *out);
+void omr_procedural_free(omr_procedural *p);
+
+/* Reference Camera constructor (src/renderer/Camera.h:12-43), double precision:
+ * fills the derived screen vectors of an omr_camera. */
+void omr_camera_from_lookat(uint32_t width, uint32_t height, const double pos[3], const double dir[3],
+                            const double up[3], double screen_height, double dist_to_screen, double focal_distance,
+                            double aperture_radius, omr_camera *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,340 @@
+// omr_capi.cpp -- extern "C" entry points (declared in include/omr_capi.h).
+#include <hip/hip_runtime_api.h>
+
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/omr_capi.h"
+#include "omr_engine.h"
+
+struct omr_ctx {
+  std::unique_ptr<omr::Engine> eng;
+  std::string err;
+  omr_render_stats last{};
+  // grow-only device scratch for per-call accumulators and outputs
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
+namespace {
+
+int set_err(omr_ctx *c, int code, const std::string &m) {
+  if (c) c->err = m;
+  return code;
+}
+int from_engine(omr_ctx *c, int code) {
+  if (code != OMR_OK && c) c->err = c->eng->err;
+  return code;
+}
+
+int check_job(omr_ctx *c, const omr_camera *cam, const omr_render_params *p, uint64_t *n_out) {
+  if (!cam || !p) return set_err(c, OMR_ERR_INVALID, "null camera or params");
+  if (cam->width == 0 || cam->height == 0) return set_err(c, OMR_ERR_INVALID, "image size is zero");
+  if (p->supersamples == 0 || p->supersamples > 64) return set_err(c, OMR_ERR_INVALID, "supersamples must be 1..64");
+  if (p->spp_end <= p->spp_begin) return set_err(c, OMR_ERR_INVALID, "spp_end must exceed spp_begin");
+  if (p->max_depth > 255) return set_err(c, OMR_ERR_INVALID, "max_depth must be <= 255 (0 = unlimited)");
+  uint64_t n = (uint64_t)cam->width * cam->height * p->supersamples * p->supersamples *
+               (uint64_t)(p->spp_end - p->spp_begin);
+  if (n > 0xFFFFFFFFull) return set_err(c, OMR_ERR_INVALID, "job exceeds 2^32-1 samples; split spp into batches");
+  *n_out = n;
+  return OMR_OK;
+}
+
+int ensure_scratch(omr_ctx *c, size_t bytes) {
+  if (bytes <= c->scratch_bytes) return OMR_OK;
+  if (c->scratch) hipFree(c->scratch);
+  c->scratch = nullptr;
+  c->scratch_bytes = 0;
+  if (hipMalloc(&c->scratch, bytes) != hipSuccess) return set_err(c, OMR_ERR_NOMEM, "device scratch allocation failed");
+  c->scratch_bytes = bytes;
+  return OMR_OK;
+}
+
+// Plain and batched rendering share this: one persistent wavefront run.
+int render_impl(omr_ctx *c, size_t n, const omr_scene_id *scenes, const omr_camera *cams,
+                const omr_render_params *params, float *const *outs, int64_t *sums_out) {
+  if (!c) return OMR_ERR_INVALID;
+  if (c->eng->server_running()) return set_err(c, OMR_ERR_STATE, "server mode is running");
+  if (n == 0) return OMR_OK;
+  std::vector<omr::JobRun> jobs(n);
+  size_t bytes = 0;
+  std::vector<size_t> off_acc(n), off_out(n);
+  for (size_t i = 0; i < n; i++) {
+    uint64_t ns;
+    int rc = check_job(c, &cams[i], &params[i], &ns);
+    if (rc) return rc;
+    if (!c->eng->scene(scenes[i])) return set_err(c, OMR_ERR_NOTFOUND, "unknown scene id");
+    jobs[i].scene = (uint32_t)scenes[i];
+    jobs[i].cam = cams[i];
+    jobs[i].params = params[i];
+    jobs[i].n_samples = (uint32_t)ns;
+    size_t px = (size_t)cams[i].width * cams[i].height;
+    off_acc[i] = bytes;
+    bytes += (px * 3 * 8 + 255) & ~(size_t)255;
+    off_out[i] = bytes;
+    bytes += (px * 3 * 4 + 255) & ~(size_t)255;
+  }
+  std::lock_guard<std::mutex> g(c->eng->mu);
+  hipSetDevice(c->eng->device_id());
+  int rc = ensure_scratch(c, bytes);
+  if (rc) return rc;
+  for (size_t i = 0; i < n; i++) {
+    jobs[i].d_accum = (unsigned long long *)((char *)c->scratch + off_acc[i]);
+    jobs[i].d_out = (float *)((char *)c->scratch + off_out[i]);
+  }
+  omr_render_stats st{};
+  rc = c->eng->run_jobs(jobs, &st);
+  if (rc) return from_engine(c, rc);
+  for (size_t i = 0; i < n; i++) {
+    size_t px = (size_t)cams[i].width * cams[i].height;
+    hipError_t e;
+    if (sums_out) e = hipMemcpy(sums_out, jobs[i].d_accum, px * 3 * 8, hipMemcpyDeviceToHost);
+    else e = hipMemcpy(outs[i], jobs[i].d_out, px * 3 * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return set_err(c, OMR_ERR_HIP, std::string("result copy: ") + hipGetErrorString(e));
+  }
+  c->last = st;
+  return OMR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int omr_create(omr_ctx **ctx, int device) {
+  if (!ctx) return OMR_ERR_INVALID;
+  *ctx = nullptr;
+  auto c = std::make_unique<omr_ctx>();
+  c->eng = std::make_unique<omr::Engine>(device);
+  if (!c->eng->ok()) return OMR_ERR_NODEVICE;
+  *ctx = c.release();
+  return OMR_OK;
+}
+
+void omr_destroy(omr_ctx *ctx) {
+  if (!ctx) return;
+  if (ctx->scratch) {
+    hipSetDevice(ctx->eng->device_id());
+    hipFree(ctx->scratch);
+  }
+  delete ctx;
+}
+
+const char *omr_last_error(omr_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int omr_device_info(omr_ctx *ctx, int *n_cu, uint64_t *hbm_bytes, char *name, size_t name_len) {
+  if (!ctx) return OMR_ERR_INVALID;
+  if (n_cu) *n_cu = ctx->eng->n_cu();
+  if (hbm_bytes) *hbm_bytes = ctx->eng->hbm_bytes();
+  if (name && name_len) {
+    std::strncpy(name, ctx->eng->device_name().c_str(), name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  return OMR_OK;
+}
+
+int omr_scene_upload(omr_ctx *ctx, const omr_scene_desc *desc, omr_scene_id *id) {
+  if (!ctx) return OMR_ERR_INVALID;
+  return from_engine(ctx, ctx->eng->upload_scene(desc, id));
+}
+
+int omr_scene_release(omr_ctx *ctx, omr_scene_id id) {
+  if (!ctx) return OMR_ERR_INVALID;
+  return from_engine(ctx, ctx->eng->release_scene(id));
+}
+
+int omr_scene_info_get(omr_ctx *ctx, omr_scene_id id, omr_scene_info *info) {
+  if (!ctx || !info) return OMR_ERR_INVALID;
+  omr::SceneHost *s = ctx->eng->scene(id);
+  if (!s) return set_err(ctx, OMR_ERR_NOTFOUND, "unknown scene id");
+  info->n_nodes = (uint32_t)s->bvh.nodes.size();
+  info->depth = s->bvh.depth;
+  info->stack_bound = s->bvh.stack_bound;
+  info->n_prims = (uint32_t)s->bvh.prims.size();
+  info->device_bytes = s->bytes;
+  info->build_ms = s->bvh.build_ms;
+  return OMR_OK;
+}
+
+int omr_scene_bvh(omr_ctx *ctx, omr_scene_id id, const void **nodes, size_t *n_nodes, const uint32_t **prim_order,
+                  size_t *n_order) {
+  if (!ctx || !nodes || !n_nodes || !prim_order || !n_order) return OMR_ERR_INVALID;
+  omr::SceneHost *s = ctx->eng->scene(id);
+  if (!s) return set_err(ctx, OMR_ERR_NOTFOUND, "unknown scene id");
+  *nodes = s->bvh.nodes.data();
+  *n_nodes = s->bvh.nodes.size();
+  *prim_order = s->bvh.prim_order.data();
+  *n_order = s->bvh.prim_order.size();
+  return OMR_OK;
+}
+
+struct omr_bvh {
+  omr::BuildOutput out;
+};
+
+int omr_bvh_build(const omr_scene_desc *d, omr_bvh **out) {
+  if (!d || !out) return OMR_ERR_INVALID;
+  *out = nullptr;
+  omr::BuildInput bi;
+  bi.n_tri = d->n_triangles;
+  bi.tri = d->triangles;
+  bi.n_sph = d->n_spheres;
+  bi.sph = d->spheres;
+  if ((bi.n_tri && !bi.tri) || (bi.n_sph && !bi.sph)) return OMR_ERR_INVALID;
+  auto b = std::make_unique<omr_bvh>();
+  std::string err;
+  if (!omr::build_wide_bvh(bi, b->out, err)) return OMR_ERR_INVALID;
+  *out = b.release();
+  return OMR_OK;
+}
+
+int omr_bvh_data(const omr_bvh *b, const void **nodes, size_t *n_nodes, const uint32_t **prim_order, size_t *n_order,
+                 uint32_t *depth, uint32_t *stack_bound) {
+  if (!b) return OMR_ERR_INVALID;
+  if (nodes) *nodes = b->out.nodes.data();
+  if (n_nodes) *n_nodes = b->out.nodes.size();
+  if (prim_order) *prim_order = b->out.prim_order.data();
+  if (n_order) *n_order = b->out.prim_order.size();
+  if (depth) *depth = b->out.depth;
+  if (stack_bound) *stack_bound = b->out.stack_bound;
+  return OMR_OK;
+}
+
+void omr_bvh_free(omr_bvh *b) { delete b; }
+
+int omr_scene_env_tables(omr_ctx *ctx, omr_scene_id id, uint64_t *row, uint64_t *marg) {
+  if (!ctx || !row || !marg) return OMR_ERR_INVALID;
+  return from_engine(ctx, ctx->eng->env_tables_host(id, row, marg));
+}
+
+int omr_trace_closest_device(omr_ctx *ctx, omr_scene_id scene, const float *d_rays, size_t n, int32_t *d_prim_id,
+                             float *d_t) {
+  if (!ctx || (n && (!d_rays || !d_prim_id || !d_t))) return OMR_ERR_INVALID;
+  std::lock_guard<std::mutex> g(ctx->eng->mu);
+  return from_engine(ctx, ctx->eng->trace(scene, d_rays, n, d_prim_id, d_t, nullptr));
+}
+
+int omr_trace_stats_device(omr_ctx *ctx, omr_scene_id scene, const float *d_rays, size_t n, int32_t *d_prim_id,
+                           float *d_t, uint64_t *node_visits, uint64_t *prim_tests) {
+  if (!ctx || (n && (!d_rays || !d_prim_id || !d_t))) return OMR_ERR_INVALID;
+  std::lock_guard<std::mutex> g(ctx->eng->mu);
+  uint64_t s2[2] = {0, 0};
+  int rc = ctx->eng->trace(scene, d_rays, n, d_prim_id, d_t, s2);
+  if (rc) return from_engine(ctx, rc);
+  if (node_visits) *node_visits = s2[0];
+  if (prim_tests) *prim_tests = s2[1];
+  return OMR_OK;
+}
+
+int omr_trace_closest(omr_ctx *ctx, omr_scene_id scene, const float *rays, size_t n, int32_t *prim_id, float *t) {
+  if (!ctx || (n && (!rays || !prim_id || !t))) return OMR_ERR_INVALID;
+  if (n == 0) return OMR_OK;
+  std::lock_guard<std::mutex> g(ctx->eng->mu);
+  hipSetDevice(ctx->eng->device_id());
+  float *d_r = nullptr, *d_t = nullptr;
+  int32_t *d_p = nullptr;
+  auto cleanup = [&]() {
+    if (d_r) hipFree(d_r);
+    if (d_t) hipFree(d_t);
+    if (d_p) hipFree(d_p);
+  };
+  if (hipMalloc(&d_r, n * 32) != hipSuccess || hipMalloc(&d_t, n * 4) != hipSuccess ||
+      hipMalloc(&d_p, n * 4) != hipSuccess) {
+    cleanup();
+    return set_err(ctx, OMR_ERR_NOMEM, "ray buffers");
+  }
+  int rc = OMR_OK;
+  if (hipMemcpy(d_r, rays, n * 32, hipMemcpyHostToDevice) != hipSuccess) rc = set_err(ctx, OMR_ERR_HIP, "ray upload");
+  if (!rc) rc = from_engine(ctx, ctx->eng->trace(scene, d_r, n, d_p, d_t, nullptr));
+  if (!rc && (hipMemcpy(prim_id, d_p, n * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(t, d_t, n * 4, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = set_err(ctx, OMR_ERR_HIP, "hit download");
+  cleanup();
+  return rc;
+}
+
+int omr_render(omr_ctx *ctx, omr_scene_id scene, const omr_camera *cam, const omr_render_params *params,
+               float *out_rgb) {
+  if (!out_rgb) return set_err(ctx, OMR_ERR_INVALID, "null output");
+  float *outs[1] = {out_rgb};
+  return render_impl(ctx, 1, &scene, cam, params, outs, nullptr);
+}
+
+int omr_render_sums(omr_ctx *ctx, omr_scene_id scene, const omr_camera *cam, const omr_render_params *params,
+                    int64_t *out_sums) {
+  if (!out_sums) return set_err(ctx, OMR_ERR_INVALID, "null output");
+  float *outs[1] = {nullptr};
+  return render_impl(ctx, 1, &scene, cam, params, outs, out_sums);
+}
+
+int omr_render_batch(omr_ctx *ctx, size_t n, const omr_scene_id *scenes, const omr_camera *cams,
+                     const omr_render_params *params, float *const *outs) {
+  if (n && (!scenes || !cams || !params || !outs)) return set_err(ctx, OMR_ERR_INVALID, "null argument");
+  for (size_t i = 0; i < n; i++)
+    if (!outs[i]) return set_err(ctx, OMR_ERR_INVALID, "null output");
+  return render_impl(ctx, n, scenes, cams, params, outs, nullptr);
+}
+
+int omr_last_render_stats(omr_ctx *ctx, omr_render_stats *stats) {
+  if (!ctx || !stats) return OMR_ERR_INVALID;
+  *stats = ctx->last;
+  return OMR_OK;
+}
+
+int omr_set_profiling(omr_ctx *ctx, int enable) {
+  if (!ctx) return OMR_ERR_INVALID;
+  std::lock_guard<std::mutex> g(ctx->eng->mu);
+  ctx->eng->profile = enable != 0;
+  return OMR_OK;
+}
+
+int omr_get_tuning(omr_ctx *ctx, omr_tuning *t) {
+  if (!ctx || !t) return OMR_ERR_INVALID;
+  *t = ctx->eng->tuning;
+  return OMR_OK;
+}
+
+int omr_set_tuning(omr_ctx *ctx, const omr_tuning *t) {
+  if (!ctx || !t) return OMR_ERR_INVALID;
+  if (ctx->eng->server_running()) return set_err(ctx, OMR_ERR_STATE, "server mode is running");
+  return from_engine(ctx, ctx->eng->set_tuning(*t));
+}
+
+int omr_server_start(omr_ctx *ctx, const omr_server_config *cfg) {
+  if (!ctx) return OMR_ERR_INVALID;
+  omr_server_config c{};
+  if (cfg) c = *cfg;
+  return from_engine(ctx, ctx->eng->server_start(c));
+}
+int omr_server_stop(omr_ctx *ctx) {
+  if (!ctx) return OMR_ERR_INVALID;
+  return from_engine(ctx, ctx->eng->server_stop());
+}
+int omr_job_submit(omr_ctx *ctx, omr_scene_id scene, const omr_camera *cam, const omr_render_params *params,
+                   float *out_rgb, omr_job_id *job) {
+  if (!ctx || !out_rgb || !job) return OMR_ERR_INVALID;
+  uint64_t ns;
+  int rc = check_job(nullptr, cam, params, &ns);
+  if (rc) return rc;
+  return ctx->eng->submit(scene, *cam, *params, out_rgb, job);
+}
+int omr_job_poll(omr_ctx *ctx, omr_job_id job, int *state) {
+  if (!ctx || !state) return OMR_ERR_INVALID;
+  return ctx->eng->poll(job, state);
+}
+int omr_job_wait(omr_ctx *ctx, omr_job_id job, int64_t timeout_ms) {
+  if (!ctx) return OMR_ERR_INVALID;
+  return ctx->eng->wait(job, timeout_ms);
+}
+int omr_job_cancel(omr_ctx *ctx, omr_job_id job) {
+  if (!ctx) return OMR_ERR_INVALID;
+  return ctx->eng->cancel(job);
+}
+int omr_server_stats_get(omr_ctx *ctx, omr_server_stats *stats) {
+  if (!ctx || !stats) return OMR_ERR_INVALID;
+  return ctx->eng->server_stats(stats);
+}
+
+}  // extern "C"

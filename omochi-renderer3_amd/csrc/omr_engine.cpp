@@ -1,0 +1,565 @@
+// omr_engine.cpp -- device runtime: scenes, path pool, bounce-step loop.
+#include "omr_engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace omr {
+
+#define OMR_HIP(call)                                  \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return hipfail(e_, #call);   \
+  } while (0)
+
+int Engine::fail(int code, const std::string &msg) {
+  err = msg;
+  return code;
+}
+
+int Engine::hipfail(hipError_t e, const char *what) {
+  err = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? OMR_ERR_NOMEM : OMR_ERR_HIP;
+}
+
+Engine::Engine(int device) : device_(device) {
+  tuning.node_width = 4;
+  tuning.wg_size = 128;
+  tuning.blocks_per_cu = 8;
+  tuning.pool_paths = 1u << 21;
+  tuning.steps_per_sync = 8;
+  tuning.use_graph = 1;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+    err = "no HIP device " + std::to_string(device);
+    return;
+  }
+  if (hipSetDevice(device) != hipSuccess) { err = "hipSetDevice failed"; return; }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) { err = "hipGetDeviceProperties failed"; return; }
+  n_cu_ = prop.multiProcessorCount;
+  hbm_ = prop.totalGlobalMem;
+  name_ = prop.gcnArchName;
+  if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) { err = "stream"; return; }
+  if (hipEventCreate(&ev_a_) != hipSuccess || hipEventCreate(&ev_b_) != hipSuccess) { err = "event"; return; }
+  if (hipMalloc(&d_scenes_, kMaxScenes * sizeof(SceneDev)) != hipSuccess) { err = "scene table"; return; }
+  if (hipMemset(d_scenes_, 0, kMaxScenes * sizeof(SceneDev)) != hipSuccess) { err = "scene table"; return; }
+  if (hipMalloc(&d_ctrl_, sizeof(Ctrl)) != hipSuccess) { err = "ctrl"; return; }
+  if (hipHostMalloc(&h_ctrl_, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess) { err = "ctrl pinned"; return; }
+  if (hipMalloc(&d_jobs_, kJobRing * sizeof(JobDev)) != hipSuccess) { err = "jobs"; return; }
+  if (hipMalloc(&d_done_, kJobRing * sizeof(unsigned long long)) != hipSuccess) { err = "done"; return; }
+  if (hipHostMalloc(&h_done_, kJobRing * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+    err = "done pinned";
+    return;
+  }
+  std::memset(h_ctrl_, 0, sizeof(Ctrl));
+  init_ok_ = true;
+}
+
+Engine::~Engine() {
+  if (server_on_.load()) server_stop();
+  if (!init_ok_) return;
+  hipSetDevice(device_);
+  hipStreamSynchronize(stream_);
+  drop_graphs();
+  for (auto &s : scenes_)
+    if (s) free_scene_mem(*s);
+  PoolPtrs &p = pool_;
+  void *bufs[] = {p.ray_o, p.ray_d, p.misc, p.rng, p.hit, p.pdf, p.pend, p.queue[0], p.queue[1], p.shadow};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  if (d_gstack_) hipFree(d_gstack_);
+  hipFree(d_scenes_);
+  hipFree(d_ctrl_);
+  hipFree(d_jobs_);
+  hipFree(d_done_);
+  hipHostFree(h_ctrl_);
+  hipHostFree(h_done_);
+  hipEventDestroy(ev_a_);
+  hipEventDestroy(ev_b_);
+  hipStreamDestroy(stream_);
+}
+
+void Engine::drop_graphs() {
+  for (auto &g : graph_) {
+    if (g) hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+  graph_steps_ = 0;
+}
+
+int Engine::set_tuning(const omr_tuning &t) {
+  if (t.node_width != 4) return fail(OMR_ERR_INVALID, "node_width must be 4");
+  if (t.wg_size != 64 && t.wg_size != 128 && t.wg_size != 256) return fail(OMR_ERR_INVALID, "wg_size must be 64/128/256");
+  if (t.blocks_per_cu < 1 || t.blocks_per_cu > 64) return fail(OMR_ERR_INVALID, "blocks_per_cu out of range");
+  if (t.pool_paths < 64 || t.pool_paths > (1u << 26)) return fail(OMR_ERR_INVALID, "pool_paths out of range");
+  if (t.steps_per_sync < 2 || t.steps_per_sync > 256) return fail(OMR_ERR_INVALID, "steps_per_sync out of range");
+  std::lock_guard<std::mutex> g(mu);
+  hipSetDevice(device_);
+  hipStreamSynchronize(stream_);
+  drop_graphs();
+  tuning = t;
+  tuning.steps_per_sync = (t.steps_per_sync + 1u) & ~1u;
+  return OMR_OK;
+}
+
+LaunchShape Engine::shape() const {
+  LaunchShape s;
+  s.wg = tuning.wg_size;
+  s.grid = (uint32_t)n_cu_ * tuning.blocks_per_cu;
+  uint32_t need = (pool_cap_ + s.wg - 1) / s.wg;
+  if (s.grid > need) s.grid = std::max(1u, need);
+  return s;
+}
+
+KernelArgs Engine::args() const {
+  KernelArgs a;
+  a.ctrl = d_ctrl_;
+  a.jobs = d_jobs_;
+  a.scenes = d_scenes_;
+  a.p = pool_;
+  a.p.gstack = d_gstack_;
+  a.p.gcap = gcap_;
+  return a;
+}
+
+int Engine::ensure_pool() {
+  uint32_t P = tuning.pool_paths;
+  if (pool_cap_ == P && pool_.ray_o) return OMR_OK;
+  drop_graphs();
+  PoolPtrs &p = pool_;
+  void *bufs[] = {p.ray_o, p.ray_d, p.misc, p.rng, p.hit, p.pdf, p.pend, p.queue[0], p.queue[1], p.shadow};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  p = PoolPtrs();
+  pool_cap_ = 0;
+  OMR_HIP(hipMalloc(&p.ray_o, (size_t)P * 16));
+  OMR_HIP(hipMalloc(&p.ray_d, (size_t)P * 16));
+  OMR_HIP(hipMalloc(&p.misc, (size_t)P * 16));
+  OMR_HIP(hipMalloc(&p.rng, (size_t)P * 16));
+  OMR_HIP(hipMalloc(&p.hit, (size_t)P * 16));
+  OMR_HIP(hipMalloc(&p.pdf, (size_t)P * 4));
+  OMR_HIP(hipMalloc(&p.pend, (size_t)P * 3 * 48));
+  OMR_HIP(hipMalloc(&p.queue[0], (size_t)P * 4));
+  OMR_HIP(hipMalloc(&p.queue[1], (size_t)P * 4));
+  OMR_HIP(hipMalloc(&p.shadow, (size_t)P * 48));
+  OMR_HIP(hipMemset(p.misc, 0, (size_t)P * 16));
+  p.pool = P;
+  pool_cap_ = P;
+  return OMR_OK;
+}
+
+int Engine::ensure_gstack(uint32_t need_entries, size_t threads) {
+  uint32_t cap = need_entries > kShortStack ? need_entries - kShortStack : 1u;
+  if (d_gstack_ && cap <= gcap_ && threads <= gstack_threads_) return OMR_OK;
+  drop_graphs();
+  cap = std::max(cap, gcap_);
+  threads = std::max(threads, gstack_threads_);
+  if (d_gstack_) hipFree(d_gstack_);
+  d_gstack_ = nullptr;
+  OMR_HIP(hipMalloc(&d_gstack_, threads * cap * sizeof(uint2)));
+  gcap_ = cap;
+  gstack_threads_ = threads;
+  return OMR_OK;
+}
+
+void Engine::free_scene_mem(SceneHost &s) {
+  for (void *p : s.allocs)
+    if (p) hipFree(p);
+  s.allocs.clear();
+}
+
+SceneHost *Engine::scene(omr_scene_id id) {
+  if (id < 0 || (size_t)id >= scenes_.size()) return nullptr;
+  return scenes_[(size_t)id].get();
+}
+
+int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
+  if (!d || !id) return fail(OMR_ERR_INVALID, "null argument");
+  size_t nt = d->n_triangles, ns = d->n_spheres, n = nt + ns;
+  if (n == 0) return fail(OMR_ERR_INVALID, "scene has no primitives");
+  if (nt && !d->triangles) return fail(OMR_ERR_INVALID, "triangles is NULL");
+  if (ns && !d->spheres) return fail(OMR_ERR_INVALID, "spheres is NULL");
+  if (!d->prim_material || !d->materials || d->n_materials == 0) return fail(OMR_ERR_INVALID, "materials missing");
+  if (d->n_materials > 0x00FFFFFFu) return fail(OMR_ERR_INVALID, "too many materials");
+  if (n > 0xFFFFFFFFull) return fail(OMR_ERR_INVALID, "too many primitives");
+  for (size_t i = 0; i < d->n_materials; i++) {
+    const omr_material &m = d->materials[i];
+    if (m.type > 2) return fail(OMR_ERR_INVALID, "material " + std::to_string(i) + ": bad type");
+    for (int c = 0; c < 3; c++)
+      if (!std::isfinite(m.color[c]) || !std::isfinite(m.emission[c]) || m.emission[c] < 0.0f)
+        return fail(OMR_ERR_INVALID, "material " + std::to_string(i) + ": bad color/emission");
+    if (m.type == 2 && !(m.ior > 0.0f)) return fail(OMR_ERR_INVALID, "material " + std::to_string(i) + ": bad ior");
+  }
+  std::vector<uint32_t> pm(n);
+  for (size_t i = 0; i < n; i++) {
+    uint32_t f = d->prim_material[i];
+    uint32_t c = d->prim_material_count ? d->prim_material_count[i] : 1u;
+    if (c < 1 || c > 256 || (size_t)f + c > d->n_materials)
+      return fail(OMR_ERR_INVALID, "primitive " + std::to_string(i) + ": bad material range");
+    pm[i] = f | ((c - 1u) << 24);
+  }
+  if (d->env_rgb && (d->env_width == 0 || d->env_height == 0)) return fail(OMR_ERR_INVALID, "env size is zero");
+
+  auto sh = std::make_unique<SceneHost>();
+  BuildInput bi;
+  bi.n_tri = nt;
+  bi.tri = d->triangles;
+  bi.n_sph = ns;
+  bi.sph = d->spheres;
+  std::string berr;
+  if (!build_wide_bvh(bi, sh->bvh, berr)) return fail(OMR_ERR_INVALID, berr);
+
+  // per-triangle shading normals: n0, n1-n0, n2-n0 padded to 12 floats
+  std::vector<float> tn(nt * 12, 0.0f);
+  for (size_t i = 0; i < nt; i++) {
+    float *o = &tn[i * 12];
+    if (d->tri_normals) {
+      const float *s = d->tri_normals + 9 * i;
+      for (int a = 0; a < 3; a++) { o[a] = s[a]; o[4 + a] = s[3 + a]; o[8 + a] = s[6 + a]; }
+    } else {
+      // Polygon::CalculateNormal (src/renderer/Polygon.h:129-133): (v1-v0)x(v2-v0), normalized
+      const float *t = d->triangles + 9 * i;
+      double e1[3] = {t[3], t[4], t[5]}, e2[3] = {t[6], t[7], t[8]};
+      double c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+      double l = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+      for (int a = 0; a < 3; a++) o[a] = l > 0 ? (float)(c[a] / l) : 0.0f;
+    }
+  }
+  std::vector<MatRec> mats(d->n_materials);
+  for (size_t i = 0; i < d->n_materials; i++) {
+    const omr_material &m = d->materials[i];
+    MatRec &r = mats[i];
+    std::memset(&r, 0, sizeof(r));
+    r.type = m.type;
+    for (int c = 0; c < 3; c++) { r.color[c] = m.color[c]; r.emission[c] = m.emission[c]; }
+    r.ior = m.ior;
+    r.rate = m.rate;
+  }
+
+  std::lock_guard<std::mutex> g(mu);
+  hipSetDevice(device_);
+  size_t slot = scenes_.size();
+  if (slot >= kMaxScenes) return fail(OMR_ERR_INVALID, "too many scenes (max 64)");
+  SceneHost &S = *sh;
+  auto dup = [&](const void *src, size_t bytes, void **dst) -> int {
+    *dst = nullptr;
+    if (bytes == 0) return OMR_OK;
+    OMR_HIP(hipMalloc(dst, bytes));
+    S.allocs.push_back(*dst);
+    S.bytes += bytes;
+    if (src) OMR_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return OMR_OK;
+  };
+  void *p_nodes, *p_prims, *p_tn = nullptr, *p_sph = nullptr, *p_pm, *p_mats;
+  int rc;
+  if ((rc = dup(S.bvh.nodes.data(), S.bvh.nodes.size() * sizeof(WNode4), &p_nodes)) ||
+      (rc = dup(S.bvh.prims.data(), S.bvh.prims.size() * sizeof(PrimRec), &p_prims)) ||
+      (rc = dup(tn.data(), tn.size() * sizeof(float), &p_tn)) ||
+      (rc = dup(d->spheres, ns * 4 * sizeof(float), &p_sph)) || (rc = dup(pm.data(), n * 4, &p_pm)) ||
+      (rc = dup(mats.data(), mats.size() * sizeof(MatRec), &p_mats))) {
+    free_scene_mem(S);
+    return rc;
+  }
+  SceneDev &D = S.dev;
+  D.nodes = (const WNode4 *)p_nodes;
+  D.prims = (const PrimRec *)p_prims;
+  D.tri_n = (const float *)p_tn;
+  D.sph = (const float *)p_sph;
+  D.prim_mat = (const uint32_t *)p_pm;
+  D.mats = (const MatRec *)p_mats;
+  D.n_tri = (uint32_t)nt;
+  D.n_sph = (uint32_t)ns;
+  D.n_nodes = (uint32_t)S.bvh.nodes.size();
+  D.stack_bound = S.bvh.stack_bound;
+  if (d->env_rgb) {
+    uint32_t w = d->env_width, h = d->env_height;
+    void *p_env, *p_w, *p_max, *p_q, *p_row, *p_marg;
+    if ((rc = dup(d->env_rgb, (size_t)w * h * 3 * sizeof(float), &p_env)) ||
+        (rc = dup(nullptr, (size_t)w * h * sizeof(float), &p_w)) || (rc = dup(nullptr, 16, &p_max)) ||
+        (rc = dup(nullptr, (size_t)w * h * 8, &p_q)) || (rc = dup(nullptr, (size_t)h * (w + 1) * 8, &p_row)) ||
+        (rc = dup(nullptr, (size_t)(h + 1) * 8, &p_marg))) {
+      free_scene_mem(S);
+      return rc;
+    }
+    OMR_HIP(hipMemset(p_max, 0, 16));
+    hipError_t e = k::env_tables((const float *)p_env, w, h, (float *)p_w, (unsigned int *)p_max,
+                                 (unsigned long long *)p_q, (unsigned long long *)p_row, (unsigned long long *)p_marg,
+                                 stream_);
+    if (e != hipSuccess) { free_scene_mem(S); return hipfail(e, "env tables"); }
+    OMR_HIP(hipStreamSynchronize(stream_));
+    D.env = (const float *)p_env;
+    D.env_q = (const unsigned long long *)p_q;
+    D.env_row = (const unsigned long long *)p_row;
+    D.env_marg = (const unsigned long long *)p_marg;
+    D.env_w = w;
+    D.env_h = h;
+    S.has_env = true;
+    S.env_w = w;
+    S.env_h = h;
+  }
+  OMR_HIP(hipMemcpy(d_scenes_ + slot, &D, sizeof(SceneDev), hipMemcpyHostToDevice));
+  scenes_.push_back(std::move(sh));
+  *id = (omr_scene_id)slot;
+  return OMR_OK;
+}
+
+int Engine::release_scene(omr_scene_id id) {
+  std::lock_guard<std::mutex> g(mu);
+  SceneHost *s = scene(id);
+  if (!s) return fail(OMR_ERR_NOTFOUND, "unknown scene");
+  if (s->users.load() > 0) return fail(OMR_ERR_STATE, "scene is in use by live jobs");
+  hipSetDevice(device_);
+  hipStreamSynchronize(stream_);
+  free_scene_mem(*s);
+  SceneDev z{};
+  hipMemcpy(d_scenes_ + id, &z, sizeof(SceneDev), hipMemcpyHostToDevice);
+  scenes_[(size_t)id].reset();
+  return OMR_OK;
+}
+
+int Engine::env_tables_host(omr_scene_id id, uint64_t *row, uint64_t *marg) {
+  std::lock_guard<std::mutex> g(mu);
+  SceneHost *s = scene(id);
+  if (!s) return fail(OMR_ERR_NOTFOUND, "unknown scene");
+  if (!s->has_env) return fail(OMR_ERR_STATE, "scene has no environment");
+  hipSetDevice(device_);
+  OMR_HIP(hipMemcpy(row, s->dev.env_row, (size_t)s->env_h * (s->env_w + 1) * 8, hipMemcpyDeviceToHost));
+  OMR_HIP(hipMemcpy(marg, s->dev.env_marg, (size_t)(s->env_h + 1) * 8, hipMemcpyDeviceToHost));
+  return OMR_OK;
+}
+
+int Engine::trace(omr_scene_id id, const float *d_rays, size_t n, int32_t *d_prim, float *d_t, uint64_t *stats2) {
+  SceneHost *s = scene(id);
+  if (!s) return fail(OMR_ERR_NOTFOUND, "unknown scene");
+  if (n > 0xFFFFFFFFull) return fail(OMR_ERR_INVALID, "too many rays");
+  if (n == 0) return OMR_OK;
+  hipSetDevice(device_);
+  const uint32_t wg = 128;
+  uint32_t grid = (uint32_t)std::min<size_t>((n + wg - 1) / wg, (size_t)n_cu_ * 16);
+  int rc = ensure_gstack(s->bvh.stack_bound, (size_t)grid * wg);
+  if (rc) return rc;
+  unsigned long long *d_cnt = nullptr;
+  if (stats2) {
+    OMR_HIP(hipMalloc(&d_cnt, 16));
+    OMR_HIP(hipMemsetAsync(d_cnt, 0, 16, stream_));
+  }
+  hipError_t e = k::trace_rays(d_scenes_ + id, d_rays, (uint32_t)n, d_prim, d_t, d_cnt, d_gstack_, gcap_, grid, wg,
+                               stream_);
+  if (e != hipSuccess) {
+    if (d_cnt) hipFree(d_cnt);
+    return hipfail(e, "trace_rays");
+  }
+  OMR_HIP(hipStreamSynchronize(stream_));
+  if (stats2) {
+    unsigned long long c[2];
+    OMR_HIP(hipMemcpy(c, d_cnt, 16, hipMemcpyDeviceToHost));
+    stats2[0] = c[0];
+    stats2[1] = c[1];
+    hipFree(d_cnt);
+  }
+  return OMR_OK;
+}
+
+int Engine::collect_profile() {
+  for (size_t i = 0; i < prof_kind_.size(); i++) {
+    float ms = 0.0f;
+    OMR_HIP(hipEventElapsedTime(&ms, prof_ev_[2 * i], prof_ev_[2 * i + 1]));
+    prof_ms_[prof_kind_[i]] += ms;
+    if (prof_kind_[i] == 0) prof_extend_launches_++;
+  }
+  prof_kind_.clear();
+  return OMR_OK;
+}
+
+int Engine::launch_steps(uint32_t steps, bool with_shadow) {
+  KernelArgs a = args();
+  LaunchShape sh = shape();
+  if (profile) {
+    // one event pair per launch, every kernel directly on the stream
+    size_t need = (size_t)steps * 3 * 2;
+    while (prof_ev_.size() < need) {
+      hipEvent_t e;
+      OMR_HIP(hipEventCreate(&e));
+      prof_ev_.push_back(e);
+    }
+    prof_kind_.clear();
+    auto timed = [&](int kind, auto &&fn) -> hipError_t {
+      size_t i = prof_kind_.size();
+      hipError_t e = hipEventRecord(prof_ev_[2 * i], stream_);
+      if (e != hipSuccess) return e;
+      e = fn();
+      if (e != hipSuccess) return e;
+      prof_kind_.push_back(kind);
+      return hipEventRecord(prof_ev_[2 * i + 1], stream_);
+    };
+    for (uint32_t s = 0; s < steps; s++) {
+      int parity = (int)(s & 1u);
+      hipError_t e = timed(0, [&] { return k::extend(a, sh, parity, true, stream_); });
+      if (e == hipSuccess) e = timed(1, [&] { return k::shade(a, sh, parity, stream_); });
+      if (e == hipSuccess && with_shadow) e = timed(2, [&] { return k::shadow(a, sh, stream_); });
+      if (e != hipSuccess) return hipfail(e, "profiled bounce step launch");
+    }
+    return OMR_OK;
+  }
+  auto enqueue = [&]() -> hipError_t {
+    for (uint32_t s = 0; s < steps; s++) {
+      int parity = (int)(s & 1u);
+      hipError_t e = k::extend(a, sh, parity, false, stream_);
+      if (e != hipSuccess) return e;
+      e = k::shade(a, sh, parity, stream_);
+      if (e != hipSuccess) return e;
+      if (with_shadow) {
+        e = k::shadow(a, sh, stream_);
+        if (e != hipSuccess) return e;
+      }
+    }
+    return hipSuccess;
+  };
+  if (!tuning.use_graph) {
+    hipError_t e = enqueue();
+    if (e != hipSuccess) return hipfail(e, "bounce step launch");
+    return OMR_OK;
+  }
+  int gi = with_shadow ? 1 : 0;
+  if (!graph_[gi] || graph_steps_ != steps) {
+    if (graph_steps_ != steps) drop_graphs();
+    hipGraph_t g = nullptr;
+    OMR_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue();
+    hipError_t e2 = hipStreamEndCapture(stream_, &g);
+    if (e != hipSuccess) return hipfail(e, "graph capture");
+    if (e2 != hipSuccess) return hipfail(e2, "hipStreamEndCapture");
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return hipfail(e, "hipGraphInstantiate");
+    graph_[gi] = ex;
+    graph_steps_ = steps;
+  }
+  OMR_HIP(hipGraphLaunch(graph_[gi], stream_));
+  return OMR_OK;
+}
+
+static uint64_t job_samples(const omr_camera &c, const omr_render_params &p) {
+  return (uint64_t)c.width * c.height * p.supersamples * p.supersamples * (uint64_t)(p.spp_end - p.spp_begin);
+}
+
+int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (jobs.size() > kJobRing) return fail(OMR_ERR_INVALID, "too many jobs in one batch (max 1024)");
+  hipSetDevice(device_);
+  int rc = ensure_pool();
+  if (rc) return rc;
+  uint32_t need_stack = 1;
+  bool any_env_is = false;
+  for (auto &j : jobs) {
+    SceneHost *s = scene((omr_scene_id)j.scene);
+    if (!s) return fail(OMR_ERR_NOTFOUND, "unknown scene");
+    need_stack = std::max(need_stack, s->bvh.stack_bound);
+    if (j.params.env_is && s->has_env) any_env_is = true;
+  }
+  LaunchShape sh = shape();
+  rc = ensure_gstack(need_stack, (size_t)sh.grid * sh.wg);
+  if (rc) return rc;
+
+  std::vector<JobDev> jd(jobs.size());
+  unsigned long long total = 0;
+  for (size_t i = 0; i < jobs.size(); i++) {
+    JobRun &j = jobs[i];
+    JobDev &D = jd[i];
+    std::memset(&D, 0, sizeof(D));
+    D.sample_base = total;
+    D.n_samples = j.n_samples;
+    D.width = j.cam.width;
+    D.height = j.cam.height;
+    D.ss = j.params.supersamples;
+    D.nspp = j.params.spp_end - j.params.spp_begin;
+    D.spp_begin = j.params.spp_begin;
+    D.max_depth = j.params.max_depth;
+    D.rr_depth = j.params.rr_depth;
+    D.flags = j.params.env_is ? 1u : 0u;
+    D.scene = j.scene;
+    D.seed = j.params.seed;
+    for (int c = 0; c < 3; c++) {
+      D.cam_pos[c] = j.cam.position[c];
+      D.cam_c[c] = j.cam.screen_center[c];
+      D.cam_x[c] = j.cam.screen_x[c];
+      D.cam_y[c] = j.cam.screen_y[c];
+    }
+    D.aperture = j.cam.aperture_radius;
+    D.focal = j.cam.focal_distance;
+    D.accum = j.d_accum;
+    D.done = d_done_ + i;
+    j.ring = (uint32_t)i;
+    total += j.n_samples;
+  }
+  Ctrl c;
+  std::memset(&c, 0, sizeof(c));
+  c.total_samples = total;
+  c.job_lo = 0;
+  c.job_hi = (uint32_t)jobs.size();
+  OMR_HIP(hipMemcpyAsync(d_jobs_, jd.data(), jd.size() * sizeof(JobDev), hipMemcpyHostToDevice, stream_));
+  OMR_HIP(hipMemcpyAsync(d_ctrl_, &c, sizeof(Ctrl), hipMemcpyHostToDevice, stream_));
+  OMR_HIP(hipMemsetAsync(d_done_, 0, kJobRing * sizeof(unsigned long long), stream_));
+  OMR_HIP(hipMemsetAsync(pool_.misc, 0, (size_t)pool_cap_ * 16, stream_));
+  for (auto &j : jobs)
+    OMR_HIP(hipMemsetAsync(j.d_accum, 0, (size_t)j.cam.width * j.cam.height * 3 * 8, stream_));
+  OMR_HIP(hipEventRecord(ev_a_, stream_));
+  KernelArgs a = args();
+  hipError_t e = k::refill(a, sh, 0, stream_);
+  if (e != hipSuccess) return hipfail(e, "refill");
+  uint64_t steps_done = 0;
+  const uint32_t chunk = tuning.steps_per_sync;
+  prof_ms_[0] = prof_ms_[1] = prof_ms_[2] = 0.0;
+  prof_extend_launches_ = 0;
+  for (uint64_t it = 0;; it++) {
+    double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (it > 2000000ull || el > 600.0) return fail(OMR_ERR_STATE, "bounce loop did not terminate");
+    rc = launch_steps(chunk, any_env_is);
+    if (rc) return rc;
+    steps_done += chunk;
+    OMR_HIP(hipMemcpyAsync(h_ctrl_, d_ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_));
+    OMR_HIP(hipStreamSynchronize(stream_));
+    if (profile && (rc = collect_profile())) return rc;
+    if (h_ctrl_->qcount[0] == 0u && h_ctrl_->next_sample >= h_ctrl_->total_samples) break;
+  }
+  OMR_HIP(hipEventRecord(ev_b_, stream_));
+  for (auto &j : jobs) {
+    if (j.d_out) {
+      double nspp = (double)j.params.supersamples * j.params.supersamples * (double)(j.params.spp_end - j.params.spp_begin);
+      e = k::resolve(j.d_accum, j.d_out, j.cam.width * j.cam.height, nspp, stream_);
+      if (e != hipSuccess) return hipfail(e, "resolve");
+    }
+  }
+  OMR_HIP(hipMemcpyAsync(h_done_, d_done_, jobs.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
+  OMR_HIP(hipStreamSynchronize(stream_));
+  for (size_t i = 0; i < jobs.size(); i++) {
+    if (h_done_[i] != jobs[i].n_samples)
+      return fail(OMR_ERR_STATE, "job " + std::to_string(i) + " completed " + std::to_string(h_done_[i]) + " of " +
+                                     std::to_string(jobs[i].n_samples) + " samples");
+    jobs[i].done = true;
+  }
+  if (st) {
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, ev_a_, ev_b_);
+    st->segments = h_ctrl_->segments;
+    st->shadow_rays = h_ctrl_->shadow_rays;
+    st->bounce_steps = steps_done;
+    st->gpu_ms = ms;
+    st->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    st->extend_ms = prof_ms_[0];
+    st->shade_ms = prof_ms_[1];
+    st->shadow_ms = prof_ms_[2];
+    st->extend_launches = prof_extend_launches_;
+    st->node_visits = h_ctrl_->node_visits;
+    st->prim_tests = h_ctrl_->prim_tests;
+  }
+  return OMR_OK;
+}
+
+uint64_t job_samples_u64(const omr_camera &c, const omr_render_params &p) { return job_samples(c, p); }
+
+}  // namespace omr

@@ -1,0 +1,163 @@
+// omr_engine.h -- device runtime behind the C ABI: resident scenes, the
+// wavefront path pool, the job ring and the bounce-step loop.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/omr_capi.h"
+#include "omr_bvh.h"
+#include "omr_kernels.h"
+#include "omr_layout.h"
+
+namespace omr {
+
+constexpr uint32_t kMaxScenes = 64;
+
+struct SceneHost {
+  BuildOutput bvh;
+  SceneDev dev{};
+  std::vector<void *> allocs;
+  size_t bytes = 0;
+  bool has_env = false;
+  uint32_t env_w = 0, env_h = 0;
+  std::atomic<int> users{0};  // live jobs referencing the scene
+};
+
+// One job as the runtime sees it (plain, batched or served).
+struct JobRun {
+  uint32_t scene = 0;
+  omr_camera cam{};
+  omr_render_params params{};
+  uint32_t n_samples = 0;
+  unsigned long long *d_accum = nullptr;  // W*H*3 fixed-point sums
+  float *d_out = nullptr;                 // W*H*3 resolved means (optional)
+  uint32_t ring = 0;                      // ring slot while live
+  uint64_t logical = 0;                   // logical ring index
+  bool done = false;
+};
+
+class Engine {
+ public:
+  explicit Engine(int device);
+  ~Engine();
+  bool ok() const { return init_ok_; }
+
+  std::string err;
+  std::mutex mu;  // guards the stream and all device state
+
+  int device_id() const { return device_; }
+  int n_cu() const { return n_cu_; }
+  size_t hbm_bytes() const { return hbm_; }
+  const std::string &device_name() const { return name_; }
+
+  int upload_scene(const omr_scene_desc *desc, omr_scene_id *id);
+  int release_scene(omr_scene_id id);
+  SceneHost *scene(omr_scene_id id);
+
+  int trace(omr_scene_id id, const float *d_rays, size_t n, int32_t *d_prim, float *d_t, uint64_t *stats2);
+
+  // Runs jobs to completion on the shared queues (plain = one job).
+  int run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st);
+
+  omr_tuning tuning;
+  int set_tuning(const omr_tuning &t);
+  bool profile = false;
+
+  // ---- served mode ----
+  int server_start(const omr_server_config &cfg);
+  int server_stop();
+  int submit(omr_scene_id scene, const omr_camera &cam, const omr_render_params &p, float *out, omr_job_id *id);
+  int poll(omr_job_id id, int *state);
+  int wait(omr_job_id id, int64_t timeout_ms);
+  int cancel(omr_job_id id);
+  int server_stats(omr_server_stats *s);
+  bool server_running() const { return server_on_.load(); }
+
+  int env_tables_host(omr_scene_id id, uint64_t *row, uint64_t *marg);
+
+ private:
+  bool init_ok_ = false;
+  int device_ = 0;
+  int n_cu_ = 0;
+  size_t hbm_ = 0;
+  std::string name_;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr;
+
+  std::vector<std::unique_ptr<SceneHost>> scenes_;
+  SceneDev *d_scenes_ = nullptr;
+
+  // pool
+  PoolPtrs pool_{};
+  uint32_t pool_cap_ = 0;
+  uint32_t grid_threads_ = 0;
+  Ctrl *d_ctrl_ = nullptr;
+  Ctrl *h_ctrl_ = nullptr;               // pinned readback
+  JobDev *d_jobs_ = nullptr;
+  unsigned long long *d_done_ = nullptr; // kJobRing counters
+  unsigned long long *h_done_ = nullptr; // pinned readback
+  uint2 *d_gstack_ = nullptr;
+  uint32_t gcap_ = 0;
+  size_t gstack_threads_ = 0;
+
+  // graphs: [with shadow kernel]
+  hipGraphExec_t graph_[2] = {nullptr, nullptr};
+  uint32_t graph_steps_ = 0;
+
+  int ensure_pool();
+  int ensure_gstack(uint32_t need_entries, size_t threads);
+  void drop_graphs();
+  int launch_steps(uint32_t steps, bool with_shadow);
+  // profiling: event pairs around each launch of the last chunk
+  std::vector<hipEvent_t> prof_ev_;
+  std::vector<int> prof_kind_;  // 0 extend, 1 shade, 2 shadow
+  double prof_ms_[3] = {0, 0, 0};
+  uint64_t prof_extend_launches_ = 0;
+  int collect_profile();
+  LaunchShape shape() const;
+  KernelArgs args() const;
+  int fail(int code, const std::string &msg);
+  int hipfail(hipError_t e, const char *what);
+  void free_scene_mem(SceneHost &s);
+
+  // ---- served mode state ----
+  struct Req {
+    omr_job_id id;
+    omr_scene_id scene;
+    omr_camera cam;
+    omr_render_params params;
+    float *out;
+    std::chrono::steady_clock::time_point t_submit;
+    int state = 0;  // 0 queued 1 running 2 done 3 cancelled 4 failed
+    size_t bytes = 0;
+    JobRun run;
+    bool cancel_req = false;
+    hipEvent_t ev_copy = nullptr;
+    bool copying = false;
+  };
+  std::thread server_;
+  std::atomic<bool> server_on_{false};
+  std::atomic<bool> server_stop_{false};
+  std::mutex smu;
+  std::condition_variable scv;
+  std::deque<std::shared_ptr<Req>> queue_;
+  std::vector<std::shared_ptr<Req>> all_;  // by id-1
+  omr_server_config scfg_{};
+  size_t used_bytes_ = 0;
+  uint64_t n_submitted_ = 0, n_done_ = 0, n_rejected_ = 0, n_cancelled_ = 0;
+  uint64_t srv_segments_ = 0, srv_shadow_ = 0;
+  double busy_ms_ = 0.0;
+  std::vector<double> latencies_;
+  void server_loop();
+};
+
+}  // namespace omr

@@ -1,0 +1,59 @@
+// omr_kernels.h -- host-side declarations of the HIP kernel launchers
+// (implemented in omr_kernels.hip).  Host code sees plain pointers only.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include "omr_layout.h"
+
+namespace omr {
+
+// Entries of the per-lane LDS traversal stack; deeper entries spill to the
+// per-thread global area (PoolPtrs::gstack, gcap entries per thread).
+constexpr uint32_t kShortStack = 16;
+
+// Device pointers of the wavefront path pool (capacity `pool` slots).
+struct PoolPtrs {
+  float *ray_o;        // float4: o.xyz, beta.x
+  float *ray_d;        // float4: d.xyz, beta.y
+  uint32_t *misc;      // uint4: beta.z bits, state word, sample index in job, job ring slot
+  uint32_t *rng;       // uint4: xorshift128 state
+  float *hit;          // float4: t, prim id bits, u, v
+  float *pdf;          // float: MIS pdf of the last Lambert bounce
+  float *pend;         // 3 pending branches x pool x 3 float4
+  uint32_t *queue[2];  // ping-pong ray queues
+  float *shadow;       // shadow queue: pool x 3 float4
+  uint2 *gstack;       // per-thread overflow traversal stacks
+  uint32_t pool;
+  uint32_t gcap;       // entries per thread in gstack
+};
+
+struct KernelArgs {
+  Ctrl *ctrl;
+  JobDev *jobs;          // ring of kJobRing entries
+  const SceneDev *scenes;
+  PoolPtrs p;
+};
+
+struct LaunchShape {
+  uint32_t grid;   // workgroups of the persistent kernels
+  uint32_t wg;     // threads per workgroup: 64, 128 or 256
+};
+
+namespace k {
+hipError_t refill(const KernelArgs &a, const LaunchShape &s, int parity, hipStream_t st);
+// stats: count node visits / primitive tests into Ctrl (profiling runs)
+hipError_t extend(const KernelArgs &a, const LaunchShape &s, int parity, bool stats, hipStream_t st);
+hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, hipStream_t st);
+hipError_t shadow(const KernelArgs &a, const LaunchShape &s, hipStream_t st);
+hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, double n_samples, hipStream_t st);
+// Closest hit for n rays (n x 8 floats).  counters (optional, device) gets
+// node visits and primitive tests.
+hipError_t trace_rays(const SceneDev *scene, const float *rays, uint32_t n, int32_t *prim, float *t,
+                      unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t grid, uint32_t wg,
+                      hipStream_t st);
+// Environment importance-sampling tables: weights, per-row scan, marginal scan.
+hipError_t env_tables(const float *env, uint32_t w, uint32_t h, float *wbuf, unsigned int *wmax_bits,
+                      unsigned long long *q, unsigned long long *row, unsigned long long *marg, hipStream_t st);
+}  // namespace k
+}  // namespace omr

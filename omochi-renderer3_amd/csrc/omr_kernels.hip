@@ -1,0 +1,1107 @@
+// omr_kernels.hip -- wavefront path-tracing kernels for gfx950 (CDNA4).
+//
+// Stages of one bounce step (all persistent, grid-stride over device-side
+// queue counts so a fixed sequence can be captured into a hipGraph):
+//   refill : idle pool slots take new samples (camera rays)      PathTracer.cpp:143-167
+//   extend : closest hit through the 4-wide BVH                  Scene.cpp:51-85, QBVH.cpp:30-162
+//   shade  : material roulette, Russian roulette, Lambert /
+//            mirror / glass, emission and environment           PathTracer.cpp:180-591
+//            accumulation, path regeneration, queue compaction
+//   shadow : any-hit visibility for environment light samples   (no reference counterpart)
+// Queues are compacted with a wave ballot + mbcnt prefix and one atomic per
+// wave; traversal stacks live in LDS (circular short stack) with a per-thread
+// global overflow area.
+//
+// Built with -ffp-contract=off: every floating-point expression below is
+// evaluated in the order written so results match the CPU restatement bit
+// for bit.
+#include <hip/hip_runtime.h>
+
+#include "omr_kernels.h"
+
+namespace omr {
+namespace {
+
+// ------------------------------------------------------------ vector math
+struct f3 {
+  float x, y, z;
+};
+__device__ inline f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ inline f3 add(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline f3 sub(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline f3 scl(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+// Vector3::dot / cross / normalize (src/tools/Vector.h:66-86)
+__device__ inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline f3 cross(f3 a, f3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ inline f3 normalize(f3 a) {
+  float len = sqrtf(dot(a, a));
+  return mk(a.x / len, a.y / len, a.z / len);
+}
+__device__ inline f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
+
+// ------------------------------------------------------------ wave helpers
+__device__ inline uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+__device__ inline uint32_t rank_in(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+// Wave-aggregated append: returns this lane's position in the queue (valid
+// when pred).  Must be reached by every active lane of the wave.
+__device__ inline uint32_t wave_append(uint32_t *counter, bool pred) {
+  unsigned long long m = __ballot(pred);
+  if (m == 0ull) return 0u;
+  uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, (int)leader);
+  return base + rank_in(m);
+}
+
+// ------------------------------------------------------------ RNG
+// Random (src/tools/Random.h:9-29): MT-style seeding recurrence + xorshift128.
+__device__ inline uint4 rng_init(uint32_t seed) {
+  uint32_t s = seed, st[4];
+#pragma unroll
+  for (uint32_t i = 1; i <= 4; i++) {
+    s = 1812433253u * (s ^ (s >> 30)) + i;
+    st[i - 1] = s;
+  }
+  return make_uint4(st[0], st[1], st[2], st[3]);
+}
+__device__ inline uint32_t rng_next(uint4 &st) {
+  uint32_t t = st.x ^ (st.x << 11);
+  st.x = st.y;
+  st.y = st.z;
+  st.z = st.w;
+  st.w = (st.w ^ (st.w >> 19)) ^ (t ^ (t >> 8));
+  return st.w;
+}
+__device__ inline float rng_float(uint4 &st) { return (float)rng_next(st) / 4294967295.0f; }
+
+__device__ inline uint32_t mix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  h *= 0x846ca68bu;
+  h ^= h >> 16;
+  return h;
+}
+__device__ inline uint32_t combine(uint32_t h, uint32_t v) { return mix32(h ^ (v + 0x9e3779b9u + (h << 6) + (h >> 2))); }
+// Per-path seed: hash(seed, pixel, sub-pixel, sample number, branch).
+__device__ inline uint32_t path_seed(unsigned long long seed, uint32_t pixel, uint32_t sub, uint32_t sample,
+                                     uint32_t branch) {
+  uint32_t h = mix32((uint32_t)seed ^ 0x6f6d7233u);
+  h = combine(h, (uint32_t)(seed >> 32));
+  h = combine(h, pixel);
+  h = combine(h, sub);
+  h = combine(h, sample);
+  h = combine(h, branch);
+  return h;
+}
+
+// ------------------------------------------------------------ elementary functions
+// sin/cos of 2*pi*x: quadrant reduction, Taylor polynomials on [-pi/4, pi/4].
+__device__ inline void sincos_turns(float x, float &s_out, float &c_out) {
+  float q = floorf(x * 4.0f + 0.5f);
+  float f = x - q * 0.25f;
+  float a = f * kTwoPi;
+  float a2 = a * a;
+  float sp = a + a * a2 * (-0.16666667f + a2 * (0.0083333333f + a2 * (-0.00019841270f + a2 * 2.7557319e-6f)));
+  float cp = 1.0f + a2 * (-0.5f + a2 * (0.041666668f + a2 * (-0.0013888889f + a2 * 2.4801587e-5f)));
+  int qi = ((int)q) & 3;
+  float s, c;
+  if (qi == 0) { s = sp; c = cp; }
+  else if (qi == 1) { s = cp; c = -sp; }
+  else if (qi == 2) { s = -sp; c = -cp; }
+  else { s = -cp; c = sp; }
+  s_out = s;
+  c_out = c;
+}
+__device__ inline float asin_small(float z) {
+  float z2 = z * z;
+  float p = 0.0083903358f;
+  p = 0.0097616095f + z2 * p;
+  p = 0.011551800f + z2 * p;
+  p = 0.013964844f + z2 * p;
+  p = 0.017352764f + z2 * p;
+  p = 0.022372159f + z2 * p;
+  p = 0.030381944f + z2 * p;
+  p = 0.044642858f + z2 * p;
+  p = 0.075f + z2 * p;
+  p = 0.16666667f + z2 * p;
+  return z + z * z2 * p;
+}
+__device__ inline float acos_f(float x) {
+  if (!(x > -1.0f)) x = -1.0f;
+  if (x > 1.0f) x = 1.0f;
+  float ax = fabsf(x);
+  float r;
+  if (ax <= 0.5f) {
+    r = 1.5707964f - asin_small(ax);
+    if (x < 0.0f) r = kPi - r;
+  } else {
+    float s = sqrtf((1.0f - ax) * 0.5f);
+    r = 2.0f * asin_small(s);
+    if (x < 0.0f) r = kPi - r;
+  }
+  return r;
+}
+
+// ------------------------------------------------------------ primitives
+// Polygon::CheckIntersection (src/renderer/Polygon.h:147-190), single-sided.
+__device__ inline bool hit_triangle(f3 o, f3 d, f3 p0, f3 e1, f3 e2, float &t_out, float &u_out, float &v_out) {
+  f3 P = cross(d, e2);
+  float det = dot(P, e1);
+  if (!(det > kEps)) return false;
+  f3 T = sub(o, p0);
+  float u = dot(P, T);
+  if (!(u >= 0.0f && u <= det)) return false;
+  f3 Q = cross(T, e1);
+  float v = dot(Q, d);
+  if (!(v >= 0.0f && u + v <= det)) return false;
+  float t = dot(Q, e2) / det;
+  if (!(t >= kEps)) return false;
+  t_out = t;
+  u_out = u / det;
+  v_out = v / det;
+  return true;
+}
+// Sphere::CheckIntersection (src/renderer/Sphere.h:21-48) with the
+// cancellation-free discriminant r^2 - |(c-x) - b v|^2 (|v| = 1).
+__device__ inline bool hit_sphere(f3 x, f3 v, f3 c, float r, float &t_out) {
+  f3 oc = sub(c, x);
+  float b = dot(v, oc);
+  f3 perp = sub(oc, scl(v, b));
+  float D = r * r - dot(perp, perp);
+  if (D < 0.0f) return false;
+  float sD = sqrtf(D);
+  float t1 = sD + b;
+  float t2 = -sD + b;
+  if (t1 <= kEps && t2 <= kEps) return false;
+  t_out = (t2 > kEps) ? t2 : t1;
+  return true;
+}
+
+// ------------------------------------------------------------ traversal
+struct Hit {
+  float t, u, v;
+  int32_t prim;
+};
+
+// Circular LDS short stack of SHORT entries per lane, spilling the oldest
+// entries to a per-thread global area.  Entry = (child code, entry distance).
+template <int WG, int SHORT>
+struct Stack {
+  uint2 *lds;
+  uint2 *g;
+  uint32_t sp;
+  __device__ inline void push(uint2 e) {
+    uint32_t slot = sp % SHORT;
+    uint2 *cell = &lds[slot * WG + threadIdx.x];
+    if (sp >= SHORT) g[sp - SHORT] = *cell;
+    *cell = e;
+    sp++;
+  }
+  __device__ inline uint2 pop() {
+    sp--;
+    uint32_t slot = sp % SHORT;
+    uint2 *cell = &lds[slot * WG + threadIdx.x];
+    uint2 e = *cell;
+    if (sp >= SHORT) *cell = g[sp - SHORT];
+    return e;
+  }
+};
+
+__device__ inline void cswap(float &ka, int32_t &ca, float &kb, int32_t &cb) {
+  if (ka > kb) {
+    float tk = ka; ka = kb; kb = tk;
+    int32_t tc = ca; ca = cb; cb = tc;
+  }
+}
+
+// Closest (ANY = false) or any (ANY = true) hit in [EPS, best.t).  Restates
+// QBVH::CheckIntersection (src/renderer/QBVH.cpp:30-144): slab test of four
+// boxes per node (BoundingBox.h:127-172), nearest-first order, leaves tested
+// as they are reached, pruning by the closest distance so far; ties on t go
+// to the smaller primitive id so the result does not depend on the order.
+template <int WG, int SHORT, bool ANY, bool STATS>
+__device__ inline void traverse(const SceneDev &S, f3 o, f3 d, Hit &best, Stack<WG, SHORT> &stk, uint32_t &nvis,
+                                uint32_t &ntest) {
+  const float ox = o.x, oy = o.y, oz = o.z;
+  // QBVH.cpp:56-60: zero direction component -> INF
+  const float ix = (d.x == 0.0f) ? kInf : 1.0f / d.x;
+  const float iy = (d.y == 0.0f) ? kInf : 1.0f / d.y;
+  const float iz = (d.z == 0.0f) ? kInf : 1.0f / d.z;
+  const bool sx = !(d.x >= 0.0f), sy = !(d.y >= 0.0f), sz = !(d.z >= 0.0f);
+  const float4 *nodes = reinterpret_cast<const float4 *>(S.nodes);
+  const float4 *prims = reinterpret_cast<const float4 *>(S.prims);
+  stk.sp = 0;
+  int32_t cur = 0;
+  for (;;) {
+    bool need_pop;
+    if (!child_is_leaf(cur)) {
+      const float4 *np = nodes + (size_t)cur * 8;
+      float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+      int4 ch = *reinterpret_cast<const int4 *>(np + 6);
+      if (STATS) nvis++;
+      float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, hi_x[4] = {hx.x, hx.y, hx.z, hx.w};
+      float lo_y[4] = {ly.x, ly.y, ly.z, ly.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w};
+      float lo_z[4] = {lz.x, lz.y, lz.z, lz.w}, hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
+      int32_t code[4] = {ch.x, ch.y, ch.z, ch.w};
+      float key[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        float tmin = 0.0f, tmax = best.t;
+        float n = ((sx ? hi_x[k] : lo_x[k]) - ox) * ix;
+        float f = ((sx ? lo_x[k] : hi_x[k]) - ox) * ix;
+        tmin = n > tmin ? n : tmin;
+        tmax = f < tmax ? f : tmax;
+        n = ((sy ? hi_y[k] : lo_y[k]) - oy) * iy;
+        f = ((sy ? lo_y[k] : hi_y[k]) - oy) * iy;
+        tmin = n > tmin ? n : tmin;
+        tmax = f < tmax ? f : tmax;
+        n = ((sz ? hi_z[k] : lo_z[k]) - oz) * iz;
+        f = ((sz ? lo_z[k] : hi_z[k]) - oz) * iz;
+        tmin = n > tmin ? n : tmin;
+        tmax = f < tmax ? f : tmax;
+        bool h = (code[k] != -1) && (tmin <= tmax * kSlabInflate);
+        key[k] = h ? tmin : __builtin_huge_valf();
+      }
+      // sorting network, nearest first
+      cswap(key[0], code[0], key[1], code[1]);
+      cswap(key[2], code[2], key[3], code[3]);
+      cswap(key[0], code[0], key[2], code[2]);
+      cswap(key[1], code[1], key[3], code[3]);
+      cswap(key[1], code[1], key[2], code[2]);
+      const float kmiss = __builtin_huge_valf();
+      if (key[3] != kmiss) stk.push(make_uint2((uint32_t)code[3], __float_as_uint(key[3])));
+      if (key[2] != kmiss) stk.push(make_uint2((uint32_t)code[2], __float_as_uint(key[2])));
+      if (key[1] != kmiss) stk.push(make_uint2((uint32_t)code[1], __float_as_uint(key[1])));
+      if (key[0] != kmiss && !(key[0] > best.t)) {
+        cur = code[0];
+        need_pop = false;
+      } else {
+        need_pop = true;
+      }
+    } else {
+      uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
+      for (uint32_t k = 0; k < cnt; k++) {
+        const float4 *pr = prims + (size_t)(first + k) * 3;
+        float4 a = pr[0], b = pr[1];
+        uint32_t id = __float_as_uint(a.w);
+        uint32_t kind = __float_as_uint(b.w);
+        float t, u = 0.0f, v = 0.0f;
+        bool ok;
+        if (STATS) ntest++;
+        if (kind == 0u) {
+          float4 c = pr[2];
+          ok = hit_triangle(o, d, xyz(a), xyz(b), xyz(c), t, u, v);
+        } else {
+          ok = hit_sphere(o, d, xyz(a), b.x, t);
+        }
+        if (ok && (t < best.t || (t == best.t && best.prim >= 0 && (int32_t)id < best.prim))) {
+          best.t = t;
+          best.u = u;
+          best.v = v;
+          best.prim = (int32_t)id;
+        }
+      }
+      if (ANY && best.prim >= 0) return;
+      need_pop = true;
+    }
+    if (need_pop) {
+      bool found = false;
+      while (stk.sp > 0) {
+        uint2 e = stk.pop();
+        if (__uint_as_float(e.y) > best.t) continue;
+        cur = (int32_t)e.x;
+        found = true;
+        break;
+      }
+      if (!found) return;
+    }
+  }
+}
+
+// ------------------------------------------------------------ environment
+// IBL::Sample(dir) (src/renderer/IBL.h:30-45) + PhiThetaToUV (:67-71).
+__device__ inline void env_texel(f3 d, uint32_t w, uint32_t h, uint32_t &ix, uint32_t &iy) {
+  float theta = acos_f(d.y);
+  float rxz = sqrtf(d.x * d.x + d.z * d.z);
+  float phi = (rxz > 0.0f) ? acos_f(d.x / rxz) : 0.0f;
+  if (d.z < 0.0f) phi = kTwoPi - phi;
+  float u = phi / kTwoPi;
+  float v = theta / kPi;
+  ix = (uint32_t)((int)(u * (float)w) % (int)w);
+  iy = (uint32_t)((int)(v * (float)h) % (int)h);
+}
+__device__ inline f3 env_rgb(const SceneDev &S, f3 d) {
+  uint32_t ix, iy;
+  env_texel(d, S.env_w, S.env_h, ix, iy);
+  const float *p = S.env + 3 * ((size_t)iy * S.env_w + ix);
+  return mk(p[0], p[1], p[2]);
+}
+__device__ inline float env_pdf(const SceneDev &S, f3 d) {
+  unsigned long long tot = S.env_marg[S.env_h];
+  if (tot == 0ull) return 0.0f;
+  uint32_t ix, iy;
+  env_texel(d, S.env_w, S.env_h, ix, iy);
+  unsigned long long q = S.env_q[(size_t)iy * S.env_w + ix];
+  float st = sqrtf(d.x * d.x + d.z * d.z);
+  if (q == 0ull || !(st > 0.0f)) return 0.0f;
+  double p = (double)q / (double)tot * ((double)S.env_w * (double)S.env_h) / (19.739208802178716 * (double)st);
+  return (float)p;
+}
+__device__ inline uint32_t search_le(const unsigned long long *a, uint32_t n, unsigned long long key) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] <= key) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+__device__ inline bool env_sample4(const SceneDev &S, float u1, float u2, float u3, float u4, f3 &dir) {
+  uint32_t W = S.env_w, H = S.env_h;
+  unsigned long long tot = S.env_marg[H];
+  if (tot == 0ull) return false;
+  unsigned long long t = (unsigned long long)((double)u1 * (double)tot);
+  if (t >= tot) t = tot - 1;
+  uint32_t iy = search_le(S.env_marg, H, t);
+  const unsigned long long *row = S.env_row + (size_t)iy * (W + 1);
+  unsigned long long rt = row[W];
+  unsigned long long t2 = (unsigned long long)((double)u2 * (double)rt);
+  if (t2 >= rt) t2 = rt - 1;
+  uint32_t ix = search_le(row, W, t2);
+  float fx = ((float)ix + u3) / (float)W;
+  float fy = ((float)iy + u4) / (2.0f * (float)H);
+  if (fx >= 1.0f) fx = 0.99999994f;
+  if (fy >= 0.5f) fy = 0.49999997f;
+  float sp, cp, sth, cth;
+  sincos_turns(fx, sp, cp);
+  sincos_turns(fy, sth, cth);
+  dir = mk(sth * cp, cth, sth * sp);
+  return true;
+}
+
+// ------------------------------------------------------------ accumulation
+__device__ inline unsigned long long fix(float v) {
+  if (!(v > 0.0f)) return 0ull;
+  if (v > kFixClamp) v = kFixClamp;
+  return (unsigned long long)(long long)rintf(v * kFixScale);
+}
+__device__ inline void add_px(unsigned long long *acc, f3 beta, f3 L, float s) {
+  unsigned long long a = fix(beta.x * L.x * s), b = fix(beta.y * L.y * s), c = fix(beta.z * L.z * s);
+  if (a) atomicAdd(acc + 0, a);
+  if (b) atomicAdd(acc + 1, b);
+  if (c) atomicAdd(acc + 2, c);
+}
+
+// ------------------------------------------------------------ job lookup
+__device__ inline uint32_t find_job(const Ctrl *ctrl, const JobDev *jobs, unsigned long long g) {
+  uint32_t lo = ctrl->job_lo, hi = ctrl->job_hi;  // answer in [lo, hi)
+  while (hi - lo > 1) {
+    uint32_t mid = lo + ((hi - lo) >> 1);
+    if (jobs[mid % kJobRing].sample_base <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo % kJobRing;
+}
+
+// Camera::SampleRayForPixel (src/renderer/Camera.h:65-93) + CircleAperture
+// (src/renderer/Aperture.h:23-35); sub-pixel centres (PathTracer.cpp:158-161).
+struct Decomp {
+  uint32_t pixel, sub, sample_no;
+};
+__device__ inline Decomp decompose(const JobDev &J, uint32_t s) {
+  uint32_t spp_px = J.ss * J.ss * J.nspp;
+  Decomp r;
+  r.pixel = s / spp_px;
+  uint32_t rem = s - r.pixel * spp_px;
+  r.sub = rem / J.nspp;
+  r.sample_no = J.spp_begin + 1u + (rem - r.sub * J.nspp);
+  return r;
+}
+__device__ inline void camera_ray(const JobDev &J, const Decomp &dc, f3 &o, f3 &d) {
+  uint32_t W = J.width, H = J.height, ss = J.ss;
+  uint32_t r = dc.pixel / W, x = dc.pixel - r * W;
+  uint32_t y = H - 1u - r;
+  uint32_t sxi = dc.sub % ss, syi = dc.sub / ss;
+  float rx = (2.0f * (float)sxi + 1.0f) / (2.0f * (float)ss);
+  float ry = (2.0f * (float)syi + 1.0f) / (2.0f * (float)ss);
+  float xs = (float)x + rx, ys = (float)y + ry;
+  f3 pos = mk(J.cam_pos[0], J.cam_pos[1], J.cam_pos[2]);
+  f3 C = mk(J.cam_c[0], J.cam_c[1], J.cam_c[2]);
+  f3 X = mk(J.cam_x[0], J.cam_x[1], J.cam_x[2]);
+  f3 Y = mk(J.cam_y[0], J.cam_y[1], J.cam_y[2]);
+  float a = xs / (float)W - 0.5f;
+  float b = ys / (float)H - 0.5f;
+  f3 target = add(add(C, scl(X, a)), scl(Y, b));
+  f3 dir = normalize(sub(target, pos));
+  f3 org = pos;
+  if (J.aperture > 0.0f) {
+    uint4 lr = rng_init(path_seed(J.seed, dc.pixel, dc.sub, J.spp_begin, 0xFFFFFFFFu));
+    float l0 = rng_float(lr), l1 = rng_float(lr);
+    float st, ct;
+    sincos_turns(l0, st, ct);
+    float rr = J.aperture * sqrtf(l1);
+    float dx = rr * ct, dy = rr * st;
+    float ft = J.focal / fabsf(dir.z);
+    f3 focus = add(pos, scl(dir, ft));
+    org = add(pos, mk(dx, dy, 0.0f));
+    dir = normalize(sub(focus, org));
+  }
+  o = org;
+  d = dir;
+}
+
+// Wave-level grab of up to `want` lanes' worth of samples from the global
+// cursor (CAS so the cursor never passes the admitted total).  Returns this
+// lane's global sample index, or ~0ull when none is left for it.
+__device__ inline unsigned long long grab_sample(Ctrl *ctrl, bool want) {
+  unsigned long long m = __ballot(want);
+  if (m == 0ull) return ~0ull;
+  uint32_t cnt = (uint32_t)__popcll(m);
+  uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
+  unsigned long long base = 0ull, take = 0ull;
+  if (lane_id() == leader) {
+    unsigned long long total = __atomic_load_n(&ctrl->total_samples, __ATOMIC_RELAXED);
+    unsigned long long old = __atomic_load_n(&ctrl->next_sample, __ATOMIC_RELAXED);
+    for (;;) {
+      unsigned long long avail = total > old ? total - old : 0ull;
+      take = avail < cnt ? avail : (unsigned long long)cnt;
+      if (take == 0ull) break;
+      unsigned long long prev = atomicCAS(&ctrl->next_sample, old, old + take);
+      if (prev == old) break;
+      old = prev;
+    }
+    base = old;
+  }
+  base = __shfl(base, (int)leader);
+  take = __shfl(take, (int)leader);
+  uint32_t r = rank_in(m);
+  if (!want || (unsigned long long)r >= take) return ~0ull;
+  return base + r;
+}
+
+// Wave-aggregated per-job completion counting.
+__device__ inline void count_done(JobDev *jobs, bool fin, uint32_t job) {
+  bool need = fin;
+  for (;;) {
+    unsigned long long m = __ballot(need);
+    if (m == 0ull) break;
+    uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
+    uint32_t lj = __shfl(job, (int)leader);
+    bool mine = need && job == lj;
+    unsigned long long mm = __ballot(mine);
+    if (lane_id() == leader) atomicAdd(jobs[lj].done, (unsigned long long)__popcll(mm));
+    if (mine) need = false;
+  }
+}
+
+// Writes a fresh camera path for global sample g into slot.
+__device__ inline void start_sample(const KernelArgs &A, uint32_t slot, unsigned long long g) {
+  uint32_t js = find_job(A.ctrl, A.jobs, g);
+  const JobDev &J = A.jobs[js];
+  uint32_t s = (uint32_t)(g - J.sample_base);
+  Decomp dc = decompose(J, s);
+  f3 o, d;
+  camera_ray(J, dc, o, d);
+  uint4 rng = rng_init(path_seed(J.seed, dc.pixel, dc.sub, dc.sample_no, 0u));
+  reinterpret_cast<float4 *>(A.p.ray_o)[slot] = make_float4(o.x, o.y, o.z, 1.0f);
+  reinterpret_cast<float4 *>(A.p.ray_d)[slot] = make_float4(d.x, d.y, d.z, 1.0f);
+  reinterpret_cast<uint4 *>(A.p.misc)[slot] = make_uint4(__float_as_uint(1.0f), ps_pack(0, 0, 0, 0), s, js);
+  reinterpret_cast<uint4 *>(A.p.rng)[slot] = rng;
+}
+
+// ------------------------------------------------------------ kernels
+// refill: every inactive slot takes a new sample while samples remain.
+template <int WG>
+__global__ __launch_bounds__(WG) void k_refill(KernelArgs A, int parity) {
+  const uint32_t P = A.p.pool;
+  const uint32_t stride = gridDim.x * WG;
+  for (uint32_t base = blockIdx.x * WG; base < P; base += stride) {
+    uint32_t slot = base + threadIdx.x;
+    bool valid = slot < P;
+    bool idle = valid && ((reinterpret_cast<const uint4 *>(A.p.misc)[slot].y >> 19) & 1u) == 0u;
+    if (__ballot(idle) == 0ull) continue;
+    unsigned long long g = grab_sample(A.ctrl, idle);
+    bool got = g != ~0ull;
+    if (got) start_sample(A, slot, g);
+    uint32_t pos = wave_append(&A.ctrl->qcount[parity], got);
+    if (got) A.p.queue[parity][pos] = slot;
+  }
+}
+
+template <int WG, int SHORT, bool STATS>
+__global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
+  __shared__ uint2 lds[SHORT * WG];
+  const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
+  if (gtid == 0) A.ctrl->qcount[parity ^ 1] = 0u;
+  const uint32_t n = A.ctrl->qcount[parity];
+  const uint32_t stride = gridDim.x * WG;
+  Stack<WG, SHORT> stk;
+  stk.lds = lds;
+  stk.g = A.p.gstack + (size_t)gtid * A.p.gcap;
+  uint32_t segs = 0, nv = 0, nt = 0;
+  for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
+    uint32_t i = base + threadIdx.x;
+    if (i < n) {
+      uint32_t slot = A.p.queue[parity][i];
+      float4 ro = reinterpret_cast<const float4 *>(A.p.ray_o)[slot];
+      float4 rd = reinterpret_cast<const float4 *>(A.p.ray_d)[slot];
+      uint32_t js = A.p.misc[4 * (size_t)slot + 3];
+      const SceneDev &S = A.scenes[A.jobs[js].scene];
+      Hit h;
+      h.t = kInf;
+      h.u = h.v = 0.0f;
+      h.prim = -1;
+      traverse<WG, SHORT, false, STATS>(S, xyz(ro), xyz(rd), h, stk, nv, nt);
+      reinterpret_cast<float4 *>(A.p.hit)[slot] = make_float4(h.t, __int_as_float(h.prim), h.u, h.v);
+      segs++;
+    }
+  }
+  // one atomic per wave for each counter
+  unsigned long long tot = segs, a = nv, b = nt;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    tot += __shfl_xor(tot, off);
+    if (STATS) {
+      a += __shfl_xor(a, off);
+      b += __shfl_xor(b, off);
+    }
+  }
+  if (lane_id() == 0) {
+    if (tot) atomicAdd(&A.ctrl->segments, tot);
+    if (STATS && a) atomicAdd(&A.ctrl->node_visits, a);
+    if (STATS && b) atomicAdd(&A.ctrl->prim_tests, b);
+  }
+}
+
+// One shading step; mirrors the CPU restatement's per-segment loop body.
+template <int WG>
+__global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity) {
+  const uint32_t n = A.ctrl->qcount[parity];
+  const uint32_t stride = gridDim.x * WG;
+  const uint32_t P = A.p.pool;
+  float4 *ray_o = reinterpret_cast<float4 *>(A.p.ray_o);
+  float4 *ray_d = reinterpret_cast<float4 *>(A.p.ray_d);
+  uint4 *misc = reinterpret_cast<uint4 *>(A.p.misc);
+  uint4 *rngs = reinterpret_cast<uint4 *>(A.p.rng);
+  float4 *pend = reinterpret_cast<float4 *>(A.p.pend);
+  for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
+    uint32_t i = base + threadIdx.x;
+    bool valid = i < n;
+    uint32_t slot = valid ? A.p.queue[parity][i] : 0u;
+    bool cont = false, fin = false, shadow = false;
+    uint32_t js = 0;
+    f3 o = mk(0, 0, 0), d = mk(0, 0, 0), beta = mk(0, 0, 0);
+    uint32_t word = 0, sidx = 0;
+    uint4 rng = make_uint4(0, 0, 0, 0);
+    float pdf_next = 0.0f;
+    f3 sh_o = mk(0, 0, 0), sh_d = mk(0, 0, 0), sh_c = mk(0, 0, 0);
+    uint32_t sh_px = 0;
+    if (valid) {
+      float4 ro = ray_o[slot], rd = ray_d[slot];
+      uint4 ms = misc[slot];
+      float4 hr = reinterpret_cast<const float4 *>(A.p.hit)[slot];
+      rng = rngs[slot];
+      o = xyz(ro);
+      d = xyz(rd);
+      beta = mk(ro.w, rd.w, __uint_as_float(ms.x));
+      word = ms.y;
+      sidx = ms.z;
+      js = ms.w;
+      const JobDev &J = A.jobs[js];
+      const SceneDev &S = A.scenes[J.scene];
+      uint32_t depth = ps_depth(word), branch = ps_branch(word), npend = ps_pending(word);
+      bool env_on = S.env != nullptr;
+      bool nee = env_on && (J.flags & 1u) && S.env_marg[S.env_h] > 0ull;
+      uint32_t pixel = decompose(J, sidx).pixel;
+      unsigned long long *acc = J.accum + 3ull * pixel;
+      int32_t prim = __float_as_int(hr.y);
+      uint32_t mis_next = 0;
+      if (prim < 0) {
+        // PathTracer.cpp:357-363: miss -> IBL lookup or black background
+        if (env_on) {
+          f3 L = env_rgb(S, d);
+          float w = 1.0f;
+          if (ps_mis(word)) {
+            float pp = A.p.pdf[slot];
+            float pe = env_pdf(S, d);
+            w = (pp * pp) / (pp * pp + pe * pe);
+          }
+          add_px(acc, beta, L, w);
+        }
+      } else {
+        uint32_t pid = (uint32_t)prim;
+        f3 x = add(o, scl(d, hr.x));
+        f3 nrm;
+        if (pid < S.n_tri) {
+          const float4 *tn = reinterpret_cast<const float4 *>(S.tri_n) + (size_t)pid * 3;
+          float4 n0 = tn[0], n1 = tn[1], n2 = tn[2];
+          nrm = normalize(add(add(scl(xyz(n1), hr.z), scl(xyz(n2), hr.w)), xyz(n0)));
+        } else {
+          float4 sc = reinterpret_cast<const float4 *>(S.sph)[pid - S.n_tri];
+          nrm = normalize(sub(x, xyz(sc)));
+        }
+        f3 nl = dot(nrm, d) < 0.0f ? nrm : scl(nrm, -1.0f);
+        // material roulette (PathTracer.cpp:368-386)
+        uint32_t pm = S.prim_mat[pid];
+        uint32_t mfirst = pm & 0x00FFFFFFu, mcount = (pm >> 24) + 1u;
+        float value = rng_float(rng);
+        uint32_t sel = mcount - 1u;
+        for (uint32_t q = 0; q < mcount; q++) {
+          float r = S.mats[mfirst + q].rate;
+          if (value <= r) { sel = q; break; }
+          value -= r;
+        }
+        const MatRec &m = S.mats[mfirst + sel];
+        f3 col = mk(m.color[0], m.color[1], m.color[2]);
+        f3 em = mk(m.emission[0], m.emission[1], m.emission[2]);
+        // Russian roulette (PathTracer.cpp:398-416)
+        float prr = col.x > (col.y > col.z ? col.y : col.z) ? col.x : (col.y > col.z ? col.y : col.z);
+        if (J.max_depth == 0u && depth > 64u) prr *= ldexpf(1.0f, -(int)(depth - 64u));
+        bool killed = false;
+        if (depth > J.rr_depth) {
+          if (rng_float(rng) >= prr) killed = true;
+        } else {
+          prr = 1.0f;
+        }
+        add_px(acc, beta, em, 1.0f);
+        uint32_t dmax = J.max_depth == 0u ? 255u : J.max_depth;
+        bool can_continue = !killed && depth + 1u < dmax;
+        if (!killed) {
+          if (m.type == 0u) {
+            // Radiance_Lambert (PathTracer.cpp:445-500), unnormalized basis
+            if (dot(em, em) == 0.0f) {
+              f3 w = nl;
+              f3 u = fabsf(nl.x) > kEps ? cross(mk(0.0f, 1.0f, 0.0f), w) : cross(mk(1.0f, 0.0f, 0.0f), w);
+              f3 vv = cross(w, u);
+              float u1 = rng_float(rng), u2 = rng_float(rng);
+              float s1, c1;
+              sincos_turns(u1, s1, c1);
+              float r2 = sqrtf(u2), r3 = sqrtf(1.0f - u2);
+              f3 nd = normalize(add(add(scl(scl(u, r3), c1), scl(scl(vv, r3), s1)), scl(w, r2)));
+              float k = 1.0f / prr;
+              if (nee && can_continue) {
+                float e1 = rng_float(rng), e2 = rng_float(rng), e3 = rng_float(rng), e4 = rng_float(rng);
+                f3 ld;
+                if (env_sample4(S, e1, e2, e3, e4, ld)) {
+                  float c = dot(nl, ld);
+                  float pe = env_pdf(S, ld);
+                  if (c > 0.0f && pe > 0.0f) {
+                    float pb = c * kInvPi;
+                    float wl = (pe * pe) / (pe * pe + pb * pb);
+                    float s = ((pb / pe) * wl) * k;
+                    f3 L = env_rgb(S, ld);
+                    f3 bc = mk(beta.x * col.x, beta.y * col.y, beta.z * col.z);
+                    sh_c = mk(bc.x * L.x * s, bc.y * L.y * s, bc.z * L.z * s);
+                    sh_o = x;
+                    sh_d = ld;
+                    sh_px = pixel;
+                    shadow = true;
+                  }
+                }
+              }
+              if (can_continue) {
+                beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
+                if (nee) {
+                  mis_next = 1u;
+                  pdf_next = dot(nl, nd) * kInvPi;
+                }
+                o = x;
+                d = nd;
+                depth++;
+                cont = true;
+              }
+            }
+          } else if (m.type == 1u) {
+            // Radiance_Specular (PathTracer.cpp:503-520)
+            f3 r = normalize(sub(d, scl(scl(nl, 2.0f), dot(d, nl))));
+            if (can_continue) {
+              float k = 1.0f / prr;
+              beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
+              o = x;
+              d = r;
+              depth++;
+              cont = true;
+            }
+          } else {
+            // Radiance_Refraction (PathTracer.cpp:523-591)
+            bool into = dot(nrm, nl) > 0.0f;
+            f3 refl = normalize(sub(d, scl(scl(nl, 2.0f), dot(d, nl))));
+            float nr = into ? 1.0f / m.ior : m.ior / 1.0f;
+            float dd = dot(d, nl);
+            float cos2t = 1.0f - nr * nr * (1.0f - dd * dd);
+            if (cos2t < 0.0f) {
+              if (can_continue) {
+                float k = 1.0f / prr;
+                beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
+                o = x;
+                d = refl;
+                depth++;
+                cont = true;
+              }
+            } else {
+              f3 refr = normalize(sub(scl(d, nr), scl(scl(nrm, into ? 1.0f : -1.0f), dd * nr + sqrtf(cos2t))));
+              float F0 = ((m.ior - 1.0f) * (m.ior - 1.0f)) / ((m.ior + 1.0f) * (m.ior + 1.0f));
+              float c = 1.0f - (into ? -dd : -dot(refr, nl));
+              float c5 = c * c * c * c * c;
+              float Fr = F0 + (1.0f - F0) * c5;
+              float Tr = (1.0f - Fr) * (nr * nr);
+              if (depth > 2u) {
+                float prob = 0.1f + 0.8f * Fr;
+                if (rng_float(rng) < prob) {
+                  if (can_continue) {
+                    float k = Fr / (prr * prob);
+                    beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
+                    o = x;
+                    d = refl;
+                    depth++;
+                    cont = true;
+                  }
+                } else if (can_continue) {
+                  float k = Tr / (prr * (1.0f - prob));
+                  beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
+                  o = x;
+                  d = refr;
+                  depth++;
+                  cont = true;
+                }
+              } else if (can_continue) {
+                // depth <= 2: both branches; the refracted one waits in the
+                // slot's pending stack, the reflected one continues now.
+                float kt = Tr / prr;
+                f3 bt = mk(beta.x * (col.x * kt), beta.y * (col.y * kt), beta.z * (col.z * kt));
+                float4 *pe = pend + ((size_t)npend * P + slot) * 3;
+                pe[0] = make_float4(x.x, x.y, x.z, bt.x);
+                pe[1] = make_float4(refr.x, refr.y, refr.z, bt.y);
+                pe[2] = make_float4(bt.z, __uint_as_float(depth + 1u), __uint_as_float(branch | (1u << depth)), 0.0f);
+                npend++;
+                float kr = Fr / prr;
+                beta = mk(beta.x * (col.x * kr), beta.y * (col.y * kr), beta.z * (col.z * kr));
+                o = x;
+                d = refl;
+                depth++;
+                cont = true;
+              }
+            }
+          }
+        }
+      }
+      if (cont) {
+        word = ps_pack(depth, branch, npend, mis_next);
+      } else if (npend > 0u) {
+        // resume the most recent pending branch with its own RNG stream
+        npend--;
+        const float4 *pe = pend + ((size_t)npend * P + slot) * 3;
+        float4 a = pe[0], b = pe[1], c = pe[2];
+        o = xyz(a);
+        d = xyz(b);
+        beta = mk(a.w, b.w, c.x);
+        uint32_t nd = __float_as_uint(c.y), nb = __float_as_uint(c.z);
+        Decomp dc = decompose(J, sidx);
+        rng = rng_init(path_seed(J.seed, dc.pixel, dc.sub, dc.sample_no, nb));
+        word = ps_pack(nd, nb, npend, 0u);
+        cont = true;
+      } else {
+        fin = true;
+      }
+    }
+    count_done(A.jobs, fin, js);
+    // regenerate finished slots while samples remain
+    unsigned long long g = grab_sample(A.ctrl, fin);
+    bool fresh = g != ~0ull;
+    if (fresh) {
+      start_sample(A, slot, g);
+    } else if (fin) {
+      misc[slot].y = 0u;  // inactive
+    }
+    if (valid && cont) {
+      ray_o[slot] = make_float4(o.x, o.y, o.z, beta.x);
+      ray_d[slot] = make_float4(d.x, d.y, d.z, beta.y);
+      misc[slot] = make_uint4(__float_as_uint(beta.z), word, sidx, js);
+      rngs[slot] = rng;
+      if (ps_mis(word)) A.p.pdf[slot] = pdf_next;
+    }
+    bool enq = valid && (cont || fresh);
+    uint32_t pos = wave_append(&A.ctrl->qcount[parity ^ 1], enq);
+    if (enq) A.p.queue[parity ^ 1][pos] = slot;
+    uint32_t spos = wave_append(&A.ctrl->scount, shadow);
+    if (shadow) {
+      float4 *sq = reinterpret_cast<float4 *>(A.p.shadow) + (size_t)spos * 3;
+      sq[0] = make_float4(sh_o.x, sh_o.y, sh_o.z, __uint_as_float(js));
+      sq[1] = make_float4(sh_d.x, sh_d.y, sh_d.z, __uint_as_float(sh_px));
+      sq[2] = make_float4(sh_c.x, sh_c.y, sh_c.z, 0.0f);
+    }
+  }
+}
+
+template <int WG, int SHORT>
+__global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
+  __shared__ uint2 lds[SHORT * WG];
+  const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
+  const uint32_t n = A.ctrl->scount;
+  const uint32_t stride = gridDim.x * WG;
+  Stack<WG, SHORT> stk;
+  stk.lds = lds;
+  stk.g = A.p.gstack + (size_t)gtid * A.p.gcap;
+  uint32_t cnt = 0;
+  for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
+    uint32_t i = base + threadIdx.x;
+    if (i < n) {
+      const float4 *sq = reinterpret_cast<const float4 *>(A.p.shadow) + (size_t)i * 3;
+      float4 a = sq[0], b = sq[1], c = sq[2];
+      uint32_t js = __float_as_uint(a.w), px = __float_as_uint(b.w);
+      const JobDev &J = A.jobs[js];
+      const SceneDev &S = A.scenes[J.scene];
+      Hit h;
+      h.t = kInf;
+      h.u = h.v = 0.0f;
+      h.prim = -1;
+      uint32_t nv = 0, nt = 0;
+      traverse<WG, SHORT, true, false>(S, xyz(a), xyz(b), h, stk, nv, nt);
+      if (h.prim < 0) {
+        unsigned long long *acc = J.accum + 3ull * px;
+        unsigned long long qa = fix(c.x), qb = fix(c.y), qc = fix(c.z);
+        if (qa) atomicAdd(acc + 0, qa);
+        if (qb) atomicAdd(acc + 1, qb);
+        if (qc) atomicAdd(acc + 2, qc);
+      }
+      cnt++;
+    }
+  }
+  uint32_t tot = cnt;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) tot += __shfl_xor(tot, off);
+  if (lane_id() == 0 && tot) atomicAdd(&A.ctrl->shadow_rays, (unsigned long long)tot);
+  __syncthreads();
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    // the next step's shade appends afresh; this kernel is the last reader
+  }
+}
+
+// Clears the shadow queue count at the start of a step (single thread).
+__global__ void k_clear_shadow(Ctrl *ctrl) { ctrl->scount = 0u; }
+
+// mean = sum * 2^-24 / (ss^2 * samples), in double, as the CPU restatement
+__global__ void k_resolve(const unsigned long long *accum, float *out, uint32_t n, double n_samples) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)((double)(long long)accum[i] * (1.0 / 16777216.0) / n_samples);
+}
+
+template <int WG, int SHORT, bool STATS>
+__global__ __launch_bounds__(WG) void k_trace_rays(const SceneDev *scene, const float4 *rays, uint32_t n, int32_t *prim,
+                                                   float *tout, unsigned long long *counters, uint2 *gstack,
+                                                   uint32_t gcap) {
+  __shared__ uint2 lds[SHORT * WG];
+  const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
+  const uint32_t stride = gridDim.x * WG;
+  Stack<WG, SHORT> stk;
+  stk.lds = lds;
+  stk.g = gstack + (size_t)gtid * gcap;
+  const SceneDev S = *scene;
+  uint32_t nv = 0, nt = 0;
+  for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
+    uint32_t i = base + threadIdx.x;
+    if (i < n) {
+      float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
+      Hit h;
+      h.t = a.w;
+      h.u = h.v = 0.0f;
+      h.prim = -1;
+      traverse<WG, SHORT, false, STATS>(S, xyz(a), xyz(b), h, stk, nv, nt);
+      prim[i] = h.prim;
+      tout[i] = h.prim >= 0 ? h.t : kInf;
+    }
+  }
+  if (STATS) {
+    unsigned long long a = nv, b = nt;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      a += __shfl_xor(a, off);
+      b += __shfl_xor(b, off);
+    }
+    if (lane_id() == 0) {
+      atomicAdd(&counters[0], a);
+      atomicAdd(&counters[1], b);
+    }
+  }
+}
+
+// ------------------------------------------------------------ environment tables
+// Per-texel weight = luminance * sin(theta at the row centre); max by integer
+// atomicMax on the (non-negative) float bits.
+__global__ void k_env_weight(const float *env, uint32_t w, uint32_t h, float *wbuf, unsigned int *wmax_bits) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  float wv = 0.0f;
+  if (i < w * h) {
+    uint32_t iy = i / w;
+    float s, c;
+    sincos_turns(((float)iy + 0.5f) / (2.0f * (float)h), s, c);
+    const float *p = env + 3 * (size_t)i;
+    float lum = (0.2126f * p[0] + 0.7152f * p[1]) + 0.0722f * p[2];
+    wv = lum * s;
+    if (!(wv > 0.0f)) wv = 0.0f;
+    wbuf[i] = wv;
+  }
+  float m = wv;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    float o = __shfl_xor(m, off);
+    m = o > m ? o : m;
+  }
+  if (lane_id() == 0 && m > 0.0f) atomicMax(wmax_bits, __float_as_uint(m));
+}
+
+// One workgroup (256 threads) per row: quantize, inclusive scan in LDS.
+__global__ __launch_bounds__(256) void k_env_rowscan(const float *wbuf, uint32_t w, const unsigned int *wmax_bits,
+                                                     unsigned long long *q, unsigned long long *row) {
+  __shared__ unsigned long long part[256];
+  uint32_t iy = blockIdx.x;
+  float wmax = __uint_as_float(*wmax_bits);
+  double scale = wmax > 0.0f ? 4294967295.0 / (double)wmax : 0.0;
+  uint32_t per = (w + 255u) / 256u;
+  uint32_t b = threadIdx.x * per, e = b + per < w ? b + per : w;
+  unsigned long long local = 0ull;
+  for (uint32_t ix = b; ix < e; ix++) {
+    unsigned long long v = (unsigned long long)((double)wbuf[(size_t)iy * w + ix] * scale);
+    q[(size_t)iy * w + ix] = v;
+    local += v;
+  }
+  part[threadIdx.x] = local;
+  __syncthreads();
+  for (uint32_t off = 1; off < 256u; off <<= 1) {
+    unsigned long long add_v = threadIdx.x >= off ? part[threadIdx.x - off] : 0ull;
+    __syncthreads();
+    part[threadIdx.x] += add_v;
+    __syncthreads();
+  }
+  unsigned long long run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0ull;
+  unsigned long long *r = row + (size_t)iy * (w + 1);
+  if (threadIdx.x == 0) r[0] = 0ull;
+  for (uint32_t ix = b; ix < e; ix++) {
+    run += q[(size_t)iy * w + ix];
+    r[ix + 1] = run;
+  }
+}
+
+// Single workgroup: prefix over the row totals.
+__global__ __launch_bounds__(256) void k_env_margscan(const unsigned long long *row, uint32_t w, uint32_t h,
+                                                      unsigned long long *marg) {
+  __shared__ unsigned long long part[256];
+  uint32_t per = (h + 255u) / 256u;
+  uint32_t b = threadIdx.x * per, e = b + per < h ? b + per : h;
+  unsigned long long local = 0ull;
+  for (uint32_t iy = b; iy < e; iy++) local += row[(size_t)iy * (w + 1) + w];
+  part[threadIdx.x] = local;
+  __syncthreads();
+  for (uint32_t off = 1; off < 256u; off <<= 1) {
+    unsigned long long add_v = threadIdx.x >= off ? part[threadIdx.x - off] : 0ull;
+    __syncthreads();
+    part[threadIdx.x] += add_v;
+    __syncthreads();
+  }
+  unsigned long long run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0ull;
+  if (threadIdx.x == 0) marg[0] = 0ull;
+  for (uint32_t iy = b; iy < e; iy++) {
+    run += row[(size_t)iy * (w + 1) + w];
+    marg[iy + 1] = run;
+  }
+}
+
+constexpr int kShort = (int)kShortStack;
+
+template <int WG>
+hipError_t launch_extend_wg(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
+  if (stats) hipLaunchKernelGGL((k_extend<WG, kShort, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else hipLaunchKernelGGL((k_extend<WG, kShort, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+namespace k {
+
+hipError_t refill(const KernelArgs &a, const LaunchShape &s, int parity, hipStream_t st) {
+  switch (s.wg) {
+    case 64: hipLaunchKernelGGL(k_refill<64>, dim3(s.grid), dim3(64), 0, st, a, parity); break;
+    case 128: hipLaunchKernelGGL(k_refill<128>, dim3(s.grid), dim3(128), 0, st, a, parity); break;
+    default: hipLaunchKernelGGL(k_refill<256>, dim3(s.grid), dim3(256), 0, st, a, parity); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t extend(const KernelArgs &a, const LaunchShape &s, int parity, bool stats, hipStream_t st) {
+  switch (s.wg) {
+    case 64: return launch_extend_wg<64>(a, s.grid, parity, stats, st);
+    case 128: return launch_extend_wg<128>(a, s.grid, parity, stats, st);
+    default: return launch_extend_wg<256>(a, s.grid, parity, stats, st);
+  }
+}
+
+hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, hipStream_t st) {
+  hipLaunchKernelGGL(k_clear_shadow, dim3(1), dim3(1), 0, st, a.ctrl);
+  switch (s.wg) {
+    case 64: hipLaunchKernelGGL(k_shade<64>, dim3(s.grid), dim3(64), 0, st, a, parity); break;
+    case 128: hipLaunchKernelGGL(k_shade<128>, dim3(s.grid), dim3(128), 0, st, a, parity); break;
+    default: hipLaunchKernelGGL(k_shade<256>, dim3(s.grid), dim3(256), 0, st, a, parity); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t shadow(const KernelArgs &a, const LaunchShape &s, hipStream_t st) {
+  switch (s.wg) {
+    case 64: hipLaunchKernelGGL((k_shadow<64, kShort>), dim3(s.grid), dim3(64), 0, st, a); break;
+    case 128: hipLaunchKernelGGL((k_shadow<128, kShort>), dim3(s.grid), dim3(128), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_shadow<256, kShort>), dim3(s.grid), dim3(256), 0, st, a); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, double n_samples, hipStream_t st) {
+  uint32_t n = n_px * 3u;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_resolve, dim3((n + 255u) / 256u), dim3(256), 0, st, accum, out, n, n_samples);
+  return hipGetLastError();
+}
+
+hipError_t trace_rays(const SceneDev *scene, const float *rays, uint32_t n, int32_t *prim, float *t,
+                      unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t grid, uint32_t wg,
+                      hipStream_t st) {
+  const float4 *r4 = reinterpret_cast<const float4 *>(rays);
+  if (counters) {
+    hipLaunchKernelGGL((k_trace_rays<128, kShort, true>), dim3(grid), dim3(128), 0, st, scene, r4, n, prim, t, counters,
+                       gstack, gcap);
+    return hipGetLastError();
+  }
+  switch (wg) {
+    case 64:
+      hipLaunchKernelGGL((k_trace_rays<64, kShort, false>), dim3(grid), dim3(64), 0, st, scene, r4, n, prim, t, nullptr,
+                         gstack, gcap);
+      break;
+    case 256:
+      hipLaunchKernelGGL((k_trace_rays<256, kShort, false>), dim3(grid), dim3(256), 0, st, scene, r4, n, prim, t,
+                         nullptr, gstack, gcap);
+      break;
+    default:
+      hipLaunchKernelGGL((k_trace_rays<128, kShort, false>), dim3(grid), dim3(128), 0, st, scene, r4, n, prim, t,
+                         nullptr, gstack, gcap);
+      break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t env_tables(const float *env, uint32_t w, uint32_t h, float *wbuf, unsigned int *wmax_bits,
+                      unsigned long long *q, unsigned long long *row, unsigned long long *marg, hipStream_t st) {
+  uint32_t n = w * h;
+  hipLaunchKernelGGL(k_env_weight, dim3((n + 255u) / 256u), dim3(256), 0, st, env, w, h, wbuf, wmax_bits);
+  hipLaunchKernelGGL(k_env_rowscan, dim3(h), dim3(256), 0, st, wbuf, w, wmax_bits, q, row);
+  hipLaunchKernelGGL(k_env_margscan, dim3(1), dim3(256), 0, st, row, w, h, marg);
+  return hipGetLastError();
+}
+
+}  // namespace k
+}  // namespace omr

@@ -1,0 +1,124 @@
+// omr_layout.h -- data layout shared by the host runtime (g++) and the HIP
+// kernels (hipcc).  Plain structs only; every device-side record is a whole
+// number of 16-byte vectors so a lane fetches it with dwordx4 loads.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define OMR_HD __host__ __device__
+#else
+#define OMR_HD
+#endif
+
+namespace omr {
+
+// Binary32 constants of the reference (src/tools/Constant.h:4-8).
+constexpr float kInf = 0x1.fffffep+126f;  // (FLT_MAX-1)/2
+constexpr float kEps = 0.0005f;
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kTwoPi = 6.28318530717958647692f;
+constexpr float kInvPi = 0.31830988618379067154f;
+// Fixed-point accumulation: 2^24 units per radiance unit, per-event clamp.
+constexpr float kFixScale = 16777216.0f;
+constexpr float kFixClamp = 1.0e8f;
+// Slab-test far-distance inflation, 1 + 2^-20 (conservative box test).
+constexpr float kSlabInflate = 1.00000095f;
+
+// 4-wide BVH node, 128 bytes = one cache line.  Children bounds stored
+// component-major so one float4 holds the same bound of all four children.
+// child: >= 0 inner node index; -1 empty; leaf = 0x80000000 | (count-1)<<27 | first.
+struct alignas(128) WNode4 {
+  float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  int32_t child[4];
+  int32_t meta[4];  // meta[0] = depth of this node (diagnostics)
+};
+static_assert(sizeof(WNode4) == 128, "node must be one cache line");
+
+OMR_HD inline bool child_is_leaf(int32_t c) { return ((uint32_t)c & 0xC0000000u) == 0x80000000u; }
+OMR_HD inline uint32_t leaf_first(int32_t c) { return (uint32_t)c & 0x07FFFFFFu; }
+OMR_HD inline uint32_t leaf_count(int32_t c) { return (((uint32_t)c >> 27) & 7u) + 1u; }
+OMR_HD inline int32_t make_leaf(uint32_t first, uint32_t count) {
+  return (int32_t)(0x80000000u | ((count - 1u) << 27) | first);
+}
+constexpr uint32_t kMaxLeafPrims = 8;
+constexpr uint32_t kMaxLeafFirst = 0x07FFFFFFu;
+
+// Primitive record in leaf order, 48 bytes.
+//  triangle: a = v0.xyz | id,  b = e1.xyz | kind 0,  c = e2.xyz | 0
+//  sphere:   a = c.xyz  | id,  b = r,0,0  | kind 1,  c = 0
+struct alignas(16) PrimRec {
+  float a[4], b[4], c[4];
+};
+
+// Material record, 48 bytes (Material, src/renderer/Material.h:11-36, plus the
+// normalized multi-material rate, src/renderer/SceneObject.h:17-30).
+struct alignas(16) MatRec {
+  uint32_t type;
+  float color[3];
+  float emission[3];
+  float ior;
+  float rate;
+  uint32_t pad[3];
+};
+
+// Device-side view of one resident scene.
+struct SceneDev {
+  const WNode4 *nodes;
+  const PrimRec *prims;        // leaf order
+  const float *tri_n;          // per original triangle: 12 floats (n0,_, dn1,_, dn2,_)
+  const float *sph;            // per original sphere: 4 floats (c, r)
+  const uint32_t *prim_mat;    // per original primitive: first | (count-1) << 24
+  const MatRec *mats;
+  const float *env;            // env_h x env_w x 3 or null
+  const unsigned long long *env_q;     // env_h x env_w
+  const unsigned long long *env_row;   // env_h x (env_w+1)
+  const unsigned long long *env_marg;  // env_h+1
+  uint32_t n_tri, n_sph;
+  uint32_t env_w, env_h;
+  uint32_t n_nodes;
+  uint32_t stack_bound;        // max traversal stack entries
+};
+
+// One render job in the device job ring.
+struct JobDev {
+  unsigned long long sample_base;  // global index of the job's first sample
+  uint32_t n_samples;              // W*H*ss^2*(spp_end-spp_begin)
+  uint32_t width, height, ss;
+  uint32_t nspp, spp_begin;
+  uint32_t max_depth, rr_depth;
+  uint32_t flags;                  // bit0: environment importance sampling
+  uint32_t scene;
+  unsigned long long seed;
+  float cam_pos[3], cam_c[3], cam_x[3], cam_y[3];
+  float aperture, focal;
+  unsigned long long *accum;       // W*H*3 fixed-point sums
+  unsigned long long *done;        // completed samples
+};
+
+// Control block in device memory (one per engine).
+struct Ctrl {
+  unsigned long long next_sample;   // global sample cursor (CAS-advanced)
+  unsigned long long total_samples; // samples admitted so far (host-written)
+  unsigned long long segments;      // closest-hit queries traced
+  unsigned long long shadow_rays;   // environment shadow rays traced
+  unsigned long long node_visits;   // stats kernels only
+  unsigned long long prim_tests;    // stats kernels only
+  uint32_t qcount[2];               // ping-pong ray queue counts
+  uint32_t scount;                  // shadow queue count
+  uint32_t job_lo, job_hi;          // live job window (logical ring indices)
+  uint32_t pad[3];
+};
+
+constexpr uint32_t kJobRing = 1024;
+
+// Per-slot packed path state word (misc.y).
+//  bits 0-7 depth, 8-15 branch, 16-17 pending count, 18 prev-lambert (MIS), 19 active
+OMR_HD inline uint32_t ps_depth(uint32_t w) { return w & 0xFFu; }
+OMR_HD inline uint32_t ps_branch(uint32_t w) { return (w >> 8) & 0xFFu; }
+OMR_HD inline uint32_t ps_pending(uint32_t w) { return (w >> 16) & 3u; }
+OMR_HD inline uint32_t ps_mis(uint32_t w) { return (w >> 18) & 1u; }
+OMR_HD inline uint32_t ps_pack(uint32_t depth, uint32_t branch, uint32_t pend, uint32_t mis) {
+  return depth | (branch << 8) | (pend << 16) | (mis << 18) | (1u << 19);
+}
+
+}  // namespace omr

@@ -1,0 +1,294 @@
+// OmochiRenderer.cpp -- drop-in classes over the C ABI.
+#include "OmochiRenderer.h"
+
+#include <chrono>
+#include <cstring>
+#include <iostream>
+#include <mutex>
+#include <sstream>
+
+namespace OmochiRenderer {
+
+omr_ctx *DefaultContext() {
+  static std::once_flag once;
+  static omr_ctx *ctx = nullptr;
+  std::call_once(once, [] {
+    if (omr_create(&ctx, 0) != OMR_OK) ctx = nullptr;
+  });
+  return ctx;
+}
+
+void SceneObject::AddMaterial(const Material &m, double rate, bool doNormalizeRate) {
+  materials_.emplace_back(m, rate);
+  if (doNormalizeRate) {
+    double total = 0;
+    for (auto &e : materials_) total += e.rate_;
+    for (auto &e : materials_) e.rate_ /= total;
+  }
+}
+
+Polygon::Polygon(const Vector3 &pos1, const Vector3 &pos2, const Vector3 &pos3, const Vector3 &uv1, const Vector3 &uv2,
+                 const Vector3 &uv3, const Vector3 &normal1, const Vector3 &normal2, const Vector3 &normal3,
+                 const Material &mat, const Vector3 &pos)
+    : SceneObject(mat) {
+  m_posAndEdges[0] = pos1;
+  m_posAndEdges[1] = pos2 - pos1;
+  m_posAndEdges[2] = pos3 - pos1;
+  m_uvOrigAndEdges[0] = uv1;
+  m_uvOrigAndEdges[1] = uv2 - uv1;
+  m_uvOrigAndEdges[2] = uv3 - uv1;
+  m_normalAndDiffs[0] = normal1;
+  m_normalAndDiffs[1] = normal2 - normal1;
+  m_normalAndDiffs[2] = normal3 - normal1;
+  position_ = pos;
+}
+
+void Polygon::Transform(const Vector3 &pos, const Vector3 &scale) {
+  position_ = pos;
+  for (int i = 0; i < 3; i++) {
+    m_posAndEdges[i].x *= scale.x;
+    m_posAndEdges[i].y *= scale.y;
+    m_posAndEdges[i].z *= scale.z;
+  }
+}
+
+Camera::Camera(size_t w, size_t h, const Vector3 &pos, const Vector3 &dir, const Vector3 &up, double screenH,
+               double dist, double focal)
+    : m_width(w), m_height(h), m_camPos(pos), m_camDir(dir), m_camUp(up), m_nearScreenHeight(screenH),
+      m_distToScreen(dist), m_focalDistance(focal) {
+  // Camera::Camera (src/renderer/Camera.h:12-43)
+  m_camDir.normalize();
+  m_camUp.normalize();
+  double nearW = m_nearScreenHeight * (double)m_width / (double)m_height;
+  m_screenCenter = m_camPos + m_camDir * m_distToScreen;
+  m_screenXaxis = m_camDir.cross(m_camUp);
+  m_screenXaxis.normalize();
+  m_screenXaxis *= nearW;
+  m_screenYaxis = m_screenXaxis.cross(m_camDir);
+  m_screenYaxis.normalize();
+  m_screenYaxis *= m_nearScreenHeight;
+}
+
+omr_camera Camera::ToDevice() const {
+  omr_camera c;
+  std::memset(&c, 0, sizeof(c));
+  c.width = (uint32_t)m_width;
+  c.height = (uint32_t)m_height;
+  const Vector3 *v[4] = {&m_camPos, &m_screenCenter, &m_screenXaxis, &m_screenYaxis};
+  float *dst[4] = {c.position, c.screen_center, c.screen_x, c.screen_y};
+  for (int i = 0; i < 4; i++) {
+    dst[i][0] = (float)v[i]->x;
+    dst[i][1] = (float)v[i]->y;
+    dst[i][2] = (float)v[i]->z;
+  }
+  c.aperture_radius = m_aperture ? (float)m_aperture->Radius() : 0.0f;
+  c.focal_distance = (float)m_focalDistance;
+  return c;
+}
+
+Scene::~Scene() {
+  if (m_ctx && m_id >= 0) omr_scene_release(m_ctx, m_id);
+  for (auto &o : m_objects)
+    if (o.doDelete) delete o.obj;
+}
+
+void Scene::AddObject(SceneObject *obj, bool doDelete, bool) {
+  m_objects.push_back({obj, doDelete});
+  if (m_ctx && m_id >= 0) {  // contents changed: re-upload on next use
+    omr_scene_release(m_ctx, m_id);
+    m_id = -1;
+  }
+}
+
+void Scene::ConstructQBVH() {
+  omr_ctx *c = DefaultContext();
+  if (c) Upload(c);
+}
+
+omr_scene_id Scene::Upload(omr_ctx *ctx) const {
+  if (m_ctx == ctx && m_id >= 0) return m_id;
+  std::vector<float> tri, nrm, sph;
+  std::vector<uint32_t> pm_tri, pc_tri, pm_sph, pc_sph;
+  std::vector<omr_material> mats;
+  std::vector<SceneObject *> tris, sphs;
+  auto add_mats = [&](const SceneObject *o, std::vector<uint32_t> &pm, std::vector<uint32_t> &pc) {
+    pm.push_back((uint32_t)mats.size());
+    pc.push_back((uint32_t)o->GetMaterialCount());
+    for (size_t i = 0; i < o->GetMaterialCount(); i++) {
+      const SceneObject::MultiMaterialElem *e = o->GetMaterial(i);
+      omr_material m;
+      std::memset(&m, 0, sizeof(m));
+      m.type = (uint32_t)e->material_.reflection_type;
+      const Color &col = e->material_.color, &em = e->material_.emission;
+      m.color[0] = (float)col.x; m.color[1] = (float)col.y; m.color[2] = (float)col.z;
+      m.emission[0] = (float)em.x; m.emission[1] = (float)em.y; m.emission[2] = (float)em.z;
+      m.ior = (float)e->material_.refraction_rate;
+      m.rate = (float)e->rate_;
+      mats.push_back(m);
+    }
+  };
+  for (auto &ow : m_objects) {
+    SceneObject *o = ow.obj;
+    if (o->GetMaterialCount() == 0) continue;
+    if (auto *p = dynamic_cast<Polygon *>(o)) {
+      const Vector3 *pe = p->PosAndEdges(), *nd = p->NormalAndDiffs();
+      Vector3 p0 = pe[0] + p->GetPosition();  // Polygon.h:158: T = o - (p0 + position_)
+      float t9[9] = {(float)p0.x, (float)p0.y, (float)p0.z, (float)pe[1].x, (float)pe[1].y, (float)pe[1].z,
+                     (float)pe[2].x, (float)pe[2].y, (float)pe[2].z};
+      float n9[9] = {(float)nd[0].x, (float)nd[0].y, (float)nd[0].z, (float)nd[1].x, (float)nd[1].y, (float)nd[1].z,
+                     (float)nd[2].x, (float)nd[2].y, (float)nd[2].z};
+      tri.insert(tri.end(), t9, t9 + 9);
+      nrm.insert(nrm.end(), n9, n9 + 9);
+      add_mats(o, pm_tri, pc_tri);
+      tris.push_back(o);
+    } else if (auto *s = dynamic_cast<Sphere *>(o)) {
+      const Vector3 &c = s->GetPosition();
+      float s4[4] = {(float)c.x, (float)c.y, (float)c.z, (float)s->GetRadius()};
+      sph.insert(sph.end(), s4, s4 + 4);
+      add_mats(o, pm_sph, pc_sph);
+      sphs.push_back(o);
+    }
+  }
+  std::vector<uint32_t> pm(pm_tri), pc(pc_tri);
+  pm.insert(pm.end(), pm_sph.begin(), pm_sph.end());
+  pc.insert(pc.end(), pc_sph.begin(), pc_sph.end());
+  omr_scene_desc d;
+  std::memset(&d, 0, sizeof(d));
+  d.n_triangles = tris.size();
+  d.triangles = tri.data();
+  d.tri_normals = nrm.data();
+  d.n_spheres = sphs.size();
+  d.spheres = sph.data();
+  d.prim_material = pm.data();
+  d.prim_material_count = pc.data();
+  d.n_materials = mats.size();
+  d.materials = mats.data();
+  if (m_ibl) {
+    d.env_width = m_ibl->Width();
+    d.env_height = m_ibl->Height();
+    d.env_rgb = m_ibl->Pixels().data();
+  }
+  omr_scene_id id = -1;
+  if (omr_scene_upload(ctx, &d, &id) != OMR_OK) {
+    std::cerr << "scene upload failed: " << omr_last_error(ctx) << std::endl;
+    return -1;
+  }
+  m_prims = tris;
+  m_prims.insert(m_prims.end(), sphs.begin(), sphs.end());
+  m_ctx = ctx;
+  m_id = id;
+  return id;
+}
+
+bool Scene::CheckIntersection(const Ray &ray, IntersectionInformation &info) const {
+  info.distance = (double)0x1.fffffep+126f;
+  info.object = nullptr;
+  omr_ctx *c = m_ctx ? m_ctx : DefaultContext();
+  if (!c || Upload(c) < 0) return false;
+  float r[8] = {(float)ray.orig.x, (float)ray.orig.y, (float)ray.orig.z, 0x1.fffffep+126f,
+                (float)ray.dir.x, (float)ray.dir.y, (float)ray.dir.z, 0.0f};
+  int32_t prim = -1;
+  float t = 0;
+  if (omr_trace_closest(c, m_id, r, 1, &prim, &t) != OMR_OK || prim < 0) return false;
+  info.distance = t;
+  info.position = ray(t);
+  info.object = m_prims[(size_t)prim];
+  return true;
+}
+
+// Floor helpers: two polygons per quad, same vertices, uvs and normals as
+// Scene::AddFloor* (src/scenes/Scene.cpp:88-193).
+#define OMR_FLOOR(name, a0, a1, a2, n0, b0, b1, b2)                                                          \
+  std::vector<SceneObject *> Scene::name(double s1, double s2, const Vector3 &position, const Material &m) { \
+    const double h1 = s1 / 2, h2 = s2 / 2;                                                                   \
+    (void)h1; (void)h2;                                                                                      \
+    std::vector<SceneObject *> objs;                                                                         \
+    objs.push_back(new Polygon(a0, a1, a2, Vector3(), Vector3(), Vector3(), n0, n0, n0, m, position));        \
+    objs.push_back(new Polygon(b0, b1, b2, Vector3(), Vector3(), Vector3(), n0, n0, n0, m, position));        \
+    AddObject(objs[0]);                                                                                      \
+    AddObject(objs[1]);                                                                                      \
+    return objs;                                                                                             \
+  }
+OMR_FLOOR(AddFloorXZ_yUp, Vector3(-h1, 0, -h2), Vector3(-h1, 0, h2), Vector3(h1, 0, -h2), Vector3(0, 1, 0),
+          Vector3(h1, 0, -h2), Vector3(-h1, 0, h2), Vector3(h1, 0, h2))
+OMR_FLOOR(AddFloorXY_zUp, Vector3(-h1, h2, 0), Vector3(-h1, -h2, 0), Vector3(h1, -h2, 0), Vector3(0, 0, 1),
+          Vector3(h1, h2, 0), Vector3(-h1, h2, 0), Vector3(h1, -h2, 0))
+OMR_FLOOR(AddFloorYZ_xUp, Vector3(0, h1, -h2), Vector3(0, h1, h2), Vector3(0, -h1, h2), Vector3(1, 0, 0),
+          Vector3(0, -h1, -h2), Vector3(0, h1, -h2), Vector3(0, -h1, h2))
+OMR_FLOOR(AddFloorXZ_yDown, Vector3(h1, 0, -h2), Vector3(-h1, 0, h2), Vector3(-h1, 0, -h2), Vector3(0, -1, 0),
+          Vector3(h1, 0, h2), Vector3(-h1, 0, h2), Vector3(h1, 0, -h2))
+OMR_FLOOR(AddFloorXY_zDown, Vector3(h1, -h2, 0), Vector3(-h1, -h2, 0), Vector3(-h1, h2, 0), Vector3(0, 0, -1),
+          Vector3(h1, -h2, 0), Vector3(-h1, h2, 0), Vector3(h1, h2, 0))
+OMR_FLOOR(AddFloorYZ_xDown, Vector3(0, -h1, h2), Vector3(0, h1, h2), Vector3(0, h1, -h2), Vector3(-1, 0, 0),
+          Vector3(0, -h1, h2), Vector3(0, h1, -h2), Vector3(0, -h1, -h2))
+#undef OMR_FLOOR
+
+PathTracer::PathTracer(const Camera &camera, int samples, int supersamples)
+    : m_camera(camera), m_min(samples), m_max(samples), m_step(1), m_ss(supersamples) {}
+
+PathTracer::PathTracer(const Camera &camera, int min_samples, int max_samples, int step, int supersamples,
+                       RenderingFinishCallbackFunction callback)
+    : m_camera(camera), m_min(min_samples), m_max(max_samples), m_step(step), m_ss(supersamples),
+      m_callback(callback) {}
+
+PathTracer::~PathTracer() {}
+
+// PathTracer::RenderScene (src/renderer/PathTracer.cpp:55-85): progressive
+// sample batches, running mean img*prev/next + batch_mean*(next-prev)/next.
+void PathTracer::RenderScene(const Scene &scene) {
+  m_error.clear();
+  const size_t W = m_camera.GetScreenWidth(), H = m_camera.GetScreenHeight();
+  m_result.reset(new ScreenPixels());
+  m_result->InitializePixels(W, H);
+  m_ctx = DefaultContext();
+  if (!m_ctx) { m_error = "no GPU context"; return; }
+  omr_scene_id id = scene.Upload(m_ctx);
+  if (id < 0) { m_error = omr_last_error(m_ctx); return; }
+  omr_camera cam = m_camera.ToDevice();
+  std::vector<float> batch(W * H * 3);
+  m_previousSamples = 0;
+  double minutes = 0;
+  for (m_currentSamples = m_min; m_currentSamples <= m_max && m_enableRendering; m_currentSamples += m_step) {
+    auto t0 = std::chrono::steady_clock::now();
+    omr_render_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.supersamples = (uint32_t)m_ss;
+    p.spp_begin = (uint32_t)m_previousSamples;
+    p.spp_end = (uint32_t)m_currentSamples;
+    p.max_depth = m_maxDepth;
+    p.rr_depth = m_rrDepth;
+    p.env_is = m_nee ? 1u : 0u;
+    p.seed = m_seed;
+    if (omr_render(m_ctx, id, &cam, &p, batch.data()) != OMR_OK) {
+      m_error = omr_last_error(m_ctx);
+      return;
+    }
+    omr_render_stats st;
+    omr_last_render_stats(m_ctx, &st);
+    m_segments += st.segments;
+    double prev = m_previousSamples, next = m_currentSamples;
+    Color *px = m_result->GetScreen();
+    for (size_t y = 0; y < H; y++) {
+      // output row r = H-1-y holds screen row y; ScreenPixels stores y at H-y-1
+      const float *row = &batch[(H - 1 - y) * W * 3];
+      for (size_t x = 0; x < W; x++) {
+        Color &c = px[x + (H - y - 1) * 2 * W];
+        Color b(row[3 * x], row[3 * x + 1], row[3 * x + 2]);
+        c = c * (prev / next) + b * ((next - prev) / next);
+      }
+    }
+    m_previousSamples = m_currentSamples;
+    minutes += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / 60.0;
+    if (m_callback) m_callback(m_currentSamples, m_result->GetScreen(), 2 * W, H, minutes);
+  }
+}
+
+std::string PathTracer::GetCurrentRenderingInfo() const {
+  std::stringstream ss;
+  ss << "(width, height) = (" << m_camera.GetScreenWidth() << ", " << m_camera.GetScreenHeight() << ")" << std::endl;
+  ss << "previous samples / pixel = " << m_previousSamples << "x(" << m_ss << "x" << m_ss << ")" << std::endl;
+  ss << "segments traced = " << m_segments << std::endl;
+  return ss.str();
+}
+
+}  // namespace OmochiRenderer

@@ -1,0 +1,319 @@
+// OmochiRenderer.h -- C++ drop-in for the reference's renderer interface,
+// backed by the MI355X library through include/omr_capi.h.
+//
+// Same names, argument meaning and error behaviour as the reference types
+// (paths relative to the reference checkout):
+//   Vector3 / Color        src/tools/Vector.h:9-103, src/renderer/Color.h
+//   Ray                    src/renderer/Ray.h:7-20
+//   Material               src/renderer/Material.h:11-36
+//   SceneObject            src/renderer/SceneObject.h:17-95 (multi-material rates)
+//   Sphere, Polygon        src/renderer/Sphere.h:11-82, src/renderer/Polygon.h:11-125
+//   SphereLight            src/renderer/SphereLight.h:8-79 (an emissive sphere; NEE is off)
+//   Scene                  src/scenes/Scene.h:18-124 (AddObject, AddFloor*, ConstructQBVH)
+//   Camera, CircleAperture src/renderer/Camera.h:10-117, src/renderer/Aperture.h:9-39
+//   ScreenPixels           src/renderer/ScreenPixels.hpp / .cpp (2W x H, row-flipped)
+//   Renderer, PathTracer   src/renderer/Renderer.h:10-23, src/renderer/PathTracer.h:18-88
+//   SceneFactory(Manager)  src/scenes/SceneFactory.h:7-48
+// The path tracing itself runs on the GPU; Scene::CheckIntersection runs the
+// device closest-hit kernel for single rays (slow; for compatibility).
+#pragma once
+#include <atomic>
+#include <cmath>
+#include <functional>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/omr_capi.h"
+
+namespace OmochiRenderer {
+
+class Vector3 {
+ public:
+  double x, y, z;
+  Vector3() : x(0), y(0), z(0) {}
+  Vector3(double x_, double y_, double z_) : x(x_), y(y_), z(z_) {}
+  Vector3 &operator+=(const Vector3 &r) { x += r.x; y += r.y; z += r.z; return *this; }
+  Vector3 &operator-=(const Vector3 &r) { x -= r.x; y -= r.y; z -= r.z; return *this; }
+  Vector3 &operator*=(double v) { x *= v; y *= v; z *= v; return *this; }
+  Vector3 &operator/=(double v) { x /= v; y /= v; z /= v; return *this; }
+  bool operator==(const Vector3 &v) const { return x == v.x && y == v.y && z == v.z; }
+  Vector3 operator*(double v) const { return Vector3(x * v, y * v, z * v); }
+  Vector3 operator/(double v) const { return Vector3(x / v, y / v, z / v); }
+  Vector3 operator+(const Vector3 &v) const { return Vector3(x + v.x, y + v.y, z + v.z); }
+  Vector3 operator-(const Vector3 &v) const { return Vector3(x - v.x, y - v.y, z - v.z); }
+  double dot(const Vector3 &r) const { return x * r.x + y * r.y + z * r.z; }
+  Vector3 cross(const Vector3 &r) const { return Vector3(y * r.z - z * r.y, z * r.x - x * r.z, x * r.y - y * r.x); }
+  double lengthSq() const { return x * x + y * y + z * z; }
+  double length() const { return std::sqrt(lengthSq()); }
+  Vector3 &normalize() { return (*this) /= length(); }
+  static const Vector3 &Zero() { static Vector3 z; return z; }
+  static const Vector3 &One() { static Vector3 o(1, 1, 1); return o; }
+};
+typedef Vector3 Color;
+
+class Ray {
+ public:
+  Ray(const Vector3 &o, const Vector3 &d) : orig(o), dir(d) {}
+  Vector3 operator()(double t) const { return orig + dir * t; }
+  Vector3 orig, dir;
+};
+
+class Material {
+ public:
+  enum REFLECTION_TYPE { REFLECTION_TYPE_LAMBERT, REFLECTION_TYPE_SPECULAR, REFLECTION_TYPE_REFRACTION };
+  Material(REFLECTION_TYPE type = REFLECTION_TYPE_LAMBERT, const Vector3 &emission_ = Vector3(),
+           const Vector3 &color_ = Vector3(), double refraction_rate_ = 0.0)
+      : reflection_type(type), emission(emission_), color(color_), refraction_rate(refraction_rate_) {}
+  REFLECTION_TYPE reflection_type;
+  Color emission;
+  Color color;
+  double refraction_rate;
+};
+
+class SceneObject {
+ public:
+  struct MultiMaterialElem {
+    Material material_;
+    double rate_;
+    MultiMaterialElem(const Material &m, double r = 1.0) : material_(m), rate_(r) {}
+  };
+  explicit SceneObject(const Material &m) { AddMaterial(m, 1.0, false); }
+  SceneObject() {}
+  virtual ~SceneObject() {}
+  // SceneObject::AddMaterial (src/renderer/SceneObject.h:48-67): rates renormalized
+  void AddMaterial(const Material &m, double rate, bool doNormalizeRate = true);
+  size_t GetMaterialCount() const { return materials_.size(); }
+  const MultiMaterialElem *GetMaterial(size_t i) const { return i < materials_.size() ? &materials_[i] : nullptr; }
+  const Vector3 &GetPosition() const { return position_; }
+
+ protected:
+  std::vector<MultiMaterialElem> materials_;
+  Vector3 position_;
+  friend class Scene;
+};
+
+class Sphere : public SceneObject {
+ public:
+  Sphere(double r, const Vector3 &pos, const Material &m) : SceneObject(m), m_radius(r) { position_ = pos; }
+  double GetRadius() const { return m_radius; }
+
+ protected:
+  double m_radius;
+};
+
+class LightBase {
+ public:
+  virtual ~LightBase() {}
+};
+// With next-event estimation off (the reference default) a sphere light is an
+// emissive sphere.
+class SphereLight : public Sphere, public LightBase {
+ public:
+  SphereLight(double r, const Vector3 &pos, const Material &m) : Sphere(r, pos, m) {}
+};
+
+class Polygon : public SceneObject {
+ public:
+  Polygon(const Vector3 &pos1, const Vector3 &pos2, const Vector3 &pos3, const Vector3 &uv1, const Vector3 &uv2,
+          const Vector3 &uv3, const Vector3 &normal1, const Vector3 &normal2, const Vector3 &normal3,
+          const Material &mat, const Vector3 &pos);
+  // Polygon::Transform (src/renderer/Polygon.h:51-61), rotation omitted: scale
+  // then translate through position_.
+  void Transform(const Vector3 &pos, const Vector3 &scale = Vector3::One());
+  const Vector3 *PosAndEdges() const { return m_posAndEdges; }
+  const Vector3 *NormalAndDiffs() const { return m_normalAndDiffs; }
+
+ private:
+  Vector3 m_posAndEdges[3];
+  Vector3 m_uvOrigAndEdges[3];
+  Vector3 m_normalAndDiffs[3];
+};
+
+class Aperture {
+ public:
+  virtual ~Aperture() {}
+  virtual double Radius() const = 0;
+};
+class CircleAperture : public Aperture {
+ public:
+  explicit CircleAperture(double radius = 0.0) : m_radius(radius) {}
+  double Radius() const override { return m_radius; }
+
+ private:
+  double m_radius;
+};
+
+class Camera {
+ public:
+  Camera(size_t screenWidth, size_t screenHeight, const Vector3 &cameraPos = Vector3(50.0, 52.0, 220.0),
+         const Vector3 &cameraDir = Vector3(0.0, -0.04, -1.0), const Vector3 &cameraUp = Vector3(0.0, 1.0, 0.0),
+         double screenHeightIn3Dcoordinate = 30.0, double distanceToScreen = 40, double focalDistance = 200);
+  void SetAperture(std::shared_ptr<Aperture> a) { m_aperture = a; }
+  size_t GetScreenWidth() const { return m_width; }
+  size_t GetScreenHeight() const { return m_height; }
+  const Vector3 &GetCameraPosition() const { return m_camPos; }
+  const Vector3 &GetScreenCenterPosition() const { return m_screenCenter; }
+  const Vector3 &GetScreenAxisXvector() const { return m_screenXaxis; }
+  const Vector3 &GetScreenAxisYvector() const { return m_screenYaxis; }
+  omr_camera ToDevice() const;
+
+ private:
+  size_t m_width, m_height;
+  Vector3 m_camPos, m_camDir, m_camUp;
+  double m_nearScreenHeight, m_distToScreen, m_focalDistance;
+  Vector3 m_screenXaxis, m_screenYaxis, m_screenCenter;
+  std::shared_ptr<Aperture> m_aperture;
+};
+
+// Latlong environment (IBL, src/renderer/IBL.h:14-45) from an in-memory image.
+class IBL {
+ public:
+  IBL(uint32_t w, uint32_t h, std::vector<float> rgb) : m_w(w), m_h(h), m_rgb(std::move(rgb)) {}
+  uint32_t Width() const { return m_w; }
+  uint32_t Height() const { return m_h; }
+  const std::vector<float> &Pixels() const { return m_rgb; }
+
+ private:
+  uint32_t m_w, m_h;
+  std::vector<float> m_rgb;
+};
+
+class Scene {
+ public:
+  struct IntersectionInformation {
+    double distance = 0;
+    Vector3 position, normal;
+    SceneObject *object = nullptr;
+  };
+  virtual ~Scene();
+  void ConstructBVH() { ConstructQBVH(); }
+  // Builds the wide BVH and keeps the scene resident on the GPU.
+  void ConstructQBVH();
+  bool CheckIntersection(const Ray &ray, IntersectionInformation &info) const;
+  const IBL *GetIBL() const { return m_ibl.get(); }
+  virtual Color Background() const { return Color(0, 0, 0); }
+  virtual bool IsValid() const { return true; }
+
+  // Device handle for the current contents (uploads on first use).
+  omr_scene_id Upload(omr_ctx *ctx) const;
+
+ protected:
+  Scene() {}
+  void AddObject(SceneObject *obj, bool doDelete = true, bool containedInBVH = true);
+  std::vector<SceneObject *> AddFloorXZ_yUp(double size_x, double size_z, const Vector3 &position, const Material &m);
+  std::vector<SceneObject *> AddFloorXY_zUp(double size_x, double size_y, const Vector3 &position, const Material &m);
+  std::vector<SceneObject *> AddFloorYZ_xUp(double size_y, double size_z, const Vector3 &position, const Material &m);
+  std::vector<SceneObject *> AddFloorXZ_yDown(double size_x, double size_z, const Vector3 &position, const Material &m);
+  std::vector<SceneObject *> AddFloorXY_zDown(double size_x, double size_y, const Vector3 &position, const Material &m);
+  std::vector<SceneObject *> AddFloorYZ_xDown(double size_y, double size_z, const Vector3 &position, const Material &m);
+  void SetIBL(std::unique_ptr<IBL> ibl) { m_ibl = std::move(ibl); }
+
+ private:
+  struct Owned {
+    SceneObject *obj;
+    bool doDelete;
+  };
+  std::vector<Owned> m_objects;
+  std::unique_ptr<IBL> m_ibl;
+  mutable omr_ctx *m_ctx = nullptr;
+  mutable omr_scene_id m_id = -1;
+  mutable std::vector<SceneObject *> m_prims;  // primitive id -> object
+};
+
+class ScreenPixels {
+ public:
+  // InitializePixels(w, h, 2, 1) layout: 2W x H, row y stored at (H-y-1)
+  void InitializePixels(size_t w, size_t h) {
+    m_w = w;
+    m_h = h;
+    m_pixels.assign(2 * w * h, Color());
+  }
+  size_t CalcScreenWidth() const { return 2 * m_w; }
+  size_t CalcScreenHeight() const { return m_h; }
+  const Color *GetScreen() const { return m_pixels.data(); }
+  Color *GetScreen() { return m_pixels.data(); }
+
+ private:
+  size_t m_w = 0, m_h = 0;
+  std::vector<Color> m_pixels;
+};
+
+class Renderer {
+ protected:
+  std::atomic<bool> m_enableRendering{true};
+
+ public:
+  virtual ~Renderer() {}
+  virtual void RenderScene(const Scene &scene) = 0;
+  virtual const ScreenPixels *GetResult() const = 0;
+  virtual const int GetCurrentSampleCount() const = 0;
+  virtual std::string GetCurrentRenderingInfo() const { return ""; }
+  void StopRendering() { m_enableRendering = false; }
+};
+
+// PathTracer (src/renderer/PathTracer.h:18-88) on the GPU.
+class PathTracer : public Renderer {
+ public:
+  typedef std::function<void(int samples, const Color *result, size_t width, size_t height,
+                             double accumulatedRenderingTime)>
+      RenderingFinishCallbackFunction;
+  PathTracer(const Camera &camera, int samples, int supersamples);
+  PathTracer(const Camera &camera, int min_samples, int max_samples, int step, int supersamples,
+             RenderingFinishCallbackFunction callback);
+  virtual ~PathTracer();
+  // Next-event estimation is off in the reference's working path; enabling it
+  // turns on environment importance sampling (shadow rays) here.
+  void EnableNextEventEstimation(bool enable = true) { m_nee = enable; }
+  void SetMaxThreadCount(size_t) {}  // threads are GPU lanes
+  // Depth parameters (reference constants MinDepth = 5, MaxDepth = 64).
+  void SetDepth(uint32_t rr_depth, uint32_t max_depth) { m_rrDepth = rr_depth; m_maxDepth = max_depth; }
+  void SetSeed(uint64_t seed) { m_seed = seed; }
+  void RenderScene(const Scene &scene) override;
+  const ScreenPixels *GetResult() const override { return m_result.get(); }
+  const int GetCurrentSampleCount() const override { return m_currentSamples; }
+  std::string GetCurrentRenderingInfo() const override;
+  // Last error from the library (empty when fine).
+  const std::string &LastError() const { return m_error; }
+
+ private:
+  Camera m_camera;
+  int m_min, m_max, m_step, m_ss;
+  int m_currentSamples = 0, m_previousSamples = 0;
+  RenderingFinishCallbackFunction m_callback;
+  std::unique_ptr<ScreenPixels> m_result;
+  bool m_nee = false;
+  uint32_t m_rrDepth = 5, m_maxDepth = 0;
+  uint64_t m_seed = 0x6F6D7233ull;
+  uint64_t m_segments = 0;
+  omr_ctx *m_ctx = nullptr;
+  std::string m_error;
+};
+
+// SceneFactory / SceneFactoryManager (src/scenes/SceneFactory.h:7-48)
+class SceneFactory {
+ public:
+  virtual ~SceneFactory() {}
+  virtual std::shared_ptr<Scene> Create(const std::string &info) const = 0;
+};
+class SceneFactoryManager {
+ public:
+  static SceneFactoryManager &GetInstance() {
+    static SceneFactoryManager m;
+    return m;
+  }
+  void Register(const std::string &name, std::shared_ptr<SceneFactory> f) { m_factories[name] = f; }
+  std::shared_ptr<SceneFactory> Get(const std::string &name) const {
+    auto it = m_factories.find(name);
+    return it == m_factories.end() ? nullptr : it->second;
+  }
+
+ private:
+  std::map<std::string, std::shared_ptr<SceneFactory>> m_factories;
+};
+
+// Process-wide library context used by the drop-in classes (device 0).
+omr_ctx *DefaultContext();
+
+}  // namespace OmochiRenderer

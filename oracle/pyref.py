@@ -1,0 +1,86 @@
+"""ctypes binding of oracle/_ref/libref.so: the reference's own code compiled
+from its sources (see oracle/Makefile and oracle/ref_harness.cpp).
+
+Only the fixture script under tests/golden/ uses this, in a container that has
+the reference checkout; nothing on the GPU host loads it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_ref", "libref.so")
+
+
+def load():
+    if not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-C", HERE, "ref"], check=True, capture_output=True)
+    lib = C.CDLL(LIB_PATH)
+    vp, sz = C.c_void_p, C.c_size_t
+    lib.ref_random.argtypes = [C.c_uint32, sz, vp, vp]
+    lib.ref_random.restype = None
+    lib.ref_scene_new.argtypes = [vp, vp, sz, vp, sz, vp, vp, vp, C.c_int, C.c_char_p]
+    lib.ref_scene_new.restype = vp
+    lib.ref_scene_free.argtypes = [vp]
+    lib.ref_scene_free.restype = None
+    lib.ref_trace.argtypes = [vp, vp, sz, vp, vp]
+    lib.ref_trace.restype = None
+    lib.ref_env_texel.argtypes = [C.c_char_p, sz, vp, vp, vp, vp]
+    lib.ref_render.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp, vp, C.c_double, C.c_double, C.c_uint32,
+                               C.c_uint32, vp, vp]
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def random(seed, n):
+    lib = load()
+    nx = np.empty(n, dtype=np.uint32)
+    nd = np.empty(n, dtype=np.float64)
+    lib.ref_random(seed, n, _p(nx), _p(nd))
+    return nx, nd
+
+
+class RefScene:
+    def __init__(self, scene, bvh=False, hdr=None):
+        self.lib = load()
+        self.scene = scene
+        self.h = self.lib.ref_scene_new(_p(scene.tri), _p(scene.tri_n), scene.tri.shape[0], _p(scene.sph),
+                                        scene.sph.shape[0], _p(scene.prim_mat), _p(scene.prim_mat_count),
+                                        _p(scene.mats), 1 if bvh else 0, (hdr or "").encode())
+
+    def close(self):
+        if self.h:
+            self.lib.ref_scene_free(self.h)
+            self.h = None
+
+    def trace(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        prim = np.empty(rays.shape[0], dtype=np.int32)
+        t = np.empty(rays.shape[0], dtype=np.float64)
+        self.lib.ref_trace(self.h, _p(rays), rays.shape[0], _p(prim), _p(t))
+        return prim, t
+
+    def render(self, w, h, pos, direction, up, screen_h, dist, ss, batches):
+        mean = np.empty((h, w, 3), dtype=np.float64)
+        var = np.empty((h, w, 3), dtype=np.float64)
+        P, D, U = (np.asarray(v, dtype=np.float64) for v in (pos, direction, up))
+        self.lib.ref_render(self.h, w, h, _p(P), _p(D), _p(U), screen_h, dist, ss, batches, _p(mean), _p(var))
+        return mean, var
+
+
+def env_texels(hdr, dirs):
+    lib = load()
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+    n = dirs.shape[0]
+    ix = np.empty(n, dtype=np.int32)
+    iy = np.empty(n, dtype=np.int32)
+    wh = np.empty(2, dtype=np.int32)
+    rc = lib.ref_env_texel(hdr.encode(), n, _p(dirs), _p(ix), _p(iy), _p(wh))
+    if rc != 0:
+        raise RuntimeError("HDR read failed")
+    return ix, iy, int(wh[0]), int(wh[1])

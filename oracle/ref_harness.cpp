@@ -1,0 +1,167 @@
+// ref_harness.cpp -- C entry points over the reference's own code, used only
+// to write golden vectors in a container that has the reference checkout.
+// It includes the reference headers from $(REF)/src and links the reference
+// translation units listed in oracle/Makefile; nothing of the reference is
+// copied here.  The output goes to oracle/_ref/ (never committed, never run
+// on the GPU host).
+#include "stdafx.h"
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "renderer/IBL.h"
+#include "renderer/PathTracer.h"
+#include "renderer/Polygon.h"
+#include "renderer/Sphere.h"
+#include "scenes/Scene.h"
+#include "tools/Random.h"
+
+using namespace OmochiRenderer;
+
+namespace {
+
+struct Mat48 {
+  uint32_t type;
+  float color[3];
+  float emission[3];
+  float ior;
+  float rate;
+  uint32_t pad[3];
+};
+
+Material to_ref(const Mat48 &m) {
+  return Material((Material::REFLECTION_TYPE)m.type, Vector3(m.emission[0], m.emission[1], m.emission[2]),
+                  Vector3(m.color[0], m.color[1], m.color[2]), m.ior);
+}
+
+class HarnessScene : public Scene {
+ public:
+  std::vector<SceneObject *> prims;
+  HarnessScene(const float *tri, const float *tri_n, size_t n_tri, const float *sph, size_t n_sph,
+               const uint32_t *pm, const uint32_t *pc, const Mat48 *mats, int bvh) {
+    auto add_mats = [&](SceneObject *o, size_t i) {
+      uint32_t f = pm[i], c = pc ? pc[i] : 1u;
+      for (uint32_t k = 1; k < c; k++)
+        o->AddMaterial(to_ref(mats[f + k]), (double)mats[f + k].rate / (double)mats[f].rate, k + 1 == c);
+    };
+    for (size_t i = 0; i < n_tri; i++) {
+      const float *t = tri + 9 * i, *n = tri_n + 9 * i;
+      Vector3 p0(t[0], t[1], t[2]), e1(t[3], t[4], t[5]), e2(t[6], t[7], t[8]);
+      Vector3 n0(n[0], n[1], n[2]), d1(n[3], n[4], n[5]), d2(n[6], n[7], n[8]);
+      Polygon *p = new Polygon(p0, p0 + e1, p0 + e2, Vector3(), Vector3(), Vector3(), n0, n0 + d1, n0 + d2,
+                               to_ref(mats[pm[i]]), Vector3(0, 0, 0));
+      add_mats(p, i);
+      AddObject(p);
+      prims.push_back(p);
+    }
+    for (size_t i = 0; i < n_sph; i++) {
+      const float *s = sph + 4 * i;
+      Sphere *o = new Sphere(s[3], Vector3(s[0], s[1], s[2]), to_ref(mats[pm[n_tri + i]]));
+      add_mats(o, n_tri + i);
+      AddObject(o);
+      prims.push_back(o);
+    }
+    if (bvh) ConstructBVH();
+  }
+  void SetEnv(const char *hdr) { m_ibl.reset(new IBL(hdr)); }
+};
+
+}  // namespace
+
+extern "C" {
+
+// Random (src/tools/Random.h): n values of next() and, from a second
+// generator with the same seed, n values of nextDouble().
+void ref_random(uint32_t seed, size_t n, uint32_t *out_next, double *out_double) {
+  Random a(seed), b(seed);
+  for (size_t i = 0; i < n; i++) {
+    out_next[i] = a.next();
+    out_double[i] = b.nextDouble();
+  }
+}
+
+void *ref_scene_new(const float *tri, const float *tri_n, size_t n_tri, const float *sph, size_t n_sph,
+                    const uint32_t *pm, const uint32_t *pc, const void *mats, int bvh, const char *hdr) {
+  HarnessScene *s = new HarnessScene(tri, tri_n, n_tri, sph, n_sph, pm, pc, (const Mat48 *)mats, bvh);
+  if (hdr && hdr[0]) s->SetEnv(hdr);
+  return s;
+}
+
+void ref_scene_free(void *s) { delete (HarnessScene *)s; }
+
+// Scene::CheckIntersection on n rays (float o.xyz, tmax, d.xyz, pad): the
+// primitive index (-1 on a miss) and the FP64 distance.
+void ref_trace(void *sp, const float *rays, size_t n, int32_t *prim, double *t) {
+  HarnessScene *s = (HarnessScene *)sp;
+  for (size_t i = 0; i < n; i++) {
+    const float *r = rays + 8 * i;
+    Ray ray(Vector3(r[0], r[1], r[2]), Vector3(r[4], r[5], r[6]));
+    Scene::IntersectionInformation info;
+    bool hit = s->CheckIntersection(ray, info);
+    prim[i] = -1;
+    t[i] = info.hit.distance;
+    if (hit) {
+      for (size_t k = 0; k < s->prims.size(); k++)
+        if (s->prims[k] == info.object) { prim[i] = (int32_t)k; break; }
+    }
+  }
+}
+
+// IBL::Sample (src/renderer/IBL.h:30-45): texel of each direction, read off
+// the address of the returned pixel.
+int ref_env_texel(const char *hdr, size_t n, const double *dirs, int32_t *ix, int32_t *iy, int32_t *wh) {
+  HDRImage img;
+  if (!img.ReadFromRadianceFile(hdr)) return -1;
+  IBL ibl(hdr);
+  const double e = 1e-9;
+  const Color *base = &ibl.Sample(Vector3(std::sin(e), std::cos(e), 0.0));
+  int W = (int)img.GetWidth();
+  wh[0] = W;
+  wh[1] = (int)img.GetHeight();
+  for (size_t i = 0; i < n; i++) {
+    const Color *p = &ibl.Sample(Vector3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+    long idx = (long)(p - base);
+    ix[i] = (int32_t)(idx % W);
+    iy[i] = (int32_t)(idx / W);
+  }
+  return 0;
+}
+
+// PathTracer::RenderScene (src/renderer/PathTracer.cpp:55-85) with one
+// thread: `batches` sample batches of ss x ss samples each.  Writes the
+// per-pixel mean and variance of the batch means (top-down rows, W x H x 3).
+int ref_render(void *sp, uint32_t W, uint32_t H, const double *pos, const double *dir, const double *up,
+               double screen_h, double dist, uint32_t ss, uint32_t batches, double *mean, double *var) {
+  HarnessScene *s = (HarnessScene *)sp;
+  Camera cam(W, H, Vector3(pos[0], pos[1], pos[2]), Vector3(dir[0], dir[1], dir[2]), Vector3(up[0], up[1], up[2]),
+             screen_h, dist, 200.0);
+  std::vector<double> prev((size_t)W * H * 3, 0.0), m1((size_t)W * H * 3, 0.0), m2((size_t)W * H * 3, 0.0);
+  PathTracer::RenderingFinishCallbackFunction cb = [&](int samples, const Color *img, size_t w2, size_t h, double) {
+    // left W x H half of the 2W x H buffer; stored row (H-1-y) is screen row y
+    for (size_t r = 0; r < h; r++)
+      for (size_t x = 0; x < W; x++) {
+        const Color &c = img[x + r * w2];
+        double cur[3] = {c.x, c.y, c.z};
+        for (int k = 0; k < 3; k++) {
+          size_t i = (r * W + x) * 3 + k;
+          double batch = cur[k] * samples - prev[i] * (samples - 1);
+          prev[i] = cur[k];
+          m1[i] += batch;
+          m2[i] += batch * batch;
+        }
+      }
+  };
+  PathTracer pt(cam, 1, (int)batches, 1, (int)ss, cb);
+  pt.SetMaxThreadCount(1);
+  pt.RenderScene(*s);
+  for (size_t i = 0; i < m1.size(); i++) {
+    double mu = m1[i] / batches;
+    mean[i] = mu;
+    var[i] = batches > 1 ? (m2[i] - batches * mu * mu) / (batches - 1) : 0.0;
+  }
+  return 0;
+}
+
+}  // extern "C"
