@@ -153,9 +153,12 @@ int ref_render(void *sp, uint32_t W, uint32_t H, const double *pos, const double
         }
       }
   };
-  PathTracer pt(cam, 1, (int)batches, 1, (int)ss, cb);
-  pt.SetMaxThreadCount(1);
-  pt.RenderScene(*s);
+  // Heap-allocated and never destroyed: ~PathTracer applies delete[] to the
+  // ScreenPixels it allocated with new (src/renderer/PathTracer.cpp:50-53,62),
+  // which aborts under glibc.
+  PathTracer *pt = new PathTracer(cam, 1, (int)batches, 1, (int)ss, cb);
+  pt->SetMaxThreadCount(1);
+  pt->RenderScene(*s);
   for (size_t i = 0; i < m1.size(); i++) {
     double mu = m1[i] / batches;
     mean[i] = mu;

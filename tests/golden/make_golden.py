@@ -1,0 +1,80 @@
+"""Writes the golden fixtures in tests/golden/ from the reference's own code.
+
+Runs only in a container that has the reference checkout: it builds
+oracle/_ref/libref.so from the reference sources (oracle/Makefile) and calls
+the reference's Random, Scene::CheckIntersection (Polygon / Sphere, FP64),
+IBL::Sample and PathTracer::RenderScene through oracle/ref_harness.cpp.
+Inputs are regenerated from seeds by the tests; the fixtures hold the
+reference outputs plus a SHA-256 of the inputs so drift is detected.
+
+    python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "omochi-renderer3_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import golden_inputs as gi  # noqa: E402
+import omr  # noqa: E402
+import pyref  # noqa: E402
+
+REF_HDR = "/root/reference/input_data/Barce_Rooftop_C_Env.hdr"
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def rng_kat():
+    out = {}
+    for seed in gi.RNG_SEEDS:
+        nx, nd = pyref.random(seed, gi.RNG_N)
+        out[str(seed)] = {"next": [int(v) for v in nx], "next_double": [float(v) for v in nd]}
+    with open(os.path.join(HERE, "rng_kat.json"), "w") as f:
+        json.dump(out, f)
+
+
+def hits():
+    for name, (kind, n, seed, bvh) in gi.HIT_SETS.items():
+        scene = omr.procedural(kind, width=64, height=64)
+        rays = gi.hit_rays(scene, n, seed)
+        r = pyref.RefScene(scene, bvh=bvh)
+        prim, t = r.trace(rays)
+        r.close()
+        np.savez_compressed(os.path.join(HERE, f"hits_{name}.npz"), prim=prim, t=t, rays_sha256=sha(rays))
+        print(name, "hits", int((prim >= 0).sum()), "of", n)
+
+
+def env_texels():
+    dirs = gi.env_dirs()
+    ix, iy, w, h = pyref.env_texels(REF_HDR, dirs)
+    np.savez_compressed(os.path.join(HERE, "env_texels.npz"), ix=ix.astype(np.int16), iy=iy.astype(np.int16),
+                        w=w, h=h, dirs_sha256=sha(dirs))
+    print("env texels", w, h)
+
+
+def render_stats():
+    scene = omr.procedural(0, width=64, height=64)
+    r = pyref.RefScene(scene, bvh=False)
+    c = gi.REF_RENDER
+    mean, var = r.render(c["w"], c["h"], c["pos"], c["dir"], (0, 1, 0), c["screen_h"], c["dist"], c["ss"],
+                         c["batches"])
+    r.close()
+    np.savez_compressed(os.path.join(HERE, "ref_render_c0.npz"), mean=mean.astype(np.float32),
+                        var=var.astype(np.float32), **{k: np.asarray(v) for k, v in c.items()})
+    print("ref render mean", float(mean.mean()))
+
+
+if __name__ == "__main__":
+    rng_kat()
+    hits()
+    env_texels()
+    render_stats()

@@ -1,0 +1,131 @@
+"""CPU-only checks of the product library: it loads, exports every entry
+point include/omr_capi.h declares, and its host-side parts (BVH builder,
+procedural scenes, camera) are right.  No kernel is launched here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import helpers
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    import omr
+
+    lib = omr.load()
+    with open(os.path.join(ROOT, "include", "omr_capi.h")) as f:
+        text = f.read()
+    names = set(re.findall(r"^\s*(?:int|void|const char \*)\s+(omr_\w+)\s*\(", text, flags=re.M))
+    assert len(names) >= 30
+    for n in sorted(names):
+        assert hasattr(lib, n), f"{n} is declared but not exported"
+    assert names <= set(omr.EXPORTS) | {"omr_scene_info_get"}
+
+
+def test_create_fails_loudly_without_gpu():
+    import omr
+    from conftest import gpu_available
+
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(omr.OmrError):
+        omr.Context(0)
+
+
+def test_procedural_scene_sizes():
+    import omr
+
+    c0 = omr.procedural(0)
+    assert c0.tri.shape[0] == 12 and c0.sph.shape[0] == 2
+    c1 = omr.procedural(1)
+    assert c1.tri.shape[0] == 12 + 81920
+    lam = c1.mats["type"][c1.prim_mat[12:81932]] == omr.LAMBERT
+    assert 0.45 < lam.mean() < 0.55
+    # triangles face inward / outward as required by the single-sided test
+    n = np.cross(c0.tri[:, 3:6], c0.tri[:, 6:9])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    assert np.allclose(n, c0.tri_n[:, 0:3], atol=1e-6)
+    assert (n[[4, 5], 1] > 0.99).all()   # floor faces +y (into the box)
+
+
+def test_camera_matches_reference_constructor():
+    """Camera::Camera (src/renderer/Camera.h:12-43) in closed form."""
+    import omr
+
+    cam = omr.camera_lookat(640, 360, (50, 55, 230), (0.00001, -0.15, -1.0), screen_height=30, dist=40)
+    d = np.array([0.00001, -0.15, -1.0])
+    d /= np.linalg.norm(d)
+    X = np.cross(d, [0, 1, 0])
+    X = X / np.linalg.norm(X) * (30 * 640 / 360)
+    Y = np.cross(X, d)
+    Y = Y / np.linalg.norm(Y) * 30
+    assert np.allclose(cam.screen_center, np.array([50, 55, 230]) + d * 40, atol=1e-4)
+    assert np.allclose(cam.screen_x, X, atol=1e-4) and np.allclose(cam.screen_y, Y, atol=1e-4)
+
+
+def _nodes(nb):
+    rec = np.dtype([("b", "<f4", (6, 4)), ("child", "<i4", 4), ("meta", "<i4", 4)])
+    return np.frombuffer(nb.tobytes(), dtype=rec)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_bvh_invariants(kind):
+    """Every primitive in exactly one leaf; every primitive inside its leaf's
+    box; child boxes inside the parent's slot box; leaves of 1..8."""
+    import omr
+
+    scene = omr.procedural(kind)
+    nb, order, depth, stack = omr.build_bvh(scene)
+    nodes = _nodes(nb)
+    n = scene.n_prims
+    assert np.array_equal(np.sort(order), np.arange(n, dtype=np.uint32))
+    lo, hi = helpers.scene_bounds(scene)
+    seen = np.zeros(n, np.int32)
+    tri, sph = scene.tri, scene.sph
+
+    def prim_box(i):
+        if i < tri.shape[0]:
+            p = np.stack([tri[i, 0:3], tri[i, 0:3] + tri[i, 3:6], tri[i, 0:3] + tri[i, 6:9]])
+            return p.min(0), p.max(0)
+        s = sph[i - tri.shape[0]]
+        return s[0:3] - s[3], s[0:3] + s[3]
+
+    parent_box = {0: (lo - 1e9, hi + 1e9)}
+    for ni, nd in enumerate(nodes):
+        b = nd["b"]
+        plo, phi = parent_box[ni]
+        for k in range(4):
+            c = int(nd["child"][k])
+            if c == -1:
+                continue
+            clo = np.array([b[0][k], b[2][k], b[4][k]])
+            chi = np.array([b[1][k], b[3][k], b[5][k]])
+            assert (clo >= plo - 1e-3).all() and (chi <= phi + 1e-3).all()
+            if (c & 0xC0000000) == 0x80000000 or c < -1:
+                u = c & 0xFFFFFFFF
+                first, cnt = u & 0x07FFFFFF, ((u >> 27) & 7) + 1
+                assert 1 <= cnt <= 8
+                for s in range(first, first + cnt):
+                    pid = int(order[s])
+                    seen[pid] += 1
+                    a, z = prim_box(pid)
+                    assert (a >= clo).all() and (z <= chi).all()
+            else:
+                parent_box[c] = (clo, chi)
+    assert (seen == 1).all()
+    assert stack >= depth
+
+
+def test_bvh_rejects_bad_input():
+    import omr
+
+    mats = np.zeros(1, dtype=omr.MATERIAL_DTYPE)
+    bad = omr.Scene(np.full((1, 9), np.nan), None, np.zeros((0, 4)), np.zeros(1, np.uint32), mats)
+    with pytest.raises(omr.OmrError):
+        omr.build_bvh(bad)
+    empty = omr.Scene(np.zeros((0, 9)), None, np.zeros((0, 4)), np.zeros(0, np.uint32), mats)
+    with pytest.raises(omr.OmrError):
+        omr.build_bvh(empty)
