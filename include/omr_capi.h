@@ -125,6 +125,8 @@ typedef struct {
   uint32_t pool_paths;        /* path slots in flight (max) */
   uint32_t steps_per_sync;    /* bounce steps per host check (graph length) */
   uint32_t use_graph;         /* capture bounce steps in a hipGraph */
+  uint32_t refill_every;      /* queue refill cadence: 0 = finished paths regenerate inside
+                                 shade; k = idle slots are refilled in pixel order every k steps */
 } omr_tuning;
 
 typedef struct {
@@ -195,6 +197,13 @@ int omr_set_profiling(omr_ctx *ctx, int enable);
 /* ---- tuning ---- */
 int omr_get_tuning(omr_ctx *ctx, omr_tuning *t);
 int omr_set_tuning(omr_ctx *ctx, const omr_tuning *t);
+
+/* Autotuner: coordinate search over node width, workgroup size, grid
+ * occupancy, pool size, queue refill cadence and steps per host check, each
+ * candidate scored by device-timed Mrays/s of the given jobs (run batched).
+ * Leaves the best tuning set on the context; stops after budget_ms. */
+int omr_autotune(omr_ctx *ctx, size_t n, const omr_scene_id *scenes, const omr_camera *cams,
+                 const omr_render_params *params, uint32_t budget_ms, omr_tuning *best);
 
 /* ---- served mode: continuous batching with HBM-budget admission ---- */
 int omr_server_start(omr_ctx *ctx, const omr_server_config *cfg);

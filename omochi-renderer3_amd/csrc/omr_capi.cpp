@@ -1,6 +1,7 @@
 // omr_capi.cpp -- extern "C" entry points (declared in include/omr_capi.h).
 #include <hip/hip_runtime_api.h>
 
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -300,6 +301,67 @@ int omr_set_tuning(omr_ctx *ctx, const omr_tuning *t) {
   if (!ctx || !t) return OMR_ERR_INVALID;
   if (ctx->eng->server_running()) return set_err(ctx, OMR_ERR_STATE, "server mode is running");
   return from_engine(ctx, ctx->eng->set_tuning(*t));
+}
+
+int omr_autotune(omr_ctx *ctx, size_t n, const omr_scene_id *scenes, const omr_camera *cams,
+                 const omr_render_params *params, uint32_t budget_ms, omr_tuning *best_out) {
+  if (!ctx || n == 0 || !scenes || !cams || !params) return OMR_ERR_INVALID;
+  if (ctx->eng->server_running()) return set_err(ctx, OMR_ERR_STATE, "server mode is running");
+  std::vector<std::vector<float>> bufs(n);
+  std::vector<float *> outs(n);
+  for (size_t i = 0; i < n; i++) {
+    bufs[i].resize((size_t)cams[i].width * cams[i].height * 3);
+    outs[i] = bufs[i].data();
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  auto elapsed = [&] { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
+  // score = device Mrays/s of one run of the workload
+  auto score = [&](const omr_tuning &t, double *out) -> int {
+    int rc = omr_set_tuning(ctx, &t);
+    if (rc) return rc;
+    rc = render_impl(ctx, n, scenes, cams, params, outs.data(), nullptr);
+    if (rc) return rc;
+    const omr_render_stats &s = ctx->last;
+    *out = s.gpu_ms > 0 ? (double)(s.segments + s.shadow_rays) / (s.gpu_ms * 1e3) : 0.0;
+    return OMR_OK;
+  };
+  omr_tuning best = ctx->eng->tuning;
+  double best_s = 0;
+  int rc = score(best, &best_s);  // warm-up run (allocations, graph capture)
+  if (rc) return rc;
+  rc = score(best, &best_s);
+  if (rc) return rc;
+  struct Dim {
+    uint32_t omr_tuning::*field;
+    std::vector<uint32_t> values;
+  };
+  std::vector<Dim> dims = {
+      {&omr_tuning::node_width, {4}},
+      {&omr_tuning::wg_size, {64, 128, 256}},
+      {&omr_tuning::blocks_per_cu, {2, 4, 8, 16}},
+      {&omr_tuning::pool_paths, {1u << 19, 1u << 20, 1u << 21, 1u << 22}},
+      {&omr_tuning::refill_every, {0, 1, 2, 4}},
+      {&omr_tuning::steps_per_sync, {4, 8, 16}},
+  };
+  for (int pass = 0; pass < 2; pass++) {
+    for (auto &d : dims) {
+      for (uint32_t v : d.values) {
+        if (elapsed() > budget_ms) break;
+        if (best.*(d.field) == v) continue;
+        omr_tuning t = best;
+        t.*(d.field) = v;
+        double s = 0;
+        if (score(t, &s) == OMR_OK && s > best_s * 1.01) {
+          best_s = s;
+          best = t;
+        }
+      }
+    }
+  }
+  rc = omr_set_tuning(ctx, &best);
+  if (rc) return rc;
+  if (best_out) *best_out = best;
+  return OMR_OK;
 }
 
 int omr_server_start(omr_ctx *ctx, const omr_server_config *cfg) {

@@ -31,6 +31,7 @@ Engine::Engine(int device) : device_(device) {
   tuning.pool_paths = 1u << 21;
   tuning.steps_per_sync = 8;
   tuning.use_graph = 1;
+  tuning.refill_every = 0;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
     err = "no HIP device " + std::to_string(device);
@@ -96,6 +97,7 @@ int Engine::set_tuning(const omr_tuning &t) {
   if (t.blocks_per_cu < 1 || t.blocks_per_cu > 64) return fail(OMR_ERR_INVALID, "blocks_per_cu out of range");
   if (t.pool_paths < 64 || t.pool_paths > (1u << 26)) return fail(OMR_ERR_INVALID, "pool_paths out of range");
   if (t.steps_per_sync < 2 || t.steps_per_sync > 256) return fail(OMR_ERR_INVALID, "steps_per_sync out of range");
+  if (t.refill_every > 64) return fail(OMR_ERR_INVALID, "refill_every out of range");
   std::lock_guard<std::mutex> g(mu);
   hipSetDevice(device_);
   hipStreamSynchronize(stream_);
@@ -377,6 +379,10 @@ int Engine::collect_profile() {
 int Engine::launch_steps(uint32_t steps, bool with_shadow) {
   KernelArgs a = args();
   LaunchShape sh = shape();
+  // refill_every = 0: finished slots regenerate inside shade; k > 0: they go
+  // idle and a bulk refill every k steps hands out samples in pixel order.
+  const bool regen = tuning.refill_every == 0;
+  auto refill_now = [&](uint32_t s) { return !regen && ((s + 1) % tuning.refill_every == 0 || s + 1 == steps); };
   if (profile) {
     // one event pair per launch, every kernel directly on the stream
     size_t need = (size_t)steps * 3 * 2;
@@ -398,8 +404,9 @@ int Engine::launch_steps(uint32_t steps, bool with_shadow) {
     for (uint32_t s = 0; s < steps; s++) {
       int parity = (int)(s & 1u);
       hipError_t e = timed(0, [&] { return k::extend(a, sh, parity, true, stream_); });
-      if (e == hipSuccess) e = timed(1, [&] { return k::shade(a, sh, parity, stream_); });
+      if (e == hipSuccess) e = timed(1, [&] { return k::shade(a, sh, parity, regen, stream_); });
       if (e == hipSuccess && with_shadow) e = timed(2, [&] { return k::shadow(a, sh, stream_); });
+      if (e == hipSuccess && refill_now(s)) e = k::refill(a, sh, parity ^ 1, stream_);
       if (e != hipSuccess) return hipfail(e, "profiled bounce step launch");
     }
     return OMR_OK;
@@ -409,10 +416,14 @@ int Engine::launch_steps(uint32_t steps, bool with_shadow) {
       int parity = (int)(s & 1u);
       hipError_t e = k::extend(a, sh, parity, false, stream_);
       if (e != hipSuccess) return e;
-      e = k::shade(a, sh, parity, stream_);
+      e = k::shade(a, sh, parity, regen, stream_);
       if (e != hipSuccess) return e;
       if (with_shadow) {
         e = k::shadow(a, sh, stream_);
+        if (e != hipSuccess) return e;
+      }
+      if (refill_now(s)) {
+        e = k::refill(a, sh, parity ^ 1, stream_);
         if (e != hipSuccess) return e;
       }
     }

@@ -44,7 +44,9 @@ namespace k {
 hipError_t refill(const KernelArgs &a, const LaunchShape &s, int parity, hipStream_t st);
 // stats: count node visits / primitive tests into Ctrl (profiling runs)
 hipError_t extend(const KernelArgs &a, const LaunchShape &s, int parity, bool stats, hipStream_t st);
-hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, hipStream_t st);
+// regen: finished slots take the next sample inline; otherwise they go idle
+// until the next refill.
+hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool regen, hipStream_t st);
 hipError_t shadow(const KernelArgs &a, const LaunchShape &s, hipStream_t st);
 hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, double n_samples, hipStream_t st);
 // Closest hit for n rays (n x 8 floats).  counters (optional, device) gets

@@ -581,7 +581,7 @@ __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
 
 // One shading step; mirrors the CPU restatement's per-segment loop body.
 template <int WG>
-__global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity) {
+__global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int regen) {
   const uint32_t n = A.ctrl->qcount[parity];
   const uint32_t stride = gridDim.x * WG;
   const uint32_t P = A.p.pool;
@@ -812,7 +812,7 @@ __global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity) {
     }
     count_done(A.jobs, fin, js);
     // regenerate finished slots while samples remain
-    unsigned long long g = grab_sample(A.ctrl, fin);
+    unsigned long long g = grab_sample(A.ctrl, fin && regen);
     bool fresh = g != ~0ull;
     if (fresh) {
       start_sample(A, slot, g);
@@ -1042,12 +1042,13 @@ hipError_t extend(const KernelArgs &a, const LaunchShape &s, int parity, bool st
   }
 }
 
-hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, hipStream_t st) {
+hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool regen, hipStream_t st) {
   hipLaunchKernelGGL(k_clear_shadow, dim3(1), dim3(1), 0, st, a.ctrl);
+  int r = regen ? 1 : 0;
   switch (s.wg) {
-    case 64: hipLaunchKernelGGL(k_shade<64>, dim3(s.grid), dim3(64), 0, st, a, parity); break;
-    case 128: hipLaunchKernelGGL(k_shade<128>, dim3(s.grid), dim3(128), 0, st, a, parity); break;
-    default: hipLaunchKernelGGL(k_shade<256>, dim3(s.grid), dim3(256), 0, st, a, parity); break;
+    case 64: hipLaunchKernelGGL(k_shade<64>, dim3(s.grid), dim3(64), 0, st, a, parity, r); break;
+    case 128: hipLaunchKernelGGL(k_shade<128>, dim3(s.grid), dim3(128), 0, st, a, parity, r); break;
+    default: hipLaunchKernelGGL(k_shade<256>, dim3(s.grid), dim3(256), 0, st, a, parity, r); break;
   }
   return hipGetLastError();
 }

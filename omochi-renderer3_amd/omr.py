@@ -61,7 +61,8 @@ class SceneInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("node_width", C.c_uint32), ("wg_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("pool_paths", C.c_uint32), ("steps_per_sync", C.c_uint32), ("use_graph", C.c_uint32)]
+                ("pool_paths", C.c_uint32), ("steps_per_sync", C.c_uint32), ("use_graph", C.c_uint32),
+                ("refill_every", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):
@@ -85,7 +86,7 @@ EXPORTS = [
     "omr_trace_stats_device", "omr_render", "omr_render_sums", "omr_last_render_stats", "omr_render_batch",
     "omr_get_tuning", "omr_set_tuning", "omr_server_start", "omr_server_stop", "omr_job_submit", "omr_job_poll",
     "omr_job_wait", "omr_job_cancel", "omr_server_stats_get", "omr_procedural_scene", "omr_procedural_free",
-    "omr_camera_from_lookat", "omr_bvh_build", "omr_bvh_data", "omr_bvh_free", "omr_set_profiling",
+    "omr_camera_from_lookat", "omr_bvh_build", "omr_bvh_data", "omr_bvh_free", "omr_set_profiling", "omr_autotune",
 ]
 
 _lib = None
@@ -119,6 +120,7 @@ def load():
     lib.omr_last_render_stats.argtypes = [vp, C.POINTER(RenderStats)]
     lib.omr_render_batch.argtypes = [vp, sz, vp, vp, vp, vp]
     lib.omr_set_profiling.argtypes = [vp, C.c_int]
+    lib.omr_autotune.argtypes = [vp, sz, vp, vp, vp, C.c_uint32, vp]
     lib.omr_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     lib.omr_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
     lib.omr_server_start.argtypes = [vp, C.POINTER(ServerConfig)]
@@ -356,6 +358,13 @@ class Context:
     def get_tuning(self):
         t = Tuning()
         self._check(self.lib.omr_get_tuning(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+    def autotune(self, sids, cams, prms, budget_ms=20000):
+        n = len(sids)
+        t = Tuning()
+        self._check(self.lib.omr_autotune(self.h, n, (C.c_int32 * n)(*sids), (Camera * n)(*cams),
+                                          (RenderParams * n)(*prms), budget_ms, C.byref(t)))
         return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
     def set_tuning(self, **kw):
