@@ -43,6 +43,131 @@ WORKLOADS = {
 }
 
 
+def batched_jobs(n_jobs=64, seed=0x6F6D7233):
+    """Config 3: uneven jobs over four resident scenes (Cornell, Cornell +
+    icosphere, terrain + sky, torus knot).  Pixel count log-uniform in
+    [64^2, 1024^2], spp log-uniform in [1, 256] (as ss^2 x samples), depth in
+    {4, 8, 16}; environment importance sampling where a sky exists."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for j in range(n_jobs):
+        kind = j % 4
+        side = int(round(np.exp(rng.uniform(np.log(64), np.log(1024)))))
+        spp = float(np.exp(rng.uniform(0.0, np.log(256))))
+        ss = int(min(16, max(1, np.floor(np.sqrt(spp)))))
+        k = max(1, int(round(spp / (ss * ss))))
+        depth = int(rng.choice([4, 8, 16]))
+        jobs.append(dict(kind=kind, w=side, h=side, ss=ss, spp=k, depth=depth, env_is=int(kind in (2, 3)),
+                         seed=1000 + j))
+    return jobs
+
+
+def served_requests(n, kinds, seed):
+    """Config 4 requests: pixel count log-uniform in [64^2, 1920x1080] at 16:9,
+    spp in {1, 4, 16, 64, 256} (ss = sqrt(spp)), random seeds."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(n):
+        px = float(np.exp(rng.uniform(np.log(64 * 64), np.log(1920 * 1080))))
+        w = max(16, int(round(np.sqrt(px * 16 / 9))))
+        h = max(9, int(round(px / w)))
+        ss = int(rng.choice([1, 2, 4, 8, 16]))
+        si = int(rng.integers(0, len(kinds)))
+        reqs.append(dict(scene=si, w=w, h=h, ss=ss, spp=1, depth=8, env_is=int(kinds[si][0] in (2, 3, 4)),
+                         seed=int(rng.integers(1, 2 ** 31))))
+    return reqs
+
+
+def _prm(omr, j):
+    return omr.params(ss=j["ss"], spp_end=j["spp"], max_depth=j["depth"], rr_depth=3, env_is=j["env_is"],
+                      seed=j["seed"])
+
+
+def run_batched(ctx, omr, args):
+    kinds = [0, 1, 2, 3]
+    scenes = {k: omr.procedural(k, width=64, height=64) for k in kinds}
+    sids = {k: ctx.upload(s) for k, s in scenes.items()}
+    jobs = batched_jobs()
+    cams = [omr.procedural_camera(j["kind"], j["w"], j["h"]) for j in jobs]
+    prms = [_prm(omr, j) for j in jobs]
+    ids = [sids[j["kind"]] for j in jobs]
+    tune = None
+    if not args.no_tune:
+        # tune on a reduced copy of the workload (every 4th job)
+        tune = ctx.autotune(ids[::4], cams[::4], prms[::4], budget_ms=args.tune_ms)
+    # plain: each job alone, one after another
+    t0 = time.perf_counter()
+    plain, rays_plain = [], 0
+    for i in range(len(jobs)):
+        plain.append(ctx.render(ids[i], cams[i], prms[i]))
+        st = ctx.last_stats()
+        rays_plain += st["segments"] + st["shadow_rays"]
+    t_plain = time.perf_counter() - t0
+    # batched: all jobs in one persistent wavefront run
+    for _ in range(args.warmup):
+        ctx.render_batch(ids, cams, prms)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch = ctx.render_batch(ids, cams, prms)
+    t_batch = (time.perf_counter() - t0) / args.steps
+    st = ctx.last_stats()
+    rays_batch = st["segments"] + st["shadow_rays"]
+    identical = all(np.array_equal(a, b) for a, b in zip(plain, batch))
+    return dict(jobs=jobs, plain_mrays=rays_plain / t_plain / 1e6, batch_mrays=rays_batch / t_batch / 1e6,
+                t_plain=t_plain, t_batch=t_batch, rays=rays_batch, identical=identical, tuning=tune,
+                n_prims=sum(s.n_prims for s in scenes.values()))
+
+
+def run_served(ctx, omr, args):
+    kinds = [(0, 1), (1, 1), (1, 2), (2, 1), (3, 1), (3, 2), (4, 1), (0, 2)]
+    scenes = [omr.procedural(k, seed=0x6F6D7233 + s, width=64, height=64) for k, s in kinds]
+    sids = [ctx.upload(s) for s in scenes]
+    n_tri = sum(s.tri.shape[0] for s in scenes)
+    cal = served_requests(args.requests, kinds, seed=7)
+
+    def job_args(r):
+        cam = omr.procedural_camera(kinds[r["scene"]][0], r["w"], r["h"])
+        return sids[r["scene"]], cam, _prm(omr, r)
+
+    # plain: each request alone, back to back (also gives the service rate)
+    t0 = time.perf_counter()
+    rays_plain = 0
+    alone = []
+    for r in cal:
+        alone.append(ctx.render(*job_args(r)))
+        st = ctx.last_stats()
+        rays_plain += st["segments"] + st["shadow_rays"]
+    t_plain = time.perf_counter() - t0
+    capacity = len(cal) / t_plain
+    rates = []
+    rng = np.random.default_rng(11)
+    for f in (0.5, 0.8, 1.1):
+        lam = f * capacity
+        ctx.server_start()
+        outs = [np.zeros((r["h"], r["w"], 3), np.float32) for r in cal]
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, len(cal)))
+        t_start = time.perf_counter()
+        ids = []
+        for r, o, a in zip(cal, outs, arrivals):
+            now = time.perf_counter() - t_start
+            if a > now:
+                time.sleep(a - now)
+            ids.append(ctx.submit(*job_args(r), o))
+        for i in ids:
+            if ctx.wait(i, 600000) != 0:
+                raise RuntimeError("served job failed")
+        t_all = time.perf_counter() - t_start
+        st = ctx.server_stats()
+        ctx.server_stop()
+        same = all(np.array_equal(a, b) for a, b in zip(alone, outs))
+        rays = st["segments"] + st["shadow_rays"]
+        rates.append(dict(load=f, rate_jobs_per_s=round(lam, 3), p50_ms=round(st["lat_p50_ms"], 2),
+                          p99_ms=round(st["lat_p99_ms"], 2), mrays=round(rays / st["busy_ms"] / 1e3, 2),
+                          wall_s=round(t_all, 2), identical_to_alone=same))
+    return dict(plain_mrays=rays_plain / t_plain / 1e6, capacity=capacity, rates=rates, n_tri=n_tri,
+                n_requests=len(cal))
+
+
 def cpu_baseline(scene, cam, prm, rows, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -60,15 +185,52 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS) + ["c3", "c4"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=64)
+    ap.add_argument("--no-tune", action="store_true", help="c3: skip the autotuner")
+    ap.add_argument("--tune-ms", type=int, default=20000)
+    ap.add_argument("--requests", type=int, default=32, help="c4: requests per arrival rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is design-only.",
               file=sys.stderr)
         sys.exit(2)
     import omr
+
+    if args.workload in ("c3", "c4"):
+        ctx = omr.Context(0)
+        dev = ctx.device_info()
+        if args.workload == "c3":
+            r = run_batched(ctx, omr, args)
+            out = {
+                "metric": METRIC, "value": round(r["batch_mrays"], 2), "unit": "Mrays/s", "n_gpus": 1,
+                "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r["t_batch"] * 1e3, 3),
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                "data": "synthetic",
+                "config": {"workload": "c3: 64 uneven jobs over 4 resident scenes (Cornell, Cornell + icosphere, "
+                                       "terrain + sky, torus knot), one persistent wavefront run",
+                           "mode": "batched", "primitives": r["n_prims"], "device": dev["name"]},
+                "plain_mrays": round(r["plain_mrays"], 2), "plain_ms": round(r["t_plain"] * 1e3, 1),
+                "speedup_vs_plain": round(r["batch_mrays"] / r["plain_mrays"], 3),
+                "images_identical_to_plain": r["identical"], "rays_per_step": r["rays"], "tuning": r["tuning"],
+            }
+        else:
+            r = run_served(ctx, omr, args)
+            mid = r["rates"][1]
+            out = {
+                "metric": METRIC, "value": mid["mrays"], "unit": "Mrays/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+                "ms_per_step": round(mid["wall_s"] * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                "config": {"workload": f"c4: Poisson arrivals over 8 resident scenes ({r['n_tri']} triangles), "
+                                       f"{r['n_requests']} requests per rate", "mode": "served",
+                           "device": dev["name"]},
+                "plain_mrays": round(r["plain_mrays"], 2), "capacity_jobs_per_s": round(r["capacity"], 3),
+                "rates": r["rates"], "p50_ms": mid["p50_ms"], "p99_ms": mid["p99_ms"],
+            }
+        print(json.dumps(out), flush=True)
+        ctx.close()
+        return
 
     wl = WORKLOADS[args.workload]
     scene = omr.procedural(wl["kind"], width=wl["w"], height=wl["h"])

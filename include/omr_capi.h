@@ -228,6 +228,9 @@ typedef struct {
 } omr_procedural;
 int omr_procedural_scene(int kind, uint64_t seed, uint32_t width, uint32_t height, omr_procedural *out);
 void omr_procedural_free(omr_procedural *p);
+/* The camera preset of a procedural kind at another image size (kind 4 is a
+ * 2000 x 1000 terrain, 3,996,002 triangles, with its own sky). */
+int omr_procedural_camera(int kind, uint32_t width, uint32_t height, omr_camera *out);
 
 /* Reference Camera constructor (src/renderer/Camera.h:12-43), double precision:
  * fills the derived screen vectors of an omr_camera. */

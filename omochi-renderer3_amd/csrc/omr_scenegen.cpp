@@ -313,37 +313,69 @@ extern "C" void omr_camera_from_lookat(uint32_t width, uint32_t height, const do
   out->focal_distance = (float)focal_distance;
 }
 
+namespace {
+// Camera preset of each procedural kind: position, direction, screen height.
+bool preset(int kind, double cp[3], double cd[3], double *sh) {
+  *sh = 30;
+  switch (kind) {
+    case 0:
+    case 1:
+      cp[0] = 50; cp[1] = 50; cp[2] = 260;
+      cd[0] = 0; cd[1] = 0; cd[2] = -1;
+      return true;
+    case 2:
+      cp[0] = 500; cp[1] = 140; cp[2] = -120;
+      cd[0] = 0; cd[1] = -0.32; cd[2] = 1;
+      *sh = 24;
+      return true;
+    case 3:
+      cp[0] = 0; cp[1] = 60; cp[2] = 130;
+      cd[0] = 0; cd[1] = -0.28; cd[2] = -1;
+      return true;
+    case 4:
+      cp[0] = 1000; cp[1] = 200; cp[2] = -200;
+      cd[0] = 0; cd[1] = -0.3; cd[2] = 1;
+      *sh = 24;
+      return true;
+    default:
+      return false;
+  }
+}
+}  // namespace
+
+extern "C" int omr_procedural_camera(int kind, uint32_t width, uint32_t height, omr_camera *out) {
+  double cp[3], cd[3], up[3] = {0, 1, 0}, sh;
+  if (!out || width == 0 || height == 0 || !preset(kind, cp, cd, &sh)) return OMR_ERR_INVALID;
+  omr_camera_from_lookat(width, height, cp, cd, up, sh, 40.0, 200.0, 0.0, out);
+  return OMR_OK;
+}
+
 extern "C" int omr_procedural_scene(int kind, uint64_t seed, uint32_t width, uint32_t height, omr_procedural *out) {
   if (!out || width == 0 || height == 0) return OMR_ERR_INVALID;
   std::memset(out, 0, sizeof(*out));
   Owned *o = new Owned();
   Builder &B = o->b;
-  double cp[3], cd[3], up[3] = {0, 1, 0};
-  double sh = 30, dist = 40;
   switch (kind) {
     case 0:
     case 1:
       cornell(B, true);
       if (kind == 1) icosphere(B, {50, 62, 45}, 18.0, 6, seed, 0.08);
-      cp[0] = 50; cp[1] = 50; cp[2] = 260;
-      cd[0] = 0; cd[1] = 0; cd[2] = -1;
       break;
     case 2:
       terrain(B, 1000, 500, seed);
       sky(B, 2048, 1024, seed);
-      cp[0] = 500; cp[1] = 140; cp[2] = -120;
-      cd[0] = 0; cd[1] = -0.32; cd[2] = 1;
-      sh = 24;
       break;
     case 3: {
       torus_knot(B, {0, 30, 0}, 10.0, 1000, 150, seed);
       uint32_t fl = B.material(OMR_MAT_LAMBERT, {0.6, 0.6, 0.6}, {0, 0, 0}, 0);
       B.quad({-200, 0, -200}, {-200, 0, 200}, {200, 0, 200}, {200, 0, -200}, {0, 1, 0}, fl);
       sky(B, 512, 256, seed + 11);
-      cp[0] = 0; cp[1] = 60; cp[2] = 130;
-      cd[0] = 0; cd[1] = -0.28; cd[2] = -1;
       break;
     }
+    case 4:
+      terrain(B, 2000, 1000, seed);
+      sky(B, 2048, 1024, seed + 4);
+      break;
     default:
       delete o;
       return OMR_ERR_INVALID;
@@ -363,7 +395,7 @@ extern "C" int omr_procedural_scene(int kind, uint64_t seed, uint32_t width, uin
   d.env_width = B.env_w;
   d.env_height = B.env_h;
   d.env_rgb = B.env.empty() ? nullptr : B.env.data();
-  omr_camera_from_lookat(width, height, cp, cd, up, sh, dist, 200.0, 0.0, &out->camera);
+  omr_procedural_camera(kind, width, height, &out->camera);
   out->internal = o;
   return OMR_OK;
 }

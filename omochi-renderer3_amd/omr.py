@@ -87,6 +87,7 @@ EXPORTS = [
     "omr_get_tuning", "omr_set_tuning", "omr_server_start", "omr_server_stop", "omr_job_submit", "omr_job_poll",
     "omr_job_wait", "omr_job_cancel", "omr_server_stats_get", "omr_procedural_scene", "omr_procedural_free",
     "omr_camera_from_lookat", "omr_bvh_build", "omr_bvh_data", "omr_bvh_free", "omr_set_profiling", "omr_autotune",
+    "omr_procedural_camera",
 ]
 
 _lib = None
@@ -133,6 +134,7 @@ def load():
     lib.omr_procedural_scene.argtypes = [C.c_int, u64, C.c_uint32, C.c_uint32, C.POINTER(Procedural)]
     lib.omr_procedural_free.argtypes = [C.POINTER(Procedural)]
     lib.omr_procedural_free.restype = None
+    lib.omr_procedural_camera.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.POINTER(Camera)]
     lib.omr_camera_from_lookat.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_double * 3),
                                            C.POINTER(C.c_double * 3), C.POINTER(C.c_double * 3), C.c_double,
                                            C.c_double, C.c_double, C.c_double, C.POINTER(Camera)]
@@ -232,6 +234,14 @@ def build_bvh(scene):
         return nb, ob, depth.value, sb.value
     finally:
         lib.omr_bvh_free(h)
+
+
+def procedural_camera(kind, width, height):
+    cam = Camera()
+    rc = load().omr_procedural_camera(kind, width, height, C.byref(cam))
+    if rc != OMR_OK:
+        raise OmrError(rc, "omr_procedural_camera failed")
+    return cam
 
 
 def camera_lookat(width, height, pos, direction, up=(0.0, 1.0, 0.0), screen_height=30.0, dist=40.0, focal=200.0,
