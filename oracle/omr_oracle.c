@@ -621,6 +621,13 @@ static void env_tables(orc_handle *h) {
   free(wf);
 }
 
+int orc_env_tables_raw(const orc_handle *h, uint64_t *row, uint64_t *marg) {
+  if (!h->env_q) return -1;
+  memcpy(row, h->env_row, (size_t)h->s.env_h * (h->s.env_w + 1) * sizeof(uint64_t));
+  memcpy(marg, h->env_marg, (size_t)(h->s.env_h + 1) * sizeof(uint64_t));
+  return 0;
+}
+
 int orc_env_cdf(const orc_handle *h, float *marginal, float *conditional) {
   if (!h->env_q) return -1;
   uint32_t W = h->s.env_w, H = h->s.env_h;

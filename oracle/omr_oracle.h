@@ -117,6 +117,8 @@ int orc_render_sums(const orc_handle *h, const orc_camera *cam, const orc_params
 /* Environment importance sampling tables (built on the CPU here). */
 int orc_env_cdf(const orc_handle *h, float *marginal /* env_h+1 */, float *conditional /* env_h x (env_w+1) */);
 void orc_env_sample(const orc_handle *h, float u1, float u2, float dir[3], float *pdf, uint32_t *ix, uint32_t *iy);
+/* The integer tables themselves: row prefix sums h x (w+1), marginal h+1. */
+int orc_env_tables_raw(const orc_handle *h, uint64_t *row, uint64_t *marg);
 float orc_env_pdf(const orc_handle *h, float dx, float dy, float dz);
 
 #ifdef __cplusplus

@@ -71,6 +71,7 @@ def load(build_if_missing=True):
     lib.orc_render_sums.argtypes = [vp, C.POINTER(OCamera), C.POINTER(OParams), C.c_int, C.c_int, u32, u32, vp, vp,
                                     vp]
     lib.orc_env_cdf.argtypes = [vp, vp, vp]
+    lib.orc_env_tables_raw.argtypes = [vp, vp, vp]
     lib.orc_env_sample.argtypes = [vp, C.c_float, C.c_float, vp, C.POINTER(C.c_float), C.POINTER(u32),
                                    C.POINTER(u32)]
     lib.orc_env_sample.restype = None
@@ -161,6 +162,14 @@ class Oracle:
         sums, seg, sh = self.render_sums(cam, prm, accel, threads, rows)
         n = float(prm.supersamples * prm.supersamples * (prm.spp_end - prm.spp_begin))
         return (sums.astype(np.float64) * (1.0 / 16777216.0) / n).astype(np.float32), seg, sh
+
+    def env_tables(self):
+        h, w = self.scene.env.shape[0], self.scene.env.shape[1]
+        row = np.zeros(h * (w + 1), dtype=np.uint64)
+        marg = np.zeros(h + 1, dtype=np.uint64)
+        if self.lib.orc_env_tables_raw(self.h, _p(row), _p(marg)) != 0:
+            raise RuntimeError("scene has no environment")
+        return row.reshape(h, w + 1), marg
 
     def env_pdf(self, d):
         return self.lib.orc_env_pdf(self.h, float(d[0]), float(d[1]), float(d[2]))
