@@ -127,6 +127,9 @@ typedef struct {
   uint32_t use_graph;         /* capture bounce steps in a hipGraph */
   uint32_t refill_every;      /* queue refill cadence: 0 = finished paths regenerate inside
                                  shade; k = idle slots are refilled in pixel order every k steps */
+  uint32_t fetch_budget;      /* traversal: 0 = one ray per lane per grid-stride turn; k = lanes
+                                 that finished fetch new rays after every k node/leaf visits */
+  uint32_t fetch_min;         /* idle lanes (of 64) that trigger a fetch */
 } omr_tuning;
 
 typedef struct {

@@ -32,6 +32,8 @@ Engine::Engine(int device) : device_(device) {
   tuning.steps_per_sync = 8;
   tuning.use_graph = 1;
   tuning.refill_every = 0;
+  tuning.fetch_budget = 0;
+  tuning.fetch_min = 16;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
     err = "no HIP device " + std::to_string(device);
@@ -98,6 +100,8 @@ int Engine::set_tuning(const omr_tuning &t) {
   if (t.pool_paths < 64 || t.pool_paths > (1u << 26)) return fail(OMR_ERR_INVALID, "pool_paths out of range");
   if (t.steps_per_sync < 2 || t.steps_per_sync > 256) return fail(OMR_ERR_INVALID, "steps_per_sync out of range");
   if (t.refill_every > 64) return fail(OMR_ERR_INVALID, "refill_every out of range");
+  if (t.fetch_budget > 1024) return fail(OMR_ERR_INVALID, "fetch_budget out of range");
+  if (t.fetch_budget && (t.fetch_min < 1 || t.fetch_min > 64)) return fail(OMR_ERR_INVALID, "fetch_min must be 1..64");
   std::lock_guard<std::mutex> g(mu);
   hipSetDevice(device_);
   hipStreamSynchronize(stream_);
@@ -124,6 +128,8 @@ KernelArgs Engine::args() const {
   a.p = pool_;
   a.p.gstack = d_gstack_;
   a.p.gcap = gcap_;
+  a.p.df_budget = tuning.fetch_budget;
+  a.p.df_fetch_min = tuning.fetch_min;
   return a;
 }
 
@@ -343,24 +349,21 @@ int Engine::trace(omr_scene_id id, const float *d_rays, size_t n, int32_t *d_pri
   uint32_t grid = (uint32_t)std::min<size_t>((n + wg - 1) / wg, (size_t)n_cu_ * 16);
   int rc = ensure_gstack(s->bvh.stack_bound, (size_t)grid * wg);
   if (rc) return rc;
+  // d_cnt[0..1]: node visits, primitive tests; d_cnt[2]: dynamic-fetch head
   unsigned long long *d_cnt = nullptr;
+  OMR_HIP(hipMalloc(&d_cnt, 32));
+  hipError_t e = hipMemsetAsync(d_cnt, 0, 32, stream_);
+  if (e == hipSuccess)
+    e = k::trace_rays(d_scenes_ + id, d_rays, (uint32_t)n, d_prim, d_t, stats2 ? d_cnt : nullptr, d_gstack_, gcap_,
+                      grid, wg, (uint32_t *)(d_cnt + 2), (int)tuning.fetch_budget, tuning.fetch_min, stream_);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  unsigned long long c[2] = {0, 0};
+  if (e == hipSuccess && stats2) e = hipMemcpy(c, d_cnt, 16, hipMemcpyDeviceToHost);
+  hipFree(d_cnt);
+  if (e != hipSuccess) return hipfail(e, "trace_rays");
   if (stats2) {
-    OMR_HIP(hipMalloc(&d_cnt, 16));
-    OMR_HIP(hipMemsetAsync(d_cnt, 0, 16, stream_));
-  }
-  hipError_t e = k::trace_rays(d_scenes_ + id, d_rays, (uint32_t)n, d_prim, d_t, d_cnt, d_gstack_, gcap_, grid, wg,
-                               stream_);
-  if (e != hipSuccess) {
-    if (d_cnt) hipFree(d_cnt);
-    return hipfail(e, "trace_rays");
-  }
-  OMR_HIP(hipStreamSynchronize(stream_));
-  if (stats2) {
-    unsigned long long c[2];
-    OMR_HIP(hipMemcpy(c, d_cnt, 16, hipMemcpyDeviceToHost));
     stats2[0] = c[0];
     stats2[1] = c[1];
-    hipFree(d_cnt);
   }
   return OMR_OK;
 }

@@ -26,6 +26,8 @@ struct PoolPtrs {
   uint2 *gstack;       // per-thread overflow traversal stacks
   uint32_t pool;
   uint32_t gcap;       // entries per thread in gstack
+  uint32_t df_budget;     // dynamic fetch: traversal visits between fetches (0 = grid-stride)
+  uint32_t df_fetch_min;  // dynamic fetch: idle lanes of a wave that trigger a fetch
 };
 
 struct KernelArgs {
@@ -51,9 +53,10 @@ hipError_t shadow(const KernelArgs &a, const LaunchShape &s, hipStream_t st);
 hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, double n_samples, hipStream_t st);
 // Closest hit for n rays (n x 8 floats).  counters (optional, device) gets
 // node visits and primitive tests.
+// head: zeroed fetch counter (used when budget > 0, dynamic fetch).
 hipError_t trace_rays(const SceneDev *scene, const float *rays, uint32_t n, int32_t *prim, float *t,
                       unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t grid, uint32_t wg,
-                      hipStream_t st);
+                      uint32_t *head, int budget, uint32_t fetch_min, hipStream_t st);
 // Environment importance-sampling tables: weights, per-row scan, marginal scan.
 hipError_t env_tables(const float *env, uint32_t w, uint32_t h, float *wbuf, unsigned int *wmax_bits,
                       unsigned long long *q, unsigned long long *row, unsigned long long *marg, hipStream_t st);

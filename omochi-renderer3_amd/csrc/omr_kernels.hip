@@ -225,20 +225,45 @@ __device__ inline void cswap(float &ka, int32_t &ca, float &kb, int32_t &cb) {
 // boxes per node (BoundingBox.h:127-172), nearest-first order, leaves tested
 // as they are reached, pruning by the closest distance so far; ties on t go
 // to the smaller primitive id so the result does not depend on the order.
+// The traversal is resumable: init() sets up a ray, step(k) performs at most
+// k node or leaf visits and reports whether the ray is finished.  traverse()
+// runs one ray to the end.
 template <int WG, int SHORT, bool ANY, bool STATS>
-__device__ inline void traverse(const SceneDev &S, f3 o, f3 d, Hit &best, Stack<WG, SHORT> &stk, uint32_t &nvis,
-                                uint32_t &ntest) {
+struct Trav {
+  const float4 *nodes, *prims;
+  f3 o, d;
+  float ix, iy, iz;
+  bool sx, sy, sz;
+  int32_t cur;
+  Hit best;
+
+  __device__ inline void init(const SceneDev &S, f3 o_, f3 d_, float tmax, Stack<WG, SHORT> &stk) {
+    o = o_;
+    d = d_;
+    // QBVH.cpp:56-60: zero direction component -> INF
+    ix = (d.x == 0.0f) ? kInf : 1.0f / d.x;
+    iy = (d.y == 0.0f) ? kInf : 1.0f / d.y;
+    iz = (d.z == 0.0f) ? kInf : 1.0f / d.z;
+    sx = !(d.x >= 0.0f);
+    sy = !(d.y >= 0.0f);
+    sz = !(d.z >= 0.0f);
+    nodes = reinterpret_cast<const float4 *>(S.nodes);
+    prims = reinterpret_cast<const float4 *>(S.prims);
+    best.t = tmax;
+    best.u = best.v = 0.0f;
+    best.prim = -1;
+    cur = 0;
+    stk.sp = 0;
+  }
+
+  __device__ inline bool step(Stack<WG, SHORT> &stk, int budget, uint32_t &nvis, uint32_t &ntest);
+};
+
+template <int WG, int SHORT, bool ANY, bool STATS>
+__device__ inline bool Trav<WG, SHORT, ANY, STATS>::step(Stack<WG, SHORT> &stk, int budget, uint32_t &nvis,
+                                                          uint32_t &ntest) {
   const float ox = o.x, oy = o.y, oz = o.z;
-  // QBVH.cpp:56-60: zero direction component -> INF
-  const float ix = (d.x == 0.0f) ? kInf : 1.0f / d.x;
-  const float iy = (d.y == 0.0f) ? kInf : 1.0f / d.y;
-  const float iz = (d.z == 0.0f) ? kInf : 1.0f / d.z;
-  const bool sx = !(d.x >= 0.0f), sy = !(d.y >= 0.0f), sz = !(d.z >= 0.0f);
-  const float4 *nodes = reinterpret_cast<const float4 *>(S.nodes);
-  const float4 *prims = reinterpret_cast<const float4 *>(S.prims);
-  stk.sp = 0;
-  int32_t cur = 0;
-  for (;;) {
+  for (int it = 0; it < budget; it++) {
     bool need_pop;
     if (!child_is_leaf(cur)) {
       const float4 *np = nodes + (size_t)cur * 8;
@@ -307,7 +332,7 @@ __device__ inline void traverse(const SceneDev &S, f3 o, f3 d, Hit &best, Stack<
           best.prim = (int32_t)id;
         }
       }
-      if (ANY && best.prim >= 0) return;
+      if (ANY && best.prim >= 0) return true;
       need_pop = true;
     }
     if (need_pop) {
@@ -319,9 +344,61 @@ __device__ inline void traverse(const SceneDev &S, f3 o, f3 d, Hit &best, Stack<
         found = true;
         break;
       }
-      if (!found) return;
+      if (!found) return true;
     }
   }
+  return false;
+}
+
+template <int WG, int SHORT, bool ANY, bool STATS>
+__device__ inline void traverse(const SceneDev &S, f3 o, f3 d, Hit &best, Stack<WG, SHORT> &stk, uint32_t &nvis,
+                                uint32_t &ntest) {
+  Trav<WG, SHORT, ANY, STATS> tr;
+  tr.init(S, o, d, best.t, stk);
+  tr.step(stk, 0x7FFFFFFF, nvis, ntest);
+  best = tr.best;
+}
+
+// Dynamic fetch: every lane keeps its own ray; after each slice of `budget`
+// visits, lanes whose ray finished take new rays from the queue (one atomic
+// per wave) as soon as at least `fetch_min` lanes are idle, so a wave does
+// not idle behind its slowest ray.  load(i, S, o, d, tmax, tag) reads queue
+// entry i; store(tag, hit) writes a finished ray's result.
+template <int WG, int SHORT, bool ANY, bool STATS, class Load, class Store>
+__device__ inline uint32_t dynamic_fetch(uint32_t n, uint32_t *head, Stack<WG, SHORT> &stk, int budget,
+                                         uint32_t fetch_min, Load load, Store store, uint32_t &nvis,
+                                         uint32_t &ntest) {
+  Trav<WG, SHORT, ANY, STATS> tr;
+  bool has = false, exhausted = false;
+  uint32_t tag = 0, done = 0;
+  const uint32_t active = (uint32_t)__popcll(__ballot(true));
+  for (;;) {
+    unsigned long long m = __ballot(!has);
+    uint32_t idle = (uint32_t)__popcll(m);
+    if (!exhausted && idle != 0u && (idle >= fetch_min || idle == active)) {
+      uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(head, idle);
+      base = __shfl(base, (int)leader);
+      if (base + idle >= n) exhausted = true;
+      uint32_t i = base + rank_in(m);
+      if (!has && i < n) {
+        const SceneDev *S;
+        f3 o, d;
+        float tmax;
+        load(i, S, o, d, tmax, tag);
+        tr.init(*S, o, d, tmax, stk);
+        has = true;
+      }
+    }
+    if (__ballot(has) == 0ull) break;
+    if (has && tr.step(stk, budget, nvis, ntest)) {
+      store(tag, tr.best);
+      has = false;
+      done++;
+    }
+  }
+  return done;
 }
 
 // ------------------------------------------------------------ environment
@@ -534,18 +611,39 @@ __global__ __launch_bounds__(WG) void k_refill(KernelArgs A, int parity) {
   }
 }
 
-template <int WG, int SHORT, bool STATS>
+template <int WG, int SHORT, bool STATS, bool DF>
 __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
   __shared__ uint2 lds[SHORT * WG];
   const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
-  if (gtid == 0) A.ctrl->qcount[parity ^ 1] = 0u;
+  if (gtid == 0) {
+    A.ctrl->qcount[parity ^ 1] = 0u;
+    A.ctrl->qhead[parity ^ 1] = 0u;  // fetch head of the next step's extend
+  }
   const uint32_t n = A.ctrl->qcount[parity];
   const uint32_t stride = gridDim.x * WG;
   Stack<WG, SHORT> stk;
   stk.lds = lds;
   stk.g = A.p.gstack + (size_t)gtid * A.p.gcap;
   uint32_t segs = 0, nv = 0, nt = 0;
-  for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
+  if (DF) {
+    auto load = [&](uint32_t i, const SceneDev *&S, f3 &o, f3 &d, float &tmax, uint32_t &tag) {
+      uint32_t slot = A.p.queue[parity][i];
+      float4 ro = reinterpret_cast<const float4 *>(A.p.ray_o)[slot];
+      float4 rd = reinterpret_cast<const float4 *>(A.p.ray_d)[slot];
+      uint32_t js = A.p.misc[4 * (size_t)slot + 3];
+      S = &A.scenes[A.jobs[js].scene];
+      o = xyz(ro);
+      d = xyz(rd);
+      tmax = kInf;
+      tag = slot;
+    };
+    auto store = [&](uint32_t slot, const Hit &h) {
+      reinterpret_cast<float4 *>(A.p.hit)[slot] = make_float4(h.t, __int_as_float(h.prim), h.u, h.v);
+    };
+    segs = dynamic_fetch<WG, SHORT, false, STATS>(n, &A.ctrl->qhead[parity], stk, (int)A.p.df_budget,
+                                                 A.p.df_fetch_min, load, store, nv, nt);
+  }
+  for (uint32_t base = blockIdx.x * WG; !DF && base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     if (i < n) {
       uint32_t slot = A.p.queue[parity][i];
@@ -839,7 +937,18 @@ __global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int rege
   }
 }
 
-template <int WG, int SHORT>
+__device__ inline void add_shadow(const KernelArgs &A, uint32_t i) {
+  const float4 *sq = reinterpret_cast<const float4 *>(A.p.shadow) + (size_t)i * 3;
+  float4 a = sq[0], b = sq[1], c = sq[2];
+  const JobDev &J = A.jobs[__float_as_uint(a.w)];
+  unsigned long long *acc = J.accum + 3ull * __float_as_uint(b.w);
+  unsigned long long qa = fix(c.x), qb = fix(c.y), qc = fix(c.z);
+  if (qa) atomicAdd(acc + 0, qa);
+  if (qb) atomicAdd(acc + 1, qb);
+  if (qc) atomicAdd(acc + 2, qc);
+}
+
+template <int WG, int SHORT, bool DF>
 __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
   __shared__ uint2 lds[SHORT * WG];
   const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
@@ -849,7 +958,24 @@ __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
   stk.lds = lds;
   stk.g = A.p.gstack + (size_t)gtid * A.p.gcap;
   uint32_t cnt = 0;
-  for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
+  if (DF) {
+    auto load = [&](uint32_t i, const SceneDev *&S, f3 &o, f3 &d, float &tmax, uint32_t &tag) {
+      const float4 *sq = reinterpret_cast<const float4 *>(A.p.shadow) + (size_t)i * 3;
+      float4 a = sq[0], b = sq[1];
+      S = &A.scenes[A.jobs[__float_as_uint(a.w)].scene];
+      o = xyz(a);
+      d = xyz(b);
+      tmax = kInf;
+      tag = i;
+    };
+    auto store = [&](uint32_t i, const Hit &h) {
+      if (h.prim < 0) add_shadow(A, i);
+    };
+    uint32_t nv = 0, nt = 0;
+    cnt = dynamic_fetch<WG, SHORT, true, false>(n, &A.ctrl->shead, stk, (int)A.p.df_budget, A.p.df_fetch_min,
+                                               load, store, nv, nt);
+  }
+  for (uint32_t base = blockIdx.x * WG; !DF && base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     if (i < n) {
       const float4 *sq = reinterpret_cast<const float4 *>(A.p.shadow) + (size_t)i * 3;
@@ -877,14 +1003,13 @@ __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) tot += __shfl_xor(tot, off);
   if (lane_id() == 0 && tot) atomicAdd(&A.ctrl->shadow_rays, (unsigned long long)tot);
-  __syncthreads();
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    // the next step's shade appends afresh; this kernel is the last reader
-  }
 }
 
-// Clears the shadow queue count at the start of a step (single thread).
-__global__ void k_clear_shadow(Ctrl *ctrl) { ctrl->scount = 0u; }
+// Clears the shadow queue count and fetch head at the start of a step.
+__global__ void k_clear_shadow(Ctrl *ctrl) {
+  ctrl->scount = 0u;
+  ctrl->shead = 0u;
+}
 
 // mean = sum * 2^-24 / (ss^2 * samples), in double, as the CPU restatement
 __global__ void k_resolve(const unsigned long long *accum, float *out, uint32_t n, double n_samples) {
@@ -892,10 +1017,10 @@ __global__ void k_resolve(const unsigned long long *accum, float *out, uint32_t 
   if (i < n) out[i] = (float)((double)(long long)accum[i] * (1.0 / 16777216.0) / n_samples);
 }
 
-template <int WG, int SHORT, bool STATS>
+template <int WG, int SHORT, bool STATS, bool DF>
 __global__ __launch_bounds__(WG) void k_trace_rays(const SceneDev *scene, const float4 *rays, uint32_t n, int32_t *prim,
                                                    float *tout, unsigned long long *counters, uint2 *gstack,
-                                                   uint32_t gcap) {
+                                                   uint32_t gcap, uint32_t *head, int budget, uint32_t fetch_min) {
   __shared__ uint2 lds[SHORT * WG];
   const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
   const uint32_t stride = gridDim.x * WG;
@@ -904,7 +1029,22 @@ __global__ __launch_bounds__(WG) void k_trace_rays(const SceneDev *scene, const 
   stk.g = gstack + (size_t)gtid * gcap;
   const SceneDev S = *scene;
   uint32_t nv = 0, nt = 0;
-  for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
+  if (DF) {
+    auto load = [&](uint32_t i, const SceneDev *&Sp, f3 &o, f3 &d, float &tmax, uint32_t &tag) {
+      float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
+      Sp = scene;
+      o = xyz(a);
+      d = xyz(b);
+      tmax = a.w;
+      tag = i;
+    };
+    auto store = [&](uint32_t i, const Hit &h) {
+      prim[i] = h.prim;
+      tout[i] = h.prim >= 0 ? h.t : kInf;
+    };
+    dynamic_fetch<WG, SHORT, false, STATS>(n, head, stk, budget, fetch_min, load, store, nv, nt);
+  }
+  for (uint32_t base = blockIdx.x * WG; !DF && base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     if (i < n) {
       float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
@@ -1016,8 +1156,31 @@ constexpr int kShort = (int)kShortStack;
 
 template <int WG>
 hipError_t launch_extend_wg(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
-  if (stats) hipLaunchKernelGGL((k_extend<WG, kShort, true>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else hipLaunchKernelGGL((k_extend<WG, kShort, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  bool df = a.p.df_budget > 0;
+  if (stats && df) hipLaunchKernelGGL((k_extend<WG, kShort, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else if (stats) hipLaunchKernelGGL((k_extend<WG, kShort, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else if (df) hipLaunchKernelGGL((k_extend<WG, kShort, false, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else hipLaunchKernelGGL((k_extend<WG, kShort, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  return hipGetLastError();
+}
+
+template <int WG>
+hipError_t launch_shadow_wg(const KernelArgs &a, uint32_t grid, hipStream_t st) {
+  if (a.p.df_budget > 0) hipLaunchKernelGGL((k_shadow<WG, kShort, true>), dim3(grid), dim3(WG), 0, st, a);
+  else hipLaunchKernelGGL((k_shadow<WG, kShort, false>), dim3(grid), dim3(WG), 0, st, a);
+  return hipGetLastError();
+}
+
+template <int WG, bool STATS>
+hipError_t launch_trace_wg(const SceneDev *scene, const float4 *r4, uint32_t n, int32_t *prim, float *t,
+                           unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t grid, uint32_t *head,
+                           int budget, uint32_t fetch_min, hipStream_t st) {
+  if (budget > 0)
+    hipLaunchKernelGGL((k_trace_rays<WG, kShort, STATS, true>), dim3(grid), dim3(WG), 0, st, scene, r4, n, prim, t,
+                       counters, gstack, gcap, head, budget, fetch_min);
+  else
+    hipLaunchKernelGGL((k_trace_rays<WG, kShort, STATS, false>), dim3(grid), dim3(WG), 0, st, scene, r4, n, prim, t,
+                       counters, gstack, gcap, head, budget, fetch_min);
   return hipGetLastError();
 }
 
@@ -1055,11 +1218,10 @@ hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool reg
 
 hipError_t shadow(const KernelArgs &a, const LaunchShape &s, hipStream_t st) {
   switch (s.wg) {
-    case 64: hipLaunchKernelGGL((k_shadow<64, kShort>), dim3(s.grid), dim3(64), 0, st, a); break;
-    case 128: hipLaunchKernelGGL((k_shadow<128, kShort>), dim3(s.grid), dim3(128), 0, st, a); break;
-    default: hipLaunchKernelGGL((k_shadow<256, kShort>), dim3(s.grid), dim3(256), 0, st, a); break;
+    case 64: return launch_shadow_wg<64>(a, s.grid, st);
+    case 128: return launch_shadow_wg<128>(a, s.grid, st);
+    default: return launch_shadow_wg<256>(a, s.grid, st);
   }
-  return hipGetLastError();
 }
 
 hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, double n_samples, hipStream_t st) {
@@ -1071,28 +1233,20 @@ hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, d
 
 hipError_t trace_rays(const SceneDev *scene, const float *rays, uint32_t n, int32_t *prim, float *t,
                       unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t grid, uint32_t wg,
-                      hipStream_t st) {
+                      uint32_t *head, int budget, uint32_t fetch_min, hipStream_t st) {
   const float4 *r4 = reinterpret_cast<const float4 *>(rays);
-  if (counters) {
-    hipLaunchKernelGGL((k_trace_rays<128, kShort, true>), dim3(grid), dim3(128), 0, st, scene, r4, n, prim, t, counters,
-                       gstack, gcap);
-    return hipGetLastError();
-  }
+  if (counters)
+    return launch_trace_wg<128, true>(scene, r4, n, prim, t, counters, gstack, gcap, grid, head, budget, fetch_min, st);
   switch (wg) {
     case 64:
-      hipLaunchKernelGGL((k_trace_rays<64, kShort, false>), dim3(grid), dim3(64), 0, st, scene, r4, n, prim, t, nullptr,
-                         gstack, gcap);
-      break;
+      return launch_trace_wg<64, false>(scene, r4, n, prim, t, nullptr, gstack, gcap, grid, head, budget, fetch_min, st);
     case 256:
-      hipLaunchKernelGGL((k_trace_rays<256, kShort, false>), dim3(grid), dim3(256), 0, st, scene, r4, n, prim, t,
-                         nullptr, gstack, gcap);
-      break;
+      return launch_trace_wg<256, false>(scene, r4, n, prim, t, nullptr, gstack, gcap, grid, head, budget, fetch_min,
+                                         st);
     default:
-      hipLaunchKernelGGL((k_trace_rays<128, kShort, false>), dim3(grid), dim3(128), 0, st, scene, r4, n, prim, t,
-                         nullptr, gstack, gcap);
-      break;
+      return launch_trace_wg<128, false>(scene, r4, n, prim, t, nullptr, gstack, gcap, grid, head, budget, fetch_min,
+                                         st);
   }
-  return hipGetLastError();
 }
 
 hipError_t env_tables(const float *env, uint32_t w, uint32_t h, float *wbuf, unsigned int *wmax_bits,

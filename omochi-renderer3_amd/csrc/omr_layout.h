@@ -106,7 +106,8 @@ struct Ctrl {
   uint32_t qcount[2];               // ping-pong ray queue counts
   uint32_t scount;                  // shadow queue count
   uint32_t job_lo, job_hi;          // live job window (logical ring indices)
-  uint32_t pad[3];
+  uint32_t qhead[2];                // dynamic-fetch heads of the ray queues
+  uint32_t shead;                   // dynamic-fetch head of the shadow queue
 };
 
 constexpr uint32_t kJobRing = 1024;

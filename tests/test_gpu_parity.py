@@ -107,3 +107,33 @@ def test_render_bit_exact(ctx, kind, ss, spp, depth, rr):
     assert not diff.any(), f"{diff.sum()} pixels differ"
     assert st["segments"] == seg
     ctx.release(sid)
+
+
+@pytest.mark.parametrize("tune", [dict(fetch_budget=8, fetch_min=16), dict(refill_every=2),
+                                  dict(fetch_budget=4, fetch_min=32, refill_every=1, use_graph=0, wg_size=256),
+                                  dict(wg_size=64, blocks_per_cu=2, pool_paths=4096, steps_per_sync=4)])
+def test_tuning_modes_bit_exact(ctx, tune):
+    """Every launch configuration the autotuner can pick gives the same bits:
+    dynamic-fetch traversal, bulk refill cadence, graph off, small pools."""
+    import omr
+    import pyoracle
+
+    saved = ctx.get_tuning()
+    try:
+        ctx.set_tuning(**tune)
+        scene = omr.procedural(1, width=40, height=32)
+        sid = ctx.upload(scene)
+        o = _oracle(scene, ctx, sid)
+        rays = helpers.hit_rays(scene, 50_000, seed=21)
+        gp, gt = ctx.trace(sid, rays)
+        cp, ct = o.trace(rays, pyoracle.WIDE_BVH)
+        assert np.array_equal(gp, cp)
+        prm = omr.params(ss=2, spp_end=2, max_depth=12, rr_depth=3, seed=31)
+        print("tuning", tune, flush=True)
+        g = ctx.render_sums(sid, scene.camera, prm)
+        c, seg, _ = o.render_sums(scene.camera, prm, pyoracle.WIDE_BVH)
+        assert not (g != c).any(axis=2).any()
+        assert ctx.last_stats()["segments"] == seg
+        ctx.release(sid)
+    finally:
+        ctx.set_tuning(**saved)
