@@ -187,7 +187,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS) + ["c3", "c4"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-rows", type=int, default=64)
+    ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU baseline sample (0 = all)")
     ap.add_argument("--no-tune", action="store_true", help="c3: skip the autotuner")
     ap.add_argument("--tune-ms", type=int, default=20000)
     ap.add_argument("--requests", type=int, default=32, help="c4: requests per arrival rate")
@@ -292,13 +292,14 @@ def main():
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        r0 = max(0, wl["h"] // 2 - args.cpu_rows // 2)
-        rows = (r0, r0 + args.cpu_rows)
+        nrows = args.cpu_rows if 0 < args.cpu_rows < wl["h"] else wl["h"]
+        r0 = max(0, wl["h"] // 2 - nrows // 2)
+        rows = (r0, r0 + nrows)
         cimg, cmrays, cdt = cpu_baseline(scene, scene.camera, prm, rows, threads)
         diff = img[rows[0]:rows[1]].astype(np.float64) - cimg.astype(np.float64)
         out["rmse_vs_cpu"] = float(np.sqrt(np.mean(diff * diff)))
         out["cpu_baseline"] = {"value": round(cmrays, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-                               "sample": f"rows {rows[0]}-{rows[1] - 1} of the same render ({args.cpu_rows} x "
+                               "sample": f"rows {rows[0]}-{rows[1] - 1} of the same render ({nrows} x "
                                          f"{wl['w']} px, all {wl['ss'] ** 2 * wl['spp']} spp), {cdt:.1f} s wall"}
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -52,8 +52,8 @@ Engine::Engine(int device) : device_(device) {
   if (hipMalloc(&d_ctrl_, sizeof(Ctrl)) != hipSuccess) { err = "ctrl"; return; }
   if (hipHostMalloc(&h_ctrl_, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess) { err = "ctrl pinned"; return; }
   if (hipMalloc(&d_jobs_, kJobRing * sizeof(JobDev)) != hipSuccess) { err = "jobs"; return; }
-  if (hipMalloc(&d_done_, kJobRing * sizeof(unsigned long long)) != hipSuccess) { err = "done"; return; }
-  if (hipHostMalloc(&h_done_, kJobRing * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+  if (hipMalloc(&d_done_, kJobRing * kShards * sizeof(unsigned long long)) != hipSuccess) { err = "done"; return; }
+  if (hipHostMalloc(&h_done_, kJobRing * kShards * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
     err = "done pinned";
     return;
   }
@@ -133,9 +133,23 @@ KernelArgs Engine::args() const {
   return a;
 }
 
+// Sub-queue capacity.  A producer kernel (shade or refill) runs GW waves in a
+// grid-stride loop over at most P entries, so one wave appends at most
+// 64 * ceil(P / (64 GW)) entries and a shard (ceil(GW / kShards) waves)
+// receives at most ceil(GW/kShards) * (P/GW + 64); shade and refill may both
+// append to the same queue in one step, hence the factor 2.
+static uint32_t sub_queue_cap(uint32_t P, uint32_t grid_waves) {
+  uint64_t gw = std::max<uint64_t>(1, grid_waves);
+  uint64_t per_wave = 64ull * ((P + 64ull * gw - 1) / (64ull * gw));
+  uint64_t waves = (gw + kShards - 1) / kShards;
+  return (uint32_t)std::min<uint64_t>(P, 2ull * waves * per_wave);
+}
+
 int Engine::ensure_pool() {
   uint32_t P = tuning.pool_paths;
-  if (pool_cap_ == P && pool_.ray_o) return OMR_OK;
+  uint32_t gw = (uint32_t)((uint64_t)n_cu_ * tuning.blocks_per_cu * tuning.wg_size / 64);
+  uint32_t qcap = sub_queue_cap(P, gw);
+  if (pool_cap_ == P && pool_.ray_o && pool_.qcap == qcap) return OMR_OK;
   drop_graphs();
   PoolPtrs &p = pool_;
   void *bufs[] = {p.ray_o, p.ray_d, p.misc, p.rng, p.hit, p.pdf, p.pend, p.queue[0], p.queue[1], p.shadow};
@@ -150,11 +164,12 @@ int Engine::ensure_pool() {
   OMR_HIP(hipMalloc(&p.hit, (size_t)P * 16));
   OMR_HIP(hipMalloc(&p.pdf, (size_t)P * 4));
   OMR_HIP(hipMalloc(&p.pend, (size_t)P * 3 * 48));
-  OMR_HIP(hipMalloc(&p.queue[0], (size_t)P * 4));
-  OMR_HIP(hipMalloc(&p.queue[1], (size_t)P * 4));
-  OMR_HIP(hipMalloc(&p.shadow, (size_t)P * 48));
+  OMR_HIP(hipMalloc(&p.queue[0], (size_t)kShards * qcap * 4));
+  OMR_HIP(hipMalloc(&p.queue[1], (size_t)kShards * qcap * 4));
+  OMR_HIP(hipMalloc(&p.shadow, (size_t)kShards * qcap * 48));
   OMR_HIP(hipMemset(p.misc, 0, (size_t)P * 16));
   p.pool = P;
+  p.qcap = qcap;
   pool_cap_ = P;
   return OMR_OK;
 }
@@ -349,10 +364,11 @@ int Engine::trace(omr_scene_id id, const float *d_rays, size_t n, int32_t *d_pri
   uint32_t grid = (uint32_t)std::min<size_t>((n + wg - 1) / wg, (size_t)n_cu_ * 16);
   int rc = ensure_gstack(s->bvh.stack_bound, (size_t)grid * wg);
   if (rc) return rc;
-  // d_cnt[0..1]: node visits, primitive tests; d_cnt[2]: dynamic-fetch head
+  // d_cnt[0..1]: node visits, primitive tests; then kShards dynamic-fetch heads
+  const size_t cnt_bytes = 16 + kShards * 4;
   unsigned long long *d_cnt = nullptr;
-  OMR_HIP(hipMalloc(&d_cnt, 32));
-  hipError_t e = hipMemsetAsync(d_cnt, 0, 32, stream_);
+  OMR_HIP(hipMalloc(&d_cnt, cnt_bytes));
+  hipError_t e = hipMemsetAsync(d_cnt, 0, cnt_bytes, stream_);
   if (e == hipSuccess)
     e = k::trace_rays(d_scenes_ + id, d_rays, (uint32_t)n, d_prim, d_t, stats2 ? d_cnt : nullptr, d_gstack_, gcap_,
                       grid, wg, (uint32_t *)(d_cnt + 2), (int)tuning.fetch_budget, tuning.fetch_min, stream_);
@@ -506,7 +522,7 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     D.aperture = j.cam.aperture_radius;
     D.focal = j.cam.focal_distance;
     D.accum = j.d_accum;
-    D.done = d_done_ + i;
+    D.done = d_done_ + i * kShards;
     j.ring = (uint32_t)i;
     total += j.n_samples;
   }
@@ -517,7 +533,7 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
   c.job_hi = (uint32_t)jobs.size();
   OMR_HIP(hipMemcpyAsync(d_jobs_, jd.data(), jd.size() * sizeof(JobDev), hipMemcpyHostToDevice, stream_));
   OMR_HIP(hipMemcpyAsync(d_ctrl_, &c, sizeof(Ctrl), hipMemcpyHostToDevice, stream_));
-  OMR_HIP(hipMemsetAsync(d_done_, 0, kJobRing * sizeof(unsigned long long), stream_));
+  OMR_HIP(hipMemsetAsync(d_done_, 0, kJobRing * kShards * sizeof(unsigned long long), stream_));
   OMR_HIP(hipMemsetAsync(pool_.misc, 0, (size_t)pool_cap_ * 16, stream_));
   for (auto &j : jobs)
     OMR_HIP(hipMemsetAsync(j.d_accum, 0, (size_t)j.cam.width * j.cam.height * 3 * 8, stream_));
@@ -538,7 +554,7 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     OMR_HIP(hipMemcpyAsync(h_ctrl_, d_ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_));
     OMR_HIP(hipStreamSynchronize(stream_));
     if (profile && (rc = collect_profile())) return rc;
-    if (h_ctrl_->qcount[0] == 0u && h_ctrl_->next_sample >= h_ctrl_->total_samples) break;
+    if (queue_len(*h_ctrl_, 0) == 0 && samples_exhausted(*h_ctrl_)) break;
   }
   OMR_HIP(hipEventRecord(ev_b_, stream_));
   for (auto &j : jobs) {
@@ -548,19 +564,21 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
       if (e != hipSuccess) return hipfail(e, "resolve");
     }
   }
-  OMR_HIP(hipMemcpyAsync(h_done_, d_done_, jobs.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
+  OMR_HIP(hipMemcpyAsync(h_done_, d_done_, jobs.size() * kShards * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                         stream_));
   OMR_HIP(hipStreamSynchronize(stream_));
   for (size_t i = 0; i < jobs.size(); i++) {
-    if (h_done_[i] != jobs[i].n_samples)
-      return fail(OMR_ERR_STATE, "job " + std::to_string(i) + " completed " + std::to_string(h_done_[i]) + " of " +
+    unsigned long long done = sum_shards(h_done_ + i * kShards);
+    if (done != jobs[i].n_samples)
+      return fail(OMR_ERR_STATE, "job " + std::to_string(i) + " completed " + std::to_string(done) + " of " +
                                      std::to_string(jobs[i].n_samples) + " samples");
     jobs[i].done = true;
   }
   if (st) {
     float ms = 0.0f;
     hipEventElapsedTime(&ms, ev_a_, ev_b_);
-    st->segments = h_ctrl_->segments;
-    st->shadow_rays = h_ctrl_->shadow_rays;
+    st->segments = sum_shards(h_ctrl_->segments);
+    st->shadow_rays = sum_shards(h_ctrl_->shadow_rays);
     st->bounce_steps = steps_done;
     st->gpu_ms = ms;
     st->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -568,8 +586,8 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     st->shade_ms = prof_ms_[1];
     st->shadow_ms = prof_ms_[2];
     st->extend_launches = prof_extend_launches_;
-    st->node_visits = h_ctrl_->node_visits;
-    st->prim_tests = h_ctrl_->prim_tests;
+    st->node_visits = sum_shards(h_ctrl_->node_visits);
+    st->prim_tests = sum_shards(h_ctrl_->prim_tests);
   }
   return OMR_OK;
 }

@@ -22,6 +22,23 @@ namespace omr {
 
 constexpr uint32_t kMaxScenes = 64;
 
+// Summaries of the sharded control block as read back to the host.
+inline unsigned long long sum_shards(const unsigned long long *a) {
+  unsigned long long s = 0;
+  for (uint32_t k = 0; k < kShards; k++) s += a[k];
+  return s;
+}
+inline uint64_t queue_len(const Ctrl &c, int parity) {
+  uint64_t s = 0;
+  for (uint32_t k = 0; k < kShards; k++) s += c.qcount[parity][k];
+  return s;
+}
+inline bool samples_exhausted(const Ctrl &c) {
+  for (uint32_t k = 0; k < kShards; k++)
+    if (c.next_sample[k] < shard_limit(c.total_samples, k)) return false;
+  return true;
+}
+
 struct SceneHost {
   BuildOutput bvh;
   SceneDev dev{};

@@ -21,8 +21,9 @@ struct PoolPtrs {
   float *hit;          // float4: t, prim id bits, u, v
   float *pdf;          // float: MIS pdf of the last Lambert bounce
   float *pend;         // 3 pending branches x pool x 3 float4
-  uint32_t *queue[2];  // ping-pong ray queues
-  float *shadow;       // shadow queue: pool x 3 float4
+  uint32_t *queue[2];  // ping-pong ray queues: kShards sub-queues of qcap entries
+  float *shadow;       // shadow queue: kShards sub-queues of qcap records (3 float4)
+  uint32_t qcap;       // capacity of one sub-queue
   uint2 *gstack;       // per-thread overflow traversal stacks
   uint32_t pool;
   uint32_t gcap;       // entries per thread in gstack

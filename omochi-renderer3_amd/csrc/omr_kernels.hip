@@ -58,6 +58,31 @@ __device__ inline uint32_t wave_append(uint32_t *counter, bool pred) {
   return base + rank_in(m);
 }
 
+// Shard of the calling wave (global wave index mod kShards).
+__device__ inline uint32_t wave_shard() { return (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % kShards; }
+
+// A queue is kShards sub-queues of `cap` entries.  s_pre (LDS, kShards+1)
+// receives the exclusive prefix of the sub-queue counts; the flat index i of
+// an entry maps to its sub-queue by binary search.  Whole block must call.
+__device__ inline void load_prefix(uint32_t *s_pre, const uint32_t *counts) {
+  if (threadIdx.x < kShards) s_pre[threadIdx.x + 1] = counts[threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s_pre[0] = 0u;
+    for (uint32_t k = 1; k <= kShards; k++) s_pre[k] += s_pre[k - 1];
+  }
+  __syncthreads();
+}
+__device__ inline uint32_t flat_slot(const uint32_t *s_pre, uint32_t i, uint32_t cap) {
+  uint32_t lo = 0, hi = kShards;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (s_pre[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo * cap + (i - s_pre[lo]);
+}
+
 // ------------------------------------------------------------ RNG
 // Random (src/tools/Random.h:9-29): MT-style seeding recurrence + xorshift128.
 __device__ inline uint4 rng_init(uint32_t seed) {
@@ -360,17 +385,20 @@ __device__ inline void traverse(const SceneDev &S, f3 o, f3 d, Hit &best, Stack<
 }
 
 // Dynamic fetch: every lane keeps its own ray; after each slice of `budget`
-// visits, lanes whose ray finished take new rays from the queue (one atomic
-// per wave) as soon as at least `fetch_min` lanes are idle, so a wave does
-// not idle behind its slowest ray.  load(i, S, o, d, tmax, tag) reads queue
-// entry i; store(tag, hit) writes a finished ray's result.
-template <int WG, int SHORT, bool ANY, bool STATS, class Load, class Store>
-__device__ inline uint32_t dynamic_fetch(uint32_t n, uint32_t *head, Stack<WG, SHORT> &stk, int budget,
+// visits, lanes whose ray finished take new rays (one atomic per wave) as soon
+// as at least `fetch_min` lanes are idle, so a wave does not idle behind its
+// slowest ray.  Work comes from kShards sub-queues: a wave starts on its own
+// shard's sub-queue (heads[k] counts what was taken) and moves on to the next
+// shard when that one runs dry.  count(k): entries of sub-queue k;
+// load(k, off, S, o, d, tmax, tag) reads one entry; store(tag, hit) writes a
+// finished ray's result.
+template <int WG, int SHORT, bool ANY, bool STATS, class Count, class Load, class Store>
+__device__ inline uint32_t dynamic_fetch(uint32_t *heads, Count count, Stack<WG, SHORT> &stk, int budget,
                                          uint32_t fetch_min, Load load, Store store, uint32_t &nvis,
                                          uint32_t &ntest) {
   Trav<WG, SHORT, ANY, STATS> tr;
   bool has = false, exhausted = false;
-  uint32_t tag = 0, done = 0;
+  uint32_t tag = 0, done = 0, shard = wave_shard(), visited = 0;
   const uint32_t active = (uint32_t)__popcll(__ballot(true));
   for (;;) {
     unsigned long long m = __ballot(!has);
@@ -378,20 +406,27 @@ __device__ inline uint32_t dynamic_fetch(uint32_t n, uint32_t *head, Stack<WG, S
     if (!exhausted && idle != 0u && (idle >= fetch_min || idle == active)) {
       uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
       uint32_t base = 0;
-      if (lane_id() == leader) base = atomicAdd(head, idle);
+      if (lane_id() == leader) base = atomicAdd(&heads[shard], idle);
       base = __shfl(base, (int)leader);
-      if (base + idle >= n) exhausted = true;
-      uint32_t i = base + rank_in(m);
-      if (!has && i < n) {
+      uint32_t n = count(shard);
+      uint32_t avail = base < n ? n - base : 0u;
+      uint32_t r = rank_in(m);
+      if (!has && r < avail) {
         const SceneDev *S;
         f3 o, d;
         float tmax;
-        load(i, S, o, d, tmax, tag);
+        load(shard, base + r, S, o, d, tmax, tag);
         tr.init(*S, o, d, tmax, stk);
         has = true;
       }
+      if (base + idle >= n) {  // this sub-queue is used up: move on
+        shard = (shard + 1u) % kShards;
+        if (++visited == kShards) exhausted = true;
+      }
     }
-    if (__ballot(has) == 0ull) break;
+    unsigned long long busy = __ballot(has);
+    if (busy == 0ull && exhausted) break;
+    if (busy == 0ull) continue;
     if (has && tr.step(stk, budget, nvis, ntest)) {
       store(tag, tr.best);
       has = false;
@@ -533,38 +568,39 @@ __device__ inline void camera_ray(const JobDev &J, const Decomp &dc, f3 &o, f3 &
   d = dir;
 }
 
-// Wave-level grab of up to `want` lanes' worth of samples from the global
-// cursor (CAS so the cursor never passes the admitted total).  Returns this
+// Wave-level grab of up to `want` lanes' worth of samples: one fetch-and-add
+// per wave on the wave's cursor shard k, whose j-th sample is j*kShards + k.
+// Once the admitted samples run out a cursor moves past its limit; the host
+// resets it before it admits more work (no kernel runs then).  Returns this
 // lane's global sample index, or ~0ull when none is left for it.
 __device__ inline unsigned long long grab_sample(Ctrl *ctrl, bool want) {
   unsigned long long m = __ballot(want);
   if (m == 0ull) return ~0ull;
   uint32_t cnt = (uint32_t)__popcll(m);
   uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
+  const uint32_t k = wave_shard();
   unsigned long long base = 0ull, take = 0ull;
   if (lane_id() == leader) {
-    unsigned long long total = __atomic_load_n(&ctrl->total_samples, __ATOMIC_RELAXED);
-    unsigned long long old = __atomic_load_n(&ctrl->next_sample, __ATOMIC_RELAXED);
-    for (;;) {
-      unsigned long long avail = total > old ? total - old : 0ull;
+    unsigned long long lim = shard_limit(__atomic_load_n(&ctrl->total_samples, __ATOMIC_RELAXED), k);
+    unsigned long long cur = __atomic_load_n(&ctrl->next_sample[k], __ATOMIC_RELAXED);
+    if (cur < lim) {
+      unsigned long long old = atomicAdd(&ctrl->next_sample[k], (unsigned long long)cnt);
+      unsigned long long avail = lim > old ? lim - old : 0ull;
       take = avail < cnt ? avail : (unsigned long long)cnt;
-      if (take == 0ull) break;
-      unsigned long long prev = atomicCAS(&ctrl->next_sample, old, old + take);
-      if (prev == old) break;
-      old = prev;
+      base = old;
     }
-    base = old;
   }
   base = __shfl(base, (int)leader);
   take = __shfl(take, (int)leader);
   uint32_t r = rank_in(m);
   if (!want || (unsigned long long)r >= take) return ~0ull;
-  return base + r;
+  return (base + r) * kShards + k;
 }
 
-// Wave-aggregated per-job completion counting.
+// Wave-aggregated per-job completion counting into the wave's shard.
 __device__ inline void count_done(JobDev *jobs, bool fin, uint32_t job) {
   bool need = fin;
+  const uint32_t k = wave_shard();
   for (;;) {
     unsigned long long m = __ballot(need);
     if (m == 0ull) break;
@@ -572,9 +608,16 @@ __device__ inline void count_done(JobDev *jobs, bool fin, uint32_t job) {
     uint32_t lj = __shfl(job, (int)leader);
     bool mine = need && job == lj;
     unsigned long long mm = __ballot(mine);
-    if (lane_id() == leader) atomicAdd(jobs[lj].done, (unsigned long long)__popcll(mm));
+    if (lane_id() == leader) atomicAdd(&jobs[lj].done[k], (unsigned long long)__popcll(mm));
     if (mine) need = false;
   }
+}
+
+// One atomic per wave into the wave's shard of a counter array.
+__device__ inline void wave_count(unsigned long long *shards, unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  if (lane_id() == 0 && v) atomicAdd(&shards[wave_shard()], v);
 }
 
 // Writes a fresh camera path for global sample g into slot.
@@ -606,28 +649,36 @@ __global__ __launch_bounds__(WG) void k_refill(KernelArgs A, int parity) {
     unsigned long long g = grab_sample(A.ctrl, idle);
     bool got = g != ~0ull;
     if (got) start_sample(A, slot, g);
-    uint32_t pos = wave_append(&A.ctrl->qcount[parity], got);
-    if (got) A.p.queue[parity][pos] = slot;
+    const uint32_t k = wave_shard();
+    uint32_t pos = wave_append(&A.ctrl->qcount[parity][k], got);
+    if (got) A.p.queue[parity][k * A.p.qcap + pos] = slot;
   }
 }
 
 template <int WG, int SHORT, bool STATS, bool DF>
 __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
   __shared__ uint2 lds[SHORT * WG];
+  __shared__ uint32_t s_pre[kShards + 1];
   const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
-  if (gtid == 0) {
-    A.ctrl->qcount[parity ^ 1] = 0u;
-    A.ctrl->qhead[parity ^ 1] = 0u;  // fetch head of the next step's extend
+  // this step's shade appends to the other queue; clear its counts and the
+  // next extend's fetch heads (nobody else touches them during this kernel)
+  if (blockIdx.x == 0 && threadIdx.x < kShards) {
+    A.ctrl->qcount[parity ^ 1][threadIdx.x] = 0u;
+    A.ctrl->qhead[parity ^ 1][threadIdx.x] = 0u;
   }
-  const uint32_t n = A.ctrl->qcount[parity];
+  load_prefix(s_pre, A.ctrl->qcount[parity]);
+  const uint32_t n = s_pre[kShards];
   const uint32_t stride = gridDim.x * WG;
+  const uint32_t *queue = A.p.queue[parity];
+  const uint32_t qcap = A.p.qcap;
   Stack<WG, SHORT> stk;
   stk.lds = lds;
   stk.g = A.p.gstack + (size_t)gtid * A.p.gcap;
   uint32_t segs = 0, nv = 0, nt = 0;
   if (DF) {
-    auto load = [&](uint32_t i, const SceneDev *&S, f3 &o, f3 &d, float &tmax, uint32_t &tag) {
-      uint32_t slot = A.p.queue[parity][i];
+    auto count = [&](uint32_t k) { return s_pre[k + 1] - s_pre[k]; };
+    auto load = [&](uint32_t k, uint32_t off, const SceneDev *&S, f3 &o, f3 &d, float &tmax, uint32_t &tag) {
+      uint32_t slot = queue[k * qcap + off];
       float4 ro = reinterpret_cast<const float4 *>(A.p.ray_o)[slot];
       float4 rd = reinterpret_cast<const float4 *>(A.p.ray_d)[slot];
       uint32_t js = A.p.misc[4 * (size_t)slot + 3];
@@ -640,13 +691,13 @@ __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
     auto store = [&](uint32_t slot, const Hit &h) {
       reinterpret_cast<float4 *>(A.p.hit)[slot] = make_float4(h.t, __int_as_float(h.prim), h.u, h.v);
     };
-    segs = dynamic_fetch<WG, SHORT, false, STATS>(n, &A.ctrl->qhead[parity], stk, (int)A.p.df_budget,
+    segs = dynamic_fetch<WG, SHORT, false, STATS>(A.ctrl->qhead[parity], count, stk, (int)A.p.df_budget,
                                                  A.p.df_fetch_min, load, store, nv, nt);
   }
   for (uint32_t base = blockIdx.x * WG; !DF && base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     if (i < n) {
-      uint32_t slot = A.p.queue[parity][i];
+      uint32_t slot = queue[flat_slot(s_pre, i, qcap)];
       float4 ro = reinterpret_cast<const float4 *>(A.p.ray_o)[slot];
       float4 rd = reinterpret_cast<const float4 *>(A.p.ray_d)[slot];
       uint32_t js = A.p.misc[4 * (size_t)slot + 3];
@@ -660,28 +711,21 @@ __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
       segs++;
     }
   }
-  // one atomic per wave for each counter
-  unsigned long long tot = segs, a = nv, b = nt;
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    tot += __shfl_xor(tot, off);
-    if (STATS) {
-      a += __shfl_xor(a, off);
-      b += __shfl_xor(b, off);
-    }
-  }
-  if (lane_id() == 0) {
-    if (tot) atomicAdd(&A.ctrl->segments, tot);
-    if (STATS && a) atomicAdd(&A.ctrl->node_visits, a);
-    if (STATS && b) atomicAdd(&A.ctrl->prim_tests, b);
+  wave_count(A.ctrl->segments, segs);
+  if (STATS) {
+    wave_count(A.ctrl->node_visits, nv);
+    wave_count(A.ctrl->prim_tests, nt);
   }
 }
 
 // One shading step; mirrors the CPU restatement's per-segment loop body.
 template <int WG>
 __global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int regen) {
-  const uint32_t n = A.ctrl->qcount[parity];
+  __shared__ uint32_t s_pre[kShards + 1];
+  load_prefix(s_pre, A.ctrl->qcount[parity]);
+  const uint32_t n = s_pre[kShards];
   const uint32_t stride = gridDim.x * WG;
+  const uint32_t qcap = A.p.qcap;
   const uint32_t P = A.p.pool;
   float4 *ray_o = reinterpret_cast<float4 *>(A.p.ray_o);
   float4 *ray_d = reinterpret_cast<float4 *>(A.p.ray_d);
@@ -691,7 +735,7 @@ __global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int rege
   for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     bool valid = i < n;
-    uint32_t slot = valid ? A.p.queue[parity][i] : 0u;
+    uint32_t slot = valid ? A.p.queue[parity][flat_slot(s_pre, i, qcap)] : 0u;
     bool cont = false, fin = false, shadow = false;
     uint32_t js = 0;
     f3 o = mk(0, 0, 0), d = mk(0, 0, 0), beta = mk(0, 0, 0);
@@ -925,11 +969,12 @@ __global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int rege
       if (ps_mis(word)) A.p.pdf[slot] = pdf_next;
     }
     bool enq = valid && (cont || fresh);
-    uint32_t pos = wave_append(&A.ctrl->qcount[parity ^ 1], enq);
-    if (enq) A.p.queue[parity ^ 1][pos] = slot;
-    uint32_t spos = wave_append(&A.ctrl->scount, shadow);
+    const uint32_t k = wave_shard();
+    uint32_t pos = wave_append(&A.ctrl->qcount[parity ^ 1][k], enq);
+    if (enq) A.p.queue[parity ^ 1][k * qcap + pos] = slot;
+    uint32_t spos = wave_append(&A.ctrl->scount[k], shadow);
     if (shadow) {
-      float4 *sq = reinterpret_cast<float4 *>(A.p.shadow) + (size_t)spos * 3;
+      float4 *sq = reinterpret_cast<float4 *>(A.p.shadow) + ((size_t)k * qcap + spos) * 3;
       sq[0] = make_float4(sh_o.x, sh_o.y, sh_o.z, __uint_as_float(js));
       sq[1] = make_float4(sh_d.x, sh_d.y, sh_d.z, __uint_as_float(sh_px));
       sq[2] = make_float4(sh_c.x, sh_c.y, sh_c.z, 0.0f);
@@ -937,8 +982,9 @@ __global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int rege
   }
 }
 
-__device__ inline void add_shadow(const KernelArgs &A, uint32_t i) {
-  const float4 *sq = reinterpret_cast<const float4 *>(A.p.shadow) + (size_t)i * 3;
+// Accumulates the contribution of shadow record r if its ray was unoccluded.
+__device__ inline void add_shadow(const KernelArgs &A, uint32_t r) {
+  const float4 *sq = reinterpret_cast<const float4 *>(A.p.shadow) + (size_t)r * 3;
   float4 a = sq[0], b = sq[1], c = sq[2];
   const JobDev &J = A.jobs[__float_as_uint(a.w)];
   unsigned long long *acc = J.accum + 3ull * __float_as_uint(b.w);
@@ -951,64 +997,63 @@ __device__ inline void add_shadow(const KernelArgs &A, uint32_t i) {
 template <int WG, int SHORT, bool DF>
 __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
   __shared__ uint2 lds[SHORT * WG];
+  __shared__ uint32_t s_pre[kShards + 1];
   const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
-  const uint32_t n = A.ctrl->scount;
+  load_prefix(s_pre, A.ctrl->scount);
+  const uint32_t n = s_pre[kShards];
   const uint32_t stride = gridDim.x * WG;
+  const uint32_t qcap = A.p.qcap;
   Stack<WG, SHORT> stk;
   stk.lds = lds;
   stk.g = A.p.gstack + (size_t)gtid * A.p.gcap;
   uint32_t cnt = 0;
+  auto load = [&](uint32_t k, uint32_t off, const SceneDev *&S, f3 &o, f3 &d, float &tmax, uint32_t &tag) {
+    uint32_t r = k * qcap + off;
+    const float4 *sq = reinterpret_cast<const float4 *>(A.p.shadow) + (size_t)r * 3;
+    float4 a = sq[0], b = sq[1];
+    S = &A.scenes[A.jobs[__float_as_uint(a.w)].scene];
+    o = xyz(a);
+    d = xyz(b);
+    tmax = kInf;
+    tag = r;
+  };
   if (DF) {
-    auto load = [&](uint32_t i, const SceneDev *&S, f3 &o, f3 &d, float &tmax, uint32_t &tag) {
-      const float4 *sq = reinterpret_cast<const float4 *>(A.p.shadow) + (size_t)i * 3;
-      float4 a = sq[0], b = sq[1];
-      S = &A.scenes[A.jobs[__float_as_uint(a.w)].scene];
-      o = xyz(a);
-      d = xyz(b);
-      tmax = kInf;
-      tag = i;
-    };
-    auto store = [&](uint32_t i, const Hit &h) {
-      if (h.prim < 0) add_shadow(A, i);
+    auto count = [&](uint32_t k) { return s_pre[k + 1] - s_pre[k]; };
+    auto store = [&](uint32_t r, const Hit &h) {
+      if (h.prim < 0) add_shadow(A, r);
     };
     uint32_t nv = 0, nt = 0;
-    cnt = dynamic_fetch<WG, SHORT, true, false>(n, &A.ctrl->shead, stk, (int)A.p.df_budget, A.p.df_fetch_min,
+    cnt = dynamic_fetch<WG, SHORT, true, false>(A.ctrl->shead, count, stk, (int)A.p.df_budget, A.p.df_fetch_min,
                                                load, store, nv, nt);
   }
   for (uint32_t base = blockIdx.x * WG; !DF && base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     if (i < n) {
-      const float4 *sq = reinterpret_cast<const float4 *>(A.p.shadow) + (size_t)i * 3;
-      float4 a = sq[0], b = sq[1], c = sq[2];
-      uint32_t js = __float_as_uint(a.w), px = __float_as_uint(b.w);
-      const JobDev &J = A.jobs[js];
-      const SceneDev &S = A.scenes[J.scene];
+      uint32_t r = flat_slot(s_pre, i, qcap);
+      const SceneDev *S;
+      f3 o, d;
+      float tmax;
+      uint32_t tag;
+      load(r / qcap, r % qcap, S, o, d, tmax, tag);
       Hit h;
-      h.t = kInf;
+      h.t = tmax;
       h.u = h.v = 0.0f;
       h.prim = -1;
       uint32_t nv = 0, nt = 0;
-      traverse<WG, SHORT, true, false>(S, xyz(a), xyz(b), h, stk, nv, nt);
-      if (h.prim < 0) {
-        unsigned long long *acc = J.accum + 3ull * px;
-        unsigned long long qa = fix(c.x), qb = fix(c.y), qc = fix(c.z);
-        if (qa) atomicAdd(acc + 0, qa);
-        if (qb) atomicAdd(acc + 1, qb);
-        if (qc) atomicAdd(acc + 2, qc);
-      }
+      traverse<WG, SHORT, true, false>(*S, o, d, h, stk, nv, nt);
+      if (h.prim < 0) add_shadow(A, r);
       cnt++;
     }
   }
-  uint32_t tot = cnt;
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) tot += __shfl_xor(tot, off);
-  if (lane_id() == 0 && tot) atomicAdd(&A.ctrl->shadow_rays, (unsigned long long)tot);
+  wave_count(A.ctrl->shadow_rays, cnt);
 }
 
-// Clears the shadow queue count and fetch head at the start of a step.
+// Clears the shadow sub-queue counts and fetch heads at the start of a step.
 __global__ void k_clear_shadow(Ctrl *ctrl) {
-  ctrl->scount = 0u;
-  ctrl->shead = 0u;
+  if (threadIdx.x < kShards) {
+    ctrl->scount[threadIdx.x] = 0u;
+    ctrl->shead[threadIdx.x] = 0u;
+  }
 }
 
 // mean = sum * 2^-24 / (ss^2 * samples), in double, as the CPU restatement
@@ -1030,7 +1075,11 @@ __global__ __launch_bounds__(WG) void k_trace_rays(const SceneDev *scene, const 
   const SceneDev S = *scene;
   uint32_t nv = 0, nt = 0;
   if (DF) {
-    auto load = [&](uint32_t i, const SceneDev *&Sp, f3 &o, f3 &d, float &tmax, uint32_t &tag) {
+    // sub-queue k = rays [k n / kShards, (k+1) n / kShards)
+    auto first = [&](uint32_t k) { return (uint32_t)(((unsigned long long)n * k) / kShards); };
+    auto count = [&](uint32_t k) { return first(k + 1) - first(k); };
+    auto load = [&](uint32_t k, uint32_t off, const SceneDev *&Sp, f3 &o, f3 &d, float &tmax, uint32_t &tag) {
+      uint32_t i = first(k) + off;
       float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
       Sp = scene;
       o = xyz(a);
@@ -1042,7 +1091,7 @@ __global__ __launch_bounds__(WG) void k_trace_rays(const SceneDev *scene, const 
       prim[i] = h.prim;
       tout[i] = h.prim >= 0 ? h.t : kInf;
     };
-    dynamic_fetch<WG, SHORT, false, STATS>(n, head, stk, budget, fetch_min, load, store, nv, nt);
+    dynamic_fetch<WG, SHORT, false, STATS>(head, count, stk, budget, fetch_min, load, store, nv, nt);
   }
   for (uint32_t base = blockIdx.x * WG; !DF && base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
@@ -1206,7 +1255,7 @@ hipError_t extend(const KernelArgs &a, const LaunchShape &s, int parity, bool st
 }
 
 hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool regen, hipStream_t st) {
-  hipLaunchKernelGGL(k_clear_shadow, dim3(1), dim3(1), 0, st, a.ctrl);
+  hipLaunchKernelGGL(k_clear_shadow, dim3(1), dim3(kShards), 0, st, a.ctrl);
   int r = regen ? 1 : 0;
   switch (s.wg) {
     case 64: hipLaunchKernelGGL(k_shade<64>, dim3(s.grid), dim3(64), 0, st, a, parity, r); break;

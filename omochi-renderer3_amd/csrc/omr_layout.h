@@ -92,22 +92,32 @@ struct JobDev {
   float cam_pos[3], cam_c[3], cam_x[3], cam_y[3];
   float aperture, focal;
   unsigned long long *accum;       // W*H*3 fixed-point sums
-  unsigned long long *done;        // completed samples
+  unsigned long long *done;        // completed samples, kShards partial counts
 };
+
+// Every device-wide counter that waves update in the hot loop is split into
+// kShards copies; wave w uses copy w % kShards.  A single address updated by
+// every wave serializes across the eight XCDs.
+constexpr uint32_t kShards = 32;
+
+// Samples handed out by cursor shard k are k, k + kShards, k + 2 kShards, ...
+OMR_HD inline unsigned long long shard_limit(unsigned long long total, uint32_t k) {
+  return total > k ? (total - k + kShards - 1) / kShards : 0ull;
+}
 
 // Control block in device memory (one per engine).
 struct Ctrl {
-  unsigned long long next_sample;   // global sample cursor (CAS-advanced)
-  unsigned long long total_samples; // samples admitted so far (host-written)
-  unsigned long long segments;      // closest-hit queries traced
-  unsigned long long shadow_rays;   // environment shadow rays traced
-  unsigned long long node_visits;   // stats kernels only
-  unsigned long long prim_tests;    // stats kernels only
-  uint32_t qcount[2];               // ping-pong ray queue counts
-  uint32_t scount;                  // shadow queue count
-  uint32_t job_lo, job_hi;          // live job window (logical ring indices)
-  uint32_t qhead[2];                // dynamic-fetch heads of the ray queues
-  uint32_t shead;                   // dynamic-fetch head of the shadow queue
+  unsigned long long next_sample[kShards];  // striped sample cursors (index j -> sample j*kShards + k)
+  unsigned long long total_samples;         // samples admitted so far (host-written)
+  unsigned long long segments[kShards];     // closest-hit queries traced
+  unsigned long long shadow_rays[kShards];  // environment shadow rays traced
+  unsigned long long node_visits[kShards];  // stats kernels only
+  unsigned long long prim_tests[kShards];   // stats kernels only
+  uint32_t qcount[2][kShards];              // ping-pong ray sub-queue counts
+  uint32_t scount[kShards];                 // shadow sub-queue counts
+  uint32_t qhead[2][kShards];               // dynamic-fetch heads of the ray sub-queues
+  uint32_t shead[kShards];                  // dynamic-fetch heads of the shadow sub-queues
+  uint32_t job_lo, job_hi;                  // live job window (logical ring indices)
 };
 
 constexpr uint32_t kJobRing = 1024;

@@ -151,9 +151,10 @@ void Engine::server_loop() {
   std::deque<std::shared_ptr<Req>> live;
   uint64_t lo = 0, hi = 0;          // logical ring window [lo, hi)
   unsigned long long total = 0;     // samples admitted
+  unsigned long long cursor[kShards] = {};  // device sample cursors as of the last readback
   JobDev *h_jobs = nullptr;         // pinned staging for job records
   struct Stage {
-    unsigned long long total;
+    unsigned long long cursor[kShards], total;  // same order as Ctrl::next_sample, total_samples
     uint32_t lo, hi;
   } *h_stage = nullptr;
   bool ok = true;
@@ -166,7 +167,7 @@ void Engine::server_loop() {
       Ctrl c;
       std::memset(&c, 0, sizeof(c));
       ok = hipMemcpy(d_ctrl_, &c, sizeof(Ctrl), hipMemcpyHostToDevice) == hipSuccess &&
-           hipMemset(d_done_, 0, kJobRing * sizeof(unsigned long long)) == hipSuccess &&
+           hipMemset(d_done_, 0, kJobRing * kShards * sizeof(unsigned long long)) == hipSuccess &&
            hipMemset(pool_.misc, 0, (size_t)pool_cap_ * 16) == hipSuccess;
     }
   }
@@ -212,6 +213,7 @@ void Engine::server_loop() {
     }
     std::unique_lock<std::mutex> dev(mu);
     if (!admit.empty()) {
+      const unsigned long long total_before = total;
       if (live.empty()) busy_t0 = std::chrono::steady_clock::now();
       uint32_t need_stack = 1;
       for (auto &r : admit) {
@@ -232,7 +234,7 @@ void Engine::server_loop() {
           return;
         }
         hipMemsetAsync(j.d_accum, 0, px * 3 * 8, stream_);
-        hipMemsetAsync(d_done_ + j.ring, 0, sizeof(unsigned long long), stream_);
+        hipMemsetAsync(d_done_ + (size_t)j.ring * kShards, 0, kShards * sizeof(unsigned long long), stream_);
         JobDev &D = h_jobs[j.ring];
         std::memset(&D, 0, sizeof(D));
         D.sample_base = total;
@@ -256,7 +258,7 @@ void Engine::server_loop() {
         D.aperture = j.cam.aperture_radius;
         D.focal = j.cam.focal_distance;
         D.accum = j.d_accum;
-        D.done = d_done_ + j.ring;
+        D.done = d_done_ + (size_t)j.ring * kShards;
         hipMemcpyAsync(d_jobs_ + j.ring, &D, sizeof(JobDev), hipMemcpyHostToDevice, stream_);
         total += j.n_samples;
         live.push_back(r);
@@ -267,10 +269,13 @@ void Engine::server_loop() {
         fail_all("traversal stack allocation failed");
         return;
       }
+      // the cursor may have run past the old total once work ran out: new
+      // samples start exactly at the old total
+      for (uint32_t k = 0; k < kShards; k++) h_stage->cursor[k] = std::min(cursor[k], shard_limit(total_before, k));
       h_stage->total = total;
       h_stage->lo = (uint32_t)lo;
       h_stage->hi = (uint32_t)hi;
-      hipMemcpyAsync(&d_ctrl_->total_samples, &h_stage->total, 8, hipMemcpyHostToDevice, stream_);
+      hipMemcpyAsync(&d_ctrl_->next_sample[0], &h_stage->cursor[0], (kShards + 1) * 8, hipMemcpyHostToDevice, stream_);
       hipMemcpyAsync(&d_ctrl_->job_lo, &h_stage->lo, 8, hipMemcpyHostToDevice, stream_);
       k::refill(args(), sh, 0, stream_);
     }
@@ -287,17 +292,18 @@ void Engine::server_loop() {
       return;
     }
     hipMemcpyAsync(h_ctrl_, d_ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_);
-    hipMemcpyAsync(h_done_, d_done_, kJobRing * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_);
+    hipMemcpyAsync(h_done_, d_done_, kJobRing * kShards * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_);
     if (hipStreamSynchronize(stream_) != hipSuccess) {
       dev.unlock();
       fail_all("device error in the bounce loop");
       return;
     }
+    for (uint32_t k = 0; k < kShards; k++) cursor[k] = h_ctrl_->next_sample[k];
     // ---- 3. completions
     std::vector<std::shared_ptr<Req>> finished;
     for (auto &r : live) {
       JobRun &j = r->run;
-      if (j.done || h_done_[j.ring] != j.n_samples) continue;
+      if (j.done || sum_shards(h_done_ + (size_t)j.ring * kShards) != j.n_samples) continue;
       j.done = true;
       double nspp = (double)j.params.supersamples * j.params.supersamples *
                     (double)(j.params.spp_end - j.params.spp_begin);
@@ -320,8 +326,8 @@ void Engine::server_loop() {
     dev.unlock();
     {
       std::lock_guard<std::mutex> g(smu);
-      srv_segments_ = h_ctrl_->segments;
-      srv_shadow_ = h_ctrl_->shadow_rays;
+      srv_segments_ = sum_shards(h_ctrl_->segments);
+      srv_shadow_ = sum_shards(h_ctrl_->shadow_rays);
       for (auto &r : finished) {
         used_bytes_ -= r->bytes;
         if (SceneHost *s = scene(r->scene)) s->users--;
