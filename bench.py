@@ -191,6 +191,11 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="c3: skip the autotuner")
     ap.add_argument("--tune-ms", type=int, default=20000)
     ap.add_argument("--requests", type=int, default=32, help="c4: requests per arrival rate")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="time the renders without per-launch events (no roofline timing)")
+    ap.add_argument("--tune", default="", help="launch tuning overrides, e.g. wg_size=64,blocks_per_cu=16")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB device-to-device three times (hardware-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is design-only.",
@@ -238,9 +243,11 @@ def main():
     dev = ctx.device_info()
     sid = ctx.upload(scene)
     info = ctx.scene_info(sid)
+    if args.tune:
+        ctx.set_tuning(**{k: int(v) for k, v in (kv.split("=") for kv in args.tune.split(","))})
     prm = omr.params(ss=wl["ss"], spp_end=wl["spp"], max_depth=wl["depth"], rr_depth=wl["rr"], env_is=wl["env_is"],
                      seed=0x6F6D7233)
-    ctx.set_profiling(True)
+    ctx.set_profiling(not args.no_profile)
     for _ in range(args.warmup):
         ctx.render(sid, scene.camera, prm)
     seg = shadow = 0
@@ -286,7 +293,7 @@ def main():
                    "resolution": f"{wl['w']}x{wl['h']}", "spp": wl["ss"] ** 2 * wl["spp"],
                    "max_depth": wl["depth"], "rr_depth": wl["rr"], "env_is": wl["env_is"],
                    "primitives": scene.n_prims, "bvh_nodes": info["n_nodes"], "device": dev["name"],
-                   "n_cu": dev["n_cu"]},
+                   "n_cu": dev["n_cu"], "tuning": ctx.get_tuning()},
         "segments_per_step": seg // args.steps, "shadow_rays_per_step": shadow // args.steps,
         "roofline": roofline,
     }
@@ -303,6 +310,30 @@ def main():
                                          f"{wl['w']} px, all {wl['ss'] ** 2 * wl['spp']} spp), {cdt:.1f} s wall"}
     print(json.dumps(out), flush=True)
     ctx.close()
+    if args.calib_copy > 0:
+        copy_calibration(args.calib_copy << 20)
+
+
+def copy_calibration(nbytes):
+    """Known-size traffic for scaling hardware byte counters: one fill of
+    nbytes and three device-to-device copies of nbytes (each reads and writes
+    nbytes once)."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(src, 1, C.c_size_t(nbytes)) != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(3):
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
 
 
 if __name__ == "__main__":

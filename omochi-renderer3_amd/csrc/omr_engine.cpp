@@ -124,6 +124,7 @@ KernelArgs Engine::args() const {
   KernelArgs a;
   a.ctrl = d_ctrl_;
   a.jobs = d_jobs_;
+  a.done = d_done_;
   a.scenes = d_scenes_;
   a.p = pool_;
   a.p.gstack = d_gstack_;
@@ -364,14 +365,14 @@ int Engine::trace(omr_scene_id id, const float *d_rays, size_t n, int32_t *d_pri
   uint32_t grid = (uint32_t)std::min<size_t>((n + wg - 1) / wg, (size_t)n_cu_ * 16);
   int rc = ensure_gstack(s->bvh.stack_bound, (size_t)grid * wg);
   if (rc) return rc;
-  // d_cnt[0..1]: node visits, primitive tests; then kShards dynamic-fetch heads
-  const size_t cnt_bytes = 16 + kShards * 4;
+  // d_cnt[0..1]: node visits, primitive tests; one line on, the dynamic-fetch heads
+  const size_t cnt_bytes = kLine + sizeof(Sharded<uint32_t>);
   unsigned long long *d_cnt = nullptr;
   OMR_HIP(hipMalloc(&d_cnt, cnt_bytes));
   hipError_t e = hipMemsetAsync(d_cnt, 0, cnt_bytes, stream_);
   if (e == hipSuccess)
     e = k::trace_rays(d_scenes_ + id, d_rays, (uint32_t)n, d_prim, d_t, stats2 ? d_cnt : nullptr, d_gstack_, gcap_,
-                      grid, wg, (uint32_t *)(d_cnt + 2), (int)tuning.fetch_budget, tuning.fetch_min, stream_);
+                      grid, wg, reinterpret_cast<Sharded<uint32_t> *>(d_cnt + kLine / 8), (int)tuning.fetch_budget, tuning.fetch_min, stream_);
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
   unsigned long long c[2] = {0, 0};
   if (e == hipSuccess && stats2) e = hipMemcpy(c, d_cnt, 16, hipMemcpyDeviceToHost);
@@ -522,7 +523,6 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     D.aperture = j.cam.aperture_radius;
     D.focal = j.cam.focal_distance;
     D.accum = j.d_accum;
-    D.done = d_done_ + i * kShards;
     j.ring = (uint32_t)i;
     total += j.n_samples;
   }
@@ -564,11 +564,11 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
       if (e != hipSuccess) return hipfail(e, "resolve");
     }
   }
-  OMR_HIP(hipMemcpyAsync(h_done_, d_done_, jobs.size() * kShards * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+  OMR_HIP(hipMemcpyAsync(h_done_, d_done_, kJobRing * kShards * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                          stream_));
   OMR_HIP(hipStreamSynchronize(stream_));
   for (size_t i = 0; i < jobs.size(); i++) {
-    unsigned long long done = sum_shards(h_done_ + i * kShards);
+    unsigned long long done = job_done(h_done_, (uint32_t)i);
     if (done != jobs[i].n_samples)
       return fail(OMR_ERR_STATE, "job " + std::to_string(i) + " completed " + std::to_string(done) + " of " +
                                      std::to_string(jobs[i].n_samples) + " samples");

@@ -23,9 +23,16 @@ namespace omr {
 constexpr uint32_t kMaxScenes = 64;
 
 // Summaries of the sharded control block as read back to the host.
-inline unsigned long long sum_shards(const unsigned long long *a) {
+inline unsigned long long sum_shards(const Sharded<unsigned long long> &a) {
   unsigned long long s = 0;
   for (uint32_t k = 0; k < kShards; k++) s += a[k];
+  return s;
+}
+// Completed samples of the job in ring slot `ring` (host copy of the
+// kShards x kJobRing counter array).
+inline unsigned long long job_done(const unsigned long long *done, uint32_t ring) {
+  unsigned long long s = 0;
+  for (uint32_t k = 0; k < kShards; k++) s += done[done_index(k, ring)];
   return s;
 }
 inline uint64_t queue_len(const Ctrl &c, int parity) {

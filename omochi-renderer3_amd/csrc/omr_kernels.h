@@ -34,6 +34,7 @@ struct PoolPtrs {
 struct KernelArgs {
   Ctrl *ctrl;
   JobDev *jobs;          // ring of kJobRing entries
+  unsigned long long *done;  // completed-sample counters, done_index(shard, ring)
   const SceneDev *scenes;
   PoolPtrs p;
 };
@@ -51,13 +52,15 @@ hipError_t extend(const KernelArgs &a, const LaunchShape &s, int parity, bool st
 // until the next refill.
 hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool regen, hipStream_t st);
 hipError_t shadow(const KernelArgs &a, const LaunchShape &s, hipStream_t st);
+// Zeroes the completed-sample counters of ring slot `ring` (every shard).
+hipError_t clear_done(unsigned long long *done, uint32_t ring, hipStream_t st);
 hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, double n_samples, hipStream_t st);
 // Closest hit for n rays (n x 8 floats).  counters (optional, device) gets
 // node visits and primitive tests.
-// head: zeroed fetch counter (used when budget > 0, dynamic fetch).
+// head: zeroed fetch heads (used when budget > 0, dynamic fetch).
 hipError_t trace_rays(const SceneDev *scene, const float *rays, uint32_t n, int32_t *prim, float *t,
                       unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t grid, uint32_t wg,
-                      uint32_t *head, int budget, uint32_t fetch_min, hipStream_t st);
+                      Sharded<uint32_t> *head, int budget, uint32_t fetch_min, hipStream_t st);
 // Environment importance-sampling tables: weights, per-row scan, marginal scan.
 hipError_t env_tables(const float *env, uint32_t w, uint32_t h, float *wbuf, unsigned int *wmax_bits,
                       unsigned long long *q, unsigned long long *row, unsigned long long *marg, hipStream_t st);

@@ -41,6 +41,15 @@ __device__ inline f3 normalize(f3 a) {
 }
 __device__ inline f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
 
+// 16-byte load through the global address space.  Scene pointers are read
+// from device memory, so the compiler cannot prove where they point and would
+// emit flat loads (which also count against the LDS wait counter).
+typedef float v4f __attribute__((ext_vector_type(4)));
+__device__ inline float4 ld4(const float4 *p) {
+  v4f v = *(__attribute__((address_space(1))) const v4f *)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // ------------------------------------------------------------ wave helpers
 __device__ inline uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 __device__ inline uint32_t rank_in(unsigned long long m) {
@@ -64,7 +73,7 @@ __device__ inline uint32_t wave_shard() { return (blockIdx.x * (blockDim.x >> 6)
 // A queue is kShards sub-queues of `cap` entries.  s_pre (LDS, kShards+1)
 // receives the exclusive prefix of the sub-queue counts; the flat index i of
 // an entry maps to its sub-queue by binary search.  Whole block must call.
-__device__ inline void load_prefix(uint32_t *s_pre, const uint32_t *counts) {
+__device__ inline void load_prefix(uint32_t *s_pre, const Sharded<uint32_t> &counts) {
   if (threadIdx.x < kShards) s_pre[threadIdx.x + 1] = counts[threadIdx.x];
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -175,7 +184,11 @@ __device__ inline float acos_f(float x) {
 
 // ------------------------------------------------------------ primitives
 // Polygon::CheckIntersection (src/renderer/Polygon.h:147-190), single-sided.
-__device__ inline bool hit_triangle(f3 o, f3 d, f3 p0, f3 e1, f3 e2, float &t_out, float &u_out, float &v_out) {
+// u_out, v_out are the unnormalized barycentrics and det_out their divisor:
+// the traversal divides once, for the closest hit only (u/det is the same
+// value whenever it is computed).
+__device__ inline bool hit_triangle(f3 o, f3 d, f3 p0, f3 e1, f3 e2, float &t_out, float &u_out, float &v_out,
+                                    float &det_out) {
   f3 P = cross(d, e2);
   float det = dot(P, e1);
   if (!(det > kEps)) return false;
@@ -188,8 +201,9 @@ __device__ inline bool hit_triangle(f3 o, f3 d, f3 p0, f3 e1, f3 e2, float &t_ou
   float t = dot(Q, e2) / det;
   if (!(t >= kEps)) return false;
   t_out = t;
-  u_out = u / det;
-  v_out = v / det;
+  u_out = u;
+  v_out = v;
+  det_out = det;
   return true;
 }
 // Sphere::CheckIntersection (src/renderer/Sphere.h:21-48) with the
@@ -258,9 +272,15 @@ struct Trav {
   const float4 *nodes, *prims;
   f3 o, d;
   float ix, iy, iz;
-  bool sx, sy, sz;
+  uint32_t nx, ny, nz;  // 1 when the ray runs towards -axis: the near slab is the upper bound
   int32_t cur;
-  Hit best;
+  Hit best;   // u, v unnormalized until the traversal ends
+  float det;  // divisor of best.u, best.v (1 for spheres)
+
+  __device__ inline void finish() {
+    best.u = best.u / det;
+    best.v = best.v / det;
+  }
 
   __device__ inline void init(const SceneDev &S, f3 o_, f3 d_, float tmax, Stack<WG, SHORT> &stk) {
     o = o_;
@@ -269,14 +289,15 @@ struct Trav {
     ix = (d.x == 0.0f) ? kInf : 1.0f / d.x;
     iy = (d.y == 0.0f) ? kInf : 1.0f / d.y;
     iz = (d.z == 0.0f) ? kInf : 1.0f / d.z;
-    sx = !(d.x >= 0.0f);
-    sy = !(d.y >= 0.0f);
-    sz = !(d.z >= 0.0f);
+    nx = !(d.x >= 0.0f) ? 1u : 0u;
+    ny = !(d.y >= 0.0f) ? 1u : 0u;
+    nz = !(d.z >= 0.0f) ? 1u : 0u;
     nodes = reinterpret_cast<const float4 *>(S.nodes);
     prims = reinterpret_cast<const float4 *>(S.prims);
     best.t = tmax;
     best.u = best.v = 0.0f;
     best.prim = -1;
+    det = 1.0f;
     cur = 0;
     stk.sp = 0;
   }
@@ -291,28 +312,32 @@ __device__ inline bool Trav<WG, SHORT, ANY, STATS>::step(Stack<WG, SHORT> &stk, 
   for (int it = 0; it < budget; it++) {
     bool need_pop;
     if (!child_is_leaf(cur)) {
+      // node = lo.x, hi.x, lo.y, hi.y, lo.z, hi.z (4 children each), child
+      // codes: the near and far slab of each axis are picked by address
       const float4 *np = nodes + (size_t)cur * 8;
-      float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-      int4 ch = *reinterpret_cast<const int4 *>(np + 6);
+      float4 nX = ld4(np + nx), fX = ld4(np + (nx ^ 1u));
+      float4 nY = ld4(np + 2u + ny), fY = ld4(np + 2u + (ny ^ 1u));
+      float4 nZ = ld4(np + 4u + nz), fZ = ld4(np + 4u + (nz ^ 1u));
+      float4 chf = ld4(np + 6);
       if (STATS) nvis++;
-      float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, hi_x[4] = {hx.x, hx.y, hx.z, hx.w};
-      float lo_y[4] = {ly.x, ly.y, ly.z, ly.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w};
-      float lo_z[4] = {lz.x, lz.y, lz.z, lz.w}, hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
-      int32_t code[4] = {ch.x, ch.y, ch.z, ch.w};
+      float near_x[4] = {nX.x, nX.y, nX.z, nX.w}, far_x[4] = {fX.x, fX.y, fX.z, fX.w};
+      float near_y[4] = {nY.x, nY.y, nY.z, nY.w}, far_y[4] = {fY.x, fY.y, fY.z, fY.w};
+      float near_z[4] = {nZ.x, nZ.y, nZ.z, nZ.w}, far_z[4] = {fZ.x, fZ.y, fZ.z, fZ.w};
+      int32_t code[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z), __float_as_int(chf.w)};
       float key[4];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         float tmin = 0.0f, tmax = best.t;
-        float n = ((sx ? hi_x[k] : lo_x[k]) - ox) * ix;
-        float f = ((sx ? lo_x[k] : hi_x[k]) - ox) * ix;
+        float n = (near_x[k] - ox) * ix;
+        float f = (far_x[k] - ox) * ix;
         tmin = n > tmin ? n : tmin;
         tmax = f < tmax ? f : tmax;
-        n = ((sy ? hi_y[k] : lo_y[k]) - oy) * iy;
-        f = ((sy ? lo_y[k] : hi_y[k]) - oy) * iy;
+        n = (near_y[k] - oy) * iy;
+        f = (far_y[k] - oy) * iy;
         tmin = n > tmin ? n : tmin;
         tmax = f < tmax ? f : tmax;
-        n = ((sz ? hi_z[k] : lo_z[k]) - oz) * iz;
-        f = ((sz ? lo_z[k] : hi_z[k]) - oz) * iz;
+        n = (near_z[k] - oz) * iz;
+        f = (far_z[k] - oz) * iz;
         tmin = n > tmin ? n : tmin;
         tmax = f < tmax ? f : tmax;
         bool h = (code[k] != -1) && (tmin <= tmax * kSlabInflate);
@@ -335,18 +360,26 @@ __device__ inline bool Trav<WG, SHORT, ANY, STATS>::step(Stack<WG, SHORT> &stk, 
         need_pop = true;
       }
     } else {
+      // primitive records are 3 x 16 bytes; the next record is loaded while
+      // the current one is tested
       uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
+      const float4 *pr = prims + (size_t)first * 3;
+      float4 a = ld4(pr), b = ld4(pr + 1), c = ld4(pr + 2);
       for (uint32_t k = 0; k < cnt; k++) {
-        const float4 *pr = prims + (size_t)(first + k) * 3;
-        float4 a = pr[0], b = pr[1];
+        float4 na = a, nb = b, nc = c;
+        if (k + 1u < cnt) {
+          const float4 *pn = pr + (size_t)(k + 1u) * 3;
+          na = ld4(pn);
+          nb = ld4(pn + 1);
+          nc = ld4(pn + 2);
+        }
         uint32_t id = __float_as_uint(a.w);
         uint32_t kind = __float_as_uint(b.w);
-        float t, u = 0.0f, v = 0.0f;
+        float t, u = 0.0f, v = 0.0f, dt = 1.0f;
         bool ok;
         if (STATS) ntest++;
         if (kind == 0u) {
-          float4 c = pr[2];
-          ok = hit_triangle(o, d, xyz(a), xyz(b), xyz(c), t, u, v);
+          ok = hit_triangle(o, d, xyz(a), xyz(b), xyz(c), t, u, v, dt);
         } else {
           ok = hit_sphere(o, d, xyz(a), b.x, t);
         }
@@ -354,10 +387,17 @@ __device__ inline bool Trav<WG, SHORT, ANY, STATS>::step(Stack<WG, SHORT> &stk, 
           best.t = t;
           best.u = u;
           best.v = v;
+          det = dt;
           best.prim = (int32_t)id;
         }
+        a = na;
+        b = nb;
+        c = nc;
       }
-      if (ANY && best.prim >= 0) return true;
+      if (ANY && best.prim >= 0) {
+        finish();
+        return true;
+      }
       need_pop = true;
     }
     if (need_pop) {
@@ -369,7 +409,10 @@ __device__ inline bool Trav<WG, SHORT, ANY, STATS>::step(Stack<WG, SHORT> &stk, 
         found = true;
         break;
       }
-      if (!found) return true;
+      if (!found) {
+        finish();
+        return true;
+      }
     }
   }
   return false;
@@ -393,7 +436,7 @@ __device__ inline void traverse(const SceneDev &S, f3 o, f3 d, Hit &best, Stack<
 // load(k, off, S, o, d, tmax, tag) reads one entry; store(tag, hit) writes a
 // finished ray's result.
 template <int WG, int SHORT, bool ANY, bool STATS, class Count, class Load, class Store>
-__device__ inline uint32_t dynamic_fetch(uint32_t *heads, Count count, Stack<WG, SHORT> &stk, int budget,
+__device__ inline uint32_t dynamic_fetch(Sharded<uint32_t> &heads, Count count, Stack<WG, SHORT> &stk, int budget,
                                          uint32_t fetch_min, Load load, Store store, uint32_t &nvis,
                                          uint32_t &ntest) {
   Trav<WG, SHORT, ANY, STATS> tr;
@@ -597,24 +640,38 @@ __device__ inline unsigned long long grab_sample(Ctrl *ctrl, bool want) {
   return (base + r) * kShards + k;
 }
 
-// Wave-aggregated per-job completion counting into the wave's shard.
-__device__ inline void count_done(JobDev *jobs, bool fin, uint32_t job) {
+// Per-wave completion counting: a wave keeps a running count for one job and
+// adds it to that job's counter in the wave's shard when the job changes and
+// when the kernel ends, so a launch costs about one atomic per wave.  Called
+// with every lane of the wave active.
+struct DoneAcc {
+  uint32_t job = ~0u;
+  uint32_t n = 0u;
+};
+__device__ inline void done_flush(unsigned long long *done, DoneAcc &acc) {
+  if (acc.n != 0u && lane_id() == 0u) atomicAdd(&done[done_index(wave_shard(), acc.job)], (unsigned long long)acc.n);
+  acc.n = 0u;
+}
+__device__ inline void count_done(unsigned long long *done, DoneAcc &acc, bool fin, uint32_t job) {
   bool need = fin;
-  const uint32_t k = wave_shard();
   for (;;) {
     unsigned long long m = __ballot(need);
     if (m == 0ull) break;
     uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
     uint32_t lj = __shfl(job, (int)leader);
     bool mine = need && job == lj;
-    unsigned long long mm = __ballot(mine);
-    if (lane_id() == leader) atomicAdd(&jobs[lj].done[k], (unsigned long long)__popcll(mm));
+    uint32_t c = (uint32_t)__popcll(__ballot(mine));
+    if (lj != acc.job) {
+      done_flush(done, acc);
+      acc.job = lj;
+    }
+    acc.n += c;
     if (mine) need = false;
   }
 }
 
-// One atomic per wave into the wave's shard of a counter array.
-__device__ inline void wave_count(unsigned long long *shards, unsigned long long v) {
+// One atomic per wave into the wave's shard of a counter.
+__device__ inline void wave_count(Sharded<unsigned long long> &shards, unsigned long long v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
   if (lane_id() == 0 && v) atomicAdd(&shards[wave_shard()], v);
@@ -732,6 +789,7 @@ __global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int rege
   uint4 *misc = reinterpret_cast<uint4 *>(A.p.misc);
   uint4 *rngs = reinterpret_cast<uint4 *>(A.p.rng);
   float4 *pend = reinterpret_cast<float4 *>(A.p.pend);
+  DoneAcc dacc;
   for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     bool valid = i < n;
@@ -952,7 +1010,7 @@ __global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int rege
         fin = true;
       }
     }
-    count_done(A.jobs, fin, js);
+    count_done(A.done, dacc, fin, js);
     // regenerate finished slots while samples remain
     unsigned long long g = grab_sample(A.ctrl, fin && regen);
     bool fresh = g != ~0ull;
@@ -980,6 +1038,7 @@ __global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int rege
       sq[2] = make_float4(sh_c.x, sh_c.y, sh_c.z, 0.0f);
     }
   }
+  done_flush(A.done, dacc);
 }
 
 // Accumulates the contribution of shadow record r if its ray was unoccluded.
@@ -1056,6 +1115,10 @@ __global__ void k_clear_shadow(Ctrl *ctrl) {
   }
 }
 
+__global__ void k_clear_done(unsigned long long *done, uint32_t ring) {
+  if (threadIdx.x < kShards) done[done_index(threadIdx.x, ring)] = 0ull;
+}
+
 // mean = sum * 2^-24 / (ss^2 * samples), in double, as the CPU restatement
 __global__ void k_resolve(const unsigned long long *accum, float *out, uint32_t n, double n_samples) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1065,7 +1128,7 @@ __global__ void k_resolve(const unsigned long long *accum, float *out, uint32_t 
 template <int WG, int SHORT, bool STATS, bool DF>
 __global__ __launch_bounds__(WG) void k_trace_rays(const SceneDev *scene, const float4 *rays, uint32_t n, int32_t *prim,
                                                    float *tout, unsigned long long *counters, uint2 *gstack,
-                                                   uint32_t gcap, uint32_t *head, int budget, uint32_t fetch_min) {
+                                                   uint32_t gcap, Sharded<uint32_t> *head, int budget, uint32_t fetch_min) {
   __shared__ uint2 lds[SHORT * WG];
   const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
   const uint32_t stride = gridDim.x * WG;
@@ -1091,7 +1154,7 @@ __global__ __launch_bounds__(WG) void k_trace_rays(const SceneDev *scene, const 
       prim[i] = h.prim;
       tout[i] = h.prim >= 0 ? h.t : kInf;
     };
-    dynamic_fetch<WG, SHORT, false, STATS>(head, count, stk, budget, fetch_min, load, store, nv, nt);
+    dynamic_fetch<WG, SHORT, false, STATS>(*head, count, stk, budget, fetch_min, load, store, nv, nt);
   }
   for (uint32_t base = blockIdx.x * WG; !DF && base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
@@ -1222,7 +1285,8 @@ hipError_t launch_shadow_wg(const KernelArgs &a, uint32_t grid, hipStream_t st) 
 
 template <int WG, bool STATS>
 hipError_t launch_trace_wg(const SceneDev *scene, const float4 *r4, uint32_t n, int32_t *prim, float *t,
-                           unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t grid, uint32_t *head,
+                           unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t grid,
+                           Sharded<uint32_t> *head,
                            int budget, uint32_t fetch_min, hipStream_t st) {
   if (budget > 0)
     hipLaunchKernelGGL((k_trace_rays<WG, kShort, STATS, true>), dim3(grid), dim3(WG), 0, st, scene, r4, n, prim, t,
@@ -1273,6 +1337,10 @@ hipError_t shadow(const KernelArgs &a, const LaunchShape &s, hipStream_t st) {
   }
 }
 
+hipError_t clear_done(unsigned long long *done, uint32_t ring, hipStream_t st) {
+  hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(kShards), 0, st, done, ring);
+  return hipGetLastError();
+}
 hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, double n_samples, hipStream_t st) {
   uint32_t n = n_px * 3u;
   if (n == 0) return hipSuccess;
@@ -1282,7 +1350,7 @@ hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, d
 
 hipError_t trace_rays(const SceneDev *scene, const float *rays, uint32_t n, int32_t *prim, float *t,
                       unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t grid, uint32_t wg,
-                      uint32_t *head, int budget, uint32_t fetch_min, hipStream_t st) {
+                      Sharded<uint32_t> *head, int budget, uint32_t fetch_min, hipStream_t st) {
   const float4 *r4 = reinterpret_cast<const float4 *>(rays);
   if (counters)
     return launch_trace_wg<128, true>(scene, r4, n, prim, t, counters, gstack, gcap, grid, head, budget, fetch_min, st);

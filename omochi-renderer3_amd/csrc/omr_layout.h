@@ -92,13 +92,25 @@ struct JobDev {
   float cam_pos[3], cam_c[3], cam_x[3], cam_y[3];
   float aperture, focal;
   unsigned long long *accum;       // W*H*3 fixed-point sums
-  unsigned long long *done;        // completed samples, kShards partial counts
 };
 
 // Every device-wide counter that waves update in the hot loop is split into
-// kShards copies; wave w uses copy w % kShards.  A single address updated by
-// every wave serializes across the eight XCDs.
+// kShards copies; wave w uses copy w % kShards.  Device-scope atomics on one
+// cache line are serialized at the memory side (tens of ns each), so each copy
+// sits on its own kLine-byte line: copies packed into one line would still
+// serialize.
 constexpr uint32_t kShards = 32;
+constexpr uint32_t kLine = 256;
+
+template <class T>
+struct Sharded {
+  struct alignas(kLine) Cell {
+    T v;
+  };
+  Cell c[kShards];
+  OMR_HD T &operator[](uint32_t k) { return c[k].v; }
+  OMR_HD const T &operator[](uint32_t k) const { return c[k].v; }
+};
 
 // Samples handed out by cursor shard k are k, k + kShards, k + 2 kShards, ...
 OMR_HD inline unsigned long long shard_limit(unsigned long long total, uint32_t k) {
@@ -107,20 +119,24 @@ OMR_HD inline unsigned long long shard_limit(unsigned long long total, uint32_t 
 
 // Control block in device memory (one per engine).
 struct Ctrl {
-  unsigned long long next_sample[kShards];  // striped sample cursors (index j -> sample j*kShards + k)
-  unsigned long long total_samples;         // samples admitted so far (host-written)
-  unsigned long long segments[kShards];     // closest-hit queries traced
-  unsigned long long shadow_rays[kShards];  // environment shadow rays traced
-  unsigned long long node_visits[kShards];  // stats kernels only
-  unsigned long long prim_tests[kShards];   // stats kernels only
-  uint32_t qcount[2][kShards];              // ping-pong ray sub-queue counts
-  uint32_t scount[kShards];                 // shadow sub-queue counts
-  uint32_t qhead[2][kShards];               // dynamic-fetch heads of the ray sub-queues
-  uint32_t shead[kShards];                  // dynamic-fetch heads of the shadow sub-queues
-  uint32_t job_lo, job_hi;                  // live job window (logical ring indices)
+  Sharded<unsigned long long> next_sample;                // striped sample cursors (index j -> sample j*kShards + k)
+  alignas(kLine) unsigned long long total_samples;        // samples admitted so far (host-written)
+  alignas(kLine) uint32_t job_lo;                         // live job window [job_lo, job_hi) (logical ring
+  uint32_t job_hi;                                        // indices); the pair is copied as 8 bytes
+  Sharded<unsigned long long> segments;     // closest-hit queries traced
+  Sharded<unsigned long long> shadow_rays;  // environment shadow rays traced
+  Sharded<unsigned long long> node_visits;  // stats kernels only
+  Sharded<unsigned long long> prim_tests;   // stats kernels only
+  Sharded<uint32_t> qcount[2];              // ping-pong ray sub-queue counts
+  Sharded<uint32_t> scount;                 // shadow sub-queue counts
+  Sharded<uint32_t> qhead[2];               // dynamic-fetch heads of the ray sub-queues
+  Sharded<uint32_t> shead;                  // dynamic-fetch heads of the shadow sub-queues
 };
 
 constexpr uint32_t kJobRing = 1024;
+// Completed-sample counters: kShards x kJobRing, shard-major, so the waves of
+// one shard that finish samples of one job update one line.
+OMR_HD inline size_t done_index(uint32_t shard, uint32_t ring) { return (size_t)shard * kJobRing + ring; }
 
 // Per-slot packed path state word (misc.y).
 //  bits 0-7 depth, 8-15 branch, 16-17 pending count, 18 prev-lambert (MIS), 19 active

@@ -11,6 +11,7 @@
 // on (job seed, pixel, sub-pixel, sample, branch) and accumulation is integer.
 #include <algorithm>
 #include <chrono>
+#include <cstddef>
 #include <cstring>
 
 #include "omr_engine.h"
@@ -153,10 +154,16 @@ void Engine::server_loop() {
   unsigned long long total = 0;     // samples admitted
   unsigned long long cursor[kShards] = {};  // device sample cursors as of the last readback
   JobDev *h_jobs = nullptr;         // pinned staging for job records
-  struct Stage {
-    unsigned long long cursor[kShards], total;  // same order as Ctrl::next_sample, total_samples
+  struct Stage {  // same layout as Ctrl::next_sample, total_samples (one copy updates both)
+    Sharded<unsigned long long> cursor;
+    alignas(kLine) unsigned long long total;
     uint32_t lo, hi;
   } *h_stage = nullptr;
+  static_assert(offsetof(Stage, total) - offsetof(Stage, cursor) ==
+                    offsetof(Ctrl, total_samples) - offsetof(Ctrl, next_sample),
+                "staged cursors must mirror the control block");
+  static_assert(offsetof(Ctrl, job_hi) == offsetof(Ctrl, job_lo) + 4 && offsetof(Stage, hi) == offsetof(Stage, lo) + 4,
+                "the job window is copied as one 8-byte pair");
   bool ok = true;
   {
     std::lock_guard<std::mutex> g(mu);
@@ -234,7 +241,7 @@ void Engine::server_loop() {
           return;
         }
         hipMemsetAsync(j.d_accum, 0, px * 3 * 8, stream_);
-        hipMemsetAsync(d_done_ + (size_t)j.ring * kShards, 0, kShards * sizeof(unsigned long long), stream_);
+        k::clear_done(d_done_, j.ring, stream_);
         JobDev &D = h_jobs[j.ring];
         std::memset(&D, 0, sizeof(D));
         D.sample_base = total;
@@ -258,7 +265,6 @@ void Engine::server_loop() {
         D.aperture = j.cam.aperture_radius;
         D.focal = j.cam.focal_distance;
         D.accum = j.d_accum;
-        D.done = d_done_ + (size_t)j.ring * kShards;
         hipMemcpyAsync(d_jobs_ + j.ring, &D, sizeof(JobDev), hipMemcpyHostToDevice, stream_);
         total += j.n_samples;
         live.push_back(r);
@@ -275,7 +281,8 @@ void Engine::server_loop() {
       h_stage->total = total;
       h_stage->lo = (uint32_t)lo;
       h_stage->hi = (uint32_t)hi;
-      hipMemcpyAsync(&d_ctrl_->next_sample[0], &h_stage->cursor[0], (kShards + 1) * 8, hipMemcpyHostToDevice, stream_);
+      hipMemcpyAsync(&d_ctrl_->next_sample, &h_stage->cursor, offsetof(Stage, total) + 8, hipMemcpyHostToDevice,
+                     stream_);
       hipMemcpyAsync(&d_ctrl_->job_lo, &h_stage->lo, 8, hipMemcpyHostToDevice, stream_);
       k::refill(args(), sh, 0, stream_);
     }
@@ -303,7 +310,7 @@ void Engine::server_loop() {
     std::vector<std::shared_ptr<Req>> finished;
     for (auto &r : live) {
       JobRun &j = r->run;
-      if (j.done || sum_shards(h_done_ + (size_t)j.ring * kShards) != j.n_samples) continue;
+      if (j.done || job_done(h_done_, j.ring) != j.n_samples) continue;
       j.done = true;
       double nspp = (double)j.params.supersamples * j.params.supersamples *
                     (double)(j.params.spp_end - j.params.spp_begin);
