@@ -130,6 +130,8 @@ typedef struct {
   uint32_t fetch_budget;      /* traversal: 0 = one ray per lane per grid-stride turn; k = lanes
                                  that finished fetch new rays after every k node/leaf visits */
   uint32_t fetch_min;         /* idle lanes (of 64) that trigger a fetch */
+  uint32_t stack_short;       /* traversal stack entries per lane kept in LDS: 8 or 16 (deeper
+                                 entries spill to a per-thread global area) */
 } omr_tuning;
 
 typedef struct {

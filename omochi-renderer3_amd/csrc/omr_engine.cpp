@@ -25,15 +25,18 @@ int Engine::hipfail(hipError_t e, const char *what) {
 }
 
 Engine::Engine(int device) : device_(device) {
+  // defaults: the best point of a sweep on MI355X (Cornell box + 82k-triangle
+  // icosphere, 512x512, 64 spp)
   tuning.node_width = 4;
-  tuning.wg_size = 128;
-  tuning.blocks_per_cu = 8;
-  tuning.pool_paths = 1u << 21;
+  tuning.wg_size = 64;
+  tuning.blocks_per_cu = 16;
+  tuning.pool_paths = 1u << 22;
   tuning.steps_per_sync = 8;
   tuning.use_graph = 1;
   tuning.refill_every = 0;
-  tuning.fetch_budget = 0;
+  tuning.fetch_budget = 8;
   tuning.fetch_min = 16;
+  tuning.stack_short = kShortStackMax;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
     err = "no HIP device " + std::to_string(device);
@@ -102,6 +105,8 @@ int Engine::set_tuning(const omr_tuning &t) {
   if (t.refill_every > 64) return fail(OMR_ERR_INVALID, "refill_every out of range");
   if (t.fetch_budget > 1024) return fail(OMR_ERR_INVALID, "fetch_budget out of range");
   if (t.fetch_budget && (t.fetch_min < 1 || t.fetch_min > 64)) return fail(OMR_ERR_INVALID, "fetch_min must be 1..64");
+  if (t.stack_short != kShortStackMin && t.stack_short != kShortStackMax)
+    return fail(OMR_ERR_INVALID, "stack_short must be 8 or 16");
   std::lock_guard<std::mutex> g(mu);
   hipSetDevice(device_);
   hipStreamSynchronize(stream_);
@@ -131,6 +136,7 @@ KernelArgs Engine::args() const {
   a.p.gcap = gcap_;
   a.p.df_budget = tuning.fetch_budget;
   a.p.df_fetch_min = tuning.fetch_min;
+  a.p.stack_short = tuning.stack_short;
   return a;
 }
 
@@ -176,7 +182,7 @@ int Engine::ensure_pool() {
 }
 
 int Engine::ensure_gstack(uint32_t need_entries, size_t threads) {
-  uint32_t cap = need_entries > kShortStack ? need_entries - kShortStack : 1u;
+  uint32_t cap = need_entries > tuning.stack_short ? need_entries - tuning.stack_short : 1u;
   if (d_gstack_ && cap <= gcap_ && threads <= gstack_threads_) return OMR_OK;
   drop_graphs();
   cap = std::max(cap, gcap_);
@@ -372,7 +378,8 @@ int Engine::trace(omr_scene_id id, const float *d_rays, size_t n, int32_t *d_pri
   hipError_t e = hipMemsetAsync(d_cnt, 0, cnt_bytes, stream_);
   if (e == hipSuccess)
     e = k::trace_rays(d_scenes_ + id, d_rays, (uint32_t)n, d_prim, d_t, stats2 ? d_cnt : nullptr, d_gstack_, gcap_,
-                      grid, wg, reinterpret_cast<Sharded<uint32_t> *>(d_cnt + kLine / 8), (int)tuning.fetch_budget, tuning.fetch_min, stream_);
+                      tuning.stack_short, grid, wg, reinterpret_cast<Sharded<uint32_t> *>(d_cnt + kLine / 8),
+                      (int)tuning.fetch_budget, tuning.fetch_min, stream_);
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
   unsigned long long c[2] = {0, 0};
   if (e == hipSuccess && stats2) e = hipMemcpy(c, d_cnt, 16, hipMemcpyDeviceToHost);

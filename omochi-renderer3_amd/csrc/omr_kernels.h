@@ -8,9 +8,10 @@
 
 namespace omr {
 
-// Entries of the per-lane LDS traversal stack; deeper entries spill to the
-// per-thread global area (PoolPtrs::gstack, gcap entries per thread).
-constexpr uint32_t kShortStack = 16;
+// Entries of the per-lane LDS traversal stack (8 or 16, a launch choice);
+// deeper entries spill to the per-thread global area (PoolPtrs::gstack, gcap
+// entries per thread).
+constexpr uint32_t kShortStackMin = 8, kShortStackMax = 16;
 
 // Device pointers of the wavefront path pool (capacity `pool` slots).
 struct PoolPtrs {
@@ -29,6 +30,7 @@ struct PoolPtrs {
   uint32_t gcap;       // entries per thread in gstack
   uint32_t df_budget;     // dynamic fetch: traversal visits between fetches (0 = grid-stride)
   uint32_t df_fetch_min;  // dynamic fetch: idle lanes of a wave that trigger a fetch
+  uint32_t stack_short;   // LDS traversal stack entries per lane (8 or 16)
 };
 
 struct KernelArgs {
@@ -59,8 +61,8 @@ hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, d
 // node visits and primitive tests.
 // head: zeroed fetch heads (used when budget > 0, dynamic fetch).
 hipError_t trace_rays(const SceneDev *scene, const float *rays, uint32_t n, int32_t *prim, float *t,
-                      unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t grid, uint32_t wg,
-                      Sharded<uint32_t> *head, int budget, uint32_t fetch_min, hipStream_t st);
+                      unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t stack_short, uint32_t grid,
+                      uint32_t wg, Sharded<uint32_t> *head, int budget, uint32_t fetch_min, hipStream_t st);
 // Environment importance-sampling tables: weights, per-row scan, marginal scan.
 hipError_t env_tables(const float *env, uint32_t w, uint32_t h, float *wbuf, unsigned int *wmax_bits,
                       unsigned long long *q, unsigned long long *row, unsigned long long *marg, hipStream_t st);

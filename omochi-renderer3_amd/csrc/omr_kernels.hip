@@ -1264,37 +1264,54 @@ __global__ __launch_bounds__(256) void k_env_margscan(const unsigned long long *
   }
 }
 
-constexpr int kShort = (int)kShortStack;
-
+template <int WG, int SHORT>
+hipError_t launch_extend_ws(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
+  bool df = a.p.df_budget > 0;
+  if (stats && df) hipLaunchKernelGGL((k_extend<WG, SHORT, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else if (stats) hipLaunchKernelGGL((k_extend<WG, SHORT, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else if (df) hipLaunchKernelGGL((k_extend<WG, SHORT, false, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else hipLaunchKernelGGL((k_extend<WG, SHORT, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  return hipGetLastError();
+}
 template <int WG>
 hipError_t launch_extend_wg(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
-  bool df = a.p.df_budget > 0;
-  if (stats && df) hipLaunchKernelGGL((k_extend<WG, kShort, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else if (stats) hipLaunchKernelGGL((k_extend<WG, kShort, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else if (df) hipLaunchKernelGGL((k_extend<WG, kShort, false, true>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else hipLaunchKernelGGL((k_extend<WG, kShort, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
-  return hipGetLastError();
+  if (a.p.stack_short == kShortStackMin) return launch_extend_ws<WG, (int)kShortStackMin>(a, grid, parity, stats, st);
+  return launch_extend_ws<WG, (int)kShortStackMax>(a, grid, parity, stats, st);
 }
 
+template <int WG, int SHORT>
+hipError_t launch_shadow_ws(const KernelArgs &a, uint32_t grid, hipStream_t st) {
+  if (a.p.df_budget > 0) hipLaunchKernelGGL((k_shadow<WG, SHORT, true>), dim3(grid), dim3(WG), 0, st, a);
+  else hipLaunchKernelGGL((k_shadow<WG, SHORT, false>), dim3(grid), dim3(WG), 0, st, a);
+  return hipGetLastError();
+}
 template <int WG>
 hipError_t launch_shadow_wg(const KernelArgs &a, uint32_t grid, hipStream_t st) {
-  if (a.p.df_budget > 0) hipLaunchKernelGGL((k_shadow<WG, kShort, true>), dim3(grid), dim3(WG), 0, st, a);
-  else hipLaunchKernelGGL((k_shadow<WG, kShort, false>), dim3(grid), dim3(WG), 0, st, a);
-  return hipGetLastError();
+  if (a.p.stack_short == kShortStackMin) return launch_shadow_ws<WG, (int)kShortStackMin>(a, grid, st);
+  return launch_shadow_ws<WG, (int)kShortStackMax>(a, grid, st);
 }
 
-template <int WG, bool STATS>
-hipError_t launch_trace_wg(const SceneDev *scene, const float4 *r4, uint32_t n, int32_t *prim, float *t,
+template <int WG, int SHORT, bool STATS>
+hipError_t launch_trace_ws(const SceneDev *scene, const float4 *r4, uint32_t n, int32_t *prim, float *t,
                            unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t grid,
-                           Sharded<uint32_t> *head,
-                           int budget, uint32_t fetch_min, hipStream_t st) {
+                           Sharded<uint32_t> *head, int budget, uint32_t fetch_min, hipStream_t st) {
   if (budget > 0)
-    hipLaunchKernelGGL((k_trace_rays<WG, kShort, STATS, true>), dim3(grid), dim3(WG), 0, st, scene, r4, n, prim, t,
+    hipLaunchKernelGGL((k_trace_rays<WG, SHORT, STATS, true>), dim3(grid), dim3(WG), 0, st, scene, r4, n, prim, t,
                        counters, gstack, gcap, head, budget, fetch_min);
   else
-    hipLaunchKernelGGL((k_trace_rays<WG, kShort, STATS, false>), dim3(grid), dim3(WG), 0, st, scene, r4, n, prim, t,
+    hipLaunchKernelGGL((k_trace_rays<WG, SHORT, STATS, false>), dim3(grid), dim3(WG), 0, st, scene, r4, n, prim, t,
                        counters, gstack, gcap, head, budget, fetch_min);
   return hipGetLastError();
+}
+template <int WG, bool STATS>
+hipError_t launch_trace_wg(const SceneDev *scene, const float4 *r4, uint32_t n, int32_t *prim, float *t,
+                           unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t stack_short,
+                           uint32_t grid, Sharded<uint32_t> *head, int budget, uint32_t fetch_min, hipStream_t st) {
+  if (stack_short == kShortStackMin)
+    return launch_trace_ws<WG, (int)kShortStackMin, STATS>(scene, r4, n, prim, t, counters, gstack, gcap, grid, head,
+                                                            budget, fetch_min, st);
+  return launch_trace_ws<WG, (int)kShortStackMax, STATS>(scene, r4, n, prim, t, counters, gstack, gcap, grid, head,
+                                                          budget, fetch_min, st);
 }
 
 }  // namespace
@@ -1349,20 +1366,22 @@ hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, d
 }
 
 hipError_t trace_rays(const SceneDev *scene, const float *rays, uint32_t n, int32_t *prim, float *t,
-                      unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t grid, uint32_t wg,
-                      Sharded<uint32_t> *head, int budget, uint32_t fetch_min, hipStream_t st) {
+                      unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t stack_short, uint32_t grid,
+                      uint32_t wg, Sharded<uint32_t> *head, int budget, uint32_t fetch_min, hipStream_t st) {
   const float4 *r4 = reinterpret_cast<const float4 *>(rays);
   if (counters)
-    return launch_trace_wg<128, true>(scene, r4, n, prim, t, counters, gstack, gcap, grid, head, budget, fetch_min, st);
+    return launch_trace_wg<128, true>(scene, r4, n, prim, t, counters, gstack, gcap, stack_short, grid, head, budget,
+                                      fetch_min, st);
   switch (wg) {
     case 64:
-      return launch_trace_wg<64, false>(scene, r4, n, prim, t, nullptr, gstack, gcap, grid, head, budget, fetch_min, st);
+      return launch_trace_wg<64, false>(scene, r4, n, prim, t, nullptr, gstack, gcap, stack_short, grid, head, budget,
+                                        fetch_min, st);
     case 256:
-      return launch_trace_wg<256, false>(scene, r4, n, prim, t, nullptr, gstack, gcap, grid, head, budget, fetch_min,
-                                         st);
+      return launch_trace_wg<256, false>(scene, r4, n, prim, t, nullptr, gstack, gcap, stack_short, grid, head, budget,
+                                         fetch_min, st);
     default:
-      return launch_trace_wg<128, false>(scene, r4, n, prim, t, nullptr, gstack, gcap, grid, head, budget, fetch_min,
-                                         st);
+      return launch_trace_wg<128, false>(scene, r4, n, prim, t, nullptr, gstack, gcap, stack_short, grid, head, budget,
+                                         fetch_min, st);
   }
 }
 

@@ -48,24 +48,30 @@ def test_env_tables_bit_exact(ctx):
     ctx.release(sid)
 
 
-@pytest.mark.parametrize("env_is", [0, 1])
-def test_render_env_bit_exact(ctx, env_is):
+@pytest.mark.parametrize("env_is,tune", [(0, {}), (1, {}), (1, dict(stack_short=8, fetch_budget=8))])
+def test_render_env_bit_exact(ctx, env_is, tune):
     """Torus knot under an HDR sky: lookup on escape, and with importance
-    sampling + shadow rays + MIS."""
+    sampling + shadow rays + MIS (also with the 8-entry LDS stack and
+    dynamic fetch in the shadow kernel)."""
     import omr
     import pyoracle
 
-    scene = omr.procedural(3, width=40, height=30)
-    sid = ctx.upload(scene)
-    prm = omr.params(ss=1, spp_end=2, max_depth=6, rr_depth=3, env_is=env_is, seed=77)
-    print(f"render knot env_is={env_is}", flush=True)
-    g = ctx.render_sums(sid, scene.camera, prm)
-    st = ctx.last_stats()
-    c, seg, sh = _oracle(scene, ctx, sid).render_sums(scene.camera, prm, pyoracle.WIDE_BVH)
-    assert not (g != c).any(axis=2).any(), f"{(g != c).any(axis=2).sum()} pixels differ"
-    assert st["segments"] == seg and st["shadow_rays"] == sh
-    assert (sh > 0) == bool(env_is)
-    ctx.release(sid)
+    saved = ctx.get_tuning()
+    try:
+        ctx.set_tuning(**tune)
+        scene = omr.procedural(3, width=40, height=30)
+        sid = ctx.upload(scene)
+        prm = omr.params(ss=1, spp_end=2, max_depth=6, rr_depth=3, env_is=env_is, seed=77)
+        print(f"render knot env_is={env_is} tuning {tune}", flush=True)
+        g = ctx.render_sums(sid, scene.camera, prm)
+        st = ctx.last_stats()
+        c, seg, sh = _oracle(scene, ctx, sid).render_sums(scene.camera, prm, pyoracle.WIDE_BVH)
+        assert not (g != c).any(axis=2).any(), f"{(g != c).any(axis=2).sum()} pixels differ"
+        assert st["segments"] == seg and st["shadow_rays"] == sh
+        assert (sh > 0) == bool(env_is)
+        ctx.release(sid)
+    finally:
+        ctx.set_tuning(**saved)
 
 
 def test_render_aperture_and_multimaterial_bit_exact(ctx):
@@ -111,10 +117,13 @@ def test_render_bit_exact(ctx, kind, ss, spp, depth, rr):
 
 @pytest.mark.parametrize("tune", [dict(fetch_budget=8, fetch_min=16), dict(refill_every=2),
                                   dict(fetch_budget=4, fetch_min=32, refill_every=1, use_graph=0, wg_size=256),
-                                  dict(wg_size=64, blocks_per_cu=2, pool_paths=4096, steps_per_sync=4)])
+                                  dict(wg_size=64, blocks_per_cu=2, pool_paths=4096, steps_per_sync=4),
+                                  dict(stack_short=8), dict(stack_short=8, fetch_budget=8, wg_size=64,
+                                                            blocks_per_cu=16)])
 def test_tuning_modes_bit_exact(ctx, tune):
     """Every launch configuration the autotuner can pick gives the same bits:
-    dynamic-fetch traversal, bulk refill cadence, graph off, small pools."""
+    dynamic-fetch traversal, bulk refill cadence, graph off, small pools,
+    the 8-entry LDS stack (more entries spill to global memory)."""
     import omr
     import pyoracle
 
