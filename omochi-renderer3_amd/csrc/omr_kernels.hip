@@ -325,21 +325,25 @@ __device__ inline bool Trav<WG, SHORT, ANY, STATS>::step(Stack<WG, SHORT> &stk, 
       float near_z[4] = {nZ.x, nZ.y, nZ.z, nZ.w}, far_z[4] = {fZ.x, fZ.y, fZ.z, fZ.w};
       int32_t code[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z), __float_as_int(chf.w)};
       float key[4];
+      // tmin = n > tmin ? n : tmin (and tmax likewise) as one v_max / v_min:
+      // equal for every input here (a NaN operand leaves the other one in
+      // both forms; n, f are never signalling NaNs; +0 and -0 compare equal
+      // in every later test)
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         float tmin = 0.0f, tmax = best.t;
         float n = (near_x[k] - ox) * ix;
         float f = (far_x[k] - ox) * ix;
-        tmin = n > tmin ? n : tmin;
-        tmax = f < tmax ? f : tmax;
+        tmin = __builtin_fmaxf(n, tmin);
+        tmax = __builtin_fminf(f, tmax);
         n = (near_y[k] - oy) * iy;
         f = (far_y[k] - oy) * iy;
-        tmin = n > tmin ? n : tmin;
-        tmax = f < tmax ? f : tmax;
+        tmin = __builtin_fmaxf(n, tmin);
+        tmax = __builtin_fminf(f, tmax);
         n = (near_z[k] - oz) * iz;
         f = (far_z[k] - oz) * iz;
-        tmin = n > tmin ? n : tmin;
-        tmax = f < tmax ? f : tmax;
+        tmin = __builtin_fmaxf(n, tmin);
+        tmax = __builtin_fminf(f, tmax);
         bool h = (code[k] != -1) && (tmin <= tmax * kSlabInflate);
         key[k] = h ? tmin : __builtin_huge_valf();
       }
@@ -431,8 +435,11 @@ __device__ inline void traverse(const SceneDev &S, f3 o, f3 d, Hit &best, Stack<
 // visits, lanes whose ray finished take new rays (one atomic per wave) as soon
 // as at least `fetch_min` lanes are idle, so a wave does not idle behind its
 // slowest ray.  Work comes from kShards sub-queues: a wave starts on its own
-// shard's sub-queue (heads[k] counts what was taken) and moves on to the next
-// shard when that one runs dry.  count(k): entries of sub-queue k;
+// shard's sub-queue (heads[k] counts what was taken) and, when that one runs
+// dry, reads every head at once (one lane per shard, plain loads: heads only
+// grow, so a shard seen used up is used up) and moves to the next shard that
+// still has entries -- no atomics on drained sub-queues at the end of a
+// launch.  Must be called by whole waves.  count(k): entries of sub-queue k;
 // load(k, off, S, o, d, tmax, tag) reads one entry; store(tag, hit) writes a
 // finished ray's result.
 template <int WG, int SHORT, bool ANY, bool STATS, class Count, class Load, class Store>
@@ -441,7 +448,7 @@ __device__ inline uint32_t dynamic_fetch(Sharded<uint32_t> &heads, Count count, 
                                          uint32_t &ntest) {
   Trav<WG, SHORT, ANY, STATS> tr;
   bool has = false, exhausted = false;
-  uint32_t tag = 0, done = 0, shard = wave_shard(), visited = 0;
+  uint32_t tag = 0, done = 0, shard = wave_shard();
   const uint32_t active = (uint32_t)__popcll(__ballot(true));
   for (;;) {
     unsigned long long m = __ballot(!has);
@@ -462,9 +469,17 @@ __device__ inline uint32_t dynamic_fetch(Sharded<uint32_t> &heads, Count count, 
         tr.init(*S, o, d, tmax, stk);
         has = true;
       }
-      if (base + idle >= n) {  // this sub-queue is used up: move on
-        shard = (shard + 1u) % kShards;
-        if (++visited == kShards) exhausted = true;
+      if (base + idle >= n) {  // this sub-queue is used up: find one with entries left
+        const uint32_t lane = lane_id();
+        bool left = false;
+        if (lane < kShards) left = __hip_atomic_load(&heads[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < count(lane);
+        unsigned long long lm = __ballot(left);
+        if (lm == 0ull) {
+          exhausted = true;
+        } else {
+          unsigned long long after = lm & ~((2ull << shard) - 1ull);  // shards past the current one first
+          shard = (uint32_t)(__ffsll((long long)(after ? after : lm)) - 1);
+        }
       }
     }
     unsigned long long busy = __ballot(has);
