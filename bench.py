@@ -250,8 +250,8 @@ def main():
     ctx.set_profiling(not args.no_profile)
     for _ in range(args.warmup):
         ctx.render(sid, scene.camera, prm)
-    seg = shadow = 0
-    ext_ms = shade_ms = shadow_ms = 0.0
+    seg = shadow = drain_seg = drained = 0
+    ext_ms = shade_ms = shadow_ms = drain_ms = 0.0
     launches = nodes = prims = 0
     img = None
     t0 = time.perf_counter()
@@ -266,10 +266,14 @@ def main():
         launches += st["extend_launches"]
         nodes += st["node_visits"]
         prims += st["prim_tests"]
+        drain_seg += st["drain_segments"]
+        drained += st["drained_paths"]
+        drain_ms += st["drain_ms"]
     dt = time.perf_counter() - t0
     rays = seg + shadow
     value = rays / dt / 1e6
-    alg_bytes = B_SEG * seg + B_NODE * nodes + B_PRIM * prims
+    ext_seg = seg - drain_seg  # node and primitive counts cover the extend launches only
+    alg_bytes = B_SEG * ext_seg + B_NODE * nodes + B_PRIM * prims
     achieved = alg_bytes / (ext_ms / 1e3) / 1e9 if ext_ms > 0 else 0.0
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"extend_{args.workload}_traffic.json")
@@ -281,9 +285,10 @@ def main():
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
         "kernel": "k_extend", "avg_launch_ms": round(ext_ms / max(1, launches), 4),
         "alg_bytes_per_launch": int(alg_bytes / max(1, launches)),
-        "nodes_per_segment": round(nodes / max(1, seg), 3), "prims_per_segment": round(prims / max(1, seg), 3),
+        "nodes_per_segment": round(nodes / max(1, ext_seg), 3), "prims_per_segment": round(prims / max(1, ext_seg), 3),
         "kernel_ms_per_step": {"extend": round(ext_ms / args.steps, 3), "shade": round(shade_ms / args.steps, 3),
-                               "shadow": round(shadow_ms / args.steps, 3)},
+                               "shadow": round(shadow_ms / args.steps, 3), "drain": round(drain_ms / args.steps, 3)},
+        "drain": {"paths_per_step": drained // args.steps, "segments_per_step": drain_seg // args.steps},
     }
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,

@@ -105,7 +105,11 @@ typedef struct {
    * from events around every launch, and traversal work counted by extend */
   double extend_ms, shade_ms, shadow_ms;
   uint64_t extend_launches;
-  uint64_t node_visits, prim_tests;
+  uint64_t node_visits, prim_tests;  /* extend launches only */
+  /* drain (omr_tuning.drain_below): paths handed to it, their closest-hit
+   * queries (part of `segments`), and its device time when profiling */
+  uint64_t drained_paths, drain_segments;
+  double drain_ms;
 } omr_render_stats;
 
 typedef struct {
@@ -132,6 +136,9 @@ typedef struct {
   uint32_t fetch_min;         /* idle lanes (of 64) that trigger a fetch */
   uint32_t stack_short;       /* traversal stack entries per lane kept in LDS: 8 or 16 (deeper
                                  entries spill to a per-thread global area) */
+  uint32_t drain_below;       /* once no samples are left to hand out and at most this many paths
+                                 are queued, one drain launch runs them all to the end instead of
+                                 further extend/shade steps (0 = never) */
 } omr_tuning;
 
 typedef struct {

@@ -344,6 +344,7 @@ int omr_autotune(omr_ctx *ctx, size_t n, const omr_scene_id *scenes, const omr_c
       {&omr_tuning::fetch_budget, {0, 4, 8, 16}},
       {&omr_tuning::fetch_min, {8, 16, 32}},
       {&omr_tuning::stack_short, {8, 16}},
+      {&omr_tuning::drain_below, {0, 1u << 14, 1u << 16, 1u << 18}},
       {&omr_tuning::steps_per_sync, {4, 8, 16}},
   };
   for (int pass = 0; pass < 2; pass++) {

@@ -37,6 +37,7 @@ Engine::Engine(int device) : device_(device) {
   tuning.fetch_budget = 8;
   tuning.fetch_min = 16;
   tuning.stack_short = kShortStackMax;
+  tuning.drain_below = 1u << 20;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
     err = "no HIP device " + std::to_string(device);
@@ -49,7 +50,11 @@ Engine::Engine(int device) : device_(device) {
   hbm_ = prop.totalGlobalMem;
   name_ = prop.gcnArchName;
   if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) { err = "stream"; return; }
-  if (hipEventCreate(&ev_a_) != hipSuccess || hipEventCreate(&ev_b_) != hipSuccess) { err = "event"; return; }
+  if (hipEventCreate(&ev_a_) != hipSuccess || hipEventCreate(&ev_b_) != hipSuccess ||
+      hipEventCreate(&ev_c_) != hipSuccess || hipEventCreate(&ev_d_) != hipSuccess) {
+    err = "event";
+    return;
+  }
   if (hipMalloc(&d_scenes_, kMaxScenes * sizeof(SceneDev)) != hipSuccess) { err = "scene table"; return; }
   if (hipMemset(d_scenes_, 0, kMaxScenes * sizeof(SceneDev)) != hipSuccess) { err = "scene table"; return; }
   if (hipMalloc(&d_ctrl_, sizeof(Ctrl)) != hipSuccess) { err = "ctrl"; return; }
@@ -85,6 +90,8 @@ Engine::~Engine() {
   hipHostFree(h_done_);
   hipEventDestroy(ev_a_);
   hipEventDestroy(ev_b_);
+  hipEventDestroy(ev_c_);
+  hipEventDestroy(ev_d_);
   hipStreamDestroy(stream_);
 }
 
@@ -107,6 +114,7 @@ int Engine::set_tuning(const omr_tuning &t) {
   if (t.fetch_budget && (t.fetch_min < 1 || t.fetch_min > 64)) return fail(OMR_ERR_INVALID, "fetch_min must be 1..64");
   if (t.stack_short != kShortStackMin && t.stack_short != kShortStackMax)
     return fail(OMR_ERR_INVALID, "stack_short must be 8 or 16");
+  if (t.drain_below > t.pool_paths) return fail(OMR_ERR_INVALID, "drain_below above pool_paths");
   std::lock_guard<std::mutex> g(mu);
   hipSetDevice(device_);
   hipStreamSynchronize(stream_);
@@ -552,6 +560,8 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
   const uint32_t chunk = tuning.steps_per_sync;
   prof_ms_[0] = prof_ms_[1] = prof_ms_[2] = 0.0;
   prof_extend_launches_ = 0;
+  uint64_t drained = 0;
+  double drain_ms = 0.0;
   for (uint64_t it = 0;; it++) {
     double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (it > 2000000ull || el > 600.0) return fail(OMR_ERR_STATE, "bounce loop did not terminate");
@@ -561,7 +571,25 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     OMR_HIP(hipMemcpyAsync(h_ctrl_, d_ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_));
     OMR_HIP(hipStreamSynchronize(stream_));
     if (profile && (rc = collect_profile())) return rc;
-    if (queue_len(*h_ctrl_, 0) == 0 && samples_exhausted(*h_ctrl_)) break;
+    const uint64_t queued = queue_len(*h_ctrl_, 0);
+    const bool exhausted = samples_exhausted(*h_ctrl_);
+    if (queued == 0 && exhausted) break;
+    if (exhausted && queued <= tuning.drain_below) {
+      // the live paths (queue 0 after an even number of steps) run to the end in one launch
+      drained = queued;
+      if (profile) OMR_HIP(hipEventRecord(ev_c_, stream_));
+      e = k::drain(a, sh, 0, stream_);
+      if (e != hipSuccess) return hipfail(e, "drain");
+      if (profile) OMR_HIP(hipEventRecord(ev_d_, stream_));
+      OMR_HIP(hipMemcpyAsync(h_ctrl_, d_ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_));
+      OMR_HIP(hipStreamSynchronize(stream_));
+      if (profile) {
+        float ms = 0.0f;
+        OMR_HIP(hipEventElapsedTime(&ms, ev_c_, ev_d_));
+        drain_ms = ms;
+      }
+      break;
+    }
   }
   OMR_HIP(hipEventRecord(ev_b_, stream_));
   for (auto &j : jobs) {
@@ -595,6 +623,9 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     st->extend_launches = prof_extend_launches_;
     st->node_visits = sum_shards(h_ctrl_->node_visits);
     st->prim_tests = sum_shards(h_ctrl_->prim_tests);
+    st->drained_paths = drained;
+    st->drain_segments = sum_shards(h_ctrl_->drain_segments);
+    st->drain_ms = drain_ms;
   }
   return OMR_OK;
 }

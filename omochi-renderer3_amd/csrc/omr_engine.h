@@ -115,7 +115,7 @@ class Engine {
   size_t hbm_ = 0;
   std::string name_;
   hipStream_t stream_ = nullptr;
-  hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr;
+  hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr, ev_c_ = nullptr, ev_d_ = nullptr;
 
   std::vector<std::unique_ptr<SceneHost>> scenes_;
   SceneDev *d_scenes_ = nullptr;

@@ -790,7 +790,244 @@ __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
   }
 }
 
-// One shading step; mirrors the CPU restatement's per-segment loop body.
+// Path state between segments (kept in the slot arrays between launches).
+struct PathState {
+  f3 o, d, beta;
+  uint32_t word;  // ps_pack(depth, branch, pending, mis)
+  uint32_t sidx;  // sample index within the job
+  uint32_t js;    // job ring slot
+  uint4 rng;
+  float pdf;      // MIS: BSDF pdf of the last Lambert bounce (when ps_mis(word))
+};
+// An environment light sample waiting for its visibility test.
+struct ShadowRay {
+  f3 o, d, c;
+  uint32_t px;
+};
+
+// Shades the segment of slot `slot` that ended at hr (t, primitive id bits,
+// u, v): environment and emission contributions go to the job's pixel sums
+// and st becomes the next segment's state.  Mirrors the CPU restatement's
+// per-segment loop body.  pdf_mem: where the slot's last Lambert pdf is
+// stored (null: st.pdf).  Returns true while the path, or one of its pending
+// glass branches, continues; `shadow` is set when an environment sample needs
+// a visibility ray.
+__device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathState &st, float4 hr,
+                                     const float *pdf_mem, bool &shadow, ShadowRay &sh) {
+  float4 *pend = reinterpret_cast<float4 *>(A.p.pend);
+  const uint32_t P = A.p.pool;
+  bool cont = false;
+  f3 o = st.o, d = st.d, beta = st.beta;
+  uint4 rng = st.rng;
+  uint32_t word = st.word;
+  const uint32_t sidx = st.sidx;
+  float pdf_next = 0.0f;
+  const JobDev &J = A.jobs[st.js];
+  const SceneDev &S = A.scenes[J.scene];
+  uint32_t depth = ps_depth(word), branch = ps_branch(word), npend = ps_pending(word);
+  bool env_on = S.env != nullptr;
+  bool nee = env_on && (J.flags & 1u) && S.env_marg[S.env_h] > 0ull;
+  uint32_t pixel = decompose(J, sidx).pixel;
+  unsigned long long *acc = J.accum + 3ull * pixel;
+  int32_t prim = __float_as_int(hr.y);
+  uint32_t mis_next = 0;
+  if (prim < 0) {
+    // PathTracer.cpp:357-363: miss -> IBL lookup or black background
+    if (env_on) {
+      f3 L = env_rgb(S, d);
+      float w = 1.0f;
+      if (ps_mis(word)) {
+        float pp = pdf_mem ? pdf_mem[slot] : st.pdf;
+        float pe = env_pdf(S, d);
+        w = (pp * pp) / (pp * pp + pe * pe);
+      }
+      add_px(acc, beta, L, w);
+    }
+  } else {
+    uint32_t pid = (uint32_t)prim;
+    f3 x = add(o, scl(d, hr.x));
+    f3 nrm;
+    if (pid < S.n_tri) {
+      const float4 *tn = reinterpret_cast<const float4 *>(S.tri_n) + (size_t)pid * 3;
+      float4 n0 = tn[0], n1 = tn[1], n2 = tn[2];
+      nrm = normalize(add(add(scl(xyz(n1), hr.z), scl(xyz(n2), hr.w)), xyz(n0)));
+    } else {
+      float4 sc = reinterpret_cast<const float4 *>(S.sph)[pid - S.n_tri];
+      nrm = normalize(sub(x, xyz(sc)));
+    }
+    f3 nl = dot(nrm, d) < 0.0f ? nrm : scl(nrm, -1.0f);
+    // material roulette (PathTracer.cpp:368-386)
+    uint32_t pm = S.prim_mat[pid];
+    uint32_t mfirst = pm & 0x00FFFFFFu, mcount = (pm >> 24) + 1u;
+    float value = rng_float(rng);
+    uint32_t sel = mcount - 1u;
+    for (uint32_t q = 0; q < mcount; q++) {
+      float r = S.mats[mfirst + q].rate;
+      if (value <= r) { sel = q; break; }
+      value -= r;
+    }
+    const MatRec &m = S.mats[mfirst + sel];
+    f3 col = mk(m.color[0], m.color[1], m.color[2]);
+    f3 em = mk(m.emission[0], m.emission[1], m.emission[2]);
+    // Russian roulette (PathTracer.cpp:398-416)
+    float prr = col.x > (col.y > col.z ? col.y : col.z) ? col.x : (col.y > col.z ? col.y : col.z);
+    if (J.max_depth == 0u && depth > 64u) prr *= ldexpf(1.0f, -(int)(depth - 64u));
+    bool killed = false;
+    if (depth > J.rr_depth) {
+      if (rng_float(rng) >= prr) killed = true;
+    } else {
+      prr = 1.0f;
+    }
+    add_px(acc, beta, em, 1.0f);
+    uint32_t dmax = J.max_depth == 0u ? 255u : J.max_depth;
+    bool can_continue = !killed && depth + 1u < dmax;
+    if (!killed) {
+      if (m.type == 0u) {
+        // Radiance_Lambert (PathTracer.cpp:445-500), unnormalized basis
+        if (dot(em, em) == 0.0f) {
+          f3 w = nl;
+          f3 u = fabsf(nl.x) > kEps ? cross(mk(0.0f, 1.0f, 0.0f), w) : cross(mk(1.0f, 0.0f, 0.0f), w);
+          f3 vv = cross(w, u);
+          float u1 = rng_float(rng), u2 = rng_float(rng);
+          float s1, c1;
+          sincos_turns(u1, s1, c1);
+          float r2 = sqrtf(u2), r3 = sqrtf(1.0f - u2);
+          f3 nd = normalize(add(add(scl(scl(u, r3), c1), scl(scl(vv, r3), s1)), scl(w, r2)));
+          float k = 1.0f / prr;
+          if (nee && can_continue) {
+            float e1 = rng_float(rng), e2 = rng_float(rng), e3 = rng_float(rng), e4 = rng_float(rng);
+            f3 ld;
+            if (env_sample4(S, e1, e2, e3, e4, ld)) {
+              float c = dot(nl, ld);
+              float pe = env_pdf(S, ld);
+              if (c > 0.0f && pe > 0.0f) {
+                float pb = c * kInvPi;
+                float wl = (pe * pe) / (pe * pe + pb * pb);
+                float s = ((pb / pe) * wl) * k;
+                f3 L = env_rgb(S, ld);
+                f3 bc = mk(beta.x * col.x, beta.y * col.y, beta.z * col.z);
+                sh.c = mk(bc.x * L.x * s, bc.y * L.y * s, bc.z * L.z * s);
+                sh.o = x;
+                sh.d = ld;
+                sh.px = pixel;
+                shadow = true;
+              }
+            }
+          }
+          if (can_continue) {
+            beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
+            if (nee) {
+              mis_next = 1u;
+              pdf_next = dot(nl, nd) * kInvPi;
+            }
+            o = x;
+            d = nd;
+            depth++;
+            cont = true;
+          }
+        }
+      } else if (m.type == 1u) {
+        // Radiance_Specular (PathTracer.cpp:503-520)
+        f3 r = normalize(sub(d, scl(scl(nl, 2.0f), dot(d, nl))));
+        if (can_continue) {
+          float k = 1.0f / prr;
+          beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
+          o = x;
+          d = r;
+          depth++;
+          cont = true;
+        }
+      } else {
+        // Radiance_Refraction (PathTracer.cpp:523-591)
+        bool into = dot(nrm, nl) > 0.0f;
+        f3 refl = normalize(sub(d, scl(scl(nl, 2.0f), dot(d, nl))));
+        float nr = into ? 1.0f / m.ior : m.ior / 1.0f;
+        float dd = dot(d, nl);
+        float cos2t = 1.0f - nr * nr * (1.0f - dd * dd);
+        if (cos2t < 0.0f) {
+          if (can_continue) {
+            float k = 1.0f / prr;
+            beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
+            o = x;
+            d = refl;
+            depth++;
+            cont = true;
+          }
+        } else {
+          f3 refr = normalize(sub(scl(d, nr), scl(scl(nrm, into ? 1.0f : -1.0f), dd * nr + sqrtf(cos2t))));
+          float F0 = ((m.ior - 1.0f) * (m.ior - 1.0f)) / ((m.ior + 1.0f) * (m.ior + 1.0f));
+          float c = 1.0f - (into ? -dd : -dot(refr, nl));
+          float c5 = c * c * c * c * c;
+          float Fr = F0 + (1.0f - F0) * c5;
+          float Tr = (1.0f - Fr) * (nr * nr);
+          if (depth > 2u) {
+            float prob = 0.1f + 0.8f * Fr;
+            if (rng_float(rng) < prob) {
+              if (can_continue) {
+                float k = Fr / (prr * prob);
+                beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
+                o = x;
+                d = refl;
+                depth++;
+                cont = true;
+              }
+            } else if (can_continue) {
+              float k = Tr / (prr * (1.0f - prob));
+              beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
+              o = x;
+              d = refr;
+              depth++;
+              cont = true;
+            }
+          } else if (can_continue) {
+            // depth <= 2: both branches; the refracted one waits in the
+            // slot's pending stack, the reflected one continues now.
+            float kt = Tr / prr;
+            f3 bt = mk(beta.x * (col.x * kt), beta.y * (col.y * kt), beta.z * (col.z * kt));
+            float4 *pe = pend + ((size_t)npend * P + slot) * 3;
+            pe[0] = make_float4(x.x, x.y, x.z, bt.x);
+            pe[1] = make_float4(refr.x, refr.y, refr.z, bt.y);
+            pe[2] = make_float4(bt.z, __uint_as_float(depth + 1u), __uint_as_float(branch | (1u << depth)), 0.0f);
+            npend++;
+            float kr = Fr / prr;
+            beta = mk(beta.x * (col.x * kr), beta.y * (col.y * kr), beta.z * (col.z * kr));
+            o = x;
+            d = refl;
+            depth++;
+            cont = true;
+          }
+        }
+      }
+    }
+  }
+  if (cont) {
+    word = ps_pack(depth, branch, npend, mis_next);
+  } else if (npend > 0u) {
+    // resume the most recent pending branch with its own RNG stream
+    npend--;
+    const float4 *pe = pend + ((size_t)npend * P + slot) * 3;
+    float4 a = pe[0], b = pe[1], c = pe[2];
+    o = xyz(a);
+    d = xyz(b);
+    beta = mk(a.w, b.w, c.x);
+    uint32_t nd = __float_as_uint(c.y), nb = __float_as_uint(c.z);
+    Decomp dc = decompose(J, sidx);
+    rng = rng_init(path_seed(J.seed, dc.pixel, dc.sub, dc.sample_no, nb));
+    word = ps_pack(nd, nb, npend, 0u);
+    cont = true;
+  }
+  st.o = o;
+  st.d = d;
+  st.beta = beta;
+  st.rng = rng;
+  st.word = word;
+  st.pdf = pdf_next;
+  return cont;
+}
+
+// One shading step over the ray queue `parity`: shade each hit, then
+// regenerate finished slots (regen) and compact the continuing paths into the
+// other queue.
 template <int WG>
 __global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int regen) {
   __shared__ uint32_t s_pre[kShards + 1];
@@ -798,234 +1035,35 @@ __global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int rege
   const uint32_t n = s_pre[kShards];
   const uint32_t stride = gridDim.x * WG;
   const uint32_t qcap = A.p.qcap;
-  const uint32_t P = A.p.pool;
   float4 *ray_o = reinterpret_cast<float4 *>(A.p.ray_o);
   float4 *ray_d = reinterpret_cast<float4 *>(A.p.ray_d);
   uint4 *misc = reinterpret_cast<uint4 *>(A.p.misc);
   uint4 *rngs = reinterpret_cast<uint4 *>(A.p.rng);
-  float4 *pend = reinterpret_cast<float4 *>(A.p.pend);
   DoneAcc dacc;
   for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     bool valid = i < n;
     uint32_t slot = valid ? A.p.queue[parity][flat_slot(s_pre, i, qcap)] : 0u;
     bool cont = false, fin = false, shadow = false;
-    uint32_t js = 0;
-    f3 o = mk(0, 0, 0), d = mk(0, 0, 0), beta = mk(0, 0, 0);
-    uint32_t word = 0, sidx = 0;
-    uint4 rng = make_uint4(0, 0, 0, 0);
-    float pdf_next = 0.0f;
-    f3 sh_o = mk(0, 0, 0), sh_d = mk(0, 0, 0), sh_c = mk(0, 0, 0);
-    uint32_t sh_px = 0;
+    PathState st;
+    st.js = 0;
+    ShadowRay sh;
     if (valid) {
       float4 ro = ray_o[slot], rd = ray_d[slot];
       uint4 ms = misc[slot];
       float4 hr = reinterpret_cast<const float4 *>(A.p.hit)[slot];
-      rng = rngs[slot];
-      o = xyz(ro);
-      d = xyz(rd);
-      beta = mk(ro.w, rd.w, __uint_as_float(ms.x));
-      word = ms.y;
-      sidx = ms.z;
-      js = ms.w;
-      const JobDev &J = A.jobs[js];
-      const SceneDev &S = A.scenes[J.scene];
-      uint32_t depth = ps_depth(word), branch = ps_branch(word), npend = ps_pending(word);
-      bool env_on = S.env != nullptr;
-      bool nee = env_on && (J.flags & 1u) && S.env_marg[S.env_h] > 0ull;
-      uint32_t pixel = decompose(J, sidx).pixel;
-      unsigned long long *acc = J.accum + 3ull * pixel;
-      int32_t prim = __float_as_int(hr.y);
-      uint32_t mis_next = 0;
-      if (prim < 0) {
-        // PathTracer.cpp:357-363: miss -> IBL lookup or black background
-        if (env_on) {
-          f3 L = env_rgb(S, d);
-          float w = 1.0f;
-          if (ps_mis(word)) {
-            float pp = A.p.pdf[slot];
-            float pe = env_pdf(S, d);
-            w = (pp * pp) / (pp * pp + pe * pe);
-          }
-          add_px(acc, beta, L, w);
-        }
-      } else {
-        uint32_t pid = (uint32_t)prim;
-        f3 x = add(o, scl(d, hr.x));
-        f3 nrm;
-        if (pid < S.n_tri) {
-          const float4 *tn = reinterpret_cast<const float4 *>(S.tri_n) + (size_t)pid * 3;
-          float4 n0 = tn[0], n1 = tn[1], n2 = tn[2];
-          nrm = normalize(add(add(scl(xyz(n1), hr.z), scl(xyz(n2), hr.w)), xyz(n0)));
-        } else {
-          float4 sc = reinterpret_cast<const float4 *>(S.sph)[pid - S.n_tri];
-          nrm = normalize(sub(x, xyz(sc)));
-        }
-        f3 nl = dot(nrm, d) < 0.0f ? nrm : scl(nrm, -1.0f);
-        // material roulette (PathTracer.cpp:368-386)
-        uint32_t pm = S.prim_mat[pid];
-        uint32_t mfirst = pm & 0x00FFFFFFu, mcount = (pm >> 24) + 1u;
-        float value = rng_float(rng);
-        uint32_t sel = mcount - 1u;
-        for (uint32_t q = 0; q < mcount; q++) {
-          float r = S.mats[mfirst + q].rate;
-          if (value <= r) { sel = q; break; }
-          value -= r;
-        }
-        const MatRec &m = S.mats[mfirst + sel];
-        f3 col = mk(m.color[0], m.color[1], m.color[2]);
-        f3 em = mk(m.emission[0], m.emission[1], m.emission[2]);
-        // Russian roulette (PathTracer.cpp:398-416)
-        float prr = col.x > (col.y > col.z ? col.y : col.z) ? col.x : (col.y > col.z ? col.y : col.z);
-        if (J.max_depth == 0u && depth > 64u) prr *= ldexpf(1.0f, -(int)(depth - 64u));
-        bool killed = false;
-        if (depth > J.rr_depth) {
-          if (rng_float(rng) >= prr) killed = true;
-        } else {
-          prr = 1.0f;
-        }
-        add_px(acc, beta, em, 1.0f);
-        uint32_t dmax = J.max_depth == 0u ? 255u : J.max_depth;
-        bool can_continue = !killed && depth + 1u < dmax;
-        if (!killed) {
-          if (m.type == 0u) {
-            // Radiance_Lambert (PathTracer.cpp:445-500), unnormalized basis
-            if (dot(em, em) == 0.0f) {
-              f3 w = nl;
-              f3 u = fabsf(nl.x) > kEps ? cross(mk(0.0f, 1.0f, 0.0f), w) : cross(mk(1.0f, 0.0f, 0.0f), w);
-              f3 vv = cross(w, u);
-              float u1 = rng_float(rng), u2 = rng_float(rng);
-              float s1, c1;
-              sincos_turns(u1, s1, c1);
-              float r2 = sqrtf(u2), r3 = sqrtf(1.0f - u2);
-              f3 nd = normalize(add(add(scl(scl(u, r3), c1), scl(scl(vv, r3), s1)), scl(w, r2)));
-              float k = 1.0f / prr;
-              if (nee && can_continue) {
-                float e1 = rng_float(rng), e2 = rng_float(rng), e3 = rng_float(rng), e4 = rng_float(rng);
-                f3 ld;
-                if (env_sample4(S, e1, e2, e3, e4, ld)) {
-                  float c = dot(nl, ld);
-                  float pe = env_pdf(S, ld);
-                  if (c > 0.0f && pe > 0.0f) {
-                    float pb = c * kInvPi;
-                    float wl = (pe * pe) / (pe * pe + pb * pb);
-                    float s = ((pb / pe) * wl) * k;
-                    f3 L = env_rgb(S, ld);
-                    f3 bc = mk(beta.x * col.x, beta.y * col.y, beta.z * col.z);
-                    sh_c = mk(bc.x * L.x * s, bc.y * L.y * s, bc.z * L.z * s);
-                    sh_o = x;
-                    sh_d = ld;
-                    sh_px = pixel;
-                    shadow = true;
-                  }
-                }
-              }
-              if (can_continue) {
-                beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
-                if (nee) {
-                  mis_next = 1u;
-                  pdf_next = dot(nl, nd) * kInvPi;
-                }
-                o = x;
-                d = nd;
-                depth++;
-                cont = true;
-              }
-            }
-          } else if (m.type == 1u) {
-            // Radiance_Specular (PathTracer.cpp:503-520)
-            f3 r = normalize(sub(d, scl(scl(nl, 2.0f), dot(d, nl))));
-            if (can_continue) {
-              float k = 1.0f / prr;
-              beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
-              o = x;
-              d = r;
-              depth++;
-              cont = true;
-            }
-          } else {
-            // Radiance_Refraction (PathTracer.cpp:523-591)
-            bool into = dot(nrm, nl) > 0.0f;
-            f3 refl = normalize(sub(d, scl(scl(nl, 2.0f), dot(d, nl))));
-            float nr = into ? 1.0f / m.ior : m.ior / 1.0f;
-            float dd = dot(d, nl);
-            float cos2t = 1.0f - nr * nr * (1.0f - dd * dd);
-            if (cos2t < 0.0f) {
-              if (can_continue) {
-                float k = 1.0f / prr;
-                beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
-                o = x;
-                d = refl;
-                depth++;
-                cont = true;
-              }
-            } else {
-              f3 refr = normalize(sub(scl(d, nr), scl(scl(nrm, into ? 1.0f : -1.0f), dd * nr + sqrtf(cos2t))));
-              float F0 = ((m.ior - 1.0f) * (m.ior - 1.0f)) / ((m.ior + 1.0f) * (m.ior + 1.0f));
-              float c = 1.0f - (into ? -dd : -dot(refr, nl));
-              float c5 = c * c * c * c * c;
-              float Fr = F0 + (1.0f - F0) * c5;
-              float Tr = (1.0f - Fr) * (nr * nr);
-              if (depth > 2u) {
-                float prob = 0.1f + 0.8f * Fr;
-                if (rng_float(rng) < prob) {
-                  if (can_continue) {
-                    float k = Fr / (prr * prob);
-                    beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
-                    o = x;
-                    d = refl;
-                    depth++;
-                    cont = true;
-                  }
-                } else if (can_continue) {
-                  float k = Tr / (prr * (1.0f - prob));
-                  beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
-                  o = x;
-                  d = refr;
-                  depth++;
-                  cont = true;
-                }
-              } else if (can_continue) {
-                // depth <= 2: both branches; the refracted one waits in the
-                // slot's pending stack, the reflected one continues now.
-                float kt = Tr / prr;
-                f3 bt = mk(beta.x * (col.x * kt), beta.y * (col.y * kt), beta.z * (col.z * kt));
-                float4 *pe = pend + ((size_t)npend * P + slot) * 3;
-                pe[0] = make_float4(x.x, x.y, x.z, bt.x);
-                pe[1] = make_float4(refr.x, refr.y, refr.z, bt.y);
-                pe[2] = make_float4(bt.z, __uint_as_float(depth + 1u), __uint_as_float(branch | (1u << depth)), 0.0f);
-                npend++;
-                float kr = Fr / prr;
-                beta = mk(beta.x * (col.x * kr), beta.y * (col.y * kr), beta.z * (col.z * kr));
-                o = x;
-                d = refl;
-                depth++;
-                cont = true;
-              }
-            }
-          }
-        }
-      }
-      if (cont) {
-        word = ps_pack(depth, branch, npend, mis_next);
-      } else if (npend > 0u) {
-        // resume the most recent pending branch with its own RNG stream
-        npend--;
-        const float4 *pe = pend + ((size_t)npend * P + slot) * 3;
-        float4 a = pe[0], b = pe[1], c = pe[2];
-        o = xyz(a);
-        d = xyz(b);
-        beta = mk(a.w, b.w, c.x);
-        uint32_t nd = __float_as_uint(c.y), nb = __float_as_uint(c.z);
-        Decomp dc = decompose(J, sidx);
-        rng = rng_init(path_seed(J.seed, dc.pixel, dc.sub, dc.sample_no, nb));
-        word = ps_pack(nd, nb, npend, 0u);
-        cont = true;
-      } else {
-        fin = true;
-      }
+      st.rng = rngs[slot];
+      st.o = xyz(ro);
+      st.d = xyz(rd);
+      st.beta = mk(ro.w, rd.w, __uint_as_float(ms.x));
+      st.word = ms.y;
+      st.sidx = ms.z;
+      st.js = ms.w;
+      st.pdf = 0.0f;
+      cont = shade_segment(A, slot, st, hr, A.p.pdf, shadow, sh);
+      fin = !cont;
     }
-    count_done(A.done, dacc, fin, js);
+    count_done(A.done, dacc, fin, st.js);
     // regenerate finished slots while samples remain
     unsigned long long g = grab_sample(A.ctrl, fin && regen);
     bool fresh = g != ~0ull;
@@ -1035,11 +1073,11 @@ __global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int rege
       misc[slot].y = 0u;  // inactive
     }
     if (valid && cont) {
-      ray_o[slot] = make_float4(o.x, o.y, o.z, beta.x);
-      ray_d[slot] = make_float4(d.x, d.y, d.z, beta.y);
-      misc[slot] = make_uint4(__float_as_uint(beta.z), word, sidx, js);
-      rngs[slot] = rng;
-      if (ps_mis(word)) A.p.pdf[slot] = pdf_next;
+      ray_o[slot] = make_float4(st.o.x, st.o.y, st.o.z, st.beta.x);
+      ray_d[slot] = make_float4(st.d.x, st.d.y, st.d.z, st.beta.y);
+      misc[slot] = make_uint4(__float_as_uint(st.beta.z), st.word, st.sidx, st.js);
+      rngs[slot] = st.rng;
+      if (ps_mis(st.word)) A.p.pdf[slot] = st.pdf;
     }
     bool enq = valid && (cont || fresh);
     const uint32_t k = wave_shard();
@@ -1048,9 +1086,9 @@ __global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int rege
     uint32_t spos = wave_append(&A.ctrl->scount[k], shadow);
     if (shadow) {
       float4 *sq = reinterpret_cast<float4 *>(A.p.shadow) + ((size_t)k * qcap + spos) * 3;
-      sq[0] = make_float4(sh_o.x, sh_o.y, sh_o.z, __uint_as_float(js));
-      sq[1] = make_float4(sh_d.x, sh_d.y, sh_d.z, __uint_as_float(sh_px));
-      sq[2] = make_float4(sh_c.x, sh_c.y, sh_c.z, 0.0f);
+      sq[0] = make_float4(sh.o.x, sh.o.y, sh.o.z, __uint_as_float(st.js));
+      sq[1] = make_float4(sh.d.x, sh.d.y, sh.d.z, __uint_as_float(sh.px));
+      sq[2] = make_float4(sh.c.x, sh.c.y, sh.c.z, 0.0f);
     }
   }
   done_flush(A.done, dacc);
@@ -1120,6 +1158,101 @@ __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
     }
   }
   wave_count(A.ctrl->shadow_rays, cnt);
+}
+
+// Drain: once every admitted sample has been handed out and the ray queue
+// `parity` is short, each lane carries its paths to the end -- closest hit,
+// shading, environment visibility, pending glass branches -- in one launch,
+// instead of one extend/shade pair per remaining bounce, each as long as its
+// slowest ray.  Same per-path arithmetic and RNG streams as the step
+// kernels, and integer accumulation, so the result bits are the same.  The
+// queues are left stale: the run ends after a drain.
+template <int WG, int SHORT>
+__global__ __launch_bounds__(WG) void k_drain(KernelArgs A, int parity) {
+  __shared__ uint2 lds[SHORT * WG];
+  __shared__ uint32_t s_pre[kShards + 1];
+  const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
+  load_prefix(s_pre, A.ctrl->qcount[parity]);
+  const uint32_t n = s_pre[kShards];
+  const uint32_t stride = gridDim.x * WG;
+  const uint32_t qcap = A.p.qcap;
+  Stack<WG, SHORT> stk;
+  stk.lds = lds;
+  stk.g = A.p.gstack + (size_t)gtid * A.p.gcap;
+  const float4 *ray_o = reinterpret_cast<const float4 *>(A.p.ray_o);
+  const float4 *ray_d = reinterpret_cast<const float4 *>(A.p.ray_d);
+  uint4 *misc = reinterpret_cast<uint4 *>(A.p.misc);
+  const uint4 *rngs = reinterpret_cast<const uint4 *>(A.p.rng);
+  uint32_t segs = 0, shadows = 0, nv = 0, nt = 0;  // node/primitive statistics cover extend only
+  uint32_t djob = ~0u, dcnt = 0;  // this lane's completions of job djob not yet added
+  for (uint32_t i = gtid; i < n; i += stride) {
+    uint32_t slot = A.p.queue[parity][flat_slot(s_pre, i, qcap)];
+    float4 ro = ray_o[slot], rd = ray_d[slot];
+    uint4 ms = misc[slot];
+    PathState st;
+    st.rng = rngs[slot];
+    st.o = xyz(ro);
+    st.d = xyz(rd);
+    st.beta = mk(ro.w, rd.w, __uint_as_float(ms.x));
+    st.word = ms.y;
+    st.sidx = ms.z;
+    st.js = ms.w;
+    st.pdf = ps_mis(st.word) ? A.p.pdf[slot] : 0.0f;
+    const JobDev &J = A.jobs[st.js];
+    const SceneDev &S = A.scenes[J.scene];
+    for (;;) {
+      Hit h;
+      h.t = kInf;
+      h.u = h.v = 0.0f;
+      h.prim = -1;
+      traverse<WG, SHORT, false, false>(S, st.o, st.d, h, stk, nv, nt);
+      segs++;
+      bool shadow = false;
+      ShadowRay sh;
+      bool cont = shade_segment(A, slot, st, make_float4(h.t, __int_as_float(h.prim), h.u, h.v), nullptr, shadow, sh);
+      if (shadow) {
+        Hit hs;
+        hs.t = kInf;
+        hs.u = hs.v = 0.0f;
+        hs.prim = -1;
+        uint32_t snv = 0, snt = 0;
+        traverse<WG, SHORT, true, false>(S, sh.o, sh.d, hs, stk, snv, snt);
+        shadows++;
+        if (hs.prim < 0) {
+          unsigned long long *acc = J.accum + 3ull * sh.px;
+          unsigned long long qa = fix(sh.c.x), qb = fix(sh.c.y), qc = fix(sh.c.z);
+          if (qa) atomicAdd(acc + 0, qa);
+          if (qb) atomicAdd(acc + 1, qb);
+          if (qc) atomicAdd(acc + 2, qc);
+        }
+      }
+      if (!cont) break;
+    }
+    misc[slot].y = 0u;  // inactive
+    if (st.js != djob) {
+      if (dcnt) atomicAdd(&A.done[done_index(wave_shard(), djob)], (unsigned long long)dcnt);
+      djob = st.js;
+      dcnt = 0;
+    }
+    dcnt++;
+  }
+  // the remaining completion counts, one atomic per job present in the wave
+  bool need = dcnt != 0u;
+  for (;;) {
+    unsigned long long m = __ballot(need);
+    if (m == 0ull) break;
+    uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
+    uint32_t lj = __shfl(djob, (int)leader);
+    bool mine = need && djob == lj;
+    unsigned long long c = mine ? dcnt : 0u;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
+    if (lane_id() == leader) atomicAdd(&A.done[done_index(wave_shard(), lj)], c);
+    if (mine) need = false;
+  }
+  wave_count(A.ctrl->segments, segs);
+  wave_count(A.ctrl->drain_segments, segs);
+  wave_count(A.ctrl->shadow_rays, shadows);
 }
 
 // Clears the shadow sub-queue counts and fetch heads at the start of a step.
@@ -1306,6 +1439,15 @@ hipError_t launch_shadow_wg(const KernelArgs &a, uint32_t grid, hipStream_t st) 
   return launch_shadow_ws<WG, (int)kShortStackMax>(a, grid, st);
 }
 
+template <int WG>
+hipError_t launch_drain_wg(const KernelArgs &a, uint32_t grid, int parity, hipStream_t st) {
+  if (a.p.stack_short == kShortStackMin)
+    hipLaunchKernelGGL((k_drain<WG, (int)kShortStackMin>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else
+    hipLaunchKernelGGL((k_drain<WG, (int)kShortStackMax>), dim3(grid), dim3(WG), 0, st, a, parity);
+  return hipGetLastError();
+}
+
 template <int WG, int SHORT, bool STATS>
 hipError_t launch_trace_ws(const SceneDev *scene, const float4 *r4, uint32_t n, int32_t *prim, float *t,
                            unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t grid,
@@ -1366,6 +1508,14 @@ hipError_t shadow(const KernelArgs &a, const LaunchShape &s, hipStream_t st) {
     case 64: return launch_shadow_wg<64>(a, s.grid, st);
     case 128: return launch_shadow_wg<128>(a, s.grid, st);
     default: return launch_shadow_wg<256>(a, s.grid, st);
+  }
+}
+
+hipError_t drain(const KernelArgs &a, const LaunchShape &s, int parity, hipStream_t st) {
+  switch (s.wg) {
+    case 64: return launch_drain_wg<64>(a, s.grid, parity, st);
+    case 128: return launch_drain_wg<128>(a, s.grid, parity, st);
+    default: return launch_drain_wg<256>(a, s.grid, parity, st);
   }
 }
 

@@ -127,6 +127,7 @@ struct Ctrl {
   Sharded<unsigned long long> shadow_rays;  // environment shadow rays traced
   Sharded<unsigned long long> node_visits;  // stats kernels only
   Sharded<unsigned long long> prim_tests;   // stats kernels only
+  Sharded<unsigned long long> drain_segments;  // closest-hit queries traced by the drain kernel
   Sharded<uint32_t> qcount[2];              // ping-pong ray sub-queue counts
   Sharded<uint32_t> scount;                 // shadow sub-queue counts
   Sharded<uint32_t> qhead[2];               // dynamic-fetch heads of the ray sub-queues

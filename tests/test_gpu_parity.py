@@ -48,7 +48,8 @@ def test_env_tables_bit_exact(ctx):
     ctx.release(sid)
 
 
-@pytest.mark.parametrize("env_is,tune", [(0, {}), (1, {}), (1, dict(stack_short=8, fetch_budget=8))])
+@pytest.mark.parametrize("env_is,tune", [(0, {}), (1, {}), (1, dict(stack_short=8, fetch_budget=8)),
+                                         (1, dict(drain_below=1 << 20))])
 def test_render_env_bit_exact(ctx, env_is, tune):
     """Torus knot under an HDR sky: lookup on escape, and with importance
     sampling + shadow rays + MIS (also with the 8-entry LDS stack and
@@ -69,6 +70,8 @@ def test_render_env_bit_exact(ctx, env_is, tune):
         assert not (g != c).any(axis=2).any(), f"{(g != c).any(axis=2).sum()} pixels differ"
         assert st["segments"] == seg and st["shadow_rays"] == sh
         assert (sh > 0) == bool(env_is)
+        if tune.get("drain_below"):
+            assert st["drained_paths"] > 0
         ctx.release(sid)
     finally:
         ctx.set_tuning(**saved)
@@ -119,11 +122,13 @@ def test_render_bit_exact(ctx, kind, ss, spp, depth, rr):
                                   dict(fetch_budget=4, fetch_min=32, refill_every=1, use_graph=0, wg_size=256),
                                   dict(wg_size=64, blocks_per_cu=2, pool_paths=4096, steps_per_sync=4),
                                   dict(stack_short=8), dict(stack_short=8, fetch_budget=8, wg_size=64,
-                                                            blocks_per_cu=16)])
+                                                            blocks_per_cu=16),
+                                  dict(drain_below=1 << 20), dict(drain_below=2000, pool_paths=8192)])
 def test_tuning_modes_bit_exact(ctx, tune):
     """Every launch configuration the autotuner can pick gives the same bits:
     dynamic-fetch traversal, bulk refill cadence, graph off, small pools,
-    the 8-entry LDS stack (more entries spill to global memory)."""
+    the 8-entry LDS stack (more entries spill to global memory), the drain
+    launch that finishes the last paths in one go."""
     import omr
     import pyoracle
 
@@ -142,7 +147,10 @@ def test_tuning_modes_bit_exact(ctx, tune):
         g = ctx.render_sums(sid, scene.camera, prm)
         c, seg, _ = o.render_sums(scene.camera, prm, pyoracle.WIDE_BVH)
         assert not (g != c).any(axis=2).any()
-        assert ctx.last_stats()["segments"] == seg
+        st = ctx.last_stats()
+        assert st["segments"] == seg
+        if tune.get("drain_below"):
+            assert st["drained_paths"] > 0 and 0 < st["drain_segments"] <= seg
         ctx.release(sid)
     finally:
         ctx.set_tuning(**saved)
