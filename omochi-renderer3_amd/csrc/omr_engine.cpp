@@ -114,7 +114,6 @@ int Engine::set_tuning(const omr_tuning &t) {
   if (t.fetch_budget && (t.fetch_min < 1 || t.fetch_min > 64)) return fail(OMR_ERR_INVALID, "fetch_min must be 1..64");
   if (t.stack_short != kShortStackMin && t.stack_short != kShortStackMax)
     return fail(OMR_ERR_INVALID, "stack_short must be 8 or 16");
-  if (t.drain_below > t.pool_paths) return fail(OMR_ERR_INVALID, "drain_below above pool_paths");
   std::lock_guard<std::mutex> g(mu);
   hipSetDevice(device_);
   hipStreamSynchronize(stream_);
