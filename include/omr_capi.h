@@ -55,7 +55,7 @@ typedef struct {
 
 /* Scene contents handed to omr_scene_upload; replaces the object lists of
  * Scene (src/scenes/Scene.h:112-116) built by AddObject / AddModel.
- * Primitive ids: triangles 0..n_triangles-1, then spheres. */
+ * Primitive ids: triangles 0..n_triangles-1, then spheres, then planes. */
 typedef struct {
   size_t n_triangles;
   const float *triangles;        /* n x 9: v0.xyz, e1.xyz = v1-v0, e2.xyz = v2-v0 (Polygon.h:103-105,112) */
@@ -68,6 +68,10 @@ typedef struct {
   const omr_material *materials;
   uint32_t env_width, env_height;  /* latlong environment (IBL, src/renderer/IBL.h:14-45) */
   const float *env_rgb;            /* env_height rows x env_width x 3, row 0 = +y; NULL = black */
+  size_t n_planes;                 /* infinite floors y = planes[i], two-sided, outside the BVH and
+                                      tested after it (InfiniteFloor, src/renderer/InfiniteFloor.h:10-45;
+                                      Scene::CheckIntersection's not-in-BVH list, src/scenes/Scene.cpp:72-80) */
+  const float *planes;
 } omr_scene_desc;
 
 /* Camera (src/renderer/Camera.h:12-43) in its derived form; the drop-in
@@ -243,6 +247,38 @@ void omr_procedural_free(omr_procedural *p);
 /* The camera preset of a procedural kind at another image size (kind 4 is a
  * 2000 x 1000 terrain, 3,996,002 triangles, with its own sky). */
 int omr_procedural_camera(int kind, uint32_t width, uint32_t height, omr_camera *out);
+
+/* ---- file formats (host only: no context or GPU needed).  Errors:
+ * omr_io_last_error() of the calling thread. ---- */
+const char *omr_io_last_error(void);
+/* Releases memory returned by the loaders below. */
+void omr_free(void *p);
+/* HDRImage::ReadFromRadianceFile (src/tools/HDRImage.cpp:12-27, 51-186):
+ * Radiance RGBE, run-length encoded or flat, into *rgb = height rows x width
+ * x 3 floats (the file's first scanline first).  Release with omr_free. */
+int omr_hdr_read(const char *path, uint32_t *width, uint32_t *height, float **rgb);
+/* HDRImage::WriteToRadianceFile (src/tools/HDRImage.cpp:29-49, 188-314):
+ * RGBE with run-length encoded scanlines, byte for byte as the reference
+ * writes them. */
+int omr_hdr_write(const char *path, uint32_t width, uint32_t height, const float *rgb);
+/* A scene read from a file: desc points into memory the handle owns (until
+ * omr_loaded_scene_free).  Texture references are read but not used: the
+ * shading path takes material colours. */
+typedef struct {
+  omr_scene_desc desc;
+  size_t n_objects;               /* objects of the scene file (a mesh counts its polygons) */
+  void *internal;
+} omr_loaded_scene;
+/* SceneFromExternalFile (src/scenes/SceneFromExternalFile.cpp:46-531): the
+ * reference's .scene text format, with meshes (Wavefront OBJ/MTL), spheres,
+ * sphere lights, floors, infinite floors and a Radiance IBL. */
+int omr_scene_file_load(const char *path, omr_loaded_scene *out);
+/* Model::ReadFromObj + Model::Transform (src/renderer/Model.cpp:50-182,
+ * 38-48): one OBJ mesh placed by translation, per-axis scale and rotation
+ * (radians, applied x, then y, then z as in SceneFromExternalFile.cpp:523-526). */
+int omr_obj_load(const char *path, const double position[3], const double scale[3], const double rotation[3],
+                 int flip_v, omr_loaded_scene *out);
+void omr_loaded_scene_free(omr_loaded_scene *s);
 
 /* Reference Camera constructor (src/renderer/Camera.h:12-43), double precision:
  * fills the derived screen vectors of an omr_camera. */

@@ -215,13 +215,17 @@ SceneHost *Engine::scene(omr_scene_id id) {
 
 int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
   if (!d || !id) return fail(OMR_ERR_INVALID, "null argument");
-  size_t nt = d->n_triangles, ns = d->n_spheres, n = nt + ns;
-  if (n == 0) return fail(OMR_ERR_INVALID, "scene has no primitives");
+  size_t nt = d->n_triangles, ns = d->n_spheres, n = nt + ns, npl = d->n_planes, nall = n + npl;
+  if (n == 0) return fail(OMR_ERR_INVALID, "scene has no triangles or spheres");
   if (nt && !d->triangles) return fail(OMR_ERR_INVALID, "triangles is NULL");
   if (ns && !d->spheres) return fail(OMR_ERR_INVALID, "spheres is NULL");
+  if (npl && !d->planes) return fail(OMR_ERR_INVALID, "planes is NULL");
+  if (npl > 64) return fail(OMR_ERR_INVALID, "too many planes (max 64)");
+  for (size_t i = 0; i < npl; i++)
+    if (!std::isfinite(d->planes[i])) return fail(OMR_ERR_INVALID, "plane " + std::to_string(i) + " is not finite");
   if (!d->prim_material || !d->materials || d->n_materials == 0) return fail(OMR_ERR_INVALID, "materials missing");
   if (d->n_materials > 0x00FFFFFFu) return fail(OMR_ERR_INVALID, "too many materials");
-  if (n > 0xFFFFFFFFull) return fail(OMR_ERR_INVALID, "too many primitives");
+  if (nall > 0x7FFFFFFFull) return fail(OMR_ERR_INVALID, "too many primitives");
   for (size_t i = 0; i < d->n_materials; i++) {
     const omr_material &m = d->materials[i];
     if (m.type > 2) return fail(OMR_ERR_INVALID, "material " + std::to_string(i) + ": bad type");
@@ -230,8 +234,8 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
         return fail(OMR_ERR_INVALID, "material " + std::to_string(i) + ": bad color/emission");
     if (m.type == 2 && !(m.ior > 0.0f)) return fail(OMR_ERR_INVALID, "material " + std::to_string(i) + ": bad ior");
   }
-  std::vector<uint32_t> pm(n);
-  for (size_t i = 0; i < n; i++) {
+  std::vector<uint32_t> pm(nall);
+  for (size_t i = 0; i < nall; i++) {
     uint32_t f = d->prim_material[i];
     uint32_t c = d->prim_material_count ? d->prim_material_count[i] : 1u;
     if (c < 1 || c > 256 || (size_t)f + c > d->n_materials)
@@ -290,13 +294,14 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
     if (src) OMR_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
     return OMR_OK;
   };
-  void *p_nodes, *p_prims, *p_tn = nullptr, *p_sph = nullptr, *p_pm, *p_mats;
+  void *p_nodes, *p_prims, *p_tn = nullptr, *p_sph = nullptr, *p_pm, *p_mats, *p_pl = nullptr;
   int rc;
   if ((rc = dup(S.bvh.nodes.data(), S.bvh.nodes.size() * sizeof(WNode4), &p_nodes)) ||
       (rc = dup(S.bvh.prims.data(), S.bvh.prims.size() * sizeof(PrimRec), &p_prims)) ||
       (rc = dup(tn.data(), tn.size() * sizeof(float), &p_tn)) ||
-      (rc = dup(d->spheres, ns * 4 * sizeof(float), &p_sph)) || (rc = dup(pm.data(), n * 4, &p_pm)) ||
-      (rc = dup(mats.data(), mats.size() * sizeof(MatRec), &p_mats))) {
+      (rc = dup(d->spheres, ns * 4 * sizeof(float), &p_sph)) || (rc = dup(pm.data(), nall * 4, &p_pm)) ||
+      (rc = dup(mats.data(), mats.size() * sizeof(MatRec), &p_mats)) ||
+      (rc = dup(d->planes, npl * sizeof(float), &p_pl))) {
     free_scene_mem(S);
     return rc;
   }
@@ -309,6 +314,8 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
   D.mats = (const MatRec *)p_mats;
   D.n_tri = (uint32_t)nt;
   D.n_sph = (uint32_t)ns;
+  D.planes = (const float *)p_pl;
+  D.n_planes = (uint32_t)npl;
   D.n_nodes = (uint32_t)S.bvh.nodes.size();
   D.stack_bound = S.bvh.stack_bound;
   if (d->env_rgb) {

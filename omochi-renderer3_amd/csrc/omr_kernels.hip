@@ -222,6 +222,16 @@ __device__ inline bool hit_sphere(f3 x, f3 v, f3 c, float r, float &t_out) {
   return true;
 }
 
+// InfiniteFloor::CheckIntersection (src/renderer/InfiniteFloor.h:20-38):
+// the plane y = const, both sides.
+__device__ inline bool hit_plane(f3 o, f3 d, float y, float &t_out) {
+  if (fabsf(d.y) < kEps) return false;
+  float t = (y - o.y) / d.y;
+  if (t < kEps) return false;
+  t_out = t;
+  return true;
+}
+
 // ------------------------------------------------------------ traversal
 struct Hit {
   float t, u, v;
@@ -276,10 +286,23 @@ struct Trav {
   int32_t cur;
   Hit best;   // u, v unnormalized until the traversal ends
   float det;  // divisor of best.u, best.v (1 for spheres)
+  const float *planes;  // infinite floors, ids plane_base + k, tested after the BVH
+  uint32_t n_planes, plane_base;
 
   __device__ inline void finish() {
     best.u = best.u / det;
     best.v = best.v / det;
+    // objects outside the BVH come after it (Scene.cpp:72-80)
+    for (uint32_t k = 0; k < n_planes; k++) {
+      float t;
+      if (!hit_plane(o, d, planes[k], t)) continue;
+      const int32_t id = (int32_t)(plane_base + k);
+      if (t < best.t || (t == best.t && best.prim >= 0 && id < best.prim)) {
+        best.t = t;
+        best.u = best.v = 0.0f;
+        best.prim = id;
+      }
+    }
   }
 
   __device__ inline void init(const SceneDev &S, f3 o_, f3 d_, float tmax, Stack<WG, SHORT> &stk) {
@@ -294,6 +317,9 @@ struct Trav {
     nz = !(d.z >= 0.0f) ? 1u : 0u;
     nodes = reinterpret_cast<const float4 *>(S.nodes);
     prims = reinterpret_cast<const float4 *>(S.prims);
+    planes = S.planes;
+    n_planes = S.n_planes;
+    plane_base = S.n_tri + S.n_sph;
     best.t = tmax;
     best.u = best.v = 0.0f;
     best.prim = -1;
@@ -851,9 +877,11 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
       const float4 *tn = reinterpret_cast<const float4 *>(S.tri_n) + (size_t)pid * 3;
       float4 n0 = tn[0], n1 = tn[1], n2 = tn[2];
       nrm = normalize(add(add(scl(xyz(n1), hr.z), scl(xyz(n2), hr.w)), xyz(n0)));
-    } else {
+    } else if (pid < S.n_tri + S.n_sph) {
       float4 sc = reinterpret_cast<const float4 *>(S.sph)[pid - S.n_tri];
       nrm = normalize(sub(x, xyz(sc)));
+    } else {
+      nrm = mk(0.0f, d.y < 0.0f ? 1.0f : -1.0f, 0.0f);  // InfiniteFloor.h:34-36: facing the ray
     }
     f3 nl = dot(nrm, d) < 0.0f ? nrm : scl(nrm, -1.0f);
     // material roulette (PathTracer.cpp:368-386)

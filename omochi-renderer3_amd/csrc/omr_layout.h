@@ -74,6 +74,8 @@ struct SceneDev {
   const unsigned long long *env_row;   // env_h x (env_w+1)
   const unsigned long long *env_marg;  // env_h+1
   uint32_t n_tri, n_sph;
+  const float *planes;         // infinite floors y = planes[i]; ids n_tri + n_sph + i
+  uint32_t n_planes;
   uint32_t env_w, env_h;
   uint32_t n_nodes;
   uint32_t stack_bound;        // max traversal stack entries

@@ -43,13 +43,21 @@ Polygon::Polygon(const Vector3 &pos1, const Vector3 &pos2, const Vector3 &pos3, 
   position_ = pos;
 }
 
-void Polygon::Transform(const Vector3 &pos, const Vector3 &scale) {
+void Polygon::Transform(const Vector3 &pos, const Vector3 &scale, const Matrix &rot) {
   position_ = pos;
   for (int i = 0; i < 3; i++) {
+    m_posAndEdges[i] = rot.Apply(m_posAndEdges[i]);
     m_posAndEdges[i].x *= scale.x;
     m_posAndEdges[i].y *= scale.y;
     m_posAndEdges[i].z *= scale.z;
+    m_normalAndDiffs[i] = rot.Apply(m_normalAndDiffs[i]);
   }
+}
+
+Vector3 Polygon::CalculateNormal(const Vector3 &v0, const Vector3 &v1, const Vector3 &v2) {
+  Vector3 n((v1 - v0).cross(v2 - v0));
+  n.normalize();
+  return n;
 }
 
 Camera::Camera(size_t w, size_t h, const Vector3 &pos, const Vector3 &dir, const Vector3 &up, double screenH,
@@ -105,14 +113,33 @@ void Scene::ConstructQBVH() {
   if (c) Upload(c);
 }
 
-omr_scene_id Scene::Upload(omr_ctx *ctx) const {
-  if (m_ctx == ctx && m_id >= 0) return m_id;
-  std::vector<float> tri, nrm, sph;
-  std::vector<uint32_t> pm_tri, pc_tri, pm_sph, pc_sph;
-  std::vector<omr_material> mats;
-  std::vector<SceneObject *> tris, sphs;
+void Scene::AddModel(Model *model, bool doDelete, bool containedInBVH) {
+  // every polygon, material by material (Scene.h:78-88); a material of the
+  // same value as an earlier one shares its list and is added once
+  std::vector<const Model::PolygonList *> seen;
+  for (size_t i = 0; i < model->GetMaterialCount(); i++) {
+    const Model::PolygonList &pl = model->GetPolygonList(model->GetMaterial(i));
+    bool dup = false;
+    for (auto *q : seen) dup = dup || q == &pl;
+    if (dup) continue;
+    seen.push_back(&pl);
+    for (Polygon *p : pl) AddObject(p, false, containedInBVH);
+  }
+  if (doDelete) m_models.emplace_back(model);
+}
+
+SceneObject *Scene::AddInfiniteFLoor(double y, const Material &material) {
+  SceneObject *o = new InfiniteFloor(y, material);
+  AddObject(o, true, false);
+  return o;
+}
+
+void Scene::Describe(SceneArrays &a) const {
+  a = SceneArrays();
+  std::vector<uint32_t> pm_sph, pc_sph, pm_pln, pc_pln;
+  std::vector<SceneObject *> sphs, plns;
   auto add_mats = [&](const SceneObject *o, std::vector<uint32_t> &pm, std::vector<uint32_t> &pc) {
-    pm.push_back((uint32_t)mats.size());
+    pm.push_back((uint32_t)a.mats.size());
     pc.push_back((uint32_t)o->GetMaterialCount());
     for (size_t i = 0; i < o->GetMaterialCount(); i++) {
       const SceneObject::MultiMaterialElem *e = o->GetMaterial(i);
@@ -124,7 +151,7 @@ omr_scene_id Scene::Upload(omr_ctx *ctx) const {
       m.emission[0] = (float)em.x; m.emission[1] = (float)em.y; m.emission[2] = (float)em.z;
       m.ior = (float)e->material_.refraction_rate;
       m.rate = (float)e->rate_;
-      mats.push_back(m);
+      a.mats.push_back(m);
     }
   };
   for (auto &ow : m_objects) {
@@ -137,44 +164,58 @@ omr_scene_id Scene::Upload(omr_ctx *ctx) const {
                      (float)pe[2].x, (float)pe[2].y, (float)pe[2].z};
       float n9[9] = {(float)nd[0].x, (float)nd[0].y, (float)nd[0].z, (float)nd[1].x, (float)nd[1].y, (float)nd[1].z,
                      (float)nd[2].x, (float)nd[2].y, (float)nd[2].z};
-      tri.insert(tri.end(), t9, t9 + 9);
-      nrm.insert(nrm.end(), n9, n9 + 9);
-      add_mats(o, pm_tri, pc_tri);
-      tris.push_back(o);
+      a.tri.insert(a.tri.end(), t9, t9 + 9);
+      a.nrm.insert(a.nrm.end(), n9, n9 + 9);
+      add_mats(o, a.prim_mat, a.prim_mat_count);
+      a.prims.push_back(o);
     } else if (auto *s = dynamic_cast<Sphere *>(o)) {
       const Vector3 &c = s->GetPosition();
       float s4[4] = {(float)c.x, (float)c.y, (float)c.z, (float)s->GetRadius()};
-      sph.insert(sph.end(), s4, s4 + 4);
+      a.sph.insert(a.sph.end(), s4, s4 + 4);
       add_mats(o, pm_sph, pc_sph);
       sphs.push_back(o);
+    } else if (auto *f = dynamic_cast<InfiniteFloor *>(o)) {
+      a.planes.push_back((float)f->GetY());
+      add_mats(o, pm_pln, pc_pln);
+      plns.push_back(o);
     }
   }
-  std::vector<uint32_t> pm(pm_tri), pc(pc_tri);
-  pm.insert(pm.end(), pm_sph.begin(), pm_sph.end());
-  pc.insert(pc.end(), pc_sph.begin(), pc_sph.end());
-  omr_scene_desc d;
+  a.prim_mat.insert(a.prim_mat.end(), pm_sph.begin(), pm_sph.end());
+  a.prim_mat_count.insert(a.prim_mat_count.end(), pc_sph.begin(), pc_sph.end());
+  a.prim_mat.insert(a.prim_mat.end(), pm_pln.begin(), pm_pln.end());
+  a.prim_mat_count.insert(a.prim_mat_count.end(), pc_pln.begin(), pc_pln.end());
+  a.prims.insert(a.prims.end(), sphs.begin(), sphs.end());
+  a.prims.insert(a.prims.end(), plns.begin(), plns.end());
+  omr_scene_desc &d = a.desc;
   std::memset(&d, 0, sizeof(d));
-  d.n_triangles = tris.size();
-  d.triangles = tri.data();
-  d.tri_normals = nrm.data();
-  d.n_spheres = sphs.size();
-  d.spheres = sph.data();
-  d.prim_material = pm.data();
-  d.prim_material_count = pc.data();
-  d.n_materials = mats.size();
-  d.materials = mats.data();
-  if (m_ibl) {
+  d.n_triangles = a.tri.size() / 9;
+  d.triangles = a.tri.data();
+  d.tri_normals = a.nrm.data();
+  d.n_spheres = a.sph.size() / 4;
+  d.spheres = a.sph.data();
+  d.n_planes = a.planes.size();
+  d.planes = a.planes.data();
+  d.prim_material = a.prim_mat.data();
+  d.prim_material_count = a.prim_mat_count.data();
+  d.n_materials = a.mats.size();
+  d.materials = a.mats.data();
+  if (m_ibl && m_ibl->IsValid()) {
     d.env_width = m_ibl->Width();
     d.env_height = m_ibl->Height();
     d.env_rgb = m_ibl->Pixels().data();
   }
+}
+
+omr_scene_id Scene::Upload(omr_ctx *ctx) const {
+  if (m_ctx == ctx && m_id >= 0) return m_id;
+  SceneArrays a;
+  Describe(a);
   omr_scene_id id = -1;
-  if (omr_scene_upload(ctx, &d, &id) != OMR_OK) {
+  if (omr_scene_upload(ctx, &a.desc, &id) != OMR_OK) {
     std::cerr << "scene upload failed: " << omr_last_error(ctx) << std::endl;
     return -1;
   }
-  m_prims = tris;
-  m_prims.insert(m_prims.end(), sphs.begin(), sphs.end());
+  m_prims = a.prims;
   m_ctx = ctx;
   m_id = id;
   return id;

@@ -14,6 +14,12 @@
 //   ScreenPixels           src/renderer/ScreenPixels.hpp / .cpp (2W x H, row-flipped)
 //   Renderer, PathTracer   src/renderer/Renderer.h:10-23, src/renderer/PathTracer.h:18-88
 //   SceneFactory(Manager)  src/scenes/SceneFactory.h:7-48
+//   Matrix                 src/tools/Matrix.h:8-65 (rotations for mesh placement)
+//   HDRImage               src/tools/HDRImage.h:8-81, HDRImage.cpp:12-314 (Radiance RGBE)
+//   ImageHandler           src/tools/ImageHandler.h (texture registry; ids only)
+//   InfiniteFloor          src/renderer/InfiniteFloor.h:10-45
+//   Model                  src/renderer/Model.h:13-63, Model.cpp:14-367 (Wavefront OBJ/MTL)
+//   SceneFromExternalFile  src/scenes/SceneFromExternalFile.h:14-52, .cpp:13-531 (.scene)
 // The path tracing itself runs on the GPU; Scene::CheckIntersection runs the
 // device closest-hit kernel for single rays (slow; for compatibility).
 #pragma once
@@ -60,16 +66,51 @@ class Ray {
   Vector3 orig, dir;
 };
 
+// Texture registry (ImageHandler, src/tools/ImageHandler.h): file name -> id.
+// Ids only: texture lookups are not part of the shading path.
+class ImageHandler {
+ public:
+  typedef int IMAGE_ID;
+  static const IMAGE_ID INVALID_IMAGE_ID = -1;
+  static ImageHandler &GetInstance();
+  IMAGE_ID LoadFromFile(const std::string &file);
+  const std::string &GetFileName(IMAGE_ID id) const;
+  size_t GetImageCount() const { return m_files.size(); }
+
+ private:
+  std::vector<std::string> m_files;
+};
+
 class Material {
  public:
   enum REFLECTION_TYPE { REFLECTION_TYPE_LAMBERT, REFLECTION_TYPE_SPECULAR, REFLECTION_TYPE_REFRACTION };
   Material(REFLECTION_TYPE type = REFLECTION_TYPE_LAMBERT, const Vector3 &emission_ = Vector3(),
-           const Vector3 &color_ = Vector3(), double refraction_rate_ = 0.0)
-      : reflection_type(type), emission(emission_), color(color_), refraction_rate(refraction_rate_) {}
+           const Vector3 &color_ = Vector3(), double refraction_rate_ = 0.0,
+           ImageHandler::IMAGE_ID texture_id_ = ImageHandler::INVALID_IMAGE_ID)
+      : reflection_type(type), emission(emission_), color(color_), refraction_rate(refraction_rate_),
+        texture_id(texture_id_) {}
+  bool operator==(const Material &m) const {
+    return reflection_type == m.reflection_type && emission == m.emission && color == m.color &&
+           refraction_rate == m.refraction_rate && texture_id == m.texture_id;
+  }
   REFLECTION_TYPE reflection_type;
   Color emission;
   Color color;
   double refraction_rate;
+  ImageHandler::IMAGE_ID texture_id;
+};
+
+// 4x4 matrix of which the upper 3x3 rotates (Matrix, src/tools/Matrix.h:8-65).
+class Matrix {
+ public:
+  Matrix();
+  explicit Matrix(const double m_[4][4]);
+  static const Matrix &Identity();
+  // rotation by `angle` radians about `axis` (normalized first)
+  static Matrix RotateAroundVector(const Vector3 &axis, double angle);
+  Vector3 Apply(const Vector3 &v) const;
+  Matrix operator*(const Matrix &rhs) const;
+  double m[4][4];
 };
 
 class SceneObject {
@@ -86,6 +127,7 @@ class SceneObject {
   void AddMaterial(const Material &m, double rate, bool doNormalizeRate = true);
   size_t GetMaterialCount() const { return materials_.size(); }
   const MultiMaterialElem *GetMaterial(size_t i) const { return i < materials_.size() ? &materials_[i] : nullptr; }
+  MultiMaterialElem *GetMaterial(size_t i) { return i < materials_.size() ? &materials_[i] : nullptr; }
   const Vector3 &GetPosition() const { return position_; }
 
  protected:
@@ -119,11 +161,14 @@ class Polygon : public SceneObject {
   Polygon(const Vector3 &pos1, const Vector3 &pos2, const Vector3 &pos3, const Vector3 &uv1, const Vector3 &uv2,
           const Vector3 &uv3, const Vector3 &normal1, const Vector3 &normal2, const Vector3 &normal3,
           const Material &mat, const Vector3 &pos);
-  // Polygon::Transform (src/renderer/Polygon.h:51-61), rotation omitted: scale
-  // then translate through position_.
-  void Transform(const Vector3 &pos, const Vector3 &scale = Vector3::One());
+  // Polygon::Transform (src/renderer/Polygon.h:51-61): rotate vertex and
+  // edges, scale per axis, translate through position_; normals rotate only.
+  void Transform(const Vector3 &pos, const Vector3 &scale = Vector3::One(), const Matrix &rot = Matrix::Identity());
   const Vector3 *PosAndEdges() const { return m_posAndEdges; }
+  const Vector3 *UVOrigAndEdges() const { return m_uvOrigAndEdges; }
   const Vector3 *NormalAndDiffs() const { return m_normalAndDiffs; }
+  // Polygon::CalculateNormal (src/renderer/Polygon.h:45-49)
+  static Vector3 CalculateNormal(const Vector3 &v0, const Vector3 &v1, const Vector3 &v2);
 
  private:
   Vector3 m_posAndEdges[3];
@@ -167,10 +212,50 @@ class Camera {
   std::shared_ptr<Aperture> m_aperture;
 };
 
-// Latlong environment (IBL, src/renderer/IBL.h:14-45) from an in-memory image.
+// Infinite horizontal plane y = const, two-sided (InfiniteFloor,
+// src/renderer/InfiniteFloor.h:10-45); never inside the BVH.
+class InfiniteFloor : public SceneObject {
+ public:
+  InfiniteFloor(double y, const Material &m) : SceneObject(m), m_y(y) {}
+  double GetY() const { return m_y; }
+
+ private:
+  double m_y;
+};
+
+// Radiance RGBE image (HDRImage, src/tools/HDRImage.h:8-81): read and write,
+// run-length encoded scanlines as the reference writes them.
+class HDRImage {
+ public:
+  bool ReadFromRadianceFile(const std::string &file);
+  bool WriteToRadianceFile(const std::string &file);
+  size_t GetWidth() const { return m_width; }
+  size_t GetHeight() const { return m_height; }
+  const Color &GetPixel(size_t x, size_t y) const { return m_image[x + y * m_width]; }
+  // replaces the pixels (row-major, row 0 first)
+  void SetImage(size_t width, size_t height, std::vector<Color> pixels);
+  static void Color2RGBE(unsigned char rgbe[4], const Color &c);
+  static void RGBE2Color(const unsigned char rgbe[4], Color &c);
+
+ private:
+  struct Header {
+    int valid = 0;  // 1 program type, 2 gamma, 4 exposure
+    std::string programtype;
+    float gamma = 0.0f, exposure = 0.0f;
+    int height = 0, width = 0;
+  };
+  std::vector<Color> m_image;
+  size_t m_width = 0, m_height = 0;
+  Header m_info;
+};
+
+// Latlong environment (IBL, src/renderer/IBL.h:14-45).
 class IBL {
  public:
   IBL(uint32_t w, uint32_t h, std::vector<float> rgb) : m_w(w), m_h(h), m_rgb(std::move(rgb)) {}
+  // IBL(hdr_filename) (IBL.h:16-26): reads a Radiance file; empty when it fails
+  explicit IBL(const std::string &hdr_filename);
+  bool IsValid() const { return m_w > 0 && m_h > 0; }
   uint32_t Width() const { return m_w; }
   uint32_t Height() const { return m_h; }
   const std::vector<float> &Pixels() const { return m_rgb; }
@@ -178,6 +263,18 @@ class IBL {
  private:
   uint32_t m_w, m_h;
   std::vector<float> m_rgb;
+};
+
+class Model;
+
+// A scene flattened into what omr_scene_upload takes: triangles, then
+// spheres, then infinite floors; `desc` points into the vectors.
+struct SceneArrays {
+  std::vector<float> tri, nrm, sph, planes;
+  std::vector<uint32_t> prim_mat, prim_mat_count;
+  std::vector<omr_material> mats;
+  std::vector<SceneObject *> prims;  // primitive id -> object
+  omr_scene_desc desc;
 };
 
 class Scene {
@@ -198,10 +295,17 @@ class Scene {
 
   // Device handle for the current contents (uploads on first use).
   omr_scene_id Upload(omr_ctx *ctx) const;
+  // The current contents as upload arrays (no device needed).
+  void Describe(SceneArrays &out) const;
 
  protected:
   Scene() {}
   void AddObject(SceneObject *obj, bool doDelete = true, bool containedInBVH = true);
+  // Scene::AddModel (src/scenes/Scene.h:78-88): every polygon of the model,
+  // material by material; the scene keeps the model alive when doDelete.
+  void AddModel(Model *model, bool doDelete = true, bool containedInBVH = true);
+  // Scene::AddInfiniteFLoor (src/scenes/Scene.h:71)
+  SceneObject *AddInfiniteFLoor(double y, const Material &material);
   std::vector<SceneObject *> AddFloorXZ_yUp(double size_x, double size_z, const Vector3 &position, const Material &m);
   std::vector<SceneObject *> AddFloorXY_zUp(double size_x, double size_y, const Vector3 &position, const Material &m);
   std::vector<SceneObject *> AddFloorYZ_xUp(double size_y, double size_z, const Vector3 &position, const Material &m);
@@ -216,10 +320,60 @@ class Scene {
     bool doDelete;
   };
   std::vector<Owned> m_objects;
+  std::vector<std::unique_ptr<Model>> m_models;
   std::unique_ptr<IBL> m_ibl;
   mutable omr_ctx *m_ctx = nullptr;
   mutable omr_scene_id m_id = -1;
   mutable std::vector<SceneObject *> m_prims;  // primitive id -> object
+};
+
+// Wavefront OBJ mesh with MTL materials (Model, src/renderer/Model.cpp:50-367):
+// faces of n >= 3 vertices as a fan, v, v/vt, v//vn, v/vt/vn indices (1-based),
+// per-face normals when a face gives none, one polygon list per material.
+class Model {
+ public:
+  typedef Polygon *PolygonPtr;
+  typedef std::vector<PolygonPtr> PolygonList;
+  Model() {}
+  ~Model();
+  bool ReadFromObj(const std::string &filename, bool flipV_of_UV = false);
+  void Transform(const Vector3 &pos, const Vector3 &scale = Vector3::One(), const Matrix &rot = Matrix::Identity());
+  const Vector3 &GetPosition() const { return m_position; }
+  size_t GetMaterialCount() const { return m_materials.size(); }
+  const Material &GetMaterial(size_t i) const { return m_materials.at(i); }
+  const PolygonList &GetPolygonList(const Material &mat) const;
+
+ private:
+  void Clear();
+  std::vector<Material> m_materials;   // one per material name, in the reference's listing order
+  std::vector<Material> m_list_keys;   // distinct material values that own a polygon list
+  std::vector<PolygonList> m_meshes;   // parallel to m_list_keys
+  Vector3 m_position;
+};
+
+// Scene read from the reference's text format (SceneFromExternalFile,
+// src/scenes/SceneFromExternalFile.cpp:13-531): header (Base Dir, IBL, Space
+// Partitioning) up to "Objects:", then blocks "Obj Mesh", "Sphere", "Floor",
+// "SphereLight", "InfiniteFloor" of "Key: value" lines.  IsValid() is false
+// when the file or any block fails to parse (messages on stderr).
+class SceneFromExternalFile : public Scene {
+ public:
+  explicit SceneFromExternalFile(const std::string &file);
+  bool IsValid() const override { return m_isValid; }
+  const std::string &GetIBLFileName() const { return m_iblFileName; }
+
+ private:
+  typedef std::pair<std::string, std::string> LinePair;
+  bool ReadFromFile(const std::string &file);
+  bool ReadHeader(const std::vector<LinePair> &lines);
+  bool ParseMaterial(const std::string &data, Material &mat);
+  bool ProcessMaterials(SceneObject *obj, std::vector<Material> &materials, std::vector<double> &rates);
+  bool ReadMesh(const std::vector<LinePair> &lines);
+  bool ReadSphere(const std::vector<LinePair> &lines, bool isLight);
+  bool ReadFloor(const std::vector<LinePair> &lines);
+  bool ReadInfiniteFloor(const std::vector<LinePair> &lines);
+  bool m_isValid = false;
+  std::string m_baseDir, m_iblFileName, m_spacePartitioningMethod;
 };
 
 class ScreenPixels {

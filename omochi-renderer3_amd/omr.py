@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libomr.so")
 
 OMR_OK = 0
+OMR_ERR_INVALID = -1
 ERRORS = {
     -1: "invalid argument", -2: "out of memory", -3: "HIP error", -4: "over HBM budget",
     -5: "not found", -6: "bad state", -7: "no device", -8: "timeout", -9: "cancelled",
@@ -33,7 +34,12 @@ class SceneDesc(C.Structure):
     _fields_ = [("n_triangles", C.c_size_t), ("triangles", C.c_void_p), ("tri_normals", C.c_void_p),
                 ("n_spheres", C.c_size_t), ("spheres", C.c_void_p), ("prim_material", C.c_void_p),
                 ("prim_material_count", C.c_void_p), ("n_materials", C.c_size_t), ("materials", C.c_void_p),
-                ("env_width", C.c_uint32), ("env_height", C.c_uint32), ("env_rgb", C.c_void_p)]
+                ("env_width", C.c_uint32), ("env_height", C.c_uint32), ("env_rgb", C.c_void_p),
+                ("n_planes", C.c_size_t), ("planes", C.c_void_p)]
+
+
+class LoadedScene(C.Structure):
+    _fields_ = [("desc", SceneDesc), ("n_objects", C.c_size_t), ("internal", C.c_void_p)]
 
 
 class Camera(C.Structure):
@@ -89,7 +95,8 @@ EXPORTS = [
     "omr_get_tuning", "omr_set_tuning", "omr_server_start", "omr_server_stop", "omr_job_submit", "omr_job_poll",
     "omr_job_wait", "omr_job_cancel", "omr_server_stats_get", "omr_procedural_scene", "omr_procedural_free",
     "omr_camera_from_lookat", "omr_bvh_build", "omr_bvh_data", "omr_bvh_free", "omr_set_profiling", "omr_autotune",
-    "omr_procedural_camera",
+    "omr_procedural_camera", "omr_io_last_error", "omr_free", "omr_hdr_read", "omr_hdr_write", "omr_scene_file_load",
+    "omr_obj_load", "omr_loaded_scene_free",
 ]
 
 _lib = None
@@ -146,6 +153,17 @@ def load():
                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     lib.omr_bvh_free.argtypes = [vp]
     lib.omr_bvh_free.restype = None
+    lib.omr_io_last_error.argtypes = []
+    lib.omr_io_last_error.restype = C.c_char_p
+    lib.omr_free.argtypes = [vp]
+    lib.omr_free.restype = None
+    lib.omr_hdr_read.argtypes = [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(vp)]
+    lib.omr_hdr_write.argtypes = [C.c_char_p, C.c_uint32, C.c_uint32, vp]
+    lib.omr_scene_file_load.argtypes = [C.c_char_p, C.POINTER(LoadedScene)]
+    lib.omr_obj_load.argtypes = [C.c_char_p, C.POINTER(C.c_double * 3), C.POINTER(C.c_double * 3),
+                                 C.POINTER(C.c_double * 3), C.c_int, C.POINTER(LoadedScene)]
+    lib.omr_loaded_scene_free.argtypes = [C.POINTER(LoadedScene)]
+    lib.omr_loaded_scene_free.restype = None
     _lib = lib
     return lib
 
@@ -157,7 +175,8 @@ def _ptr(a):
 class Scene:
     """A scene as numpy arrays (the omr_scene_desc layout)."""
 
-    def __init__(self, tri, tri_n, sph, prim_mat, mats, prim_mat_count=None, env=None, camera=None):
+    def __init__(self, tri, tri_n, sph, prim_mat, mats, prim_mat_count=None, env=None, camera=None, planes=None):
+        self.planes = np.zeros(0, np.float32) if planes is None else np.ascontiguousarray(planes, dtype=np.float32)
         self.tri = np.ascontiguousarray(tri, dtype=np.float32).reshape(-1, 9)
         self.tri_n = None if tri_n is None else np.ascontiguousarray(tri_n, dtype=np.float32).reshape(-1, 9)
         self.sph = np.ascontiguousarray(sph, dtype=np.float32).reshape(-1, 4)
@@ -169,7 +188,7 @@ class Scene:
 
     @property
     def n_prims(self):
-        return self.tri.shape[0] + self.sph.shape[0]
+        return self.tri.shape[0] + self.sph.shape[0] + self.planes.shape[0]
 
     def desc(self):
         d = SceneDesc()
@@ -185,7 +204,82 @@ class Scene:
         if self.env is not None:
             d.env_height, d.env_width = self.env.shape[0], self.env.shape[1]
             d.env_rgb = _ptr(self.env)
+        d.n_planes = self.planes.shape[0]
+        d.planes = _ptr(self.planes) if self.planes.shape[0] else None
         return d
+
+
+def _scene_from_desc(d, camera=None):
+    """numpy copy of an omr_scene_desc the library owns."""
+    nt, ns, npl = d.n_triangles, d.n_spheres, d.n_planes
+
+    def arr(ptr, n, dt):
+        if n == 0 or not ptr:
+            return np.zeros(0, dtype=dt)
+        return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(n,)).copy()
+
+    tri = arr(d.triangles, nt * 9, np.float32).reshape(-1, 9)
+    trn = arr(d.tri_normals, nt * 9, np.float32).reshape(-1, 9)
+    sph = arr(d.spheres, ns * 4, np.float32).reshape(-1, 4)
+    pm = arr(d.prim_material, nt + ns + npl, np.uint32)
+    pc = arr(d.prim_material_count, nt + ns + npl, np.uint32) if d.prim_material_count else None
+    mats = np.frombuffer(C.string_at(d.materials, d.n_materials * 48), dtype=MATERIAL_DTYPE).copy()
+    env = None
+    if d.env_rgb:
+        env = arr(d.env_rgb, d.env_width * d.env_height * 3, np.float32).reshape(d.env_height, d.env_width, 3)
+    planes = arr(d.planes, npl, np.float32)
+    return Scene(tri, trn, sph, pm, mats, pc, env, camera, planes)
+
+
+def _io_error(lib, what):
+    return OmrError(OMR_ERR_INVALID, f"{what}: {lib.omr_io_last_error().decode(errors='replace')}")
+
+
+def hdr_read(path):
+    """Radiance RGBE image -> float32 array (height, width, 3), first scanline first."""
+    lib = load()
+    w, h, p = C.c_uint32(), C.c_uint32(), C.c_void_p()
+    if lib.omr_hdr_read(os.fsencode(path), C.byref(w), C.byref(h), C.byref(p)) != OMR_OK:
+        raise _io_error(lib, "omr_hdr_read")
+    try:
+        n = w.value * h.value * 3
+        a = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), shape=(n,)).copy()
+        return a.reshape(h.value, w.value, 3)
+    finally:
+        lib.omr_free(p)
+
+
+def hdr_write(path, img):
+    """Writes a float (height, width, 3) image as run-length encoded RGBE."""
+    lib = load()
+    a = np.ascontiguousarray(img, dtype=np.float32)
+    if lib.omr_hdr_write(os.fsencode(path), a.shape[1], a.shape[0], _ptr(a)) != OMR_OK:
+        raise _io_error(lib, "omr_hdr_write")
+
+
+def load_scene_file(path):
+    """A .scene file (the reference's text format) as a Scene."""
+    lib = load()
+    ls = LoadedScene()
+    if lib.omr_scene_file_load(os.fsencode(path), C.byref(ls)) != OMR_OK:
+        raise _io_error(lib, "omr_scene_file_load")
+    try:
+        return _scene_from_desc(ls.desc)
+    finally:
+        lib.omr_loaded_scene_free(C.byref(ls))
+
+
+def load_obj(path, position=(0, 0, 0), scale=(1, 1, 1), rotation=(0, 0, 0), flip_v=False):
+    """One Wavefront OBJ mesh (with its MTL materials), placed, as a Scene."""
+    lib = load()
+    ls = LoadedScene()
+    v3 = C.c_double * 3
+    if lib.omr_obj_load(os.fsencode(path), v3(*position), v3(*scale), v3(*rotation), int(flip_v), C.byref(ls)) != OMR_OK:
+        raise _io_error(lib, "omr_obj_load")
+    try:
+        return _scene_from_desc(ls.desc)
+    finally:
+        lib.omr_loaded_scene_free(C.byref(ls))
 
 
 def procedural(kind, seed=0x6F6D7233, width=256, height=256):
@@ -196,25 +290,9 @@ def procedural(kind, seed=0x6F6D7233, width=256, height=256):
     if rc != OMR_OK:
         raise RuntimeError(f"omr_procedural_scene failed ({rc})")
     try:
-        d = p.desc
-        nt, ns = d.n_triangles, d.n_spheres
-
-        def arr(ptr, n, dt):
-            if n == 0 or not ptr:
-                return np.zeros(0, dtype=dt)
-            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(n,)).copy()
-
-        tri = arr(d.triangles, nt * 9, np.float32).reshape(-1, 9)
-        trn = arr(d.tri_normals, nt * 9, np.float32).reshape(-1, 9)
-        sph = arr(d.spheres, ns * 4, np.float32).reshape(-1, 4)
-        pm = arr(d.prim_material, nt + ns, np.uint32)
-        mats = np.frombuffer(C.string_at(d.materials, d.n_materials * 48), dtype=MATERIAL_DTYPE).copy()
-        env = None
-        if d.env_rgb:
-            env = arr(d.env_rgb, d.env_width * d.env_height * 3, np.float32).reshape(d.env_height, d.env_width, 3)
         cam = Camera()
         C.memmove(C.byref(cam), C.byref(p.camera), C.sizeof(Camera))
-        return Scene(tri, trn, sph, pm, mats, None, env, cam)
+        return _scene_from_desc(p.desc, cam)
     finally:
         lib.omr_procedural_free(C.byref(p))
 

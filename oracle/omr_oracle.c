@@ -380,6 +380,14 @@ int orc_set_wide_bvh(orc_handle *h, const void *nodes, size_t n_nodes, const uin
 /* ------------------------------------------------------------ traversal */
 typedef struct { float t, u, v; int32_t prim; } hitrec;
 
+int orc_intersect_plane(const float ray[8], float y, float *t) {
+  if (fabsf(ray[5]) < ORC_EPS) return 0;
+  float tt = (y - ray[1]) / ray[5];
+  if (tt < ORC_EPS) return 0;
+  *t = tt;
+  return 1;
+}
+
 /* closest-hit update with the tie rule: smaller t wins, equal t -> smaller id */
 static inline void consider(const orc_handle *h, const float ray[8], uint32_t pid, hitrec *best, uint64_t *nprim) {
   float t, u = 0.0f, v = 0.0f;
@@ -508,6 +516,15 @@ static void closest(const orc_handle *h, const float ray[8], int accel, hitrec *
     trace_own(h, ray, best, cnt);
   } else {
     trace_wide(h, ray, best, cnt);
+  }
+  /* objects outside the BVH after it (Scene.cpp:72-80): infinite floors */
+  for (size_t k = 0; k < h->s.n_pln; k++) {
+    float t;
+    if (!orc_intersect_plane(ray, h->s.pln[k], &t)) continue;
+    int32_t id = (int32_t)(h->n_prim + k);
+    if (t < best->t || (t == best->t && best->prim >= 0 && id < best->prim)) {
+      best->t = t; best->u = best->v = 0.0f; best->prim = id;
+    }
   }
 }
 
@@ -794,9 +811,12 @@ static void trace_sample(const render_job *J, uint32_t pixel, uint32_t subpx, ui
         /* Polygon.h:178-179: interpolated normal, normalized */
         const float *tn = S->tri_n + 9 * (size_t)pid;
         n = normalize(add(add(scl(ld3(tn + 3), hr.u), scl(ld3(tn + 6), hr.v)), ld3(tn)));
-      } else {
+      } else if (pid < S->n_tri + S->n_sph) {
         /* Sphere.h:47-48 */
         n = normalize(sub(x, ld3(S->sph + 4 * (size_t)(pid - S->n_tri))));
+      } else {
+        /* InfiniteFloor.h:34-36: +-y, facing the ray */
+        n = mk(0.0f, d.y < 0.0f ? 1.0f : -1.0f, 0.0f);
       }
       /* PathTracer.cpp:189: orient against the ray */
       v3 nl = dot(n, d) < 0.0f ? n : scl(n, -1.0f);

@@ -40,12 +40,14 @@ typedef struct {
   const float *tri_n;    /* n_tri x 9: n0.xyz, n1-n0, n2-n0     (Polygon.h:109-111) */
   size_t n_sph;
   const float *sph;      /* n_sph x 4: center.xyz, radius        (Sphere.h:13-19)  */
-  const uint32_t *prim_mat;       /* (n_tri+n_sph): first material index */
-  const uint32_t *prim_mat_count; /* (n_tri+n_sph): material count, or NULL = 1 */
+  const uint32_t *prim_mat;       /* (n_tri+n_sph+n_pln): first material index */
+  const uint32_t *prim_mat_count; /* (n_tri+n_sph+n_pln): material count, or NULL = 1 */
   size_t n_mat;
   const orc_material *mat;
   uint32_t env_w, env_h;
   const float *env;      /* env_h rows x env_w x 3 (row 0 = +y), or NULL */
+  size_t n_pln;
+  const float *pln;      /* n_pln planes y = pln[i] (InfiniteFloor.h:12-41), tested after the BVH */
 } orc_scene;
 
 typedef struct {
@@ -84,6 +86,9 @@ void orc_sincos_turns(float x, float *s, float *c);
 float orc_acos(float x);
 int orc_intersect_triangle(const float ray[8], const float tri[9], float *t, float *u, float *v);
 int orc_intersect_sphere(const float ray[8], const float sph[4], float *t);
+/* InfiniteFloor::CheckIntersection (src/renderer/InfiniteFloor.h:20-38):
+ * |d.y| >= EPS, t = (y - o.y) / d.y >= EPS. */
+int orc_intersect_plane(const float ray[8], float y, float *t);
 void orc_env_texel(float dx, float dy, float dz, uint32_t w, uint32_t h, uint32_t *ix, uint32_t *iy);
 void orc_camera_ray(const orc_camera *cam, float x, float y, const float lens[2], float o[3], float d[3]);
 

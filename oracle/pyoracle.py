@@ -17,7 +17,8 @@ BRUTE, OWN_BVH, WIDE_BVH = 0, 1, 2
 class OScene(C.Structure):
     _fields_ = [("n_tri", C.c_size_t), ("tri", C.c_void_p), ("tri_n", C.c_void_p), ("n_sph", C.c_size_t),
                 ("sph", C.c_void_p), ("prim_mat", C.c_void_p), ("prim_mat_count", C.c_void_p), ("n_mat", C.c_size_t),
-                ("mat", C.c_void_p), ("env_w", C.c_uint32), ("env_h", C.c_uint32), ("env", C.c_void_p)]
+                ("mat", C.c_void_p), ("env_w", C.c_uint32), ("env_h", C.c_uint32), ("env", C.c_void_p),
+                ("n_pln", C.c_size_t), ("pln", C.c_void_p)]
 
 
 class OCamera(C.Structure):
@@ -56,6 +57,7 @@ def load(build_if_missing=True):
     lib.orc_acos.restype = C.c_float
     lib.orc_intersect_triangle.argtypes = [vp, vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
     lib.orc_intersect_sphere.argtypes = [vp, vp, C.POINTER(C.c_float)]
+    lib.orc_intersect_plane.argtypes = [vp, C.c_float, C.POINTER(C.c_float)]
     lib.orc_env_texel.argtypes = [C.c_float, C.c_float, C.c_float, u32, u32, C.POINTER(u32), C.POINTER(u32)]
     lib.orc_env_texel.restype = None
     lib.orc_camera_ray.argtypes = [C.POINTER(OCamera), C.c_float, C.c_float, vp, vp, vp]
@@ -112,6 +114,10 @@ class Oracle:
         if scene.env is not None:
             s.env_h, s.env_w = scene.env.shape[0], scene.env.shape[1]
             s.env = _p(scene.env)
+        planes = getattr(scene, "planes", None)
+        if planes is not None and planes.shape[0]:
+            s.n_pln = planes.shape[0]
+            s.pln = _p(planes)
         self._s = s
         self.h = self.lib.orc_prepare(C.byref(s))
         self._wide = None

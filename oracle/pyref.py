@@ -84,3 +84,80 @@ def env_texels(hdr, dirs):
     if rc != 0:
         raise RuntimeError("HDR read failed")
     return ix, iy, int(wh[0]), int(wh[1])
+
+
+def _load_io(lib):
+    vp, sz = C.c_void_p, C.c_size_t
+    lib.ref_scene_add_planes.argtypes = [vp, vp, sz, vp, vp, vp, sz]
+    lib.ref_scene_add_planes.restype = None
+    lib.ref_file_scene_new.argtypes = [C.c_char_p]
+    lib.ref_file_scene_new.restype = vp
+    lib.ref_file_scene_free.argtypes = [vp]
+    lib.ref_file_scene_free.restype = None
+    lib.ref_file_scene_objects.argtypes = [vp]
+    lib.ref_file_scene_objects.restype = sz
+    lib.ref_file_trace.argtypes = [vp, vp, sz, vp, vp, vp, vp, vp]
+    lib.ref_file_trace.restype = None
+    lib.ref_hdr_read.argtypes = [C.c_char_p, vp, vp]
+    lib.ref_hdr_read.restype = C.c_int
+    lib.ref_hdr_write.argtypes = [C.c_char_p, C.c_int32, C.c_int32, vp]
+    lib.ref_hdr_write.restype = C.c_int
+    return lib
+
+
+def add_planes(ref_scene, scene):
+    """Appends scene.planes as InfiniteFloor objects (after every other
+    object, outside the BVH) to a RefScene built from the same scene."""
+    lib = _load_io(ref_scene.lib)
+    y = np.ascontiguousarray(scene.planes, dtype=np.float32)
+    first = scene.tri.shape[0] + scene.sph.shape[0]
+    lib.ref_scene_add_planes(ref_scene.h, _p(y), y.shape[0], _p(scene.prim_mat), _p(scene.prim_mat_count),
+                             _p(scene.mats), first)
+
+
+class RefFileScene:
+    """SceneFromExternalFile on a .scene file (relative names inside it
+    resolve against the current directory)."""
+
+    def __init__(self, path):
+        self.lib = _load_io(load())
+        self.h = self.lib.ref_file_scene_new(path.encode())
+        if not self.h:
+            raise RuntimeError(f"reference could not load {path}")
+
+    def objects(self):
+        return int(self.lib.ref_file_scene_objects(self.h))
+
+    def close(self):
+        if self.h:
+            self.lib.ref_file_scene_free(self.h)
+            self.h = None
+
+    def trace(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        t = np.empty(n, np.float64)
+        nrm = np.zeros((n, 3), np.float64)
+        mat = np.zeros((n, 4, 9), np.float32)
+        nmat = np.zeros(n, np.int32)
+        obj = np.empty(n, np.int64)
+        self.lib.ref_file_trace(self.h, _p(rays), n, _p(t), _p(nrm), _p(mat), _p(nmat), _p(obj))
+        return dict(t=t, normal=nrm, mat=mat, nmat=nmat, obj=obj)
+
+
+def hdr_read(path):
+    lib = _load_io(load())
+    wh = np.zeros(2, np.int32)
+    if lib.ref_hdr_read(path.encode(), _p(wh), None) != 0:
+        raise RuntimeError(f"reference could not read {path}")
+    out = np.empty((wh[1], wh[0], 3), np.float32)
+    lib.ref_hdr_read(path.encode(), _p(wh), _p(out))
+    return out
+
+
+def hdr_write(path, rgb):
+    lib = _load_io(load())
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w = rgb.shape[0], rgb.shape[1]
+    if lib.ref_hdr_write(path.encode(), w, h, _p(rgb)) != 0:
+        raise RuntimeError(f"reference could not write {path}")

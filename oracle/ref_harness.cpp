@@ -12,10 +12,13 @@
 #include <vector>
 
 #include "renderer/IBL.h"
+#include "renderer/InfiniteFloor.h"
 #include "renderer/PathTracer.h"
 #include "renderer/Polygon.h"
 #include "renderer/Sphere.h"
 #include "scenes/Scene.h"
+#include "scenes/SceneFromExternalFile.h"
+#include "tools/HDRImage.h"
 #include "tools/Random.h"
 
 using namespace OmochiRenderer;
@@ -66,6 +69,29 @@ class HarnessScene : public Scene {
     if (bvh) ConstructBVH();
   }
   void SetEnv(const char *hdr) { m_ibl.reset(new IBL(hdr)); }
+  // infinite floors outside the BVH, after every other object
+  void AddPlanes(const float *y, size_t n, const uint32_t *pm, const uint32_t *pc, const Mat48 *mats, size_t first) {
+    for (size_t i = 0; i < n; i++) {
+      InfiniteFloor *o = new InfiniteFloor(y[i], to_ref(mats[pm[first + i]]));
+      uint32_t f = pm[first + i], c = pc ? pc[first + i] : 1u;
+      for (uint32_t k = 1; k < c; k++)
+        o->AddMaterial(to_ref(mats[f + k]), (double)mats[f + k].rate / (double)mats[f].rate, k + 1 == c);
+      AddObject(o, true, false);
+      prims.push_back(o);
+    }
+  }
+};
+
+// A scene the reference reads from its .scene format, with its object list.
+class FileScene : public SceneFromExternalFile {
+ public:
+  explicit FileScene(const std::string &f) : SceneFromExternalFile(f) {}
+  size_t Count() const { return m_objects.size(); }
+  size_t IndexOf(const SceneObject *o) const {
+    for (size_t k = 0; k < m_objects.size(); k++)
+      if (m_objects[k].object == o) return k;
+    return (size_t)-1;
+  }
 };
 
 }  // namespace
@@ -90,6 +116,85 @@ void *ref_scene_new(const float *tri, const float *tri_n, size_t n_tri, const fl
 }
 
 void ref_scene_free(void *s) { delete (HarnessScene *)s; }
+
+// Adds n infinite floors y[i] (materials pm/pc indexed from `first`).
+void ref_scene_add_planes(void *s, const float *y, size_t n, const uint32_t *pm, const uint32_t *pc, const void *mats,
+                          size_t first) {
+  ((HarnessScene *)s)->AddPlanes(y, n, pm, pc, (const Mat48 *)mats, first);
+}
+
+// SceneFromExternalFile on `path` (relative names resolve against the
+// current directory, as for the reference's executable).  NULL if invalid.
+void *ref_file_scene_new(const char *path) {
+  FileScene *s = new FileScene(path);
+  if (!s->IsValid()) {
+    delete s;
+    return nullptr;
+  }
+  return s;
+}
+void ref_file_scene_free(void *s) { delete (FileScene *)s; }
+size_t ref_file_scene_objects(void *s) { return ((FileScene *)s)->Count(); }
+// Scene::CheckIntersection of the loaded scene on n rays: FP64 distance,
+// normal of the hit record, the object's material count, the object's index
+// in the scene's object list (-1 on a miss), and its first four materials as
+// floats (type, color, emission, ior, rate; 9 per material, 36 per ray).
+void ref_file_trace(void *sp, const float *rays, size_t n, double *t, double *normal, float *mats, int32_t *nmat,
+                    int64_t *obj) {
+  FileScene *s = (FileScene *)sp;
+  for (size_t i = 0; i < n; i++) {
+    const float *r = rays + 8 * i;
+    Ray ray(Vector3(r[0], r[1], r[2]), Vector3(r[4], r[5], r[6]));
+    Scene::IntersectionInformation info;
+    bool hit = s->CheckIntersection(ray, info);
+    obj[i] = -1;
+    t[i] = info.hit.distance;
+    if (!hit) continue;
+    normal[3 * i] = info.hit.normal.x;
+    normal[3 * i + 1] = info.hit.normal.y;
+    normal[3 * i + 2] = info.hit.normal.z;
+    const size_t cnt = info.object->GetMaterialCount();
+    for (size_t k = 0; k < cnt && k < 4; k++) {
+      const SceneObject::MultiMaterialElem *e = info.object->GetMaterial(k);
+      const Material &m = e->material_;
+      float *o = mats + 36 * i + 9 * k;
+      o[0] = (float)m.reflection_type;
+      o[1] = (float)m.color.x; o[2] = (float)m.color.y; o[3] = (float)m.color.z;
+      o[4] = (float)m.emission.x; o[5] = (float)m.emission.y; o[6] = (float)m.emission.z;
+      o[7] = (float)m.refraction_rate;
+      o[8] = (float)e->rate_;
+    }
+    nmat[i] = (int32_t)cnt;
+    obj[i] = (int64_t)s->IndexOf(info.object);
+  }
+}
+
+// HDRImage::ReadFromRadianceFile: dims into wh (w, h); pixels (h x w x 3,
+// first scanline first) into out when out != NULL.
+int ref_hdr_read(const char *path, int32_t *wh, float *out) {
+  HDRImage img;
+  if (!img.ReadFromRadianceFile(path)) return -1;
+  wh[0] = (int32_t)img.GetWidth();
+  wh[1] = (int32_t)img.GetHeight();
+  if (out)
+    for (size_t y = 0; y < img.GetHeight(); y++)
+      for (size_t x = 0; x < img.GetWidth(); x++) {
+        const Color &c = img.GetPixel(x, y);
+        float *o = out + 3 * (y * img.GetWidth() + x);
+        o[0] = (float)c.x; o[1] = (float)c.y; o[2] = (float)c.z;
+      }
+  return 0;
+}
+
+// HDRImage::WriteToRadianceFile of a w x h float image.
+int ref_hdr_write(const char *path, int32_t w, int32_t h, const float *rgb) {
+  HDRImage img;
+  img.m_width = (size_t)w;
+  img.m_height = (size_t)h;
+  img.m_image.resize((size_t)w * h);
+  for (size_t i = 0; i < img.m_image.size(); i++) img.m_image[i] = Color(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+  return img.WriteToRadianceFile(path) ? 0 : -1;
+}
 
 // Scene::CheckIntersection on n rays (float o.xyz, tmax, d.xyz, pad): the
 // primitive index (-1 on a miss) and the FP64 distance.

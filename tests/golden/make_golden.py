@@ -3,7 +3,9 @@
 Runs only in a container that has the reference checkout: it builds
 oracle/_ref/libref.so from the reference sources (oracle/Makefile) and calls
 the reference's Random, Scene::CheckIntersection (Polygon / Sphere, FP64),
-IBL::Sample and PathTracer::RenderScene through oracle/ref_harness.cpp.
+IBL::Sample, PathTracer::RenderScene, InfiniteFloor, HDRImage's RGBE reader
+and writer and SceneFromExternalFile (with Model's OBJ/MTL reader) through
+oracle/ref_harness.cpp.
 Inputs are regenerated from seeds by the tests; the fixtures hold the
 reference outputs plus a SHA-256 of the inputs so drift is detected.
 
@@ -27,6 +29,7 @@ import omr  # noqa: E402
 import pyref  # noqa: E402
 
 REF_HDR = "/root/reference/input_data/Barce_Rooftop_C_Env.hdr"
+SCENES = os.path.join(HERE, "scenes")
 
 
 def sha(a):
@@ -73,7 +76,62 @@ def render_stats():
     print("ref render mean", float(mean.mean()))
 
 
+def hdr_files():
+    """Reference-written RGBE files (kept as data) and the reference reader's
+    decode of the run-length encoded ones."""
+    out = {}
+    for name, (w, h, seed) in gi.HDR_SETS.items():
+        img = gi.hdr_image(w, h, seed)
+        path = os.path.join(HERE, f"rgbe_{name}.hdr")
+        pyref.hdr_write(path, img)
+        if 8 <= w <= 0x7FFF:
+            # the reference reader keeps only `width` pixels of a narrower
+            # (flat) file, so it is decoded for run-length files only
+            out[f"{name}_decoded"] = pyref.hdr_read(path)
+        out[f"{name}_input_sha256"] = sha(img)
+    sky = gi.sky_image()
+    path = os.path.join(SCENES, "sky.hdr")
+    pyref.hdr_write(path, sky)
+    out["sky_decoded"] = pyref.hdr_read(path)
+    out["sky_input_sha256"] = sha(sky)
+    np.savez_compressed(os.path.join(HERE, "rgbe_decoded.npz"), **out)
+    print("rgbe", {k: v.shape for k, v in out.items() if hasattr(v, "shape") and v.ndim})
+
+
+def plane_hits():
+    scene = gi.plane_scene()
+    kind, n, seed = gi.PLANE_HITS
+    rays = gi.hit_rays(scene, n, seed)
+    r = pyref.RefScene(scene, bvh=False)
+    pyref.add_planes(r, scene)
+    prim, t = r.trace(rays)
+    r.close()
+    np.savez_compressed(os.path.join(HERE, "hits_planes.npz"), prim=prim, t=t, rays_sha256=sha(rays))
+    npl = int((prim >= scene.tri.shape[0] + scene.sph.shape[0]).sum())
+    print("plane hits", int((prim >= 0).sum()), "of", n, "planes", npl)
+
+
+def loader_hits():
+    """Closest hits through the reference's own loading of
+    tests/golden/scenes/loader.scene (relative names resolve from there)."""
+    rays = gi.loader_rays()
+    cwd = os.getcwd()
+    os.chdir(SCENES)
+    try:
+        r = pyref.RefFileScene("loader.scene")
+        res = r.trace(rays)
+        nobj = r.objects()
+        r.close()
+    finally:
+        os.chdir(cwd)
+    np.savez_compressed(os.path.join(HERE, "hits_loader.npz"), rays_sha256=sha(rays), n_objects=nobj, **res)
+    print("loader hits", int((res["obj"] >= 0).sum()), "of", len(rays), "objects", nobj)
+
+
 if __name__ == "__main__":
+    hdr_files()
+    plane_hits()
+    loader_hits()
     rng_kat()
     hits()
     env_texels()

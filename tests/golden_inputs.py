@@ -27,3 +27,81 @@ def env_dirs():
     axes = np.array([[1, 0, 0], [-1, 0, 0], [0, 0, 1], [0, 0, -1], [0.6, 0.8, 0], [0.6, -0.8, 0], [0, 0.6, 0.8],
                      [0, -0.6, -0.8], [0.70710678, 0, 0.70710678], [-0.70710678, 0, -0.70710678]], dtype=np.float64)
     return np.concatenate([axes, d])
+
+
+# RGBE images written by the reference's HDRImage writer: name -> (w, h, seed).
+# Widths 8..32767 take the run-length encoded scanline path, others the flat one.
+HDR_SETS = {"rle": (48, 7, 5), "flat": (5, 3, 6)}
+SKY = (64, 32, 7)
+
+# Planes (infinite floors) added to the procedural Cornell box for hit vectors.
+PLANES_Y = (12.5, 71.0)
+PLANE_HITS = (0, 20000, 303)
+
+LOADER_RAYS = (30000, 404)
+
+
+def hdr_image(w, h, seed):
+    """Non-negative float image with constant runs (RLE runs of every
+    length), zeros, values below RGBE's smallest exponent and large values."""
+    rng = np.random.default_rng(seed)
+    img = (rng.random((h, w, 3)) ** 3 * 4.0).astype(np.float32)
+    for y in range(h):
+        a = int(rng.integers(0, max(1, w - 1)))
+        L = int(rng.integers(1, max(2, w // 2)))
+        img[y, a:a + L] = img[y, a]
+    img[0, : min(w, 3)] = 0.0
+    img[-1, 0] = (1e-39, 2e-38, 1e-30)
+    img[-1, -1] = (65504.0, 1.5e5, 3.0)
+    if w > 10:
+        img[1, 2:9, 1] = 0.25          # a run in one channel only
+    return img
+
+
+def sky_image():
+    """Latlong sky: a vertical gradient, a bright sun and a dark ground."""
+    w, h, seed = SKY
+    rng = np.random.default_rng(seed)
+    y = (np.arange(h) + 0.5) / h
+    x = (np.arange(w) + 0.5) / w
+    g = np.clip(1.2 - 1.1 * y, 0.05, None)[:, None] * np.ones(w)[None, :]
+    img = np.stack([0.5 * g, 0.7 * g, 1.1 * g], axis=2)
+    sx, sy = 0.3, 0.22
+    d2 = ((x[None, :] - sx) * 2) ** 2 + (y[:, None] - sy) ** 2
+    img += (60.0 * np.exp(-d2 / 0.002))[:, :, None] * np.array([1.0, 0.9, 0.7])
+    img[h // 2 + 2:] = 0.08
+    img *= 1.0 + 0.05 * rng.random(img.shape)
+    return img.astype(np.float32)
+
+
+def plane_scene():
+    """Procedural Cornell box plus two infinite floors (glass, blue lambert)."""
+    import omr
+
+    s = omr.procedural(PLANE_HITS[0], width=64, height=64)
+    mats = np.concatenate([s.mats, s.mats[[5, 1]]])
+    m0 = len(s.mats)
+    pm = np.concatenate([s.prim_mat, np.array([m0, m0 + 1], np.uint32)])
+    return omr.Scene(s.tri, s.tri_n, s.sph, pm, mats, planes=np.array(PLANES_Y, np.float32))
+
+
+def loader_rays():
+    """Rays through the test .scene: a third from a box around the objects
+    in uniform directions (many reach the infinite floor from above or
+    below), a third aimed from outside at points inside the objects' box and
+    a third at points in the box of the placed mesh."""
+    n, seed = LOADER_RAYS
+    rng = np.random.default_rng(seed)
+    lo, hi = np.array([-60.0, -20.0, -60.0]), np.array([60.0, 50.0, 40.0])
+    n1 = n // 3
+    o = [lo + (hi - lo) * rng.random((n1, 3))]
+    d = [helpers.unit_dirs(rng, n1)]
+    for (tlo, thi), m in ((([-25.0, -2.0, -40.0], [45.0, 42.0, 25.0]), n1),
+                          (([4.0, 3.0, -32.0], [26.0, 26.0, -8.0]), n - 2 * n1)):
+        tlo, thi = np.array(tlo), np.array(thi)
+        tgt = tlo + (thi - tlo) * rng.random((m, 3))
+        oo = (tlo + thi) / 2 + 90.0 * helpers.unit_dirs(rng, m)
+        dd = tgt - oo
+        o.append(oo)
+        d.append(dd / np.linalg.norm(dd, axis=1, keepdims=True))
+    return helpers.make_rays(np.concatenate(o).astype(np.float32), np.concatenate(d).astype(np.float32))

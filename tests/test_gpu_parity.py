@@ -154,3 +154,51 @@ def test_tuning_modes_bit_exact(ctx, tune):
         ctx.release(sid)
     finally:
         ctx.set_tuning(**saved)
+
+
+def test_trace_planes_bit_exact(ctx):
+    """Infinite floors (tested after the BVH, ids after the spheres)."""
+    import golden_inputs as gi
+    import pyoracle
+
+    scene = gi.plane_scene()
+    sid = ctx.upload(scene)
+    rays = helpers.hit_rays(scene, 100_000, seed=41)
+    print(f"trace planes n={rays.shape[0]}", flush=True)
+    gp, gt = ctx.trace(sid, rays)
+    cp, ct = _oracle(scene, ctx, sid).trace(rays, pyoracle.WIDE_BVH)
+    assert np.array_equal(gp, cp)
+    hit = cp >= 0
+    assert np.array_equal(gt[hit].view(np.uint32), ct[hit].view(np.uint32))
+    assert (cp >= scene.tri.shape[0] + scene.sph.shape[0]).sum() > 10_000
+    ctx.release(sid)
+
+
+@pytest.mark.parametrize("env_is", [0, 1])
+def test_render_loaded_scene_bit_exact(ctx, env_is):
+    """A scene read from the .scene format (OBJ mesh with MTL materials,
+    multi-material sphere and floor, emissive sphere, glass infinite floor,
+    RGBE sky) renders to the same bits as the restatement."""
+    import os
+
+    import omr
+    import pyoracle
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes")
+    cwd = os.getcwd()
+    os.chdir(here)
+    try:
+        scene = omr.load_scene_file("loader.scene")
+    finally:
+        os.chdir(cwd)
+    cam = omr.camera_lookat(48, 32, (0.0, 30.0, 110.0), (0.0, -0.25, -1.0), screen_height=30.0, dist=40.0)
+    sid = ctx.upload(scene)
+    prm = omr.params(ss=2, spp_end=2, max_depth=8, rr_depth=3, env_is=env_is, seed=2024)
+    print(f"render loaded scene env_is={env_is}", flush=True)
+    g = ctx.render_sums(sid, cam, prm)
+    st = ctx.last_stats()
+    c, seg, sh = _oracle(scene, ctx, sid).render_sums(cam, prm, pyoracle.WIDE_BVH)
+    assert not (g != c).any(axis=2).any(), f"{(g != c).any(axis=2).sum()} pixels differ"
+    assert st["segments"] == seg and st["shadow_rays"] == sh
+    assert g.any()
+    ctx.release(sid)
