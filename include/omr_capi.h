@@ -50,8 +50,18 @@ typedef struct {
   float emission[3];
   float ior;         /* refraction_rate */
   float rate;        /* roulette rate among the primitive's materials (normalized) */
-  uint32_t pad[3];
+  uint32_t texture;  /* 0 = none; k = the scene's textures[k-1] (Material::texture_id,
+                        src/renderer/Material.h:33): the hit colour is color x texel */
+  uint32_t pad[2];
 } omr_material;
+
+/* Texture image (Image, src/tools/Image.h:4-35): linear RGB, rows from the
+ * top of the image; sampled by uv with u, v clamped to [0, 1] and scaled by
+ * 0.99999 (Image::GetPixelByUV, :21-30).  width, height <= 32768. */
+typedef struct {
+  uint32_t width, height;
+  const float *rgb;                /* height x width x 3 */
+} omr_texture;
 
 /* Scene contents handed to omr_scene_upload; replaces the object lists of
  * Scene (src/scenes/Scene.h:112-116) built by AddObject / AddModel.
@@ -72,6 +82,11 @@ typedef struct {
                                       tested after it (InfiniteFloor, src/renderer/InfiniteFloor.h:10-45;
                                       Scene::CheckIntersection's not-in-BVH list, src/scenes/Scene.cpp:72-80) */
   const float *planes;
+  size_t n_textures;               /* textures referenced by omr_material.texture */
+  const omr_texture *textures;
+  const float *tri_uv;             /* n_triangles x 6: uv0.xy, uv1-uv0, uv2-uv0 (Polygon.h:22-24,86-96);
+                                      NULL = uv 0.  Spheres use the latlong uv of the hit normal
+                                      (Sphere.h:52-75), planes uv 0 */
 } omr_scene_desc;
 
 /* Camera (src/renderer/Camera.h:12-43) in its derived form; the drop-in
@@ -261,9 +276,19 @@ int omr_hdr_read(const char *path, uint32_t *width, uint32_t *height, float **rg
  * RGBE with run-length encoded scanlines, byte for byte as the reference
  * writes them. */
 int omr_hdr_write(const char *path, uint32_t width, uint32_t height, const float *rgb);
+/* The image library call of ImageHandler::LoadFromFile
+ * (src/tools/ImageHandler.cpp:54, stbi_load with native channels): a PNG
+ * (8-bit samples, any colour type, interlaced or not) or baseline JPEG file
+ * as 8-bit samples, rows from the top, *channels per pixel.  Release *px with
+ * omr_free. */
+int omr_image_read(const char *path, uint32_t *width, uint32_t *height, uint32_t *channels, uint8_t **px);
+/* ImageHandler::LoadFromFile (src/tools/ImageHandler.cpp:20-90) as a texture:
+ * linear RGB floats, rows from the top; .hdr files as Radiance RGBE, other
+ * files b / 255 per sample, raised to 2.2 when inverse_gamma.  Release *rgb
+ * with omr_free. */
+int omr_texture_read(const char *path, int inverse_gamma, uint32_t *width, uint32_t *height, float **rgb);
 /* A scene read from a file: desc points into memory the handle owns (until
- * omr_loaded_scene_free).  Texture references are read but not used: the
- * shading path takes material colours. */
+ * omr_loaded_scene_free), textures included. */
 typedef struct {
   omr_scene_desc desc;
   size_t n_objects;               /* objects of the scene file (a mesh counts its polygons) */

@@ -233,7 +233,19 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
       if (!std::isfinite(m.color[c]) || !std::isfinite(m.emission[c]) || m.emission[c] < 0.0f)
         return fail(OMR_ERR_INVALID, "material " + std::to_string(i) + ": bad color/emission");
     if (m.type == 2 && !(m.ior > 0.0f)) return fail(OMR_ERR_INVALID, "material " + std::to_string(i) + ": bad ior");
+    if (m.texture > d->n_textures)
+      return fail(OMR_ERR_INVALID, "material " + std::to_string(i) + ": texture index out of range");
   }
+  if (d->n_textures && !d->textures) return fail(OMR_ERR_INVALID, "textures is NULL");
+  if (d->n_textures > 4096) return fail(OMR_ERR_INVALID, "too many textures (max 4096)");
+  size_t texels = 0;
+  for (size_t i = 0; i < d->n_textures; i++) {
+    const omr_texture &t = d->textures[i];
+    if (!t.rgb || t.width == 0 || t.height == 0 || t.width > 32768 || t.height > 32768)
+      return fail(OMR_ERR_INVALID, "texture " + std::to_string(i) + ": bad size or NULL pixels");
+    texels += (size_t)t.width * t.height;
+  }
+  if (texels > 0xFFFFFFFFull) return fail(OMR_ERR_INVALID, "textures too large");
   std::vector<uint32_t> pm(nall);
   for (size_t i = 0; i < nall; i++) {
     uint32_t f = d->prim_material[i];
@@ -278,6 +290,24 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
     for (int c = 0; c < 3; c++) { r.color[c] = m.color[c]; r.emission[c] = m.emission[c]; }
     r.ior = m.ior;
     r.rate = m.rate;
+    r.tex = m.texture;
+  }
+  // texels as 4 floats (one 16-byte load) after each other, with a directory
+  std::vector<float> tex_px(texels * 4, 0.0f);
+  std::vector<TexRec> tex_rec(d->n_textures);
+  for (size_t i = 0, first = 0; i < d->n_textures; i++) {
+    const omr_texture &t = d->textures[i];
+    size_t m = (size_t)t.width * t.height;
+    tex_rec[i] = TexRec{(uint32_t)first, t.width, t.height, 0u};
+    for (size_t k = 0; k < m; k++)
+      for (int c = 0; c < 3; c++) tex_px[(first + k) * 4 + c] = t.rgb[3 * k + c];
+    first += m;
+  }
+  std::vector<float> uv;
+  if (d->tri_uv && nt) {
+    uv.assign(nt * 8, 0.0f);
+    for (size_t i = 0; i < nt; i++)
+      for (int c = 0; c < 6; c++) uv[i * 8 + c] = d->tri_uv[i * 6 + c];
   }
 
   std::lock_guard<std::mutex> g(mu);
@@ -295,13 +325,17 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
     return OMR_OK;
   };
   void *p_nodes, *p_prims, *p_tn = nullptr, *p_sph = nullptr, *p_pm, *p_mats, *p_pl = nullptr;
+  void *p_tex = nullptr, *p_trec = nullptr, *p_uv = nullptr;
   int rc;
   if ((rc = dup(S.bvh.nodes.data(), S.bvh.nodes.size() * sizeof(WNode4), &p_nodes)) ||
       (rc = dup(S.bvh.prims.data(), S.bvh.prims.size() * sizeof(PrimRec), &p_prims)) ||
       (rc = dup(tn.data(), tn.size() * sizeof(float), &p_tn)) ||
       (rc = dup(d->spheres, ns * 4 * sizeof(float), &p_sph)) || (rc = dup(pm.data(), nall * 4, &p_pm)) ||
       (rc = dup(mats.data(), mats.size() * sizeof(MatRec), &p_mats)) ||
-      (rc = dup(d->planes, npl * sizeof(float), &p_pl))) {
+      (rc = dup(d->planes, npl * sizeof(float), &p_pl)) ||
+      (rc = dup(tex_px.data(), tex_px.size() * sizeof(float), &p_tex)) ||
+      (rc = dup(tex_rec.data(), tex_rec.size() * sizeof(TexRec), &p_trec)) ||
+      (rc = dup(uv.data(), uv.size() * sizeof(float), &p_uv))) {
     free_scene_mem(S);
     return rc;
   }
@@ -316,6 +350,10 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
   D.n_sph = (uint32_t)ns;
   D.planes = (const float *)p_pl;
   D.n_planes = (uint32_t)npl;
+  D.tri_uv = (const float *)p_uv;
+  D.tex = (const float *)p_tex;
+  D.tex_rec = (const TexRec *)p_trec;
+  D.n_tex = (uint32_t)d->n_textures;
   D.n_nodes = (uint32_t)S.bvh.nodes.size();
   D.stack_bound = S.bvh.stack_bound;
   if (d->env_rgb) {

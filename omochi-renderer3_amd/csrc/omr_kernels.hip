@@ -549,6 +549,42 @@ __device__ inline float env_pdf(const SceneDev &S, f3 d) {
   double p = (double)q / (double)tot * ((double)S.env_w * (double)S.env_h) / (19.739208802178716 * (double)st);
   return (float)p;
 }
+// ------------------------------------------------------------ textures
+// Surface uv of a hit: interpolated vertex uv on triangles (Polygon.h:86-96,
+// duv1 * u + duv2 * v + uv0), the latlong uv of the normal on spheres
+// (Sphere.h:52-75: u = (atan2(n.z, n.x) + pi) / 2pi and 1 - (asin(n.y) +
+// pi/2) / pi = acos(n.y) / pi), 0 on planes (InfiniteFloor sets none).
+__device__ inline void hit_uv(const SceneDev &S, uint32_t pid, float bu, float bv, f3 nrm, float &u, float &v) {
+  u = 0.0f;
+  v = 0.0f;
+  if (pid < S.n_tri) {
+    if (S.tri_uv) {
+      const float4 *q = reinterpret_cast<const float4 *>(S.tri_uv) + (size_t)pid * 2;
+      float4 a = q[0], b = q[1];
+      u = (a.z * bu + b.x * bv) + a.x;
+      v = (a.w * bu + b.y * bv) + a.y;
+    }
+  } else if (pid < S.n_tri + S.n_sph) {
+    float rxz = sqrtf(nrm.x * nrm.x + nrm.z * nrm.z);
+    float phi = (rxz > 0.0f) ? acos_f(nrm.x / rxz) : 0.0f;
+    u = (nrm.z >= 0.0f ? kPi + phi : kPi - phi) / kTwoPi;
+    v = acos_f(nrm.y) / kPi;
+  }
+}
+// Image::GetPixelByUV (src/tools/Image.h:21-30): u, v clamped to [0, 1]
+// (NaN to 0), scaled by 0.99999, texel (int(u W), int(v H)), row 0 on top.
+__device__ inline f3 texel(const SceneDev &S, uint32_t k, float u, float v) {
+  const TexRec r = S.tex_rec[k];
+  u = u > 0.0f ? (u < 1.0f ? u : 1.0f) : 0.0f;
+  v = v > 0.0f ? (v < 1.0f ? v : 1.0f) : 0.0f;
+  uint32_t x = (uint32_t)(int)((u * 0.99999f) * (float)r.width);
+  uint32_t y = (uint32_t)(int)((v * 0.99999f) * (float)r.height);
+  x = x < r.width ? x : r.width - 1u;
+  y = y < r.height ? y : r.height - 1u;
+  float4 p = reinterpret_cast<const float4 *>(S.tex)[(size_t)r.first + (size_t)y * r.width + x];
+  return mk(p.x, p.y, p.z);
+}
+
 __device__ inline uint32_t search_le(const unsigned long long *a, uint32_t n, unsigned long long key) {
   uint32_t lo = 0, hi = n;
   while (hi - lo > 1) {
@@ -895,7 +931,15 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
       value -= r;
     }
     const MatRec &m = S.mats[mfirst + sel];
-    f3 col = mk(m.color[0], m.color[1], m.color[2]);
+    // hit colour: material colour times the texel (PathTracer.cpp:327-346,388-389)
+    const f3 col0 = mk(m.color[0], m.color[1], m.color[2]);
+    f3 col = col0;
+    if (m.tex) {
+      float tu, tv;
+      hit_uv(S, pid, hr.z, hr.w, nrm, tu, tv);
+      f3 tx = texel(S, m.tex - 1u, tu, tv);
+      col = mk(col0.x * tx.x, col0.y * tx.y, col0.z * tx.z);
+    }
     f3 em = mk(m.emission[0], m.emission[1], m.emission[2]);
     // Russian roulette (PathTracer.cpp:398-416)
     float prr = col.x > (col.y > col.z ? col.y : col.z) ? col.x : (col.y > col.z ? col.y : col.z);
@@ -973,9 +1017,10 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
         float dd = dot(d, nl);
         float cos2t = 1.0f - nr * nr * (1.0f - dd * dd);
         if (cos2t < 0.0f) {
+          // total internal reflection weighs by the untextured colour (PathTracer.cpp:543-548)
           if (can_continue) {
             float k = 1.0f / prr;
-            beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
+            beta = mk(beta.x * (col0.x * k), beta.y * (col0.y * k), beta.z * (col0.z * k));
             o = x;
             d = refl;
             depth++;

@@ -58,7 +58,13 @@ struct alignas(16) MatRec {
   float emission[3];
   float ior;
   float rate;
-  uint32_t pad[3];
+  uint32_t tex;  // 0 = none, k = texture k-1
+  uint32_t pad[2];
+};
+
+// Texture directory entry: index of the first texel and the size.
+struct TexRec {
+  uint32_t first, width, height, pad;
 };
 
 // Device-side view of one resident scene.
@@ -76,6 +82,10 @@ struct SceneDev {
   uint32_t n_tri, n_sph;
   const float *planes;         // infinite floors y = planes[i]; ids n_tri + n_sph + i
   uint32_t n_planes;
+  const float *tri_uv;         // per original triangle: 8 floats (uv0, duv1, duv2, pad) or null
+  const float *tex;            // texels of every texture: 4 floats each (linear RGB, pad)
+  const TexRec *tex_rec;       // n_tex entries
+  uint32_t n_tex;
   uint32_t env_w, env_h;
   uint32_t n_nodes;
   uint32_t stack_bound;        // max traversal stack entries

@@ -18,6 +18,7 @@
 #include <sstream>
 #include <unordered_map>
 
+#include "ImageCodecs.h"
 #include "OmochiRenderer.h"
 
 namespace OmochiRenderer {
@@ -97,21 +98,88 @@ Matrix Matrix::operator*(const Matrix &b) const {
   return r;
 }
 
+// ------------------------------------------------------------ Image
+const Color &Image::GetPixelByUV(double u, double v) const {
+  auto clamp01 = [](double x) { return x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x); };  // Utils::Clamp
+  u = clamp01(u) * 0.99999;
+  v = clamp01(v) * 0.99999;
+  size_t x = (size_t)(int)(u * (double)m_width), y = (size_t)(int)(v * (double)m_height);
+  return m_image[x + y * m_width];
+}
+
 // ------------------------------------------------------------ ImageHandler
 ImageHandler &ImageHandler::GetInstance() {
   static ImageHandler h;
   return h;
 }
-// ImageHandler::LoadFromFile (src/tools/ImageHandler.cpp:20-90): an id per
-// readable file, the same id for the same name, INVALID_IMAGE_ID when it
-// cannot be opened.
-ImageHandler::IMAGE_ID ImageHandler::LoadFromFile(const std::string &file) {
-  for (size_t i = 0; i < m_files.size(); i++)
-    if (m_files[i] == file) return (IMAGE_ID)i;
-  std::ifstream f(file.c_str(), std::ios::binary);
-  if (!f) return INVALID_IMAGE_ID;
-  m_files.push_back(file);
-  return (IMAGE_ID)(m_files.size() - 1);
+
+// ImageHandler::LoadFromFile (src/tools/ImageHandler.cpp:20-90).  A name
+// ending in ".hdr" is read as a Radiance file every time (linear, no gamma);
+// any other name that is already loaded returns its id, else the file is
+// decoded as PNG or baseline JPEG and each 8-bit sample b becomes b / 255
+// (binary32 division, as the reference) and, with doReverseGamma,
+// pow(b / 255, 2.2) (Utils::InvGammaRev, src/tools/Utils.h:103-105).
+ImageHandler::IMAGE_ID ImageHandler::LoadFromFile(const std::string &fname, bool doReverseGamma) {
+  std::unique_ptr<Image> image;
+  const bool hdr = fname.size() >= 4 && fname.compare(fname.size() - 4, 4, ".hdr") == 0;
+  if (hdr) {
+    std::unique_ptr<HDRImage> h(new HDRImage);
+    if (!h->ReadFromRadianceFile(fname)) {
+      m_error = fname + ": not a readable Radiance file";
+      return INVALID_IMAGE_ID;
+    }
+    image = std::move(h);
+  } else {
+    for (size_t i = m_files.size(); i-- > 0;)
+      if (m_files[i] == fname && m_images[i]) return (IMAGE_ID)i;
+    codec::Pixels8 px;
+    std::string err;
+    if (!codec::decode_file(fname, px, err)) {
+      m_error = fname + ": " + err;
+      return INVALID_IMAGE_ID;
+    }
+    // the reference requires 3 or 4 samples per pixel (ImageHandler.cpp:59)
+    if (px.channels < 3) {
+      m_error = fname + ": a texture needs 3 or 4 channels";
+      return INVALID_IMAGE_ID;
+    }
+    image.reset(new Image);
+    image->m_width = px.width;
+    image->m_height = px.height;
+    image->m_image.resize((size_t)px.width * px.height);
+    auto level = [doReverseGamma](uint8_t b) {
+      double x = b / 255.0f;
+      if (doReverseGamma) x = std::pow(x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x), 2.2);
+      return x;
+    };
+    for (size_t i = 0; i < image->m_image.size(); i++) {
+      const uint8_t *p = &px.px[i * px.channels];
+      image->m_image[i] = Color(level(p[0]), level(p[1]), level(p[2]));
+    }
+  }
+  m_images.push_back(std::move(image));
+  m_files.push_back(fname);
+  return (IMAGE_ID)(m_images.size() - 1);
+}
+
+ImageHandler::IMAGE_ID ImageHandler::CreateImage(size_t width, size_t height) {
+  std::unique_ptr<Image> img(new Image);
+  img->m_width = width;
+  img->m_height = height;
+  img->m_image.resize(width * height);
+  m_images.push_back(std::move(img));
+  m_files.emplace_back();
+  return (IMAGE_ID)(m_images.size() - 1);
+}
+
+Image *ImageHandler::GetImage(IMAGE_ID id) {
+  return id >= 0 && (size_t)id < m_images.size() ? m_images[(size_t)id].get() : nullptr;
+}
+const Image *ImageHandler::GetImage(IMAGE_ID id) const {
+  return id >= 0 && (size_t)id < m_images.size() ? m_images[(size_t)id].get() : nullptr;
+}
+void ImageHandler::ReleaseImage(IMAGE_ID id) {
+  if (id >= 0 && (size_t)id < m_images.size()) m_images[(size_t)id].reset();
 }
 const std::string &ImageHandler::GetFileName(IMAGE_ID id) const {
   static const std::string none;

@@ -138,6 +138,28 @@ void Scene::Describe(SceneArrays &a) const {
   a = SceneArrays();
   std::vector<uint32_t> pm_sph, pc_sph, pm_pln, pc_pln;
   std::vector<SceneObject *> sphs, plns;
+  // images used as textures, numbered in order of first use (texture = index + 1)
+  std::map<ImageHandler::IMAGE_ID, uint32_t> tex_index;
+  auto texture_of = [&](ImageHandler::IMAGE_ID id) -> uint32_t {
+    if (id == ImageHandler::INVALID_IMAGE_ID) return 0u;
+    auto it = tex_index.find(id);
+    if (it != tex_index.end()) return it->second;
+    const Image *img = ImageHandler::GetInstance().GetImage(id);
+    if (!img || img->GetWidth() == 0 || img->GetHeight() == 0) return 0u;
+    std::vector<float> rgb(img->m_image.size() * 3);
+    for (size_t i = 0; i < img->m_image.size(); i++) {
+      rgb[3 * i] = (float)img->m_image[i].x;
+      rgb[3 * i + 1] = (float)img->m_image[i].y;
+      rgb[3 * i + 2] = (float)img->m_image[i].z;
+    }
+    a.tex_rgb.push_back(std::move(rgb));
+    omr_texture t;
+    t.width = (uint32_t)img->GetWidth();
+    t.height = (uint32_t)img->GetHeight();
+    t.rgb = nullptr;  // set once tex_rgb stops growing
+    a.textures.push_back(t);
+    return tex_index[id] = (uint32_t)a.textures.size();
+  };
   auto add_mats = [&](const SceneObject *o, std::vector<uint32_t> &pm, std::vector<uint32_t> &pc) {
     pm.push_back((uint32_t)a.mats.size());
     pc.push_back((uint32_t)o->GetMaterialCount());
@@ -151,6 +173,7 @@ void Scene::Describe(SceneArrays &a) const {
       m.emission[0] = (float)em.x; m.emission[1] = (float)em.y; m.emission[2] = (float)em.z;
       m.ior = (float)e->material_.refraction_rate;
       m.rate = (float)e->rate_;
+      m.texture = texture_of(e->material_.texture_id);
       a.mats.push_back(m);
     }
   };
@@ -164,8 +187,11 @@ void Scene::Describe(SceneArrays &a) const {
                      (float)pe[2].x, (float)pe[2].y, (float)pe[2].z};
       float n9[9] = {(float)nd[0].x, (float)nd[0].y, (float)nd[0].z, (float)nd[1].x, (float)nd[1].y, (float)nd[1].z,
                      (float)nd[2].x, (float)nd[2].y, (float)nd[2].z};
+      const Vector3 *uv = p->UVOrigAndEdges();
+      float u6[6] = {(float)uv[0].x, (float)uv[0].y, (float)uv[1].x, (float)uv[1].y, (float)uv[2].x, (float)uv[2].y};
       a.tri.insert(a.tri.end(), t9, t9 + 9);
       a.nrm.insert(a.nrm.end(), n9, n9 + 9);
+      a.uv.insert(a.uv.end(), u6, u6 + 6);
       add_mats(o, a.prim_mat, a.prim_mat_count);
       a.prims.push_back(o);
     } else if (auto *s = dynamic_cast<Sphere *>(o)) {
@@ -199,6 +225,10 @@ void Scene::Describe(SceneArrays &a) const {
   d.prim_material_count = a.prim_mat_count.data();
   d.n_materials = a.mats.size();
   d.materials = a.mats.data();
+  for (size_t k = 0; k < a.textures.size(); k++) a.textures[k].rgb = a.tex_rgb[k].data();
+  d.n_textures = a.textures.size();
+  d.textures = a.textures.empty() ? nullptr : a.textures.data();
+  d.tri_uv = a.textures.empty() ? nullptr : a.uv.data();  // uv only matters with a texture
   if (m_ibl && m_ibl->IsValid()) {
     d.env_width = m_ibl->Width();
     d.env_height = m_ibl->Height();
@@ -239,29 +269,34 @@ bool Scene::CheckIntersection(const Ray &ray, IntersectionInformation &info) con
 
 // Floor helpers: two polygons per quad, same vertices, uvs and normals as
 // Scene::AddFloor* (src/scenes/Scene.cpp:88-193).
-#define OMR_FLOOR(name, a0, a1, a2, n0, b0, b1, b2)                                                          \
+#define OMR_FLOOR(name, a0, a1, a2, ta, n0, b0, b1, b2, tb)                                                 \
   std::vector<SceneObject *> Scene::name(double s1, double s2, const Vector3 &position, const Material &m) { \
     const double h1 = s1 / 2, h2 = s2 / 2;                                                                   \
     (void)h1; (void)h2;                                                                                      \
+    const double ua[6] = {ta}, ub[6] = {tb};                                                                 \
     std::vector<SceneObject *> objs;                                                                         \
-    objs.push_back(new Polygon(a0, a1, a2, Vector3(), Vector3(), Vector3(), n0, n0, n0, m, position));        \
-    objs.push_back(new Polygon(b0, b1, b2, Vector3(), Vector3(), Vector3(), n0, n0, n0, m, position));        \
+    objs.push_back(new Polygon(a0, a1, a2, Vector3(ua[0], ua[1], 0), Vector3(ua[2], ua[3], 0),              \
+                               Vector3(ua[4], ua[5], 0), n0, n0, n0, m, position));                          \
+    objs.push_back(new Polygon(b0, b1, b2, Vector3(ub[0], ub[1], 0), Vector3(ub[2], ub[3], 0),              \
+                               Vector3(ub[4], ub[5], 0), n0, n0, n0, m, position));                          \
     AddObject(objs[0]);                                                                                      \
     AddObject(objs[1]);                                                                                      \
     return objs;                                                                                             \
   }
-OMR_FLOOR(AddFloorXZ_yUp, Vector3(-h1, 0, -h2), Vector3(-h1, 0, h2), Vector3(h1, 0, -h2), Vector3(0, 1, 0),
-          Vector3(h1, 0, -h2), Vector3(-h1, 0, h2), Vector3(h1, 0, h2))
-OMR_FLOOR(AddFloorXY_zUp, Vector3(-h1, h2, 0), Vector3(-h1, -h2, 0), Vector3(h1, -h2, 0), Vector3(0, 0, 1),
-          Vector3(h1, h2, 0), Vector3(-h1, h2, 0), Vector3(h1, -h2, 0))
-OMR_FLOOR(AddFloorYZ_xUp, Vector3(0, h1, -h2), Vector3(0, h1, h2), Vector3(0, -h1, h2), Vector3(1, 0, 0),
-          Vector3(0, -h1, -h2), Vector3(0, h1, -h2), Vector3(0, -h1, h2))
-OMR_FLOOR(AddFloorXZ_yDown, Vector3(h1, 0, -h2), Vector3(-h1, 0, h2), Vector3(-h1, 0, -h2), Vector3(0, -1, 0),
-          Vector3(h1, 0, h2), Vector3(-h1, 0, h2), Vector3(h1, 0, -h2))
-OMR_FLOOR(AddFloorXY_zDown, Vector3(h1, -h2, 0), Vector3(-h1, -h2, 0), Vector3(-h1, h2, 0), Vector3(0, 0, -1),
-          Vector3(h1, -h2, 0), Vector3(-h1, h2, 0), Vector3(h1, h2, 0))
-OMR_FLOOR(AddFloorYZ_xDown, Vector3(0, -h1, h2), Vector3(0, h1, h2), Vector3(0, h1, -h2), Vector3(-1, 0, 0),
-          Vector3(0, -h1, h2), Vector3(0, h1, -h2), Vector3(0, -h1, -h2))
+#define OMR_UV(a, b, c, d, e, f) a, b, c, d, e, f
+OMR_FLOOR(AddFloorXZ_yUp, Vector3(-h1, 0, -h2), Vector3(-h1, 0, h2), Vector3(h1, 0, -h2), OMR_UV(0, 0, 0, 1, 1, 0),
+          Vector3(0, 1, 0), Vector3(h1, 0, -h2), Vector3(-h1, 0, h2), Vector3(h1, 0, h2), OMR_UV(1, 0, 0, 1, 1, 1))
+OMR_FLOOR(AddFloorXY_zUp, Vector3(-h1, h2, 0), Vector3(-h1, -h2, 0), Vector3(h1, -h2, 0), OMR_UV(0, 0, 0, 1, 1, 1),
+          Vector3(0, 0, 1), Vector3(h1, h2, 0), Vector3(-h1, h2, 0), Vector3(h1, -h2, 0), OMR_UV(1, 0, 0, 0, 1, 1))
+OMR_FLOOR(AddFloorYZ_xUp, Vector3(0, h1, -h2), Vector3(0, h1, h2), Vector3(0, -h1, h2), OMR_UV(1, 0, 0, 0, 0, 1),
+          Vector3(1, 0, 0), Vector3(0, -h1, -h2), Vector3(0, h1, -h2), Vector3(0, -h1, h2), OMR_UV(1, 1, 1, 0, 0, 1))
+OMR_FLOOR(AddFloorXZ_yDown, Vector3(h1, 0, -h2), Vector3(-h1, 0, h2), Vector3(-h1, 0, -h2), OMR_UV(1, 1, 0, 0, 0, 1),
+          Vector3(0, -1, 0), Vector3(h1, 0, h2), Vector3(-h1, 0, h2), Vector3(h1, 0, -h2), OMR_UV(1, 0, 0, 0, 1, 1))
+OMR_FLOOR(AddFloorXY_zDown, Vector3(h1, -h2, 0), Vector3(-h1, -h2, 0), Vector3(-h1, h2, 0), OMR_UV(0, 1, 1, 1, 1, 0),
+          Vector3(0, 0, -1), Vector3(h1, -h2, 0), Vector3(-h1, h2, 0), Vector3(h1, h2, 0), OMR_UV(0, 1, 1, 0, 0, 0))
+OMR_FLOOR(AddFloorYZ_xDown, Vector3(0, -h1, h2), Vector3(0, h1, h2), Vector3(0, h1, -h2), OMR_UV(1, 1, 1, 0, 0, 0),
+          Vector3(-1, 0, 0), Vector3(0, -h1, h2), Vector3(0, h1, -h2), Vector3(0, -h1, -h2), OMR_UV(1, 1, 0, 0, 0, 1))
+#undef OMR_UV
 #undef OMR_FLOOR
 
 PathTracer::PathTracer(const Camera &camera, int samples, int supersamples)

@@ -68,17 +68,44 @@ class Ray {
 
 // Texture registry (ImageHandler, src/tools/ImageHandler.h): file name -> id.
 // Ids only: texture lookups are not part of the shading path.
+// Image (src/tools/Image.h:4-35): pixels as Color, row 0 = top of the image.
+class Image {
+ public:
+  virtual ~Image() {}
+  size_t GetHeight() const { return m_height; }
+  size_t GetWidth() const { return m_width; }
+  const Color &GetPixel(size_t x, size_t y) const { return m_image[x + y * m_width]; }
+  // u, v clamped to [0, 1] and scaled by 0.99999, nearest texel (Image.h:21-30)
+  const Color &GetPixelByUV(double u, double v) const;
+
+  std::vector<Color> m_image;
+  size_t m_width = 0, m_height = 0;
+};
+
+// ImageHandler (src/tools/ImageHandler.h:8-39, ImageHandler.cpp:20-187): the
+// process-wide image table that Material::texture_id indexes.  LoadFromFile
+// reads .hdr files as Radiance RGBE (linear) and other files as 8-bit PNG or
+// baseline JPEG, scaled to [0, 1] and, by default, raised to the power 2.2;
+// the same LDR file name gives the same id.  INVALID_IMAGE_ID when the file
+// cannot be read (the reason: LastError()).
 class ImageHandler {
  public:
   typedef int IMAGE_ID;
   static const IMAGE_ID INVALID_IMAGE_ID = -1;
   static ImageHandler &GetInstance();
-  IMAGE_ID LoadFromFile(const std::string &file);
+  IMAGE_ID LoadFromFile(const std::string &fname, bool doReverseGamma = true);
+  IMAGE_ID CreateImage(size_t width, size_t height);
+  Image *GetImage(IMAGE_ID id);
+  const Image *GetImage(IMAGE_ID id) const;
+  void ReleaseImage(IMAGE_ID id);
   const std::string &GetFileName(IMAGE_ID id) const;
-  size_t GetImageCount() const { return m_files.size(); }
+  size_t GetImageCount() const { return m_images.size(); }
+  const std::string &LastError() const { return m_error; }
 
  private:
-  std::vector<std::string> m_files;
+  std::vector<std::unique_ptr<Image>> m_images;
+  std::vector<std::string> m_files;  // parallel to m_images ("" for created images)
+  std::string m_error;
 };
 
 class Material {
@@ -225,13 +252,10 @@ class InfiniteFloor : public SceneObject {
 
 // Radiance RGBE image (HDRImage, src/tools/HDRImage.h:8-81): read and write,
 // run-length encoded scanlines as the reference writes them.
-class HDRImage {
+class HDRImage : public Image {
  public:
   bool ReadFromRadianceFile(const std::string &file);
   bool WriteToRadianceFile(const std::string &file);
-  size_t GetWidth() const { return m_width; }
-  size_t GetHeight() const { return m_height; }
-  const Color &GetPixel(size_t x, size_t y) const { return m_image[x + y * m_width]; }
   // replaces the pixels (row-major, row 0 first)
   void SetImage(size_t width, size_t height, std::vector<Color> pixels);
   static void Color2RGBE(unsigned char rgbe[4], const Color &c);
@@ -244,8 +268,6 @@ class HDRImage {
     float gamma = 0.0f, exposure = 0.0f;
     int height = 0, width = 0;
   };
-  std::vector<Color> m_image;
-  size_t m_width = 0, m_height = 0;
   Header m_info;
 };
 
@@ -270,10 +292,12 @@ class Model;
 // A scene flattened into what omr_scene_upload takes: triangles, then
 // spheres, then infinite floors; `desc` points into the vectors.
 struct SceneArrays {
-  std::vector<float> tri, nrm, sph, planes;
+  std::vector<float> tri, nrm, sph, planes, uv;
   std::vector<uint32_t> prim_mat, prim_mat_count;
   std::vector<omr_material> mats;
-  std::vector<SceneObject *> prims;  // primitive id -> object
+  std::vector<SceneObject *> prims;        // primitive id -> object
+  std::vector<std::vector<float>> tex_rgb; // textured images as float RGB
+  std::vector<omr_texture> textures;
   omr_scene_desc desc;
 };
 

@@ -22,11 +22,11 @@ LAMBERT, SPECULAR, REFRACTION = 0, 1, 2
 
 class Material(C.Structure):
     _fields_ = [("type", C.c_uint32), ("color", C.c_float * 3), ("emission", C.c_float * 3),
-                ("ior", C.c_float), ("rate", C.c_float), ("pad", C.c_uint32 * 3)]
+                ("ior", C.c_float), ("rate", C.c_float), ("texture", C.c_uint32), ("pad", C.c_uint32 * 2)]
 
 
 MATERIAL_DTYPE = np.dtype([("type", "<u4"), ("color", "<f4", 3), ("emission", "<f4", 3), ("ior", "<f4"),
-                           ("rate", "<f4"), ("pad", "<u4", 3)])
+                           ("rate", "<f4"), ("texture", "<u4"), ("pad", "<u4", 2)])
 assert MATERIAL_DTYPE.itemsize == C.sizeof(Material) == 48
 
 
@@ -35,7 +35,12 @@ class SceneDesc(C.Structure):
                 ("n_spheres", C.c_size_t), ("spheres", C.c_void_p), ("prim_material", C.c_void_p),
                 ("prim_material_count", C.c_void_p), ("n_materials", C.c_size_t), ("materials", C.c_void_p),
                 ("env_width", C.c_uint32), ("env_height", C.c_uint32), ("env_rgb", C.c_void_p),
-                ("n_planes", C.c_size_t), ("planes", C.c_void_p)]
+                ("n_planes", C.c_size_t), ("planes", C.c_void_p), ("n_textures", C.c_size_t),
+                ("textures", C.c_void_p), ("tri_uv", C.c_void_p)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgb", C.c_void_p)]
 
 
 class LoadedScene(C.Structure):
@@ -96,7 +101,7 @@ EXPORTS = [
     "omr_job_wait", "omr_job_cancel", "omr_server_stats_get", "omr_procedural_scene", "omr_procedural_free",
     "omr_camera_from_lookat", "omr_bvh_build", "omr_bvh_data", "omr_bvh_free", "omr_set_profiling", "omr_autotune",
     "omr_procedural_camera", "omr_io_last_error", "omr_free", "omr_hdr_read", "omr_hdr_write", "omr_scene_file_load",
-    "omr_obj_load", "omr_loaded_scene_free",
+    "omr_obj_load", "omr_loaded_scene_free", "omr_image_read", "omr_texture_read",
 ]
 
 _lib = None
@@ -164,6 +169,10 @@ def load():
                                  C.POINTER(C.c_double * 3), C.c_int, C.POINTER(LoadedScene)]
     lib.omr_loaded_scene_free.argtypes = [C.POINTER(LoadedScene)]
     lib.omr_loaded_scene_free.restype = None
+    lib.omr_image_read.argtypes = [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                   C.POINTER(vp)]
+    lib.omr_texture_read.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                     C.POINTER(vp)]
     _lib = lib
     return lib
 
@@ -175,8 +184,13 @@ def _ptr(a):
 class Scene:
     """A scene as numpy arrays (the omr_scene_desc layout)."""
 
-    def __init__(self, tri, tri_n, sph, prim_mat, mats, prim_mat_count=None, env=None, camera=None, planes=None):
+    def __init__(self, tri, tri_n, sph, prim_mat, mats, prim_mat_count=None, env=None, camera=None, planes=None,
+                 textures=None, tri_uv=None):
         self.planes = np.zeros(0, np.float32) if planes is None else np.ascontiguousarray(planes, dtype=np.float32)
+        # textures: list of (height, width, 3) linear RGB arrays, rows from the top;
+        # materials refer to them by texture = index + 1
+        self.textures = [np.ascontiguousarray(t, dtype=np.float32) for t in (textures or [])]
+        self.tri_uv = None if tri_uv is None else np.ascontiguousarray(tri_uv, dtype=np.float32).reshape(-1, 6)
         self.tri = np.ascontiguousarray(tri, dtype=np.float32).reshape(-1, 9)
         self.tri_n = None if tri_n is None else np.ascontiguousarray(tri_n, dtype=np.float32).reshape(-1, 9)
         self.sph = np.ascontiguousarray(sph, dtype=np.float32).reshape(-1, 4)
@@ -206,6 +220,14 @@ class Scene:
             d.env_rgb = _ptr(self.env)
         d.n_planes = self.planes.shape[0]
         d.planes = _ptr(self.planes) if self.planes.shape[0] else None
+        if self.textures:
+            self._tex = (Texture * len(self.textures))()
+            for k, t in enumerate(self.textures):
+                self._tex[k].height, self._tex[k].width = t.shape[0], t.shape[1]
+                self._tex[k].rgb = _ptr(t)
+            d.n_textures = len(self.textures)
+            d.textures = C.cast(self._tex, C.c_void_p)
+        d.tri_uv = _ptr(self.tri_uv) if self.tri_uv is not None and self.tri.shape[0] else None
         return d
 
 
@@ -228,7 +250,14 @@ def _scene_from_desc(d, camera=None):
     if d.env_rgb:
         env = arr(d.env_rgb, d.env_width * d.env_height * 3, np.float32).reshape(d.env_height, d.env_width, 3)
     planes = arr(d.planes, npl, np.float32)
-    return Scene(tri, trn, sph, pm, mats, pc, env, camera, planes)
+    textures = []
+    if d.n_textures and d.textures:
+        tx = C.cast(d.textures, C.POINTER(Texture))
+        for k in range(d.n_textures):
+            w, h = tx[k].width, tx[k].height
+            textures.append(arr(tx[k].rgb, w * h * 3, np.float32).reshape(h, w, 3))
+    uv = arr(d.tri_uv, nt * 6, np.float32).reshape(-1, 6) if d.tri_uv else None
+    return Scene(tri, trn, sph, pm, mats, pc, env, camera, planes, textures, uv)
 
 
 def _io_error(lib, what):
@@ -255,6 +284,33 @@ def hdr_write(path, img):
     a = np.ascontiguousarray(img, dtype=np.float32)
     if lib.omr_hdr_write(os.fsencode(path), a.shape[1], a.shape[0], _ptr(a)) != OMR_OK:
         raise _io_error(lib, "omr_hdr_write")
+
+
+def image_read(path):
+    """PNG / baseline JPEG file -> uint8 array (height, width, channels), rows from the top."""
+    lib = load()
+    w, h, c, p = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_void_p()
+    if lib.omr_image_read(os.fsencode(path), C.byref(w), C.byref(h), C.byref(c), C.byref(p)) != OMR_OK:
+        raise _io_error(lib, "omr_image_read")
+    try:
+        n = w.value * h.value * c.value
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), shape=(n,)).copy().reshape(h.value, w.value,
+                                                                                                  c.value)
+    finally:
+        lib.omr_free(p)
+
+
+def texture_read(path, inverse_gamma=True):
+    """An image file as the renderer's texture: float32 (height, width, 3), linear."""
+    lib = load()
+    w, h, p = C.c_uint32(), C.c_uint32(), C.c_void_p()
+    if lib.omr_texture_read(os.fsencode(path), int(inverse_gamma), C.byref(w), C.byref(h), C.byref(p)) != OMR_OK:
+        raise _io_error(lib, "omr_texture_read")
+    try:
+        n = w.value * h.value * 3
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), shape=(n,)).copy().reshape(h.value, w.value, 3)
+    finally:
+        lib.omr_free(p)
 
 
 def load_scene_file(path):

@@ -588,6 +588,27 @@ void orc_env_texel(float dx, float dy, float dz, uint32_t w, uint32_t h, uint32_
   *iy = (uint32_t)((int)(v * (float)h) % (int)h);
 }
 
+/* Sphere::CheckIntersection's texture uv (src/renderer/Sphere.h:52-75):
+ * u = (atan2(n.z, n.x) + pi) / 2pi with atan2 = +-acos(n.x / |n.xz|) by the
+ * sign of n.z; v = 1 - (asin(n.y) + pi/2) / pi = acos(n.y) / pi. */
+void orc_sphere_uv(float nx, float ny, float nz, float *u, float *v) {
+  float rxz = sqrtf(nx * nx + nz * nz);
+  float phi = (rxz > 0.0f) ? orc_acos(nx / rxz) : 0.0f;
+  *u = (nz >= 0.0f ? ORC_PI + phi : ORC_PI - phi) / ORC_TWO_PI;
+  *v = orc_acos(ny) / ORC_PI;
+}
+
+/* Image::GetPixelByUV (src/tools/Image.h:21-30): Utils::Clamp to [0, 1]
+ * (NaN to 0 here), times 0.99999, texel (int(u w), int(v h)). */
+void orc_texel_xy(float u, float v, uint32_t w, uint32_t h, uint32_t *x, uint32_t *y) {
+  u = u > 0.0f ? (u < 1.0f ? u : 1.0f) : 0.0f;
+  v = v > 0.0f ? (v < 1.0f ? v : 1.0f) : 0.0f;
+  uint32_t ix = (uint32_t)(int)((u * 0.99999f) * (float)w);
+  uint32_t iy = (uint32_t)(int)((v * 0.99999f) * (float)h);
+  *x = ix < w ? ix : w - 1;
+  *y = iy < h ? iy : h - 1;
+}
+
 static inline v3 env_rgb(const orc_handle *h, v3 d) {
   uint32_t ix, iy;
   orc_env_texel(d.x, d.y, d.z, h->s.env_w, h->s.env_h, &ix, &iy);
@@ -831,7 +852,26 @@ static void trace_sample(const render_job *J, uint32_t pixel, uint32_t subpx, ui
         value -= r;
       }
       const orc_material *m = &S->mat[mfirst + sel];
-      v3 col = ld3(m->color), em = ld3(m->emission);
+      /* PathTracer.cpp:327-346,388-389: hit colour = colour x texel */
+      v3 col0 = ld3(m->color), em = ld3(m->emission);
+      v3 col = col0;
+      if (m->tex) {
+        float tu = 0.0f, tv = 0.0f;
+        if (pid < S->n_tri) {
+          if (S->tri_uv) {
+            const float *q = S->tri_uv + 6 * (size_t)pid;
+            tu = (q[2] * hr.u + q[4] * hr.v) + q[0];
+            tv = (q[3] * hr.u + q[5] * hr.v) + q[1];
+          }
+        } else if (pid < S->n_tri + S->n_sph) {
+          orc_sphere_uv(n.x, n.y, n.z, &tu, &tv);
+        }
+        const orc_texture *tx = &S->tex[m->tex - 1];
+        uint32_t ix, iy;
+        orc_texel_xy(tu, tv, tx->w, tx->h, &ix, &iy);
+        const float *px = tx->rgb + 3 * ((size_t)iy * tx->w + ix);
+        col = mk(col0.x * px[0], col0.y * px[1], col0.z * px[2]);
+      }
       /* PathTracer.cpp:398-416: Russian roulette on max(albedo) */
       float prr = col.x > (col.y > col.z ? col.y : col.z) ? col.x : (col.y > col.z ? col.y : col.z);
       if (p->max_depth == 0 && depth > 64) prr *= ldexpf(1.0f, -(int)(depth - 64));
@@ -899,9 +939,10 @@ static void trace_sample(const render_job *J, uint32_t pixel, uint32_t subpx, ui
           float dd = dot(d, nl);
           float cos2t = 1.0f - nr * nr * (1.0f - dd * dd);
           if (cos2t < 0.0f) {
+            /* total internal reflection weighs by the untextured colour (PathTracer.cpp:543-548) */
             if (can_continue) {
               float k = 1.0f / prr;
-              beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
+              beta = mk(beta.x * (col0.x * k), beta.y * (col0.y * k), beta.z * (col0.z * k));
               o = x; d = refl; depth++; cont = 1;
             }
           } else {

@@ -31,8 +31,15 @@ typedef struct {
   float emission[3];
   float ior;
   float rate;           /* normalized roulette rate */
-  uint32_t pad[3];
+  uint32_t tex;         /* 0 = none, k = texture k-1 (Material::texture_id) */
+  uint32_t pad[2];
 } orc_material;
+
+/* Texture (Image, src/tools/Image.h:4-35): linear RGB rows from the top. */
+typedef struct {
+  uint32_t w, h;
+  const float *rgb;     /* h x w x 3 */
+} orc_texture;
 
 typedef struct {
   size_t n_tri;
@@ -48,6 +55,9 @@ typedef struct {
   const float *env;      /* env_h rows x env_w x 3 (row 0 = +y), or NULL */
   size_t n_pln;
   const float *pln;      /* n_pln planes y = pln[i] (InfiniteFloor.h:12-41), tested after the BVH */
+  size_t n_tex;
+  const orc_texture *tex;
+  const float *tri_uv;   /* n_tri x 6: uv0, uv1-uv0, uv2-uv0 (Polygon.h:22-24), or NULL = uv 0 */
 } orc_scene;
 
 typedef struct {
@@ -90,6 +100,11 @@ int orc_intersect_sphere(const float ray[8], const float sph[4], float *t);
  * |d.y| >= EPS, t = (y - o.y) / d.y >= EPS. */
 int orc_intersect_plane(const float ray[8], float y, float *t);
 void orc_env_texel(float dx, float dy, float dz, uint32_t w, uint32_t h, uint32_t *ix, uint32_t *iy);
+/* Latlong uv of a sphere normal (Sphere.h:52-75): u = (atan2(n.z, n.x) + pi) / 2pi,
+ * v = 1 - (asin(n.y) + pi/2) / pi. */
+void orc_sphere_uv(float nx, float ny, float nz, float *u, float *v);
+/* Image::GetPixelByUV (src/tools/Image.h:21-30) texel coordinates. */
+void orc_texel_xy(float u, float v, uint32_t w, uint32_t h, uint32_t *x, uint32_t *y);
 void orc_camera_ray(const orc_camera *cam, float x, float y, const float lens[2], float o[3], float d[3]);
 
 /* ---- scene handle ---- */

@@ -18,7 +18,12 @@ class OScene(C.Structure):
     _fields_ = [("n_tri", C.c_size_t), ("tri", C.c_void_p), ("tri_n", C.c_void_p), ("n_sph", C.c_size_t),
                 ("sph", C.c_void_p), ("prim_mat", C.c_void_p), ("prim_mat_count", C.c_void_p), ("n_mat", C.c_size_t),
                 ("mat", C.c_void_p), ("env_w", C.c_uint32), ("env_h", C.c_uint32), ("env", C.c_void_p),
-                ("n_pln", C.c_size_t), ("pln", C.c_void_p)]
+                ("n_pln", C.c_size_t), ("pln", C.c_void_p), ("n_tex", C.c_size_t), ("tex", C.c_void_p),
+                ("tri_uv", C.c_void_p)]
+
+
+class OTexture(C.Structure):
+    _fields_ = [("w", C.c_uint32), ("h", C.c_uint32), ("rgb", C.c_void_p)]
 
 
 class OCamera(C.Structure):
@@ -60,6 +65,10 @@ def load(build_if_missing=True):
     lib.orc_intersect_plane.argtypes = [vp, C.c_float, C.POINTER(C.c_float)]
     lib.orc_env_texel.argtypes = [C.c_float, C.c_float, C.c_float, u32, u32, C.POINTER(u32), C.POINTER(u32)]
     lib.orc_env_texel.restype = None
+    lib.orc_sphere_uv.argtypes = [C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    lib.orc_sphere_uv.restype = None
+    lib.orc_texel_xy.argtypes = [C.c_float, C.c_float, u32, u32, C.POINTER(u32), C.POINTER(u32)]
+    lib.orc_texel_xy.restype = None
     lib.orc_camera_ray.argtypes = [C.POINTER(OCamera), C.c_float, C.c_float, vp, vp, vp]
     lib.orc_camera_ray.restype = None
     lib.orc_prepare.argtypes = [C.POINTER(OScene)]
@@ -118,6 +127,17 @@ class Oracle:
         if planes is not None and planes.shape[0]:
             s.n_pln = planes.shape[0]
             s.pln = _p(planes)
+        textures = getattr(scene, "textures", None) or []
+        if textures:
+            self._tex = (OTexture * len(textures))()
+            for k, t in enumerate(textures):
+                self._tex[k].h, self._tex[k].w = t.shape[0], t.shape[1]
+                self._tex[k].rgb = _p(t)
+            s.n_tex = len(textures)
+            s.tex = C.cast(self._tex, C.c_void_p)
+        tri_uv = getattr(scene, "tri_uv", None)
+        if tri_uv is not None and s.n_tri:
+            s.tri_uv = _p(tri_uv)
         self._s = s
         self.h = self.lib.orc_prepare(C.byref(s))
         self._wide = None

@@ -96,12 +96,20 @@ def _load_io(lib):
     lib.ref_file_scene_free.restype = None
     lib.ref_file_scene_objects.argtypes = [vp]
     lib.ref_file_scene_objects.restype = sz
-    lib.ref_file_trace.argtypes = [vp, vp, sz, vp, vp, vp, vp, vp]
+    lib.ref_file_trace.argtypes = [vp, vp, sz, vp, vp, vp, vp, vp, vp]
     lib.ref_file_trace.restype = None
     lib.ref_hdr_read.argtypes = [C.c_char_p, vp, vp]
     lib.ref_hdr_read.restype = C.c_int
     lib.ref_hdr_write.argtypes = [C.c_char_p, C.c_int32, C.c_int32, vp]
     lib.ref_hdr_write.restype = C.c_int
+    lib.ref_image_load.argtypes = [C.c_char_p, vp, vp]
+    lib.ref_image_load.restype = C.c_int
+    lib.ref_texture_load.argtypes = [C.c_char_p, C.c_int, vp, vp]
+    lib.ref_texture_load.restype = C.c_int
+    lib.ref_sphere_uv.argtypes = [vp, sz, vp]
+    lib.ref_sphere_uv.restype = None
+    lib.ref_texel_xy.argtypes = [C.c_int32, C.c_int32, vp, sz, vp]
+    lib.ref_texel_xy.restype = None
     return lib
 
 
@@ -141,8 +149,9 @@ class RefFileScene:
         mat = np.zeros((n, 4, 9), np.float32)
         nmat = np.zeros(n, np.int32)
         obj = np.empty(n, np.int64)
-        self.lib.ref_file_trace(self.h, _p(rays), n, _p(t), _p(nrm), _p(mat), _p(nmat), _p(obj))
-        return dict(t=t, normal=nrm, mat=mat, nmat=nmat, obj=obj)
+        uv = np.zeros((n, 2), np.float64)
+        self.lib.ref_file_trace(self.h, _p(rays), n, _p(t), _p(nrm), _p(mat), _p(nmat), _p(obj), _p(uv))
+        return dict(t=t, normal=nrm, mat=mat, nmat=nmat, obj=obj, uv=uv)
 
 
 def hdr_read(path):
@@ -161,3 +170,41 @@ def hdr_write(path, rgb):
     h, w = rgb.shape[0], rgb.shape[1]
     if lib.ref_hdr_write(path.encode(), w, h, _p(rgb)) != 0:
         raise RuntimeError(f"reference could not write {path}")
+
+
+def image_load(path):
+    """stbi_load as the reference calls it: uint8 (h, w, channels)."""
+    lib = _load_io(load())
+    whc = np.zeros(3, np.int32)
+    if lib.ref_image_load(path.encode(), _p(whc), None) != 0:
+        raise RuntimeError(f"reference could not read {path}")
+    out = np.empty((whc[1], whc[0], whc[2]), np.uint8)
+    lib.ref_image_load(path.encode(), _p(whc), _p(out))
+    return out
+
+
+def texture_load(path, inverse_gamma=True):
+    """ImageHandler::LoadFromFile's pixels as float32 (h, w, 3)."""
+    lib = _load_io(load())
+    wh = np.zeros(2, np.int32)
+    if lib.ref_texture_load(path.encode(), int(inverse_gamma), _p(wh), None) != 0:
+        raise RuntimeError(f"reference could not read {path}")
+    out = np.empty((wh[1], wh[0], 3), np.float32)
+    lib.ref_texture_load(path.encode(), int(inverse_gamma), _p(wh), _p(out))
+    return out
+
+
+def sphere_uv(normals):
+    lib = _load_io(load())
+    nrm = np.ascontiguousarray(normals, np.float64).reshape(-1, 3)
+    uv = np.empty((nrm.shape[0], 2), np.float64)
+    lib.ref_sphere_uv(_p(nrm), nrm.shape[0], _p(uv))
+    return uv
+
+
+def texel_xy(w, h, uv):
+    lib = _load_io(load())
+    uv = np.ascontiguousarray(uv, np.float64).reshape(-1, 2)
+    xy = np.empty((uv.shape[0], 2), np.int32)
+    lib.ref_texel_xy(w, h, _p(uv), uv.shape[0], _p(xy))
+    return xy

@@ -19,6 +19,8 @@
 #include "scenes/Scene.h"
 #include "scenes/SceneFromExternalFile.h"
 #include "tools/HDRImage.h"
+#include "tools/ImageHandler.h"
+#include "stb/stb_image.h"
 #include "tools/Random.h"
 
 using namespace OmochiRenderer;
@@ -140,7 +142,7 @@ size_t ref_file_scene_objects(void *s) { return ((FileScene *)s)->Count(); }
 // in the scene's object list (-1 on a miss), and its first four materials as
 // floats (type, color, emission, ior, rate; 9 per material, 36 per ray).
 void ref_file_trace(void *sp, const float *rays, size_t n, double *t, double *normal, float *mats, int32_t *nmat,
-                    int64_t *obj) {
+                    int64_t *obj, double *uv) {
   FileScene *s = (FileScene *)sp;
   for (size_t i = 0; i < n; i++) {
     const float *r = rays + 8 * i;
@@ -153,6 +155,8 @@ void ref_file_trace(void *sp, const float *rays, size_t n, double *t, double *no
     normal[3 * i] = info.hit.normal.x;
     normal[3 * i + 1] = info.hit.normal.y;
     normal[3 * i + 2] = info.hit.normal.z;
+    uv[2 * i] = info.hit.uv.x;
+    uv[2 * i + 1] = info.hit.uv.y;
     const size_t cnt = info.object->GetMaterialCount();
     for (size_t k = 0; k < cnt && k < 4; k++) {
       const SceneObject::MultiMaterialElem *e = info.object->GetMaterial(k);
@@ -184,6 +188,73 @@ int ref_hdr_read(const char *path, int32_t *wh, float *out) {
         o[0] = (float)c.x; o[1] = (float)c.y; o[2] = (float)c.z;
       }
   return 0;
+}
+
+// The reference's image library call (stbi_load with native channels, as
+// ImageHandler.cpp:54): dims and channel count into whc, 8-bit samples into
+// out when out != NULL.
+int ref_image_load(const char *path, int32_t *whc, uint8_t *out) {
+  int w, h, c;
+  unsigned char *px = stbi_load(path, &w, &h, &c, 0);
+  if (!px) return -1;
+  whc[0] = w;
+  whc[1] = h;
+  whc[2] = c;
+  if (out) std::memcpy(out, px, (size_t)w * h * c);
+  stbi_image_free(px);
+  return 0;
+}
+
+// ImageHandler::LoadFromFile + GetImage: the texture as the renderer sees it
+// (floats, rows from the top).
+int ref_texture_load(const char *path, int inverse_gamma, int32_t *wh, float *out) {
+  ImageHandler &ih = ImageHandler::GetInstance();
+  ImageHandler::IMAGE_ID id = ih.LoadFromFile(path, inverse_gamma != 0);
+  if (id == ImageHandler::INVALID_IMAGE_ID) return -1;
+  const Image *img = ih.GetImage(id);
+  wh[0] = (int32_t)img->GetWidth();
+  wh[1] = (int32_t)img->GetHeight();
+  if (out)
+    for (size_t i = 0; i < img->GetWidth() * img->GetHeight(); i++) {
+      const Color &c = img->m_image[i];
+      out[3 * i] = (float)c.x;
+      out[3 * i + 1] = (float)c.y;
+      out[3 * i + 2] = (float)c.z;
+    }
+  return 0;
+}
+
+// Sphere::CheckIntersection's texture uv (Sphere.h:52-75) for unit normals:
+// a unit sphere at the origin with a textured material, hit from outside
+// along -n at n.
+void ref_sphere_uv(const double *nrm, size_t n, double *uv) {
+  Material m(Material::REFLECTION_TYPE_LAMBERT, Vector3(0, 0, 0), Vector3(1, 1, 1));
+  m.texture_id = 0;
+  Sphere s(1.0, Vector3(0, 0, 0), m);
+  for (size_t i = 0; i < n; i++) {
+    Vector3 d(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+    Ray ray(d * 3.0, d * -1.0);
+    HitInformation hit;
+    uv[2 * i] = uv[2 * i + 1] = -1.0;
+    if (s.CheckIntersection(ray, hit)) {
+      uv[2 * i] = hit.uv.x;
+      uv[2 * i + 1] = hit.uv.y;
+    }
+  }
+}
+
+// Image::GetPixelByUV (Image.h:21-30) on a w x h image: texel column and
+// row of each (u, v), read off the address of the returned pixel.
+void ref_texel_xy(int32_t w, int32_t h, const double *uv, size_t n, int32_t *xy) {
+  Image img;
+  img.m_width = (size_t)w;
+  img.m_height = (size_t)h;
+  img.m_image.resize((size_t)w * h);
+  for (size_t i = 0; i < n; i++) {
+    long k = (long)(&img.GetPixelByUV(uv[2 * i], uv[2 * i + 1]) - img.m_image.data());
+    xy[2 * i] = (int32_t)(k % w);
+    xy[2 * i + 1] = (int32_t)(k / w);
+  }
 }
 
 // HDRImage::WriteToRadianceFile of a w x h float image.

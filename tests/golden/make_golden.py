@@ -25,11 +25,13 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import golden_inputs as gi  # noqa: E402
+import imgenc  # noqa: E402
 import omr  # noqa: E402
 import pyref  # noqa: E402
 
 REF_HDR = "/root/reference/input_data/Barce_Rooftop_C_Env.hdr"
 SCENES = os.path.join(HERE, "scenes")
+TEX = os.path.join(HERE, "tex")
 
 
 def sha(a):
@@ -112,23 +114,72 @@ def plane_hits():
 
 
 def loader_hits():
-    """Closest hits through the reference's own loading of
-    tests/golden/scenes/loader.scene (relative names resolve from there)."""
-    rays = gi.loader_rays()
-    cwd = os.getcwd()
-    os.chdir(SCENES)
-    try:
-        r = pyref.RefFileScene("loader.scene")
-        res = r.trace(rays)
-        nobj = r.objects()
-        r.close()
-    finally:
-        os.chdir(cwd)
-    np.savez_compressed(os.path.join(HERE, "hits_loader.npz"), rays_sha256=sha(rays), n_objects=nobj, **res)
-    print("loader hits", int((res["obj"] >= 0).sum()), "of", len(rays), "objects", nobj)
+    """Closest hits through the reference's own loading of the test scenes
+    in tests/golden/scenes/ (relative names resolve from there)."""
+    for which in ("loader", "textured"):
+        rays = gi.loader_rays(which)
+        cwd = os.getcwd()
+        os.chdir(SCENES)
+        try:
+            r = pyref.RefFileScene(f"{which}.scene")
+            res = r.trace(rays)
+            nobj = r.objects()
+            r.close()
+        finally:
+            os.chdir(cwd)
+        np.savez_compressed(os.path.join(HERE, f"hits_{which}.npz"), rays_sha256=sha(rays), n_objects=nobj, **res)
+        print(which, "hits", int((res["obj"] >= 0).sum()), "of", len(rays), "objects", nobj)
+
+
+def texture_lookup():
+    """Sphere::CheckIntersection's uv and Image::GetPixelByUV's texel on
+    seeded inputs."""
+    nrm, q = gi.sphere_normals(), gi.uv_queries()
+    out = {"sphere_normals_sha256": sha(nrm), "sphere_uv": pyref.sphere_uv(nrm), "uv_sha256": sha(q)}
+    for w, h in gi.TEXEL_SIZES:
+        out[f"texel_{w}x{h}"] = pyref.texel_xy(w, h, q)
+    np.savez_compressed(os.path.join(HERE, "texture_lookup.npz"), **out)
+    print("texture lookup vectors")
+
+
+def textures():
+    """PNG and JPEG files written by the encoders in imgenc.py (kept as data),
+    and the reference's decode of each: its image library (stbi_load, 8-bit
+    samples) and ImageHandler::LoadFromFile (linear floats)."""
+    os.makedirs(TEX, exist_ok=True)
+    out = {}
+    for name in gi.PNG_SETS:
+        samples, pal, trns = gi.png_samples(name)
+        o = gi.PNG_SETS[name]
+        data = imgenc.png_encode(samples, o["ctype"], palette=pal, trns=trns, interlace=o["interlace"])
+        path = os.path.join(TEX, name + ".png")
+        open(path, "wb").write(data)
+    for name, o in gi.JPEG_SETS.items():
+        w, h = gi.TEX_SIZE
+        rgb = gi.texture_image(w, h, o["seed"])
+        data = imgenc.jpeg_encode(rgb, sampling=o["sampling"], quality=o["quality"], restart=o["restart"],
+                                  separate_scans=o["separate_scans"], grey=o["grey"])
+        open(os.path.join(TEX, name + ".jpg"), "wb").write(data)
+    for f in sorted(os.listdir(TEX)):
+        path = os.path.join(TEX, f)
+        key = os.path.splitext(f)[0]
+        try:
+            out[key + "_stb"] = pyref.image_load(path)
+        except RuntimeError:
+            print("reference image library cannot read", f)
+        if key.endswith("grey") or key.endswith("grey_alpha"):
+            continue
+        try:
+            out[key + "_texture"] = pyref.texture_load(path)
+        except RuntimeError:
+            print("reference ImageHandler cannot read", f)
+    np.savez_compressed(os.path.join(HERE, "tex_decoded.npz"), **out)
+    print("textures", sorted(out))
 
 
 if __name__ == "__main__":
+    textures()
+    texture_lookup()
     hdr_files()
     plane_hits()
     loader_hits()

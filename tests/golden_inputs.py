@@ -85,19 +85,24 @@ def plane_scene():
     return omr.Scene(s.tri, s.tri_n, s.sph, pm, mats, planes=np.array(PLANES_Y, np.float32))
 
 
-def loader_rays():
-    """Rays through the test .scene: a third from a box around the objects
-    in uniform directions (many reach the infinite floor from above or
-    below), a third aimed from outside at points inside the objects' box and
-    a third at points in the box of the placed mesh."""
-    n, seed = LOADER_RAYS
+def loader_rays(which="loader"):
+    """Rays through a test .scene: a third from a box around the objects in
+    uniform directions (many reach the infinite floor from above or below),
+    a third aimed from outside at points inside the objects' box and a third
+    at points in the box of the placed mesh."""
+    n, seed = LOADER_RAYS if which == "loader" else TEXTURED_RAYS
     rng = np.random.default_rng(seed)
-    lo, hi = np.array([-60.0, -20.0, -60.0]), np.array([60.0, 50.0, 40.0])
+    if which == "loader":
+        start = ([-60.0, -20.0, -60.0], [60.0, 50.0, 40.0])
+        boxes = (([-25.0, -2.0, -40.0], [45.0, 42.0, 25.0]), ([4.0, 3.0, -32.0], [26.0, 26.0, -8.0]))
+    else:
+        start = ([-50.0, -10.0, -50.0], [50.0, 50.0, 50.0])
+        boxes = (([-22.0, -1.0, -20.0], [22.0, 18.0, 25.0]), ([-20.0, 0.0, -22.0], [0.0, 17.0, -8.0]))
+    lo, hi = np.array(start[0]), np.array(start[1])
     n1 = n // 3
     o = [lo + (hi - lo) * rng.random((n1, 3))]
     d = [helpers.unit_dirs(rng, n1)]
-    for (tlo, thi), m in ((([-25.0, -2.0, -40.0], [45.0, 42.0, 25.0]), n1),
-                          (([4.0, 3.0, -32.0], [26.0, 26.0, -8.0]), n - 2 * n1)):
+    for (tlo, thi), m in ((boxes[0], n1), (boxes[1], n - 2 * n1)):
         tlo, thi = np.array(tlo), np.array(thi)
         tgt = tlo + (thi - tlo) * rng.random((m, 3))
         oo = (tlo + thi) / 2 + 90.0 * helpers.unit_dirs(rng, m)
@@ -105,3 +110,76 @@ def loader_rays():
         o.append(oo)
         d.append(dd / np.linalg.norm(dd, axis=1, keepdims=True))
     return helpers.make_rays(np.concatenate(o).astype(np.float32), np.concatenate(d).astype(np.float32))
+
+
+TEXTURED_RAYS = (30000, 505)
+
+# Texture fixtures in tests/golden/tex/: name -> (format, width, height, seed, options)
+TEX_SIZE = (37, 21)
+
+
+def texture_image(w, h, seed):
+    """Smooth colour ramps, a hard edge and noise: rows from the top, uint8 RGB."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:h, 0:w]
+    r = 255.0 * x / max(1, w - 1)
+    g = 255.0 * y / max(1, h - 1)
+    b = np.where((x // 6 + y // 5) % 2 == 0, 230.0, 25.0)
+    img = np.stack([r, g, b], axis=2) + rng.normal(0, 6, (h, w, 3))
+    return np.clip(np.round(img), 0, 255).astype(np.uint8)
+
+
+PNG_SETS = {
+    "png_rgb": dict(ctype=2, interlace=False, seed=11),
+    "png_rgba_adam7": dict(ctype=6, interlace=True, seed=12),
+    "png_pal": dict(ctype=3, interlace=False, seed=13),
+    "png_pal_trns": dict(ctype=3, interlace=True, seed=14, trns=True),
+    "png_grey": dict(ctype=0, interlace=False, seed=15),
+    "png_grey_alpha": dict(ctype=4, interlace=False, seed=16),
+}
+JPEG_SETS = {
+    "jpg_444": dict(sampling=(1, 1), quality=90, restart=0, separate_scans=False, grey=False, seed=21),
+    "jpg_422": dict(sampling=(2, 1), quality=80, restart=0, separate_scans=False, grey=False, seed=22),
+    "jpg_420_rst": dict(sampling=(2, 2), quality=75, restart=2, separate_scans=False, grey=False, seed=23),
+    "jpg_420_scans": dict(sampling=(2, 2), quality=85, restart=0, separate_scans=True, grey=False, seed=24),
+    "jpg_grey": dict(sampling=(1, 1), quality=85, restart=3, separate_scans=False, grey=True, seed=25),
+}
+
+
+def png_samples(name):
+    """(samples (h, w, spp), palette, trns, rgba reference) for a PNG set."""
+    o = PNG_SETS[name]
+    w, h = TEX_SIZE
+    rgb = texture_image(w, h, o["seed"])
+    alpha = ((np.arange(w)[None, :] * 7 + np.arange(h)[:, None] * 3) % 256).astype(np.uint8)[..., None]
+    if o["ctype"] == 2:
+        return rgb, None, None
+    if o["ctype"] == 6:
+        return np.concatenate([rgb, alpha], axis=2), None, None
+    if o["ctype"] == 0:
+        return rgb[..., :1], None, None
+    if o["ctype"] == 4:
+        return np.concatenate([rgb[..., :1], alpha], axis=2), None, None
+    # palette: 40 colours quantized from the image
+    pal = texture_image(8, 5, o["seed"] + 100).reshape(-1, 3)
+    d = ((rgb[:, :, None, :].astype(int) - pal[None, None, :, :].astype(int)) ** 2).sum(-1)
+    idx = d.argmin(-1).astype(np.uint8)[..., None]
+    trns = (np.arange(len(pal)) * 6 % 256).astype(np.uint8)[:30] if o.get("trns") else None
+    return idx, pal, trns
+
+
+def sphere_normals():
+    """Unit normals for the sphere uv vectors (axes included)."""
+    rng = np.random.default_rng(606)
+    axes = np.array([[1, 0, 0], [-1, 0, 0], [0, 0, 1], [0, 0, -1], [0, 1, 0], [0, -1, 0]], float)
+    return np.concatenate([helpers.unit_dirs(rng, 4000).astype(np.float64), axes])
+
+
+def uv_queries():
+    """(u, v) pairs for the texel vectors, some outside [0, 1]."""
+    rng = np.random.default_rng(607)
+    edge = np.array([[0, 0], [1, 1], [0.5, 0.5], [-1, 2], [0.999999, 0.25]], float)
+    return np.concatenate([rng.random((4000, 2)) * 1.2 - 0.1, edge])
+
+
+TEXEL_SIZES = ((37, 21), (64, 32), (1, 1))

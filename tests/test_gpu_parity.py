@@ -174,11 +174,13 @@ def test_trace_planes_bit_exact(ctx):
     ctx.release(sid)
 
 
-@pytest.mark.parametrize("env_is", [0, 1])
-def test_render_loaded_scene_bit_exact(ctx, env_is):
-    """A scene read from the .scene format (OBJ mesh with MTL materials,
-    multi-material sphere and floor, emissive sphere, glass infinite floor,
-    RGBE sky) renders to the same bits as the restatement."""
+@pytest.mark.parametrize("name,env_is", [("loader", 0), ("loader", 1), ("textured", 0), ("textured", 1)])
+def test_render_loaded_scene_bit_exact(ctx, name, env_is):
+    """Scenes read from the .scene format render to the same bits as the
+    restatement: loader.scene (OBJ mesh with MTL materials, multi-material
+    sphere and floor, emissive sphere, glass infinite floor, RGBE sky) and
+    textured.scene (PNG / JPEG textures on spheres, floor and mesh, a
+    textured refractive material)."""
     import os
 
     import omr
@@ -188,13 +190,17 @@ def test_render_loaded_scene_bit_exact(ctx, env_is):
     cwd = os.getcwd()
     os.chdir(here)
     try:
-        scene = omr.load_scene_file("loader.scene")
+        scene = omr.load_scene_file(f"{name}.scene")
     finally:
         os.chdir(cwd)
-    cam = omr.camera_lookat(48, 32, (0.0, 30.0, 110.0), (0.0, -0.25, -1.0), screen_height=30.0, dist=40.0)
+    if name == "textured":
+        assert len(scene.textures) == 6
+        cam = omr.camera_lookat(48, 32, (0.0, 25.0, 80.0), (0.0, -0.3, -1.0), screen_height=30.0, dist=40.0)
+    else:
+        cam = omr.camera_lookat(48, 32, (0.0, 30.0, 110.0), (0.0, -0.25, -1.0), screen_height=30.0, dist=40.0)
     sid = ctx.upload(scene)
     prm = omr.params(ss=2, spp_end=2, max_depth=8, rr_depth=3, env_is=env_is, seed=2024)
-    print(f"render loaded scene env_is={env_is}", flush=True)
+    print(f"render {name}.scene env_is={env_is}", flush=True)
     g = ctx.render_sums(sid, cam, prm)
     st = ctx.last_stats()
     c, seg, sh = _oracle(scene, ctx, sid).render_sums(cam, prm, pyoracle.WIDE_BVH)
