@@ -19,6 +19,10 @@
  *   orc_intersect_*      compiled reference Polygon/Sphere (FP64) golden hits
  *                        + hand-built closed-form cases
  *   orc_trace            brute force == own BVH == external wide BVH
+ *   orc_intersect_plane  compiled reference InfiniteFloor golden hits
+ *                        + closed-form cases
+ *   orc_sphere_uv        compiled reference Sphere uv golden vectors + axes
+ *   orc_texel_xy         compiled reference Image::GetPixelByUV golden vectors
  *   orc_env_texel        compiled reference IBL::Sample texel golden vectors
  *   orc_env_cdf/sample   chi^2 histogram + pdf integrates to 1 (closed form)
  *   orc_render           white furnace (closed form), compiled reference
