@@ -93,8 +93,9 @@ def run_batched(ctx, omr, args):
     ids = [sids[j["kind"]] for j in jobs]
     tune = None
     if not args.no_tune:
-        # tune on a reduced copy of the workload (every 4th job)
-        tune = ctx.autotune(ids[::4], cams[::4], prms[::4], budget_ms=args.tune_ms)
+        # tune on the workload itself: a subset of the jobs favours smaller
+        # pools than the whole batch wants
+        tune = ctx.autotune(ids, cams, prms, budget_ms=args.tune_ms)
     # plain: each job alone, one after another
     t0 = time.perf_counter()
     plain, rays_plain = [], 0

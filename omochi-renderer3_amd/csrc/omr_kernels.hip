@@ -283,7 +283,8 @@ struct Trav {
   f3 o, d;
   float ix, iy, iz;
   uint32_t nx, ny, nz;  // 1 when the ray runs towards -axis: the near slab is the upper bound
-  int32_t cur;
+  int32_t cur;   // current entry: inner node (>= 0), leaf code or kNoEntry
+  int32_t pend;  // postponed leaf or kNoEntry
   Hit best;   // u, v unnormalized until the traversal ends
   float det;  // divisor of best.u, best.v (1 for spheres)
   const float *planes;  // infinite floors, ids plane_base + k, tested after the BVH
@@ -325,74 +326,106 @@ struct Trav {
     best.prim = -1;
     det = 1.0f;
     cur = 0;
+    pend = -1;
     stk.sp = 0;
   }
 
   __device__ inline bool step(Stack<WG, SHORT> &stk, int budget, uint32_t &nvis, uint32_t &ntest);
 };
 
+// Lanes of a wave move in one of two phases per trip, chosen for the whole
+// wave: a node phase, where every lane whose current entry is an inner node
+// visits it, or a leaf phase, where every lane holding a postponed leaf tests
+// its primitives.  A lane that reaches a leaf postpones it (one slot) and
+// keeps descending, so node trips run with most lanes busy and leaf trips
+// gather many leaves at once; the wave switches to a leaf trip when no lane
+// can visit a node or at least kLeafBatch/64 of its working lanes hold a
+// leaf.  The
+// primitives a ray tests and its result do not depend on the trip order (the
+// tie rule "smaller t, then smaller id" and conservative boxes).
+constexpr uint32_t kLeafBatch = 32;
+constexpr int32_t kNoEntry = -1;
+
 template <int WG, int SHORT, bool ANY, bool STATS>
 __device__ inline bool Trav<WG, SHORT, ANY, STATS>::step(Stack<WG, SHORT> &stk, int budget, uint32_t &nvis,
                                                           uint32_t &ntest) {
   const float ox = o.x, oy = o.y, oz = o.z;
+  // next stack entry that can still hold a closer hit, or kNoEntry
+  auto pop_next = [&]() -> int32_t {
+    while (stk.sp > 0) {
+      uint2 e = stk.pop();
+      if (__uint_as_float(e.y) > best.t) continue;
+      return (int32_t)e.x;
+    }
+    return kNoEntry;
+  };
+  // a leaf reached as the current entry goes to the free postponed slot
+  auto postpone = [&]() {
+    if (pend == kNoEntry && child_is_leaf(cur)) {
+      pend = cur;
+      cur = pop_next();
+    }
+  };
   for (int it = 0; it < budget; it++) {
-    bool need_pop;
-    if (!child_is_leaf(cur)) {
-      // node = lo.x, hi.x, lo.y, hi.y, lo.z, hi.z (4 children each), child
-      // codes: the near and far slab of each axis are picked by address
-      const float4 *np = nodes + (size_t)cur * 8;
-      float4 nX = ld4(np + nx), fX = ld4(np + (nx ^ 1u));
-      float4 nY = ld4(np + 2u + ny), fY = ld4(np + 2u + (ny ^ 1u));
-      float4 nZ = ld4(np + 4u + nz), fZ = ld4(np + 4u + (nz ^ 1u));
-      float4 chf = ld4(np + 6);
-      if (STATS) nvis++;
-      float near_x[4] = {nX.x, nX.y, nX.z, nX.w}, far_x[4] = {fX.x, fX.y, fX.z, fX.w};
-      float near_y[4] = {nY.x, nY.y, nY.z, nY.w}, far_y[4] = {fY.x, fY.y, fY.z, fY.w};
-      float near_z[4] = {nZ.x, nZ.y, nZ.z, nZ.w}, far_z[4] = {fZ.x, fZ.y, fZ.z, fZ.w};
-      int32_t code[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z), __float_as_int(chf.w)};
-      float key[4];
-      // tmin = n > tmin ? n : tmin (and tmax likewise) as one v_max / v_min:
-      // equal for every input here (a NaN operand leaves the other one in
-      // both forms; n, f are never signalling NaNs; +0 and -0 compare equal
-      // in every later test)
+    const bool can_visit = cur >= 0;
+    const unsigned long long vm = __ballot(can_visit);
+    const uint32_t leaves = (uint32_t)__popcll(__ballot(pend != kNoEntry));
+    const uint32_t lanes = (uint32_t)__popcll(__ballot(true));
+    if (vm != 0ull && leaves * 64u < kLeafBatch * lanes) {
+      if (can_visit) {
+        // node = lo.x, hi.x, lo.y, hi.y, lo.z, hi.z (4 children each), child
+        // codes: the near and far slab of each axis are picked by address
+        const float4 *np = nodes + (size_t)cur * 8;
+        float4 nX = ld4(np + nx), fX = ld4(np + (nx ^ 1u));
+        float4 nY = ld4(np + 2u + ny), fY = ld4(np + 2u + (ny ^ 1u));
+        float4 nZ = ld4(np + 4u + nz), fZ = ld4(np + 4u + (nz ^ 1u));
+        float4 chf = ld4(np + 6);
+        if (STATS) nvis++;
+        float near_x[4] = {nX.x, nX.y, nX.z, nX.w}, far_x[4] = {fX.x, fX.y, fX.z, fX.w};
+        float near_y[4] = {nY.x, nY.y, nY.z, nY.w}, far_y[4] = {fY.x, fY.y, fY.z, fY.w};
+        float near_z[4] = {nZ.x, nZ.y, nZ.z, nZ.w}, far_z[4] = {fZ.x, fZ.y, fZ.z, fZ.w};
+        int32_t code[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z),
+                           __float_as_int(chf.w)};
+        float key[4];
+        // tmin = n > tmin ? n : tmin (and tmax likewise) as one v_max / v_min:
+        // equal for every input here (a NaN operand leaves the other one in
+        // both forms; n, f are never signalling NaNs; +0 and -0 compare equal
+        // in every later test)
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
-        float tmin = 0.0f, tmax = best.t;
-        float n = (near_x[k] - ox) * ix;
-        float f = (far_x[k] - ox) * ix;
-        tmin = __builtin_fmaxf(n, tmin);
-        tmax = __builtin_fminf(f, tmax);
-        n = (near_y[k] - oy) * iy;
-        f = (far_y[k] - oy) * iy;
-        tmin = __builtin_fmaxf(n, tmin);
-        tmax = __builtin_fminf(f, tmax);
-        n = (near_z[k] - oz) * iz;
-        f = (far_z[k] - oz) * iz;
-        tmin = __builtin_fmaxf(n, tmin);
-        tmax = __builtin_fminf(f, tmax);
-        bool h = (code[k] != -1) && (tmin <= tmax * kSlabInflate);
-        key[k] = h ? tmin : __builtin_huge_valf();
+        for (int k = 0; k < 4; k++) {
+          float tmin = 0.0f, tmax = best.t;
+          float n = (near_x[k] - ox) * ix;
+          float f = (far_x[k] - ox) * ix;
+          tmin = __builtin_fmaxf(n, tmin);
+          tmax = __builtin_fminf(f, tmax);
+          n = (near_y[k] - oy) * iy;
+          f = (far_y[k] - oy) * iy;
+          tmin = __builtin_fmaxf(n, tmin);
+          tmax = __builtin_fminf(f, tmax);
+          n = (near_z[k] - oz) * iz;
+          f = (far_z[k] - oz) * iz;
+          tmin = __builtin_fmaxf(n, tmin);
+          tmax = __builtin_fminf(f, tmax);
+          bool h = (code[k] != -1) && (tmin <= tmax * kSlabInflate);
+          key[k] = h ? tmin : __builtin_huge_valf();
+        }
+        // sorting network, nearest first
+        cswap(key[0], code[0], key[1], code[1]);
+        cswap(key[2], code[2], key[3], code[3]);
+        cswap(key[0], code[0], key[2], code[2]);
+        cswap(key[1], code[1], key[3], code[3]);
+        cswap(key[1], code[1], key[2], code[2]);
+        const float kmiss = __builtin_huge_valf();
+        if (key[3] != kmiss) stk.push(make_uint2((uint32_t)code[3], __float_as_uint(key[3])));
+        if (key[2] != kmiss) stk.push(make_uint2((uint32_t)code[2], __float_as_uint(key[2])));
+        if (key[1] != kmiss) stk.push(make_uint2((uint32_t)code[1], __float_as_uint(key[1])));
+        cur = (key[0] != kmiss && !(key[0] > best.t)) ? code[0] : pop_next();
+        postpone();
       }
-      // sorting network, nearest first
-      cswap(key[0], code[0], key[1], code[1]);
-      cswap(key[2], code[2], key[3], code[3]);
-      cswap(key[0], code[0], key[2], code[2]);
-      cswap(key[1], code[1], key[3], code[3]);
-      cswap(key[1], code[1], key[2], code[2]);
-      const float kmiss = __builtin_huge_valf();
-      if (key[3] != kmiss) stk.push(make_uint2((uint32_t)code[3], __float_as_uint(key[3])));
-      if (key[2] != kmiss) stk.push(make_uint2((uint32_t)code[2], __float_as_uint(key[2])));
-      if (key[1] != kmiss) stk.push(make_uint2((uint32_t)code[1], __float_as_uint(key[1])));
-      if (key[0] != kmiss && !(key[0] > best.t)) {
-        cur = code[0];
-        need_pop = false;
-      } else {
-        need_pop = true;
-      }
-    } else {
+    } else if (pend != kNoEntry) {
       // primitive records are 3 x 16 bytes; the next record is loaded while
       // the current one is tested
-      uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
+      uint32_t first = leaf_first(pend), cnt = leaf_count(pend);
       const float4 *pr = prims + (size_t)first * 3;
       float4 a = ld4(pr), b = ld4(pr + 1), c = ld4(pr + 2);
       for (uint32_t k = 0; k < cnt; k++) {
@@ -428,21 +461,12 @@ __device__ inline bool Trav<WG, SHORT, ANY, STATS>::step(Stack<WG, SHORT> &stk, 
         finish();
         return true;
       }
-      need_pop = true;
+      pend = kNoEntry;
+      postpone();
     }
-    if (need_pop) {
-      bool found = false;
-      while (stk.sp > 0) {
-        uint2 e = stk.pop();
-        if (__uint_as_float(e.y) > best.t) continue;
-        cur = (int32_t)e.x;
-        found = true;
-        break;
-      }
-      if (!found) {
-        finish();
-        return true;
-      }
+    if (cur == kNoEntry && pend == kNoEntry) {
+      finish();
+      return true;
     }
   }
   return false;
@@ -468,6 +492,7 @@ __device__ inline void traverse(const SceneDev &S, f3 o, f3 d, Hit &best, Stack<
 // launch.  Must be called by whole waves.  count(k): entries of sub-queue k;
 // load(k, off, S, o, d, tmax, tag) reads one entry; store(tag, hit) writes a
 // finished ray's result.
+static_assert(kShards <= 64, "the sub-queue scan gives one lane of a wave to each shard");
 template <int WG, int SHORT, bool ANY, bool STATS, class Count, class Load, class Store>
 __device__ inline uint32_t dynamic_fetch(Sharded<uint32_t> &heads, Count count, Stack<WG, SHORT> &stk, int budget,
                                          uint32_t fetch_min, Load load, Store store, uint32_t &nvis,
@@ -689,32 +714,69 @@ __device__ inline void camera_ray(const JobDev &J, const Decomp &dc, f3 &o, f3 &
 }
 
 // Wave-level grab of up to `want` lanes' worth of samples: one fetch-and-add
-// per wave on the wave's cursor shard k, whose j-th sample is j*kShards + k.
-// Once the admitted samples run out a cursor moves past its limit; the host
-// resets it before it admits more work (no kernel runs then).  Returns this
-// lane's global sample index, or ~0ull when none is left for it.
-__device__ inline unsigned long long grab_sample(Ctrl *ctrl, bool want) {
+// per wave on a cursor shard k, whose j-th sample is j*kShards + k.  A wave
+// starts on its own shard; when that one is used up it reads every cursor at
+// once (one lane per shard) and moves to the next shard with samples left, so
+// no shard's samples wait for waves that have no work.  The admitted total is
+// read once per launch (it changes only between launches); once every shard
+// is used up the wave stops asking.  A cursor may move past its limit; the
+// host resets it before it admits more work (no kernel runs then).  Returns
+// this lane's global sample index, or ~0ull when none is left for it (lanes
+// past a shard's last sample go without until the next call).
+struct SampleCursor {
+  unsigned long long total;
+  uint32_t shard;
+  bool open;
+};
+__device__ inline SampleCursor sample_cursor(const Ctrl *ctrl) {
+  SampleCursor c;
+  c.total = __atomic_load_n(&ctrl->total_samples, __ATOMIC_RELAXED);
+  c.shard = wave_shard();
+  c.open = true;
+  return c;
+}
+// Moves sc to the next shard (after the current one, cyclically) that still
+// has samples; false, and sc closed, when none has.  Whole wave.
+__device__ inline bool next_sample_shard(const Ctrl *ctrl, SampleCursor &sc) {
+  const uint32_t lane = lane_id();
+  bool left = false;
+  if (lane < kShards)
+    left = __hip_atomic_load(&ctrl->next_sample[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+           shard_limit(sc.total, lane);
+  unsigned long long lm = __ballot(left);
+  if (lm == 0ull) {
+    sc.open = false;
+    return false;
+  }
+  unsigned long long after = lm & ~((2ull << sc.shard) - 1ull);
+  sc.shard = (uint32_t)(__ffsll((long long)(after ? after : lm)) - 1);
+  return true;
+}
+__device__ inline unsigned long long grab_sample(Ctrl *ctrl, SampleCursor &sc, bool want) {
   unsigned long long m = __ballot(want);
-  if (m == 0ull) return ~0ull;
-  uint32_t cnt = (uint32_t)__popcll(m);
-  uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
-  const uint32_t k = wave_shard();
-  unsigned long long base = 0ull, take = 0ull;
-  if (lane_id() == leader) {
-    unsigned long long lim = shard_limit(__atomic_load_n(&ctrl->total_samples, __ATOMIC_RELAXED), k);
-    unsigned long long cur = __atomic_load_n(&ctrl->next_sample[k], __ATOMIC_RELAXED);
-    if (cur < lim) {
+  if (m == 0ull || !sc.open) return ~0ull;
+  const uint32_t cnt = (uint32_t)__popcll(m);
+  const uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
+  const uint32_t r = rank_in(m);
+  for (;;) {
+    const uint32_t k = sc.shard;
+    unsigned long long base = 0ull, take = 0ull;
+    if (lane_id() == leader) {
+      unsigned long long lim = shard_limit(sc.total, k);
       unsigned long long old = atomicAdd(&ctrl->next_sample[k], (unsigned long long)cnt);
       unsigned long long avail = lim > old ? lim - old : 0ull;
       take = avail < cnt ? avail : (unsigned long long)cnt;
       base = old;
     }
+    base = __shfl(base, (int)leader);
+    take = __shfl(take, (int)leader);
+    if (take > 0ull) {
+      if (take < cnt) next_sample_shard(ctrl, sc);  // shard k is used up now
+      if (!want || (unsigned long long)r >= take) return ~0ull;
+      return (base + r) * kShards + k;
+    }
+    if (!next_sample_shard(ctrl, sc)) return ~0ull;
   }
-  base = __shfl(base, (int)leader);
-  take = __shfl(take, (int)leader);
-  uint32_t r = rank_in(m);
-  if (!want || (unsigned long long)r >= take) return ~0ull;
-  return (base + r) * kShards + k;
 }
 
 // Per-wave completion counting: a wave keeps a running count for one job and
@@ -775,12 +837,13 @@ template <int WG>
 __global__ __launch_bounds__(WG) void k_refill(KernelArgs A, int parity) {
   const uint32_t P = A.p.pool;
   const uint32_t stride = gridDim.x * WG;
+  SampleCursor sc = sample_cursor(A.ctrl);
   for (uint32_t base = blockIdx.x * WG; base < P; base += stride) {
     uint32_t slot = base + threadIdx.x;
     bool valid = slot < P;
     bool idle = valid && ((reinterpret_cast<const uint4 *>(A.p.misc)[slot].y >> 19) & 1u) == 0u;
     if (__ballot(idle) == 0ull) continue;
-    unsigned long long g = grab_sample(A.ctrl, idle);
+    unsigned long long g = grab_sample(A.ctrl, sc, idle);
     bool got = g != ~0ull;
     if (got) start_sample(A, slot, g);
     const uint32_t k = wave_shard();
@@ -1112,6 +1175,7 @@ __global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int rege
   float4 *ray_d = reinterpret_cast<float4 *>(A.p.ray_d);
   uint4 *misc = reinterpret_cast<uint4 *>(A.p.misc);
   uint4 *rngs = reinterpret_cast<uint4 *>(A.p.rng);
+  SampleCursor sc = sample_cursor(A.ctrl);
   DoneAcc dacc;
   for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
@@ -1138,7 +1202,7 @@ __global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int rege
     }
     count_done(A.done, dacc, fin, st.js);
     // regenerate finished slots while samples remain
-    unsigned long long g = grab_sample(A.ctrl, fin && regen);
+    unsigned long long g = grab_sample(A.ctrl, sc, fin && regen);
     bool fresh = g != ~0ull;
     if (fresh) {
       start_sample(A, slot, g);

@@ -111,7 +111,7 @@ struct JobDev {
 // cache line are serialized at the memory side (tens of ns each), so each copy
 // sits on its own kLine-byte line: copies packed into one line would still
 // serialize.
-constexpr uint32_t kShards = 32;
+constexpr uint32_t kShards = 64;
 constexpr uint32_t kLine = 256;
 
 template <class T>
