@@ -358,11 +358,13 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
   D.stack_bound = S.bvh.stack_bound;
   if (d->env_rgb) {
     uint32_t w = d->env_width, h = d->env_height;
-    void *p_env, *p_w, *p_max, *p_q, *p_row, *p_marg;
+    void *p_env, *p_w, *p_max, *p_q, *p_row, *p_marg, *p_rg, *p_mg, *p_rs, *p_ms;
     if ((rc = dup(d->env_rgb, (size_t)w * h * 3 * sizeof(float), &p_env)) ||
         (rc = dup(nullptr, (size_t)w * h * sizeof(float), &p_w)) || (rc = dup(nullptr, 16, &p_max)) ||
         (rc = dup(nullptr, (size_t)w * h * 8, &p_q)) || (rc = dup(nullptr, (size_t)h * (w + 1) * 8, &p_row)) ||
-        (rc = dup(nullptr, (size_t)(h + 1) * 8, &p_marg))) {
+        (rc = dup(nullptr, (size_t)(h + 1) * 8, &p_marg)) || (rc = dup(nullptr, (size_t)w * h * 8, &p_rg)) ||
+        (rc = dup(nullptr, (size_t)h * 8, &p_mg)) || (rc = dup(nullptr, (size_t)h * 8, &p_rs)) ||
+        (rc = dup(nullptr, 8, &p_ms))) {
       free_scene_mem(S);
       return rc;
     }
@@ -370,12 +372,19 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
     hipError_t e = k::env_tables((const float *)p_env, w, h, (float *)p_w, (unsigned int *)p_max,
                                  (unsigned long long *)p_q, (unsigned long long *)p_row, (unsigned long long *)p_marg,
                                  stream_);
+    if (e == hipSuccess)
+      e = k::env_guides((const unsigned long long *)p_row, (const unsigned long long *)p_marg, w, h, (uint2_t *)p_rg,
+                        (uint2_t *)p_mg, (double *)p_rs, (double *)p_ms, stream_);
     if (e != hipSuccess) { free_scene_mem(S); return hipfail(e, "env tables"); }
     OMR_HIP(hipStreamSynchronize(stream_));
     D.env = (const float *)p_env;
     D.env_q = (const unsigned long long *)p_q;
     D.env_row = (const unsigned long long *)p_row;
     D.env_marg = (const unsigned long long *)p_marg;
+    D.env_rguide = (const uint2_t *)p_rg;
+    D.env_mguide = (const uint2_t *)p_mg;
+    D.env_rscale = (const double *)p_rs;
+    D.env_mscale = (const double *)p_ms;
     D.env_w = w;
     D.env_h = h;
     S.has_env = true;

@@ -69,5 +69,8 @@ hipError_t trace_rays(const SceneDev *scene, const float *rays, uint32_t n, int3
 // Environment importance-sampling tables: weights, per-row scan, marginal scan.
 hipError_t env_tables(const float *env, uint32_t w, uint32_t h, float *wbuf, unsigned int *wmax_bits,
                       unsigned long long *q, unsigned long long *row, unsigned long long *marg, hipStream_t st);
+// Bucket directories of the row and marginal inversions (after env_tables).
+hipError_t env_guides(const unsigned long long *row, const unsigned long long *marg, uint32_t w, uint32_t h,
+                      uint2_t *rguide, uint2_t *mguide, double *rscale, double *mscale, hipStream_t st);
 }  // namespace k
 }  // namespace omr

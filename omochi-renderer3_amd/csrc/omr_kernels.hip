@@ -610,8 +610,18 @@ __device__ inline f3 texel(const SceneDev &S, uint32_t k, float u, float v) {
   return mk(p.x, p.y, p.z);
 }
 
-__device__ inline uint32_t search_le(const unsigned long long *a, uint32_t n, unsigned long long key) {
-  uint32_t lo = 0, hi = n;
+// Bucket of key k in a directory of n buckets: floor(k * scale) in binary64,
+// scale = n / total; the same expression builds and reads the directories.
+__device__ inline uint32_t env_bucket(unsigned long long k, double scale, uint32_t n) {
+  uint32_t b = (uint32_t)((double)k * scale);
+  return b < n ? b : n - 1u;
+}
+// search_le(a, n, key) restricted to the entries the key's bucket can select:
+// the same index, found in a few steps.
+__device__ inline uint32_t search_guided(const unsigned long long *a, uint32_t n, unsigned long long key,
+                                         const uint2_t *guide, double scale) {
+  const uint2_t r = guide[env_bucket(key, scale, n)];
+  uint32_t lo = r.lo, hi = r.hi + 1u;
   while (hi - lo > 1) {
     uint32_t mid = (lo + hi) >> 1;
     if (a[mid] <= key) lo = mid;
@@ -625,12 +635,12 @@ __device__ inline bool env_sample4(const SceneDev &S, float u1, float u2, float 
   if (tot == 0ull) return false;
   unsigned long long t = (unsigned long long)((double)u1 * (double)tot);
   if (t >= tot) t = tot - 1;
-  uint32_t iy = search_le(S.env_marg, H, t);
+  uint32_t iy = search_guided(S.env_marg, H, t, S.env_mguide, S.env_mscale[0]);
   const unsigned long long *row = S.env_row + (size_t)iy * (W + 1);
   unsigned long long rt = row[W];
   unsigned long long t2 = (unsigned long long)((double)u2 * (double)rt);
   if (t2 >= rt) t2 = rt - 1;
-  uint32_t ix = search_le(row, W, t2);
+  uint32_t ix = search_guided(row, W, t2, S.env_rguide + (size_t)iy * W, S.env_rscale[iy]);
   float fx = ((float)ix + u3) / (float)W;
   float fy = ((float)iy + u4) / (2.0f * (float)H);
   if (fx >= 1.0f) fx = 0.99999994f;
@@ -1311,7 +1321,6 @@ __global__ __launch_bounds__(WG) void k_drain(KernelArgs A, int parity) {
   const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
   load_prefix(s_pre, A.ctrl->qcount[parity]);
   const uint32_t n = s_pre[kShards];
-  const uint32_t stride = gridDim.x * WG;
   const uint32_t qcap = A.p.qcap;
   Stack<WG, SHORT> stk;
   stk.lds = lds;
@@ -1322,56 +1331,95 @@ __global__ __launch_bounds__(WG) void k_drain(KernelArgs A, int parity) {
   const uint4 *rngs = reinterpret_cast<const uint4 *>(A.p.rng);
   uint32_t segs = 0, shadows = 0, nv = 0, nt = 0;  // node/primitive statistics cover extend only
   uint32_t djob = ~0u, dcnt = 0;  // this lane's completions of job djob not yet added
-  for (uint32_t i = gtid; i < n; i += stride) {
-    uint32_t slot = A.p.queue[parity][flat_slot(s_pre, i, qcap)];
-    float4 ro = ray_o[slot], rd = ray_d[slot];
-    uint4 ms = misc[slot];
-    PathState st;
-    st.rng = rngs[slot];
-    st.o = xyz(ro);
-    st.d = xyz(rd);
-    st.beta = mk(ro.w, rd.w, __uint_as_float(ms.x));
-    st.word = ms.y;
-    st.sidx = ms.z;
-    st.js = ms.w;
-    st.pdf = ps_mis(st.word) ? A.p.pdf[slot] : 0.0f;
-    const JobDev &J = A.jobs[st.js];
-    const SceneDev &S = A.scenes[J.scene];
-    for (;;) {
-      Hit h;
-      h.t = kInf;
-      h.u = h.v = 0.0f;
-      h.prim = -1;
-      traverse<WG, SHORT, false, false>(S, st.o, st.d, h, stk, nv, nt);
-      segs++;
-      bool shadow = false;
-      ShadowRay sh;
-      bool cont = shade_segment(A, slot, st, make_float4(h.t, __int_as_float(h.prim), h.u, h.v), nullptr, shadow, sh);
-      if (shadow) {
-        Hit hs;
-        hs.t = kInf;
-        hs.u = hs.v = 0.0f;
-        hs.prim = -1;
-        uint32_t snv = 0, snt = 0;
-        traverse<WG, SHORT, true, false>(S, sh.o, sh.d, hs, stk, snv, snt);
-        shadows++;
-        if (hs.prim < 0) {
-          unsigned long long *acc = J.accum + 3ull * sh.px;
-          unsigned long long qa = fix(sh.c.x), qb = fix(sh.c.y), qc = fix(sh.c.z);
-          if (qa) atomicAdd(acc + 0, qa);
-          if (qb) atomicAdd(acc + 1, qb);
-          if (qc) atomicAdd(acc + 2, qc);
+  // Lanes take a new path as soon as theirs ends: a wave-level fetch on the
+  // sub-queue heads (zero here: the extend before the last shade cleared
+  // them), moving on to other sub-queues when its own is used up.
+  Sharded<uint32_t> &heads = A.ctrl->qhead[parity];
+  auto count = [&](uint32_t k) { return s_pre[k + 1] - s_pre[k]; };
+  uint32_t shard = wave_shard();
+  bool exhausted = n == 0u, have = false;
+  uint32_t slot = 0;
+  PathState st;
+  st.js = 0;
+  for (;;) {
+    unsigned long long m = __ballot(!have);
+    uint32_t idle = (uint32_t)__popcll(m);
+    if (!exhausted && idle != 0u) {
+      uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(&heads[shard], idle);
+      base = __shfl(base, (int)leader);
+      uint32_t cnt = count(shard);
+      uint32_t r = rank_in(m);
+      if (!have && base + r < cnt) {
+        slot = A.p.queue[parity][shard * qcap + base + r];
+        float4 ro = ray_o[slot], rd = ray_d[slot];
+        uint4 ms = misc[slot];
+        st.rng = rngs[slot];
+        st.o = xyz(ro);
+        st.d = xyz(rd);
+        st.beta = mk(ro.w, rd.w, __uint_as_float(ms.x));
+        st.word = ms.y;
+        st.sidx = ms.z;
+        st.js = ms.w;
+        st.pdf = ps_mis(st.word) ? A.p.pdf[slot] : 0.0f;
+        have = true;
+      }
+      if (base + idle >= cnt) {  // this sub-queue is used up: find one with entries left
+        const uint32_t lane = lane_id();
+        bool left = false;
+        if (lane < kShards) left = __hip_atomic_load(&heads[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < count(lane);
+        unsigned long long lm = __ballot(left);
+        if (lm == 0ull) {
+          exhausted = true;
+        } else {
+          unsigned long long after = lm & ~((2ull << shard) - 1ull);
+          shard = (uint32_t)(__ffsll((long long)(after ? after : lm)) - 1);
         }
       }
-      if (!cont) break;
     }
-    misc[slot].y = 0u;  // inactive
-    if (st.js != djob) {
-      if (dcnt) atomicAdd(&A.done[done_index(wave_shard(), djob)], (unsigned long long)dcnt);
-      djob = st.js;
-      dcnt = 0;
+    if (__ballot(have) == 0ull) {
+      if (exhausted) break;
+      continue;
     }
-    dcnt++;
+    if (!have) continue;
+    const JobDev &J = A.jobs[st.js];
+    const SceneDev &S = A.scenes[J.scene];
+    Hit h;
+    h.t = kInf;
+    h.u = h.v = 0.0f;
+    h.prim = -1;
+    traverse<WG, SHORT, false, false>(S, st.o, st.d, h, stk, nv, nt);
+    segs++;
+    bool shadow = false;
+    ShadowRay sh;
+    bool cont = shade_segment(A, slot, st, make_float4(h.t, __int_as_float(h.prim), h.u, h.v), nullptr, shadow, sh);
+    if (shadow) {
+      Hit hs;
+      hs.t = kInf;
+      hs.u = hs.v = 0.0f;
+      hs.prim = -1;
+      uint32_t snv = 0, snt = 0;
+      traverse<WG, SHORT, true, false>(S, sh.o, sh.d, hs, stk, snv, snt);
+      shadows++;
+      if (hs.prim < 0) {
+        unsigned long long *acc = J.accum + 3ull * sh.px;
+        unsigned long long qa = fix(sh.c.x), qb = fix(sh.c.y), qc = fix(sh.c.z);
+        if (qa) atomicAdd(acc + 0, qa);
+        if (qb) atomicAdd(acc + 1, qb);
+        if (qc) atomicAdd(acc + 2, qc);
+      }
+    }
+    if (!cont) {
+      misc[slot].y = 0u;  // inactive
+      if (st.js != djob) {
+        if (dcnt) atomicAdd(&A.done[done_index(wave_shard(), djob)], (unsigned long long)dcnt);
+        djob = st.js;
+        dcnt = 0;
+      }
+      dcnt++;
+      have = false;
+    }
   }
   // the remaining completion counts, one atomic per job present in the wave
   bool need = dcnt != 0u;
@@ -1549,6 +1597,34 @@ __global__ __launch_bounds__(256) void k_env_margscan(const unsigned long long *
   }
 }
 
+// Bucket directory of one inclusive prefix table a[0..n] (a[0] = 0): entry
+// i serves the keys [a[i], a[i+1]), whose buckets are env_bucket(a[i]) ..
+// env_bucket(a[i+1] - 1); each bucket records the first and last such entry.
+// One thread per table: the rows, then the marginal (index h).
+__global__ void k_env_guides(const unsigned long long *row, const unsigned long long *marg, uint32_t w, uint32_t h,
+                             uint2_t *rguide, uint2_t *mguide, double *rscale, double *mscale) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > h) return;
+  const bool m = t == h;
+  const unsigned long long *a = m ? marg : row + (size_t)t * (w + 1);
+  const uint32_t n = m ? h : w;
+  uint2_t *g = m ? mguide : rguide + (size_t)t * w;
+  const unsigned long long total = a[n];
+  const double scale = total > 0ull ? (double)n / (double)total : 0.0;
+  if (m) mscale[0] = scale;
+  else rscale[t] = scale;
+  for (uint32_t b = 0; b < n; b++) g[b] = uint2_t{0xFFFFFFFFu, 0u};
+  if (total == 0ull) return;
+  for (uint32_t i = 0; i < n; i++) {
+    if (a[i + 1] <= a[i]) continue;
+    uint32_t b0 = env_bucket(a[i], scale, n), b1 = env_bucket(a[i + 1] - 1ull, scale, n);
+    for (uint32_t b = b0; b <= b1; b++) {
+      if (i < g[b].lo) g[b].lo = i;
+      g[b].hi = i;  // i only grows
+    }
+  }
+}
+
 template <int WG, int SHORT>
 hipError_t launch_extend_ws(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
   bool df = a.p.df_budget > 0;
@@ -1693,6 +1769,13 @@ hipError_t env_tables(const float *env, uint32_t w, uint32_t h, float *wbuf, uns
   hipLaunchKernelGGL(k_env_weight, dim3((n + 255u) / 256u), dim3(256), 0, st, env, w, h, wbuf, wmax_bits);
   hipLaunchKernelGGL(k_env_rowscan, dim3(h), dim3(256), 0, st, wbuf, w, wmax_bits, q, row);
   hipLaunchKernelGGL(k_env_margscan, dim3(1), dim3(256), 0, st, row, w, h, marg);
+  return hipGetLastError();
+}
+
+hipError_t env_guides(const unsigned long long *row, const unsigned long long *marg, uint32_t w, uint32_t h,
+                      uint2_t *rguide, uint2_t *mguide, double *rscale, double *mscale, hipStream_t st) {
+  hipLaunchKernelGGL(k_env_guides, dim3((h + 1u + 63u) / 64u), dim3(64), 0, st, row, marg, w, h, rguide, mguide,
+                     rscale, mscale);
   return hipGetLastError();
 }
 

@@ -67,6 +67,11 @@ struct TexRec {
   uint32_t first, width, height, pad;
 };
 
+// Two 32-bit words, one 8-byte load (an entry range of a bucket directory).
+struct alignas(8) uint2_t {
+  uint32_t lo, hi;
+};
+
 // Device-side view of one resident scene.
 struct SceneDev {
   const WNode4 *nodes;
@@ -79,6 +84,13 @@ struct SceneDev {
   const unsigned long long *env_q;     // env_h x env_w
   const unsigned long long *env_row;   // env_h x (env_w+1)
   const unsigned long long *env_marg;  // env_h+1
+  // bucket directories of the inversions: for key k of a table with total T
+  // and n entries, bucket floor(k n / T) (binary64) holds the first and last
+  // entry any key of the bucket can select
+  const uint2_t *env_rguide;           // env_h x env_w
+  const uint2_t *env_mguide;           // env_h
+  const double *env_rscale;            // env_h: env_w / row total (0 for an empty row)
+  const double *env_mscale;            // 1: env_h / total
   uint32_t n_tri, n_sph;
   const float *planes;         // infinite floors y = planes[i]; ids n_tri + n_sph + i
   uint32_t n_planes;
