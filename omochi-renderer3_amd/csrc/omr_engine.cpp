@@ -29,15 +29,15 @@ Engine::Engine(int device) : device_(device) {
   // icosphere, 512x512, 64 spp)
   tuning.node_width = 4;
   tuning.wg_size = 64;
-  tuning.blocks_per_cu = 16;
-  tuning.pool_paths = 1u << 22;
+  tuning.blocks_per_cu = 19;  // the most 64-thread groups whose 16-entry LDS stacks fit in one CU
+  tuning.pool_paths = 1u << 24;
   tuning.steps_per_sync = 8;
   tuning.use_graph = 1;
   tuning.refill_every = 0;
   tuning.fetch_budget = 8;
   tuning.fetch_min = 16;
   tuning.stack_short = kShortStackMax;
-  tuning.drain_below = 1u << 20;
+  tuning.drain_below = 1u << 22;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
     err = "no HIP device " + std::to_string(device);
@@ -123,12 +123,30 @@ int Engine::set_tuning(const omr_tuning &t) {
   return OMR_OK;
 }
 
+// Sub-queue capacity.  A producer kernel running GW waves in a grid-stride
+// loop over at most P entries appends at most 64 * ceil(P / (64 GW)) entries
+// per wave, and a shard (ceil(GW / kShards) waves) receives at most
+// ceil(GW/kShards) * 64 * ceil(P / (64 GW)).  Shade (its own grid) and refill
+// (the persistent grid) may both append to one queue in one step.
+static uint64_t shard_appends(uint32_t P, uint64_t gw) {
+  gw = std::max<uint64_t>(1, gw);
+  uint64_t per_wave = 64ull * ((P + 64ull * gw - 1) / (64ull * gw));
+  return (gw + kShards - 1) / kShards * per_wave;
+}
+static uint32_t sub_queue_cap(uint32_t P, uint64_t grid_waves, uint64_t shade_waves) {
+  return (uint32_t)std::min<uint64_t>(P, shard_appends(P, grid_waves) + shard_appends(P, shade_waves));
+}
+// Workgroups of the persistent kernels and of shade for a pool of P paths.
+static void grids(uint32_t n_cu, const omr_tuning &t, uint32_t P, uint32_t &grid, uint32_t &shade_grid) {
+  uint32_t need = std::max(1u, (P + t.wg_size - 1) / t.wg_size);
+  grid = std::min(need, n_cu * t.blocks_per_cu);
+  shade_grid = std::max(grid, std::min(need, n_cu * (20u * 64u / t.wg_size)));
+}
+
 LaunchShape Engine::shape() const {
   LaunchShape s;
   s.wg = tuning.wg_size;
-  s.grid = (uint32_t)n_cu_ * tuning.blocks_per_cu;
-  uint32_t need = (pool_cap_ + s.wg - 1) / s.wg;
-  if (s.grid > need) s.grid = std::max(1u, need);
+  grids((uint32_t)n_cu_, tuning, pool_cap_, s.grid, s.shade_grid);
   return s;
 }
 
@@ -147,22 +165,11 @@ KernelArgs Engine::args() const {
   return a;
 }
 
-// Sub-queue capacity.  A producer kernel (shade or refill) runs GW waves in a
-// grid-stride loop over at most P entries, so one wave appends at most
-// 64 * ceil(P / (64 GW)) entries and a shard (ceil(GW / kShards) waves)
-// receives at most ceil(GW/kShards) * (P/GW + 64); shade and refill may both
-// append to the same queue in one step, hence the factor 2.
-static uint32_t sub_queue_cap(uint32_t P, uint32_t grid_waves) {
-  uint64_t gw = std::max<uint64_t>(1, grid_waves);
-  uint64_t per_wave = 64ull * ((P + 64ull * gw - 1) / (64ull * gw));
-  uint64_t waves = (gw + kShards - 1) / kShards;
-  return (uint32_t)std::min<uint64_t>(P, 2ull * waves * per_wave);
-}
-
 int Engine::ensure_pool() {
   uint32_t P = tuning.pool_paths;
-  uint32_t gw = (uint32_t)((uint64_t)n_cu_ * tuning.blocks_per_cu * tuning.wg_size / 64);
-  uint32_t qcap = sub_queue_cap(P, gw);
+  uint32_t grid, sgrid;
+  grids((uint32_t)n_cu_, tuning, P, grid, sgrid);
+  uint32_t qcap = sub_queue_cap(P, (uint64_t)grid * tuning.wg_size / 64, (uint64_t)sgrid * tuning.wg_size / 64);
   if (pool_cap_ == P && pool_.ray_o && pool_.qcap == qcap) return OMR_OK;
   drop_graphs();
   PoolPtrs &p = pool_;

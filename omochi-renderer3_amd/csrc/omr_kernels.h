@@ -42,8 +42,9 @@ struct KernelArgs {
 };
 
 struct LaunchShape {
-  uint32_t grid;   // workgroups of the persistent kernels
-  uint32_t wg;     // threads per workgroup: 64, 128 or 256
+  uint32_t grid;        // workgroups of the persistent kernels
+  uint32_t wg;          // threads per workgroup: 64, 128 or 256
+  uint32_t shade_grid;  // workgroups of the shade kernel: five waves per SIMD
 };
 
 namespace k {

@@ -1174,8 +1174,11 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
 // One shading step over the ray queue `parity`: shade each hit, then
 // regenerate finished slots (regen) and compact the continuing paths into the
 // other queue.
+// At most 96 VGPRs (five waves per SIMD, no spills): shade waits on memory
+// most of the time, and the extra wave hides more of it.
 template <int WG>
-__global__ __launch_bounds__(WG) void k_shade(KernelArgs A, int parity, int regen) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5))) void k_shade(KernelArgs A, int parity,
+                                                                                      int regen) {
   __shared__ uint32_t s_pre[kShards + 1];
   load_prefix(s_pre, A.ctrl->qcount[parity]);
   const uint32_t n = s_pre[kShards];
@@ -1709,9 +1712,9 @@ hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool reg
   hipLaunchKernelGGL(k_clear_shadow, dim3(1), dim3(kShards), 0, st, a.ctrl);
   int r = regen ? 1 : 0;
   switch (s.wg) {
-    case 64: hipLaunchKernelGGL(k_shade<64>, dim3(s.grid), dim3(64), 0, st, a, parity, r); break;
-    case 128: hipLaunchKernelGGL(k_shade<128>, dim3(s.grid), dim3(128), 0, st, a, parity, r); break;
-    default: hipLaunchKernelGGL(k_shade<256>, dim3(s.grid), dim3(256), 0, st, a, parity, r); break;
+    case 64: hipLaunchKernelGGL(k_shade<64>, dim3(s.shade_grid), dim3(64), 0, st, a, parity, r); break;
+    case 128: hipLaunchKernelGGL(k_shade<128>, dim3(s.shade_grid), dim3(128), 0, st, a, parity, r); break;
+    default: hipLaunchKernelGGL(k_shade<256>, dim3(s.shade_grid), dim3(256), 0, st, a, parity, r); break;
   }
   return hipGetLastError();
 }
