@@ -68,7 +68,12 @@ __device__ inline uint32_t wave_append(uint32_t *counter, bool pred) {
 }
 
 // Shard of the calling wave (global wave index mod kShards).
-__device__ inline uint32_t wave_shard() { return (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % kShards; }
+// (read through the first lane: the value is uniform across the wave, and as
+// a scalar the counter addresses built from it stay out of vector registers)
+__device__ inline uint32_t wave_shard() {
+  uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)(w % kShards));
+}
 
 // A queue is kShards sub-queues of `cap` entries.  s_pre (LDS, kShards+1)
 // receives the exclusive prefix of the sub-queue counts; the flat index i of
@@ -557,22 +562,26 @@ __device__ inline void env_texel(f3 d, uint32_t w, uint32_t h, uint32_t &ix, uin
   ix = (uint32_t)((int)(u * (float)w) % (int)w);
   iy = (uint32_t)((int)(v * (float)h) % (int)h);
 }
-__device__ inline f3 env_rgb(const SceneDev &S, f3 d) {
+// Texel index of direction d and the importance-sampling pdf there:
+// pdf = q / total * (w h) / (2 pi^2 sin(theta)) for the texel's quantized
+// weight q, sin(theta) = |d.xz| (0 without weights); env_at(i) is its radiance.
+__device__ inline size_t env_index(const SceneDev &S, f3 d) {
   uint32_t ix, iy;
   env_texel(d, S.env_w, S.env_h, ix, iy);
-  const float *p = S.env + 3 * ((size_t)iy * S.env_w + ix);
-  return mk(p[0], p[1], p[2]);
+  return (size_t)iy * S.env_w + ix;
 }
-__device__ inline float env_pdf(const SceneDev &S, f3 d) {
+__device__ inline float env_pdf_at(const SceneDev &S, size_t i, f3 d) {
   unsigned long long tot = S.env_marg[S.env_h];
   if (tot == 0ull) return 0.0f;
-  uint32_t ix, iy;
-  env_texel(d, S.env_w, S.env_h, ix, iy);
-  unsigned long long q = S.env_q[(size_t)iy * S.env_w + ix];
+  unsigned long long q = S.env_q[i];
   float st = sqrtf(d.x * d.x + d.z * d.z);
   if (q == 0ull || !(st > 0.0f)) return 0.0f;
   double p = (double)q / (double)tot * ((double)S.env_w * (double)S.env_h) / (19.739208802178716 * (double)st);
   return (float)p;
+}
+__device__ inline f3 env_at(const SceneDev &S, size_t i) {
+  const float *p = S.env + 3 * i;
+  return mk(p[0], p[1], p[2]);
 }
 // ------------------------------------------------------------ textures
 // Surface uv of a hit: interpolated vertex uv on triangles (Polygon.h:86-96,
@@ -969,14 +978,14 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
   if (prim < 0) {
     // PathTracer.cpp:357-363: miss -> IBL lookup or black background
     if (env_on) {
-      f3 L = env_rgb(S, d);
+      const size_t ei = env_index(S, d);
       float w = 1.0f;
       if (ps_mis(word)) {
         float pp = pdf_mem ? pdf_mem[slot] : st.pdf;
-        float pe = env_pdf(S, d);
+        float pe = env_pdf_at(S, ei, d);
         w = (pp * pp) / (pp * pp + pe * pe);
       }
-      add_px(acc, beta, L, w);
+      add_px(acc, beta, env_at(S, ei), w);
     }
   } else {
     uint32_t pid = (uint32_t)prim;
@@ -1044,12 +1053,13 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
             f3 ld;
             if (env_sample4(S, e1, e2, e3, e4, ld)) {
               float c = dot(nl, ld);
-              float pe = env_pdf(S, ld);
+              const size_t ei = env_index(S, ld);
+              float pe = env_pdf_at(S, ei, ld);
               if (c > 0.0f && pe > 0.0f) {
                 float pb = c * kInvPi;
                 float wl = (pe * pe) / (pe * pe + pb * pb);
                 float s = ((pb / pe) * wl) * k;
-                f3 L = env_rgb(S, ld);
+                f3 L = env_at(S, ei);
                 f3 bc = mk(beta.x * col.x, beta.y * col.y, beta.z * col.z);
                 sh.c = mk(bc.x * L.x * s, bc.y * L.y * s, bc.z * L.z * s);
                 sh.o = x;
