@@ -1351,6 +1351,12 @@ __global__ __launch_bounds__(WG) void k_drain(KernelArgs A, int parity) {
   auto count = [&](uint32_t k) { return s_pre[k + 1] - s_pre[k]; };
   uint32_t shard = wave_shard();
   bool exhausted = n == 0u, have = false;
+  // a lane's path is either tracing its next segment (a traversal sliced into
+  // `budget` visits per trip, as in dynamic fetch) or ready to be shaded; the
+  // wave shades once half of its paths are ready or none is tracing
+  bool tracing = false, ready = false;
+  const int budget = A.p.df_budget > 0 ? (int)A.p.df_budget : 8;
+  Trav<WG, SHORT, false, false> tr;
   uint32_t slot = 0;
   PathState st;
   st.js = 0;
@@ -1377,6 +1383,8 @@ __global__ __launch_bounds__(WG) void k_drain(KernelArgs A, int parity) {
         st.js = ms.w;
         st.pdf = ps_mis(st.word) ? A.p.pdf[slot] : 0.0f;
         have = true;
+        tracing = true;
+        tr.init(A.scenes[A.jobs[st.js].scene], st.o, st.d, kInf, stk);
       }
       if (base + idle >= cnt) {  // this sub-queue is used up: find one with entries left
         const uint32_t lane = lane_id();
@@ -1391,22 +1399,26 @@ __global__ __launch_bounds__(WG) void k_drain(KernelArgs A, int parity) {
         }
       }
     }
-    if (__ballot(have) == 0ull) {
+    const unsigned long long hm = __ballot(have);
+    if (hm == 0ull) {
       if (exhausted) break;
       continue;
     }
-    if (!have) continue;
+    if (tracing && tr.step(stk, budget, nv, nt)) {
+      tracing = false;
+      ready = true;
+    }
+    const unsigned long long rm = __ballot(ready);
+    if (rm == 0ull || (__ballot(tracing) != 0ull && 2u * (uint32_t)__popcll(rm) < (uint32_t)__popcll(hm))) continue;
+    if (!ready) continue;
+    ready = false;
     const JobDev &J = A.jobs[st.js];
     const SceneDev &S = A.scenes[J.scene];
-    Hit h;
-    h.t = kInf;
-    h.u = h.v = 0.0f;
-    h.prim = -1;
-    traverse<WG, SHORT, false, false>(S, st.o, st.d, h, stk, nv, nt);
     segs++;
     bool shadow = false;
     ShadowRay sh;
-    bool cont = shade_segment(A, slot, st, make_float4(h.t, __int_as_float(h.prim), h.u, h.v), nullptr, shadow, sh);
+    bool cont = shade_segment(A, slot, st, make_float4(tr.best.t, __int_as_float(tr.best.prim), tr.best.u, tr.best.v),
+                              nullptr, shadow, sh);
     if (shadow) {
       Hit hs;
       hs.t = kInf;
@@ -1423,7 +1435,10 @@ __global__ __launch_bounds__(WG) void k_drain(KernelArgs A, int parity) {
         if (qc) atomicAdd(acc + 2, qc);
       }
     }
-    if (!cont) {
+    if (cont) {
+      tracing = true;
+      tr.init(S, st.o, st.d, kInf, stk);
+    } else {
       misc[slot].y = 0u;  // inactive
       if (st.js != djob) {
         if (dcnt) atomicAdd(&A.done[done_index(wave_shard(), djob)], (unsigned long long)dcnt);
