@@ -346,13 +346,13 @@ int omr_autotune(omr_ctx *ctx, size_t n, const omr_scene_id *scenes, const omr_c
       {&omr_tuning::node_width, {4}},
       {&omr_tuning::wg_size, {64, 128, 256}},
       {&omr_tuning::blocks_per_cu, {4, 8, 16, 19}},
-      {&omr_tuning::pool_paths, {1u << 20, 1u << 22, 1u << 24, 1u << 25}},
+      {&omr_tuning::pool_paths, {1u << 22, 1u << 24, 1u << 25}},
       {&omr_tuning::refill_every, {0, 1, 2, 4}},
       {&omr_tuning::fetch_budget, {0, 4, 8, 16}},
       {&omr_tuning::fetch_min, {8, 16, 32}},
       {&omr_tuning::stack_short, {8, 16}},
-      {&omr_tuning::drain_below, {0, 1u << 18, 1u << 20, 1u << 22, 1u << 23}},
-      {&omr_tuning::steps_per_sync, {4, 8, 16}},
+      {&omr_tuning::drain_below, {0, 1u << 20, 1u << 22, 1u << 23, 1u << 24}},
+      {&omr_tuning::steps_per_sync, {2, 4, 8, 16}},
   };
   for (int pass = 0; pass < 2; pass++) {
     for (auto &d : dims) {
