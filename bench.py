@@ -30,6 +30,12 @@ HBM_PEAK_GBS = 8000.0
 # the six bound vectors and the child vector (112); per primitive test one
 # 48-byte primitive record.
 B_SEG, B_NODE, B_PRIM = 56, 112, 48
+# shade: queue index 4 + path state and hit 80 read, state 64 + queue 4 written, material 48 + shading-normal
+# record 48 read per segment; a shadow record (48) per environment sample
+B_SHADE_SEG, B_SHADOW_REC = 248, 48
+# drain: queue index 4 + path state 68 read and the state word 4 written per path; material 48 + shading-normal
+# record 48 per segment, plus the traversal's node and primitive bytes
+B_DRAIN_PATH, B_DRAIN_SEG = 76, 96
 
 WORKLOADS = {
     "c0": dict(kind=0, w=256, h=256, ss=4, spp=1, depth=8, rr=3, env_is=0,
@@ -253,7 +259,7 @@ def main():
         ctx.render(sid, scene.camera, prm)
     seg = shadow = drain_seg = drained = 0
     ext_ms = shade_ms = shadow_ms = drain_ms = 0.0
-    launches = nodes = prims = 0
+    launches = nodes = prims = dnodes = dprims = 0
     img = None
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -270,23 +276,41 @@ def main():
         drain_seg += st["drain_segments"]
         drained += st["drained_paths"]
         drain_ms += st["drain_ms"]
+        dnodes += st["drain_node_visits"]
+        dprims += st["drain_prim_tests"]
     dt = time.perf_counter() - t0
     rays = seg + shadow
     value = rays / dt / 1e6
-    ext_seg = seg - drain_seg  # node and primitive counts cover the extend launches only
-    alg_bytes = B_SEG * ext_seg + B_NODE * nodes + B_PRIM * prims
-    achieved = alg_bytes / (ext_ms / 1e3) / 1e9 if ext_ms > 0 else 0.0
+    ext_seg = seg - drain_seg  # segments of the extend launches
+    drains = args.steps if drained else 0  # one drain launch per render
+    # algorithmic bytes of each kernel over the timed renders (per-unit figures in DESIGN.md section 3)
+    alg = {
+        "k_extend": (B_SEG * ext_seg + B_NODE * nodes + B_PRIM * prims, ext_ms, launches),
+        "k_shade": (B_SHADE_SEG * ext_seg + B_SHADOW_REC * shadow, shade_ms, launches),
+        "k_drain": (B_DRAIN_PATH * drained + B_DRAIN_SEG * drain_seg + B_NODE * dnodes + B_PRIM * dprims, drain_ms,
+                    drains),
+    }
+    kernels = {}
+    for name, (b, ms, n) in alg.items():
+        if ms > 0 and n > 0:
+            a = b / (ms / 1e3) / 1e9
+            kernels[name] = {"ms_per_step": round(ms / args.steps, 3), "avg_launch_ms": round(ms / n, 4),
+                             "alg_bytes_per_launch": int(b / n), "achieved": round(a, 1),
+                             "frac": round(a / HBM_PEAK_GBS, 4)}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"]) if kernels else "k_extend"
+    kd = kernels.get(dom, {"achieved": 0.0, "frac": 0.0, "avg_launch_ms": 0.0, "alg_bytes_per_launch": 0})
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"extend_{args.workload}_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"{dom[2:]}_{args.workload}_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {
-        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-        "kernel": "k_extend", "avg_launch_ms": round(ext_ms / max(1, launches), 4),
-        "alg_bytes_per_launch": int(alg_bytes / max(1, launches)),
-        "nodes_per_segment": round(nodes / max(1, ext_seg), 3), "prims_per_segment": round(prims / max(1, ext_seg), 3),
+        "bound": "hbm", "achieved": kd["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": kd["frac"], "traffic": traffic,
+        "kernel": dom, "avg_launch_ms": kd["avg_launch_ms"], "alg_bytes_per_launch": kd["alg_bytes_per_launch"],
+        "kernels": kernels,
+        "nodes_per_segment": round((nodes + dnodes) / max(1, seg), 3),
+        "prims_per_segment": round((prims + dprims) / max(1, seg), 3),
         "kernel_ms_per_step": {"extend": round(ext_ms / args.steps, 3), "shade": round(shade_ms / args.steps, 3),
                                "shadow": round(shadow_ms / args.steps, 3), "drain": round(drain_ms / args.steps, 3)},
         "drain": {"paths_per_step": drained // args.steps, "segments_per_step": drain_seg // args.steps},

@@ -129,6 +129,7 @@ typedef struct {
    * queries (part of `segments`), and its device time when profiling */
   uint64_t drained_paths, drain_segments;
   double drain_ms;
+  uint64_t drain_node_visits, drain_prim_tests;  /* drain launch, when profiling */
 } omr_render_stats;
 
 typedef struct {

@@ -638,7 +638,7 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
       // the live paths (queue 0 after an even number of steps) run to the end in one launch
       drained = queued;
       if (profile) OMR_HIP(hipEventRecord(ev_c_, stream_));
-      e = k::drain(a, sh, 0, stream_);
+      e = k::drain(a, sh, 0, profile, stream_);
       if (e != hipSuccess) return hipfail(e, "drain");
       if (profile) OMR_HIP(hipEventRecord(ev_d_, stream_));
       OMR_HIP(hipMemcpyAsync(h_ctrl_, d_ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_));
@@ -686,6 +686,8 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     st->drained_paths = drained;
     st->drain_segments = sum_shards(h_ctrl_->drain_segments);
     st->drain_ms = drain_ms;
+    st->drain_node_visits = sum_shards(h_ctrl_->drain_node_visits);
+    st->drain_prim_tests = sum_shards(h_ctrl_->drain_prim_tests);
   }
   return OMR_OK;
 }

@@ -57,7 +57,7 @@ hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool reg
 hipError_t shadow(const KernelArgs &a, const LaunchShape &s, hipStream_t st);
 // Runs every path of ray queue `parity` to its end in one launch (closest
 // hit, shading, visibility, pending branches); only once no samples are left.
-hipError_t drain(const KernelArgs &a, const LaunchShape &s, int parity, hipStream_t st);
+hipError_t drain(const KernelArgs &a, const LaunchShape &s, int parity, bool stats, hipStream_t st);
 // Zeroes the completed-sample counters of ring slot `ring` (every shard).
 hipError_t clear_done(unsigned long long *done, uint32_t ring, hipStream_t st);
 hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, double n_samples, hipStream_t st);

@@ -1327,7 +1327,7 @@ __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
 // slowest ray.  Same per-path arithmetic and RNG streams as the step
 // kernels, and integer accumulation, so the result bits are the same.  The
 // queues are left stale: the run ends after a drain.
-template <int WG, int SHORT>
+template <int WG, int SHORT, bool STATS>
 __global__ __launch_bounds__(WG) void k_drain(KernelArgs A, int parity) {
   __shared__ uint2 lds[SHORT * WG];
   __shared__ uint32_t s_pre[kShards + 1];
@@ -1342,7 +1342,7 @@ __global__ __launch_bounds__(WG) void k_drain(KernelArgs A, int parity) {
   const float4 *ray_d = reinterpret_cast<const float4 *>(A.p.ray_d);
   uint4 *misc = reinterpret_cast<uint4 *>(A.p.misc);
   const uint4 *rngs = reinterpret_cast<const uint4 *>(A.p.rng);
-  uint32_t segs = 0, shadows = 0, nv = 0, nt = 0;  // node/primitive statistics cover extend only
+  uint32_t segs = 0, shadows = 0, nv = 0, nt = 0;  // node/primitive counts of the closest-hit traversals
   uint32_t djob = ~0u, dcnt = 0;  // this lane's completions of job djob not yet added
   // Lanes take a new path as soon as theirs ends: a wave-level fetch on the
   // sub-queue heads (zero here: the extend before the last shade cleared
@@ -1356,7 +1356,7 @@ __global__ __launch_bounds__(WG) void k_drain(KernelArgs A, int parity) {
   // wave shades once half of its paths are ready or none is tracing
   bool tracing = false, ready = false;
   const int budget = A.p.df_budget > 0 ? (int)A.p.df_budget : 8;
-  Trav<WG, SHORT, false, false> tr;
+  Trav<WG, SHORT, false, STATS> tr;
   uint32_t slot = 0;
   PathState st;
   st.js = 0;
@@ -1466,6 +1466,10 @@ __global__ __launch_bounds__(WG) void k_drain(KernelArgs A, int parity) {
   wave_count(A.ctrl->segments, segs);
   wave_count(A.ctrl->drain_segments, segs);
   wave_count(A.ctrl->shadow_rays, shadows);
+  if (STATS) {
+    wave_count(A.ctrl->drain_node_visits, nv);
+    wave_count(A.ctrl->drain_prim_tests, nt);
+  }
 }
 
 // Clears the shadow sub-queue counts and fetch heads at the start of a step.
@@ -1681,11 +1685,12 @@ hipError_t launch_shadow_wg(const KernelArgs &a, uint32_t grid, hipStream_t st) 
 }
 
 template <int WG>
-hipError_t launch_drain_wg(const KernelArgs &a, uint32_t grid, int parity, hipStream_t st) {
-  if (a.p.stack_short == kShortStackMin)
-    hipLaunchKernelGGL((k_drain<WG, (int)kShortStackMin>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else
-    hipLaunchKernelGGL((k_drain<WG, (int)kShortStackMax>), dim3(grid), dim3(WG), 0, st, a, parity);
+hipError_t launch_drain_wg(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
+  const bool s8 = a.p.stack_short == kShortStackMin;
+  if (stats && s8) hipLaunchKernelGGL((k_drain<WG, (int)kShortStackMin, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else if (stats) hipLaunchKernelGGL((k_drain<WG, (int)kShortStackMax, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else if (s8) hipLaunchKernelGGL((k_drain<WG, (int)kShortStackMin, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else hipLaunchKernelGGL((k_drain<WG, (int)kShortStackMax, false>), dim3(grid), dim3(WG), 0, st, a, parity);
   return hipGetLastError();
 }
 
@@ -1752,11 +1757,11 @@ hipError_t shadow(const KernelArgs &a, const LaunchShape &s, hipStream_t st) {
   }
 }
 
-hipError_t drain(const KernelArgs &a, const LaunchShape &s, int parity, hipStream_t st) {
+hipError_t drain(const KernelArgs &a, const LaunchShape &s, int parity, bool stats, hipStream_t st) {
   switch (s.wg) {
-    case 64: return launch_drain_wg<64>(a, s.grid, parity, st);
-    case 128: return launch_drain_wg<128>(a, s.grid, parity, st);
-    default: return launch_drain_wg<256>(a, s.grid, parity, st);
+    case 64: return launch_drain_wg<64>(a, s.grid, parity, stats, st);
+    case 128: return launch_drain_wg<128>(a, s.grid, parity, stats, st);
+    default: return launch_drain_wg<256>(a, s.grid, parity, stats, st);
   }
 }
 

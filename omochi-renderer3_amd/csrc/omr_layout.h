@@ -152,6 +152,8 @@ struct Ctrl {
   Sharded<unsigned long long> node_visits;  // stats kernels only
   Sharded<unsigned long long> prim_tests;   // stats kernels only
   Sharded<unsigned long long> drain_segments;  // closest-hit queries traced by the drain kernel
+  Sharded<unsigned long long> drain_node_visits;  // drain, stats variant only
+  Sharded<unsigned long long> drain_prim_tests;   // drain, stats variant only
   Sharded<uint32_t> qcount[2];              // ping-pong ray sub-queue counts
   Sharded<uint32_t> scount;                 // shadow sub-queue counts
   Sharded<uint32_t> qhead[2];               // dynamic-fetch heads of the ray sub-queues
