@@ -31,13 +31,13 @@ Engine::Engine(int device) : device_(device) {
   tuning.wg_size = 64;
   tuning.blocks_per_cu = 19;  // the most 64-thread groups whose 16-entry LDS stacks fit in one CU
   tuning.pool_paths = 1u << 25;
-  tuning.steps_per_sync = 4;
+  tuning.steps_per_sync = 2;
   tuning.use_graph = 1;
   tuning.refill_every = 0;
   tuning.fetch_budget = 8;
   tuning.fetch_min = 16;
   tuning.stack_short = kShortStackMax;
-  tuning.drain_below = 1u << 23;
+  tuning.drain_below = 1u << 24;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
     err = "no HIP device " + std::to_string(device);
