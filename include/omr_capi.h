@@ -145,14 +145,15 @@ typedef struct {
 typedef struct {
   uint32_t node_width;        /* 4 (128-byte nodes) */
   uint32_t wg_size;           /* 64, 128 or 256 */
-  uint32_t blocks_per_cu;     /* persistent grid = CUs x blocks_per_cu */
+  uint32_t blocks_per_cu;     /* persistent grid = CUs x blocks_per_cu (shade runs 20 waves per CU) */
   uint32_t pool_paths;        /* path slots in flight (max) */
   uint32_t steps_per_sync;    /* bounce steps per host check (graph length) */
   uint32_t use_graph;         /* capture bounce steps in a hipGraph */
   uint32_t refill_every;      /* queue refill cadence: 0 = finished paths regenerate inside
                                  shade; k = idle slots are refilled in pixel order every k steps */
   uint32_t fetch_budget;      /* traversal: 0 = one ray per lane per grid-stride turn; k = lanes
-                                 that finished fetch new rays after every k node/leaf visits */
+                                 that finished fetch new rays after every k node/leaf visits (the
+                                 drain slices its traversals the same way, 8 visits when 0) */
   uint32_t fetch_min;         /* idle lanes (of 64) that trigger a fetch */
   uint32_t stack_short;       /* traversal stack entries per lane kept in LDS: 8 or 16 (deeper
                                  entries spill to a per-thread global area) */
