@@ -49,6 +49,31 @@ __device__ inline float4 ld4(const float4 *p) {
   v4f v = *(__attribute__((address_space(1))) const v4f *)p;
   return make_float4(v.x, v.y, v.z, v.w);
 }
+// The same view for the other records that scene and job descriptors point
+// to, and for the pixel-sum atomics: flat atomics in flight would also hold
+// up every later LDS and scalar-load wait of the wave.
+template <class T>
+__device__ inline const __attribute__((address_space(1))) T *gptr(const T *p) {
+  return (const __attribute__((address_space(1))) T *)p;
+}
+template <class T>
+__device__ inline T gld(const T *p) {
+  return *gptr(p);
+}
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+__device__ inline TexRec gld(const TexRec *p) {
+  v4u v = *(__attribute__((address_space(1))) const v4u *)p;
+  return TexRec{v.x, v.y, v.z, v.w};
+}
+__device__ inline uint2_t gld(const uint2_t *p) {
+  v2u v = *(__attribute__((address_space(1))) const v2u *)p;
+  return uint2_t{v.x, v.y};
+}
+__device__ inline void gadd(unsigned long long *p, unsigned long long v) {
+  __hip_atomic_fetch_add((__attribute__((address_space(1))) unsigned long long *)p, v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // ------------------------------------------------------------ wave helpers
 __device__ inline uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
@@ -301,7 +326,7 @@ struct Trav {
     // objects outside the BVH come after it (Scene.cpp:72-80)
     for (uint32_t k = 0; k < n_planes; k++) {
       float t;
-      if (!hit_plane(o, d, planes[k], t)) continue;
+      if (!hit_plane(o, d, gld(planes + k), t)) continue;
       const int32_t id = (int32_t)(plane_base + k);
       if (t < best.t || (t == best.t && best.prim >= 0 && id < best.prim)) {
         best.t = t;
@@ -571,9 +596,9 @@ __device__ inline size_t env_index(const SceneDev &S, f3 d) {
   return (size_t)iy * S.env_w + ix;
 }
 __device__ inline float env_pdf_at(const SceneDev &S, size_t i, f3 d) {
-  unsigned long long tot = S.env_marg[S.env_h];
+  unsigned long long tot = gld(S.env_marg + S.env_h);
   if (tot == 0ull) return 0.0f;
-  unsigned long long q = S.env_q[i];
+  unsigned long long q = gld(S.env_q + i);
   float st = sqrtf(d.x * d.x + d.z * d.z);
   if (q == 0ull || !(st > 0.0f)) return 0.0f;
   double p = (double)q / (double)tot * ((double)S.env_w * (double)S.env_h) / (19.739208802178716 * (double)st);
@@ -581,7 +606,7 @@ __device__ inline float env_pdf_at(const SceneDev &S, size_t i, f3 d) {
 }
 __device__ inline f3 env_at(const SceneDev &S, size_t i) {
   const float *p = S.env + 3 * i;
-  return mk(p[0], p[1], p[2]);
+  return mk(gld(p), gld(p + 1), gld(p + 2));
 }
 // ------------------------------------------------------------ textures
 // Surface uv of a hit: interpolated vertex uv on triangles (Polygon.h:86-96,
@@ -594,7 +619,7 @@ __device__ inline void hit_uv(const SceneDev &S, uint32_t pid, float bu, float b
   if (pid < S.n_tri) {
     if (S.tri_uv) {
       const float4 *q = reinterpret_cast<const float4 *>(S.tri_uv) + (size_t)pid * 2;
-      float4 a = q[0], b = q[1];
+      float4 a = ld4(q), b = ld4(q + 1);
       u = (a.z * bu + b.x * bv) + a.x;
       v = (a.w * bu + b.y * bv) + a.y;
     }
@@ -608,14 +633,14 @@ __device__ inline void hit_uv(const SceneDev &S, uint32_t pid, float bu, float b
 // Image::GetPixelByUV (src/tools/Image.h:21-30): u, v clamped to [0, 1]
 // (NaN to 0), scaled by 0.99999, texel (int(u W), int(v H)), row 0 on top.
 __device__ inline f3 texel(const SceneDev &S, uint32_t k, float u, float v) {
-  const TexRec r = S.tex_rec[k];
+  const TexRec r = gld(S.tex_rec + k);
   u = u > 0.0f ? (u < 1.0f ? u : 1.0f) : 0.0f;
   v = v > 0.0f ? (v < 1.0f ? v : 1.0f) : 0.0f;
   uint32_t x = (uint32_t)(int)((u * 0.99999f) * (float)r.width);
   uint32_t y = (uint32_t)(int)((v * 0.99999f) * (float)r.height);
   x = x < r.width ? x : r.width - 1u;
   y = y < r.height ? y : r.height - 1u;
-  float4 p = reinterpret_cast<const float4 *>(S.tex)[(size_t)r.first + (size_t)y * r.width + x];
+  float4 p = ld4(reinterpret_cast<const float4 *>(S.tex) + ((size_t)r.first + (size_t)y * r.width + x));
   return mk(p.x, p.y, p.z);
 }
 
@@ -629,27 +654,27 @@ __device__ inline uint32_t env_bucket(unsigned long long k, double scale, uint32
 // the same index, found in a few steps.
 __device__ inline uint32_t search_guided(const unsigned long long *a, uint32_t n, unsigned long long key,
                                          const uint2_t *guide, double scale) {
-  const uint2_t r = guide[env_bucket(key, scale, n)];
+  const uint2_t r = gld(guide + env_bucket(key, scale, n));
   uint32_t lo = r.lo, hi = r.hi + 1u;
   while (hi - lo > 1) {
     uint32_t mid = (lo + hi) >> 1;
-    if (a[mid] <= key) lo = mid;
+    if (gld(a + mid) <= key) lo = mid;
     else hi = mid;
   }
   return lo;
 }
 __device__ inline bool env_sample4(const SceneDev &S, float u1, float u2, float u3, float u4, f3 &dir) {
   uint32_t W = S.env_w, H = S.env_h;
-  unsigned long long tot = S.env_marg[H];
+  unsigned long long tot = gld(S.env_marg + H);
   if (tot == 0ull) return false;
   unsigned long long t = (unsigned long long)((double)u1 * (double)tot);
   if (t >= tot) t = tot - 1;
-  uint32_t iy = search_guided(S.env_marg, H, t, S.env_mguide, S.env_mscale[0]);
+  uint32_t iy = search_guided(S.env_marg, H, t, S.env_mguide, gld(S.env_mscale));
   const unsigned long long *row = S.env_row + (size_t)iy * (W + 1);
-  unsigned long long rt = row[W];
+  unsigned long long rt = gld(row + W);
   unsigned long long t2 = (unsigned long long)((double)u2 * (double)rt);
   if (t2 >= rt) t2 = rt - 1;
-  uint32_t ix = search_guided(row, W, t2, S.env_rguide + (size_t)iy * W, S.env_rscale[iy]);
+  uint32_t ix = search_guided(row, W, t2, S.env_rguide + (size_t)iy * W, gld(S.env_rscale + iy));
   float fx = ((float)ix + u3) / (float)W;
   float fy = ((float)iy + u4) / (2.0f * (float)H);
   if (fx >= 1.0f) fx = 0.99999994f;
@@ -669,9 +694,9 @@ __device__ inline unsigned long long fix(float v) {
 }
 __device__ inline void add_px(unsigned long long *acc, f3 beta, f3 L, float s) {
   unsigned long long a = fix(beta.x * L.x * s), b = fix(beta.y * L.y * s), c = fix(beta.z * L.z * s);
-  if (a) atomicAdd(acc + 0, a);
-  if (b) atomicAdd(acc + 1, b);
-  if (c) atomicAdd(acc + 2, c);
+  if (a) gadd(acc + 0, a);
+  if (b) gadd(acc + 1, b);
+  if (c) gadd(acc + 2, c);
 }
 
 // ------------------------------------------------------------ job lookup
@@ -970,7 +995,7 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
   const SceneDev &S = A.scenes[J.scene];
   uint32_t depth = ps_depth(word), branch = ps_branch(word), npend = ps_pending(word);
   bool env_on = S.env != nullptr;
-  bool nee = env_on && (J.flags & 1u) && S.env_marg[S.env_h] > 0ull;
+  bool nee = env_on && (J.flags & 1u) && gld(S.env_marg + S.env_h) > 0ull;
   uint32_t pixel = decompose(J, sidx).pixel;
   unsigned long long *acc = J.accum + 3ull * pixel;
   int32_t prim = __float_as_int(hr.y);
@@ -993,26 +1018,26 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
     f3 nrm;
     if (pid < S.n_tri) {
       const float4 *tn = reinterpret_cast<const float4 *>(S.tri_n) + (size_t)pid * 3;
-      float4 n0 = tn[0], n1 = tn[1], n2 = tn[2];
+      float4 n0 = ld4(tn), n1 = ld4(tn + 1), n2 = ld4(tn + 2);
       nrm = normalize(add(add(scl(xyz(n1), hr.z), scl(xyz(n2), hr.w)), xyz(n0)));
     } else if (pid < S.n_tri + S.n_sph) {
-      float4 sc = reinterpret_cast<const float4 *>(S.sph)[pid - S.n_tri];
+      float4 sc = ld4(reinterpret_cast<const float4 *>(S.sph) + (pid - S.n_tri));
       nrm = normalize(sub(x, xyz(sc)));
     } else {
       nrm = mk(0.0f, d.y < 0.0f ? 1.0f : -1.0f, 0.0f);  // InfiniteFloor.h:34-36: facing the ray
     }
     f3 nl = dot(nrm, d) < 0.0f ? nrm : scl(nrm, -1.0f);
     // material roulette (PathTracer.cpp:368-386)
-    uint32_t pm = S.prim_mat[pid];
+    uint32_t pm = gld(S.prim_mat + pid);
     uint32_t mfirst = pm & 0x00FFFFFFu, mcount = (pm >> 24) + 1u;
     float value = rng_float(rng);
     uint32_t sel = mcount - 1u;
     for (uint32_t q = 0; q < mcount; q++) {
-      float r = S.mats[mfirst + q].rate;
+      float r = gptr(S.mats)[mfirst + q].rate;
       if (value <= r) { sel = q; break; }
       value -= r;
     }
-    const MatRec &m = S.mats[mfirst + sel];
+    const __attribute__((address_space(1))) MatRec &m = gptr(S.mats)[mfirst + sel];
     // hit colour: material colour times the texel (PathTracer.cpp:327-346,388-389)
     const f3 col0 = mk(m.color[0], m.color[1], m.color[2]);
     f3 col = col0;
@@ -1261,9 +1286,9 @@ __device__ inline void add_shadow(const KernelArgs &A, uint32_t r) {
   const JobDev &J = A.jobs[__float_as_uint(a.w)];
   unsigned long long *acc = J.accum + 3ull * __float_as_uint(b.w);
   unsigned long long qa = fix(c.x), qb = fix(c.y), qc = fix(c.z);
-  if (qa) atomicAdd(acc + 0, qa);
-  if (qb) atomicAdd(acc + 1, qb);
-  if (qc) atomicAdd(acc + 2, qc);
+  if (qa) gadd(acc + 0, qa);
+  if (qb) gadd(acc + 1, qb);
+  if (qc) gadd(acc + 2, qc);
 }
 
 template <int WG, int SHORT, bool DF>
@@ -1433,9 +1458,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
       if (hs.prim < 0) {
         unsigned long long *acc = J.accum + 3ull * sh.px;
         unsigned long long qa = fix(sh.c.x), qb = fix(sh.c.y), qc = fix(sh.c.z);
-        if (qa) atomicAdd(acc + 0, qa);
-        if (qb) atomicAdd(acc + 1, qb);
-        if (qc) atomicAdd(acc + 2, qc);
+        if (qa) gadd(acc + 0, qa);
+        if (qb) gadd(acc + 1, qb);
+        if (qc) gadd(acc + 2, qc);
       }
     }
     if (cont) {
