@@ -160,6 +160,11 @@ typedef struct {
   uint32_t drain_below;       /* once no samples are left to hand out and at most this many paths
                                  are queued, one drain launch runs them all to the end instead of
                                  further extend/shade steps (0 = never) */
+  uint32_t record_mb;         /* device memory (MiB) a render may use for per-sample sums (24 bytes
+                                 per sample): jobs whose records fit add their contributions there
+                                 without atomics and a gather sums them per pixel at the end; the
+                                 rest add to their pixel sums atomically (0 = atomics only).  The
+                                 results are the same bits either way */
 } omr_tuning;
 
 typedef struct {

@@ -62,8 +62,9 @@ int render_impl(omr_ctx *c, size_t n, const omr_scene_id *scenes, const omr_came
   if (c->eng->server_running()) return set_err(c, OMR_ERR_STATE, "server mode is running");
   if (n == 0) return OMR_OK;
   std::vector<omr::JobRun> jobs(n);
-  size_t bytes = 0;
-  std::vector<size_t> off_acc(n), off_out(n);
+  size_t bytes = 0, rec_bytes = 0;
+  const size_t rec_cap = (size_t)c->eng->tuning.record_mb << 20;
+  std::vector<size_t> off_acc(n), off_out(n), off_rec(n, SIZE_MAX);
   for (size_t i = 0; i < n; i++) {
     uint64_t ns;
     int rc = check_job(c, &cams[i], &params[i], &ns);
@@ -78,6 +79,13 @@ int render_impl(omr_ctx *c, size_t n, const omr_scene_id *scenes, const omr_came
     bytes += (px * 3 * 8 + 255) & ~(size_t)255;
     off_out[i] = bytes;
     bytes += (px * 3 * 4 + 255) & ~(size_t)255;
+    // per-sample records while the render's records fit in record_mb
+    const size_t rb = ((size_t)ns * 24 + 255) & ~(size_t)255;
+    if (rec_bytes + rb <= rec_cap) {
+      off_rec[i] = bytes;
+      bytes += rb;
+      rec_bytes += rb;
+    }
   }
   std::lock_guard<std::mutex> g(c->eng->mu);
   hipSetDevice(c->eng->device_id());
@@ -86,6 +94,7 @@ int render_impl(omr_ctx *c, size_t n, const omr_scene_id *scenes, const omr_came
   for (size_t i = 0; i < n; i++) {
     jobs[i].d_accum = (unsigned long long *)((char *)c->scratch + off_acc[i]);
     jobs[i].d_out = (float *)((char *)c->scratch + off_out[i]);
+    if (off_rec[i] != SIZE_MAX) jobs[i].d_recs = (unsigned long long *)((char *)c->scratch + off_rec[i]);
   }
   omr_render_stats st{};
   rc = c->eng->run_jobs(jobs, &st);

@@ -38,6 +38,7 @@ Engine::Engine(int device) : device_(device) {
   tuning.fetch_min = 16;
   tuning.stack_short = kShortStackMax;
   tuning.drain_below = 1u << 24;
+  tuning.record_mb = 32768;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
     err = "no HIP device " + std::to_string(device);
@@ -114,6 +115,7 @@ int Engine::set_tuning(const omr_tuning &t) {
   if (t.fetch_budget && (t.fetch_min < 1 || t.fetch_min > 64)) return fail(OMR_ERR_INVALID, "fetch_min must be 1..64");
   if (t.stack_short != kShortStackMin && t.stack_short != kShortStackMax)
     return fail(OMR_ERR_INVALID, "stack_short must be 8 or 16");
+  if ((uint64_t)t.record_mb << 20 > hbm_) return fail(OMR_ERR_INVALID, "record_mb exceeds the device memory");
   std::lock_guard<std::mutex> g(mu);
   hipSetDevice(device_);
   hipStreamSynchronize(stream_);
@@ -598,6 +600,7 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     D.aperture = j.cam.aperture_radius;
     D.focal = j.cam.focal_distance;
     D.accum = j.d_accum;
+    D.recs = j.d_recs;
     j.ring = (uint32_t)i;
     total += j.n_samples;
   }
@@ -610,8 +613,10 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
   OMR_HIP(hipMemcpyAsync(d_ctrl_, &c, sizeof(Ctrl), hipMemcpyHostToDevice, stream_));
   OMR_HIP(hipMemsetAsync(d_done_, 0, kJobRing * kShards * sizeof(unsigned long long), stream_));
   OMR_HIP(hipMemsetAsync(pool_.misc, 0, (size_t)pool_cap_ * 16, stream_));
-  for (auto &j : jobs)
+  for (auto &j : jobs) {
     OMR_HIP(hipMemsetAsync(j.d_accum, 0, (size_t)j.cam.width * j.cam.height * 3 * 8, stream_));
+    if (j.d_recs) OMR_HIP(hipMemsetAsync(j.d_recs, 0, (size_t)j.n_samples * 24, stream_));
+  }
   OMR_HIP(hipEventRecord(ev_a_, stream_));
   KernelArgs a = args();
   hipError_t e = k::refill(a, sh, 0, stream_);
@@ -650,6 +655,12 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
       }
       break;
     }
+  }
+  for (auto &j : jobs) {
+    if (!j.d_recs) continue;
+    e = k::gather_records(j.d_recs, j.params.supersamples * j.params.supersamples * (j.params.spp_end - j.params.spp_begin),
+                          j.cam.width * j.cam.height, j.d_accum, stream_);
+    if (e != hipSuccess) return hipfail(e, "gather_records");
   }
   OMR_HIP(hipEventRecord(ev_b_, stream_));
   for (auto &j : jobs) {

@@ -63,6 +63,7 @@ struct JobRun {
   omr_render_params params{};
   uint32_t n_samples = 0;
   unsigned long long *d_accum = nullptr;  // W*H*3 fixed-point sums
+  unsigned long long *d_recs = nullptr;   // n_samples*3 per-sample sums, or null (atomics on d_accum)
   float *d_out = nullptr;                 // W*H*3 resolved means (optional)
   uint32_t ring = 0;                      // ring slot while live
   uint64_t logical = 0;                   // logical ring index
@@ -163,6 +164,7 @@ class Engine {
     std::chrono::steady_clock::time_point t_submit;
     int state = 0;  // 0 queued 1 running 2 done 3 cancelled 4 failed
     size_t bytes = 0;
+    bool records = false;  // per-sample records instead of pixel atomics
     JobRun run;
     bool cancel_req = false;
     hipEvent_t ev_copy = nullptr;

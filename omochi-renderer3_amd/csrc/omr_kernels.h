@@ -60,6 +60,9 @@ hipError_t shadow(const KernelArgs &a, const LaunchShape &s, hipStream_t st);
 hipError_t drain(const KernelArgs &a, const LaunchShape &s, int parity, bool stats, hipStream_t st);
 // Zeroes the completed-sample counters of ring slot `ring` (every shard).
 hipError_t clear_done(unsigned long long *done, uint32_t ring, hipStream_t st);
+// accum[3p..3p+2] = sums of the records of pixel p's spp_px samples
+hipError_t gather_records(const unsigned long long *recs, uint32_t spp_px, uint32_t n_px, unsigned long long *accum,
+                          hipStream_t st);
 hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, double n_samples, hipStream_t st);
 // Closest hit for n rays (n x 8 floats).  counters (optional, device) gets
 // node visits and primitive tests.

@@ -692,11 +692,32 @@ __device__ inline unsigned long long fix(float v) {
   if (v > kFixClamp) v = kFixClamp;
   return (unsigned long long)(long long)rintf(v * kFixScale);
 }
-__device__ inline void add_px(unsigned long long *acc, f3 beta, f3 L, float s) {
-  unsigned long long a = fix(beta.x * L.x * s), b = fix(beta.y * L.y * s), c = fix(beta.z * L.z * s);
+// Adds one contribution of sample sidx (pixel `pixel`) to the job's sums.
+// Device-scope atomics on the pixel sums cost about as much as the rest of
+// a shading step on an image-based-lighting render, so jobs that have
+// per-sample records take a plain read-modify-write there instead: no two
+// lanes ever hold the same sample (its paths and pending branches run one at
+// a time in one slot, and a shadow record exists only for the step that made
+// it).
+__device__ inline void deposit(const JobDev &J, uint32_t sidx, uint32_t pixel, unsigned long long a,
+                               unsigned long long b, unsigned long long c) {
+  if ((a | b | c) == 0ull) return;
+  if (J.recs) {
+    __attribute__((address_space(1))) unsigned long long *r =
+        (__attribute__((address_space(1))) unsigned long long *)J.recs + 3ull * sidx;
+    const unsigned long long ra = r[0], rb = r[1], rc = r[2];
+    r[0] = ra + a;
+    r[1] = rb + b;
+    r[2] = rc + c;
+    return;
+  }
+  unsigned long long *acc = J.accum + 3ull * pixel;
   if (a) gadd(acc + 0, a);
   if (b) gadd(acc + 1, b);
   if (c) gadd(acc + 2, c);
+}
+__device__ inline void add_px(const JobDev &J, uint32_t sidx, uint32_t pixel, f3 beta, f3 L, float s) {
+  deposit(J, sidx, pixel, fix(beta.x * L.x * s), fix(beta.y * L.y * s), fix(beta.z * L.z * s));
 }
 
 // ------------------------------------------------------------ job lookup
@@ -997,7 +1018,6 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
   bool env_on = S.env != nullptr;
   bool nee = env_on && (J.flags & 1u) && gld(S.env_marg + S.env_h) > 0ull;
   uint32_t pixel = decompose(J, sidx).pixel;
-  unsigned long long *acc = J.accum + 3ull * pixel;
   int32_t prim = __float_as_int(hr.y);
   uint32_t mis_next = 0;
   if (prim < 0) {
@@ -1010,7 +1030,7 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
         float pe = env_pdf_at(S, ei, d);
         w = (pp * pp) / (pp * pp + pe * pe);
       }
-      add_px(acc, beta, env_at(S, ei), w);
+      add_px(J, sidx, pixel, beta, env_at(S, ei), w);
     }
   } else {
     uint32_t pid = (uint32_t)prim;
@@ -1057,7 +1077,7 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
     } else {
       prr = 1.0f;
     }
-    add_px(acc, beta, em, 1.0f);
+    add_px(J, sidx, pixel, beta, em, 1.0f);
     uint32_t dmax = J.max_depth == 0u ? 255u : J.max_depth;
     bool can_continue = !killed && depth + 1u < dmax;
     if (!killed) {
@@ -1273,7 +1293,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5))) void k_
       float4 *sq = reinterpret_cast<float4 *>(A.p.shadow) + ((size_t)k * qcap + spos) * 3;
       sq[0] = make_float4(sh.o.x, sh.o.y, sh.o.z, __uint_as_float(st.js));
       sq[1] = make_float4(sh.d.x, sh.d.y, sh.d.z, __uint_as_float(sh.px));
-      sq[2] = make_float4(sh.c.x, sh.c.y, sh.c.z, 0.0f);
+      sq[2] = make_float4(sh.c.x, sh.c.y, sh.c.z, __uint_as_float(st.sidx));
     }
   }
   done_flush(A.done, dacc);
@@ -1284,11 +1304,7 @@ __device__ inline void add_shadow(const KernelArgs &A, uint32_t r) {
   const float4 *sq = reinterpret_cast<const float4 *>(A.p.shadow) + (size_t)r * 3;
   float4 a = sq[0], b = sq[1], c = sq[2];
   const JobDev &J = A.jobs[__float_as_uint(a.w)];
-  unsigned long long *acc = J.accum + 3ull * __float_as_uint(b.w);
-  unsigned long long qa = fix(c.x), qb = fix(c.y), qc = fix(c.z);
-  if (qa) gadd(acc + 0, qa);
-  if (qb) gadd(acc + 1, qb);
-  if (qc) gadd(acc + 2, qc);
+  deposit(J, __float_as_uint(c.w), __float_as_uint(b.w), fix(c.x), fix(c.y), fix(c.z));
 }
 
 template <int WG, int SHORT, bool DF>
@@ -1455,13 +1471,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
       uint32_t snv = 0, snt = 0;
       traverse<WG, SHORT, true, false>(S, sh.o, sh.d, hs, stk, snv, snt);
       shadows++;
-      if (hs.prim < 0) {
-        unsigned long long *acc = J.accum + 3ull * sh.px;
-        unsigned long long qa = fix(sh.c.x), qb = fix(sh.c.y), qc = fix(sh.c.z);
-        if (qa) gadd(acc + 0, qa);
-        if (qb) gadd(acc + 1, qb);
-        if (qc) gadd(acc + 2, qc);
-      }
+      if (hs.prim < 0) deposit(J, st.sidx, sh.px, fix(sh.c.x), fix(sh.c.y), fix(sh.c.z));
     }
     if (cont) {
       tracing = true;
@@ -1510,6 +1520,35 @@ __global__ void k_clear_shadow(Ctrl *ctrl) {
 
 __global__ void k_clear_done(unsigned long long *done, uint32_t ring) {
   if (threadIdx.x < kShards) done[done_index(threadIdx.x, ring)] = 0ull;
+}
+
+// Pixel sums from per-sample records: sample s belongs to pixel s / spp_px,
+// so each pixel's records are contiguous; one wave sums a pixel's records
+// (lanes stride through them) and writes its three sums.
+__global__ __launch_bounds__(256) void k_gather_records(const unsigned long long *recs, uint32_t spp_px,
+                                                        uint32_t n_px, unsigned long long *accum) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t nw = gridDim.x * 4u;
+  for (uint32_t p = blockIdx.x * 4u + (threadIdx.x >> 6); p < n_px; p += nw) {
+    const __attribute__((address_space(1))) unsigned long long *r = gptr(recs) + 3ull * spp_px * p;
+    unsigned long long a = 0ull, b = 0ull, c = 0ull;
+    for (uint32_t k = lane; k < spp_px; k += 64u) {
+      a += r[3ull * k];
+      b += r[3ull * k + 1ull];
+      c += r[3ull * k + 2ull];
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      a += __shfl_xor(a, off);
+      b += __shfl_xor(b, off);
+      c += __shfl_xor(c, off);
+    }
+    if (lane == 0u) {
+      accum[3ull * p] = a;
+      accum[3ull * p + 1ull] = b;
+      accum[3ull * p + 2ull] = c;
+    }
+  }
 }
 
 // mean = sum * 2^-24 / (ss^2 * samples), in double, as the CPU restatement
@@ -1797,6 +1836,14 @@ hipError_t clear_done(unsigned long long *done, uint32_t ring, hipStream_t st) {
   hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(kShards), 0, st, done, ring);
   return hipGetLastError();
 }
+hipError_t gather_records(const unsigned long long *recs, uint32_t spp_px, uint32_t n_px, unsigned long long *accum,
+                          hipStream_t st) {
+  if (n_px == 0) return hipSuccess;
+  const uint32_t grid = n_px < 262144u ? (n_px + 3u) / 4u : 65536u;
+  hipLaunchKernelGGL(k_gather_records, dim3(grid), dim3(256), 0, st, recs, spp_px, n_px, accum);
+  return hipGetLastError();
+}
+
 hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, double n_samples, hipStream_t st) {
   uint32_t n = n_px * 3u;
   if (n == 0) return hipSuccess;

@@ -116,6 +116,12 @@ struct JobDev {
   float cam_pos[3], cam_c[3], cam_x[3], cam_y[3];
   float aperture, focal;
   unsigned long long *accum;       // W*H*3 fixed-point sums
+  // n_samples*3 fixed-point sums, one record per sample, or null.  With
+  // records every contribution goes to its sample's record by a plain
+  // read-modify-write (a sample has one owner at a time) and a gather kernel
+  // sums each pixel's records into accum at the end; without them
+  // contributions go to accum by atomics.  Integer sums: the same totals.
+  unsigned long long *recs;
 };
 
 // Every device-wide counter that waves update in the hot loop is split into

@@ -30,7 +30,7 @@ uint64_t samples_of(const omr_camera &c, const omr_render_params &p) {
 int Engine::server_start(const omr_server_config &cfg) {
   if (server_on_.load()) return fail(OMR_ERR_STATE, "server already running");
   scfg_ = cfg;
-  if (scfg_.hbm_budget_bytes == 0) scfg_.hbm_budget_bytes = 16ull << 30;
+  if (scfg_.hbm_budget_bytes == 0) scfg_.hbm_budget_bytes = 64ull << 30;
   if (scfg_.max_live_jobs == 0 || scfg_.max_live_jobs > kJobRing) scfg_.max_live_jobs = 512;
   if (scfg_.pool_paths) {
     if (scfg_.pool_paths < 64 || scfg_.pool_paths > (1u << 26)) return fail(OMR_ERR_INVALID, "pool_paths out of range");
@@ -48,6 +48,17 @@ int Engine::server_start(const omr_server_config &cfg) {
     busy_ms_ = 0;
     latencies_.clear();
   }
+  // Request buffers come from the device's default pool (stream-ordered);
+  // keep freed blocks in the pool while serving instead of handing them back
+  // to the driver at every synchronization, so admissions reuse mapped memory.
+  {
+    std::lock_guard<std::mutex> g(mu);
+    hipSetDevice(device_);
+    hipMemPool_t pool = nullptr;
+    uint64_t keep = UINT64_MAX;
+    if (hipDeviceGetDefaultMemPool(&pool, device_) == hipSuccess && pool)
+      hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
   server_stop_ = false;
   server_on_ = true;
   server_ = std::thread([this] { server_loop(); });
@@ -60,6 +71,16 @@ int Engine::server_stop() {
   scv.notify_all();
   if (server_.joinable()) server_.join();
   server_on_ = false;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    hipSetDevice(device_);
+    hipMemPool_t pool = nullptr;
+    uint64_t keep = 0;
+    if (hipDeviceGetDefaultMemPool(&pool, device_) == hipSuccess && pool) {
+      hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+      hipMemPoolTrimTo(pool, 0);
+    }
+  }
   return OMR_OK;
 }
 
@@ -76,6 +97,11 @@ int Engine::submit(omr_scene_id scene_id, const omr_camera &cam, const omr_rende
   r->t_submit = std::chrono::steady_clock::now();
   size_t px = (size_t)cam.width * cam.height;
   r->bytes = px * 3 * 8 + px * 3 * 4;
+  // per-sample records (no atomics) for requests whose records take at most a
+  // quarter of the budget and fit in record_mb; larger ones use pixel atomics
+  const size_t rec = (size_t)samples_of(cam, p) * 24;
+  r->records = rec <= scfg_.hbm_budget_bytes / 4 && rec <= ((size_t)tuning.record_mb << 20);
+  if (r->records) r->bytes += rec;
   {
     std::lock_guard<std::mutex> g(smu);
     all_.push_back(r);
@@ -234,13 +260,16 @@ void Engine::server_loop() {
         j.logical = hi - admit.size() + (&r - &admit[0]);
         j.ring = (uint32_t)(j.logical % kJobRing);
         size_t px = (size_t)r->cam.width * r->cam.height;
+        j.d_recs = nullptr;
         if (hipMallocAsync((void **)&j.d_accum, px * 3 * 8, stream_) != hipSuccess ||
-            hipMallocAsync((void **)&j.d_out, px * 3 * 4, stream_) != hipSuccess) {
+            hipMallocAsync((void **)&j.d_out, px * 3 * 4, stream_) != hipSuccess ||
+            (r->records && hipMallocAsync((void **)&j.d_recs, (size_t)j.n_samples * 24, stream_) != hipSuccess)) {
           dev.unlock();
           fail_all("device allocation for a job failed");
           return;
         }
         hipMemsetAsync(j.d_accum, 0, px * 3 * 8, stream_);
+        if (j.d_recs) hipMemsetAsync(j.d_recs, 0, (size_t)j.n_samples * 24, stream_);
         k::clear_done(d_done_, j.ring, stream_);
         JobDev &D = h_jobs[j.ring];
         std::memset(&D, 0, sizeof(D));
@@ -265,6 +294,7 @@ void Engine::server_loop() {
         D.aperture = j.cam.aperture_radius;
         D.focal = j.cam.focal_distance;
         D.accum = j.d_accum;
+        D.recs = j.d_recs;
         hipMemcpyAsync(d_jobs_ + j.ring, &D, sizeof(JobDev), hipMemcpyHostToDevice, stream_);
         total += j.n_samples;
         live.push_back(r);
@@ -314,10 +344,12 @@ void Engine::server_loop() {
       j.done = true;
       double nspp = (double)j.params.supersamples * j.params.supersamples *
                     (double)(j.params.spp_end - j.params.spp_begin);
+      if (j.d_recs) k::gather_records(j.d_recs, (uint32_t)nspp, j.cam.width * j.cam.height, j.d_accum, stream_);
       k::resolve(j.d_accum, j.d_out, j.cam.width * j.cam.height, nspp, stream_);
       hipMemcpyAsync(r->out, j.d_out, (size_t)j.cam.width * j.cam.height * 3 * 4, hipMemcpyDeviceToHost, stream_);
       hipFreeAsync(j.d_accum, stream_);
       hipFreeAsync(j.d_out, stream_);
+      if (j.d_recs) hipFreeAsync(j.d_recs, stream_);
       finished.push_back(r);
     }
     if (!finished.empty()) {
@@ -356,6 +388,7 @@ void Engine::server_loop() {
     hipStreamSynchronize(stream_);
     for (auto &r : live) {
       hipFree(r->run.d_accum);
+      if (r->run.d_recs) hipFree(r->run.d_recs);
       hipFree(r->run.d_out);
     }
     hipHostFree(h_jobs);

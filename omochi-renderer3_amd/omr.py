@@ -76,7 +76,7 @@ class Tuning(C.Structure):
     _fields_ = [("node_width", C.c_uint32), ("wg_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("pool_paths", C.c_uint32), ("steps_per_sync", C.c_uint32), ("use_graph", C.c_uint32),
                 ("refill_every", C.c_uint32), ("fetch_budget", C.c_uint32), ("fetch_min", C.c_uint32),
-                ("stack_short", C.c_uint32), ("drain_below", C.c_uint32)]
+                ("stack_short", C.c_uint32), ("drain_below", C.c_uint32), ("record_mb", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):
