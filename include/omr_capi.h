@@ -169,7 +169,7 @@ typedef struct {
 
 typedef struct {
   uint64_t pool_paths;        /* path slots (0 = default) */
-  uint64_t hbm_budget_bytes;  /* admission budget for per-job buffers (0 = 16 GiB) */
+  uint64_t hbm_budget_bytes;  /* admission budget for per-job buffers (0 = 64 GiB) */
   uint32_t max_live_jobs;     /* 0 = 512 */
 } omr_server_config;
 

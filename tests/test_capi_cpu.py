@@ -25,6 +25,27 @@ def test_library_exports_every_declared_symbol():
     assert names <= set(omr.EXPORTS) | {"omr_scene_info_get"}
 
 
+@pytest.mark.parametrize("cname,pyname", [("omr_tuning", "Tuning"), ("omr_server_config", "ServerConfig")])
+def test_ctypes_structs_follow_the_header(cname, pyname):
+    """The Python binding's structs list the header's fields, in order, with
+    the same widths (the library reads them by layout)."""
+    import ctypes as C
+    import omr
+
+    with open(os.path.join(ROOT, "include", "omr_capi.h")) as f:
+        text = f.read()
+    body = re.search(r"typedef struct \{([^{}]*)\}\s*" + cname + r";", text).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for ctype, names in re.findall(r"(uint32_t|uint64_t|int32_t|double|float)\s+([\w\s,]+);", body):
+        fields += [(n.strip(), ctype) for n in names.split(",")]
+    width = {"uint32_t": C.c_uint32, "uint64_t": C.c_uint64, "int32_t": C.c_int32, "double": C.c_double,
+             "float": C.c_float}
+    py = getattr(omr, pyname)._fields_
+    assert [n for n, _ in py] == [n for n, _ in fields]
+    assert [C.sizeof(t) for _, t in py] == [C.sizeof(width[t]) for _, t in fields]
+
+
 def test_create_fails_loudly_without_gpu():
     import omr
     from conftest import gpu_available

@@ -31,9 +31,16 @@ def _cam(omr, scene, w, h):
     return c
 
 
-def test_batch_equals_alone(ctx):
+@pytest.mark.parametrize("record_mb", [None, 0, 1])
+def test_batch_equals_alone(ctx, record_mb):
+    """Also across the two ways of summing: per-sample records (default),
+    pixel atomics (record_mb=0), and a budget that gives records to the first
+    jobs of the batch only (record_mb=1) while each job alone has them."""
     import omr
 
+    saved = ctx.get_tuning()
+    if record_mb is not None:
+        ctx.set_tuning(record_mb=record_mb)
     scenes, spec = _jobs(omr)
     sids = [ctx.upload(s) for s in scenes]
     cams, prms, ids = [], [], []
@@ -43,10 +50,13 @@ def test_batch_equals_alone(ctx):
         ids.append(sids[si])
     print("batch of", len(spec), flush=True)
     batch = ctx.render_batch(ids, cams, prms)
-    for i in range(len(spec)):
-        alone = ctx.render(ids[i], cams[i], prms[i])
-        assert np.array_equal(alone, batch[i]), f"job {i} differs"
-        assert alone.mean() > 0
+    try:
+        for i in range(len(spec)):
+            alone = ctx.render(ids[i], cams[i], prms[i])
+            assert np.array_equal(alone, batch[i]), f"job {i} differs"
+            assert alone.mean() > 0
+    finally:
+        ctx.set_tuning(**saved)
     for s in sids:
         ctx.release(s)
 
@@ -65,10 +75,13 @@ def test_progressive_batches_equal_one_shot(ctx):
     ctx.release(sid)
 
 
-def test_served_equals_alone():
+@pytest.mark.parametrize("record_mb", [None, 0])
+def test_served_equals_alone(record_mb):
     import omr
 
     ctx = omr.Context(0)
+    if record_mb is not None:
+        ctx.set_tuning(record_mb=record_mb)
     scenes, spec = _jobs(omr)
     sids = [ctx.upload(s) for s in scenes]
     jobs = []

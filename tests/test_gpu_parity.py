@@ -49,11 +49,13 @@ def test_env_tables_bit_exact(ctx):
 
 
 @pytest.mark.parametrize("env_is,tune", [(0, {}), (1, {}), (1, dict(stack_short=8, fetch_budget=8)),
-                                         (1, dict(drain_below=1 << 20))])
+                                         (1, dict(drain_below=1 << 20)), (1, dict(record_mb=0)),
+                                         (1, dict(record_mb=0, drain_below=1 << 20))])
 def test_render_env_bit_exact(ctx, env_is, tune):
     """Torus knot under an HDR sky: lookup on escape, and with importance
     sampling + shadow rays + MIS (also with the 8-entry LDS stack and
-    dynamic fetch in the shadow kernel)."""
+    dynamic fetch in the shadow kernel, and with pixel atomics instead of
+    per-sample records)."""
     import omr
     import pyoracle
 
@@ -123,7 +125,8 @@ def test_render_bit_exact(ctx, kind, ss, spp, depth, rr):
                                   dict(wg_size=64, blocks_per_cu=2, pool_paths=4096, steps_per_sync=4),
                                   dict(stack_short=8), dict(stack_short=8, fetch_budget=8, wg_size=64,
                                                             blocks_per_cu=16),
-                                  dict(drain_below=1 << 20), dict(drain_below=2000, pool_paths=8192)])
+                                  dict(drain_below=1 << 20), dict(drain_below=2000, pool_paths=8192),
+                                  dict(record_mb=0), dict(record_mb=0, refill_every=2)])
 def test_tuning_modes_bit_exact(ctx, tune):
     """Every launch configuration the autotuner can pick gives the same bits:
     dynamic-fetch traversal, bulk refill cadence, graph off, small pools,
