@@ -779,7 +779,7 @@ __device__ inline void camera_ray(const JobDev &J, const Decomp &dc, f3 &o, f3 &
 }
 
 // Wave-level grab of up to `want` lanes' worth of samples: one fetch-and-add
-// per wave on a cursor shard k, whose j-th sample is j*kShards + k.  A wave
+// per wave on a cursor shard k, whose j-th sample is shard_sample(j, k).  A wave
 // starts on its own shard; when that one is used up it reads every cursor at
 // once (one lane per shard) and moves to the next shard with samples left, so
 // no shard's samples wait for waves that have no work.  The admitted total is
@@ -838,7 +838,7 @@ __device__ inline unsigned long long grab_sample(Ctrl *ctrl, SampleCursor &sc, b
     if (take > 0ull) {
       if (take < cnt) next_sample_shard(ctrl, sc);  // shard k is used up now
       if (!want || (unsigned long long)r >= take) return ~0ull;
-      return (base + r) * kShards + k;
+      return shard_sample(base + r, k);
     }
     if (!next_sample_shard(ctrl, sc)) return ~0ull;
   }

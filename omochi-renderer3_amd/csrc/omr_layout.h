@@ -142,14 +142,26 @@ struct Sharded {
   OMR_HD const T &operator[](uint32_t k) const { return c[k].v; }
 };
 
-// Samples handed out by cursor shard k are k, k + kShards, k + 2 kShards, ...
+// Cursor shard k hands out runs of kSampleRun consecutive samples: its j-th
+// sample is in run j / kSampleRun of the shard, and run q of shard k starts
+// at (q kShards + k) kSampleRun.  A wave that takes 64 cursor positions gets
+// consecutive samples (the sub-samples of one pixel, then its neighbours):
+// rays that start alike and per-sample sums that share cache lines.
+constexpr unsigned long long kSampleRun = 64;
+OMR_HD inline unsigned long long shard_sample(unsigned long long j, uint32_t k) {
+  return (j / kSampleRun) * (kShards * kSampleRun) + (unsigned long long)k * kSampleRun + j % kSampleRun;
+}
+// Number of cursor positions of shard k whose samples are below `total`
+// (shard_sample is increasing in j).
 OMR_HD inline unsigned long long shard_limit(unsigned long long total, uint32_t k) {
-  return total > k ? (total - k + kShards - 1) / kShards : 0ull;
+  const unsigned long long round = kShards * kSampleRun, rem = total % round, lo = (unsigned long long)k * kSampleRun;
+  const unsigned long long part = rem > lo ? (rem - lo < kSampleRun ? rem - lo : kSampleRun) : 0ull;
+  return total / round * kSampleRun + part;
 }
 
 // Control block in device memory (one per engine).
 struct Ctrl {
-  Sharded<unsigned long long> next_sample;                // striped sample cursors (index j -> sample j*kShards + k)
+  Sharded<unsigned long long> next_sample;                // sample cursors (position j -> sample shard_sample(j, k))
   alignas(kLine) unsigned long long total_samples;        // samples admitted so far (host-written)
   alignas(kLine) uint32_t job_lo;                         // live job window [job_lo, job_hi) (logical ring
   uint32_t job_hi;                                        // indices); the pair is copied as 8 bytes

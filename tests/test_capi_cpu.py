@@ -150,3 +150,43 @@ def test_bvh_rejects_bad_input():
     empty = omr.Scene(np.zeros((0, 9)), None, np.zeros((0, 4)), np.zeros(0, np.uint32), mats)
     with pytest.raises(omr.OmrError):
         omr.build_bvh(empty)
+
+
+def test_sample_cursor_shards_partition_every_total(tmp_path):
+    """The cursor shards hand out each sample below the admitted total exactly
+    once, in increasing order per shard, for totals that end mid-run and
+    mid-round (the layout header compiled on the host)."""
+    import subprocess
+
+    src = tmp_path / "shards.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <vector>
+#include "omr_layout.h"
+using namespace omr;
+int main() {
+  const unsigned long long totals[] = {0, 1, 63, 64, 65, 4095, 4096, 4097, 5000, 8191, 8192, 123457, 262144};
+  for (unsigned long long T : totals) {
+    std::vector<int> seen(T, 0);
+    for (uint32_t k = 0; k < kShards; k++) {
+      unsigned long long lim = shard_limit(T, k), prev = 0;
+      for (unsigned long long j = 0; j < lim; j++) {
+        unsigned long long s = shard_sample(j, k);
+        if (s >= T || (j && s <= prev)) { printf("bad T=%llu k=%u j=%llu\n", T, k, j); return 1; }
+        seen[s]++;
+        prev = s;
+      }
+      if (shard_sample(lim, k) < T) { printf("short T=%llu k=%u\n", T, k); return 1; }
+    }
+    for (unsigned long long s = 0; s < T; s++)
+      if (seen[s] != 1) { printf("sample %llu of %llu seen %d times\n", s, T, seen[s]); return 1; }
+  }
+  printf("ok\n");
+  return 0;
+}
+''')
+    exe = tmp_path / "shards"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "omochi-renderer3_amd", "csrc"), str(src),
+                    "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
