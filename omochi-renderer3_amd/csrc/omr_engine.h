@@ -167,8 +167,7 @@ class Engine {
     bool records = false;  // per-sample records instead of pixel atomics
     JobRun run;
     bool cancel_req = false;
-    hipEvent_t ev_copy = nullptr;
-    bool copying = false;
+    hipEvent_t ev_copy = nullptr;  // its image is resolved (the copy out waits on this)
   };
   std::thread server_;
   std::atomic<bool> server_on_{false};

@@ -191,9 +191,11 @@ void Engine::server_loop() {
   static_assert(offsetof(Ctrl, job_hi) == offsetof(Ctrl, job_lo) + 4 && offsetof(Stage, hi) == offsetof(Stage, lo) + 4,
                 "the job window is copied as one 8-byte pair");
   bool ok = true;
+  hipStream_t cs = nullptr;  // result copies, beside the compute stream
   {
     std::lock_guard<std::mutex> g(mu);
     if (ensure_pool() != OMR_OK) ok = false;
+    if (ok && hipStreamCreate(&cs) != hipSuccess) ok = false;
     if (ok && hipHostMalloc(&h_jobs, kJobRing * sizeof(JobDev), hipHostMallocDefault) != hipSuccess) ok = false;
     if (ok && hipHostMalloc(&h_stage, sizeof(Stage), hipHostMallocDefault) != hipSuccess) ok = false;
     if (ok) {
@@ -216,6 +218,41 @@ void Engine::server_loop() {
     fail_all("server start: device allocation failed");
     return;
   }
+  // Requests whose image is resolved on the compute stream but not yet copied
+  // out.  Their copies run on `cs` while the compute stream works on the next
+  // chunk of steps (the copy to a caller's pageable buffer holds this thread,
+  // not the GPU); then their device buffers go back and they complete.
+  std::vector<std::shared_ptr<Req>> copying;
+  auto flush_copies = [&]() {
+    if (copying.empty()) return;
+    for (auto &r : copying) {
+      hipStreamWaitEvent(cs, r->ev_copy, 0);
+      hipMemcpyAsync(r->out, r->run.d_out, (size_t)r->cam.width * r->cam.height * 3 * 4, hipMemcpyDeviceToHost, cs);
+    }
+    hipStreamSynchronize(cs);
+    for (auto &r : copying) {
+      hipEventDestroy(r->ev_copy);
+      r->ev_copy = nullptr;
+      hipFreeAsync(r->run.d_accum, stream_);
+      hipFreeAsync(r->run.d_out, stream_);
+      if (r->run.d_recs) hipFreeAsync(r->run.d_recs, stream_);
+    }
+    std::lock_guard<std::mutex> g(smu);
+    for (auto &r : copying) {
+      used_bytes_ -= r->bytes;
+      if (SceneHost *s = scene(r->scene)) s->users--;
+      if (r->cancel_req) {
+        r->state = 3;
+        n_cancelled_++;
+      } else {
+        r->state = 2;
+        n_done_++;
+        latencies_.push_back(ms_since(r->t_submit));
+      }
+    }
+    copying.clear();
+    scv.notify_all();
+  };
   auto busy_t0 = std::chrono::steady_clock::now();
   while (!server_stop_.load()) {
     // ---- 1. admission
@@ -328,6 +365,7 @@ void Engine::server_loop() {
       fail_all(err);
       return;
     }
+    flush_copies();
     hipMemcpyAsync(h_ctrl_, d_ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_);
     hipMemcpyAsync(h_done_, d_done_, kJobRing * kShards * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_);
     if (hipStreamSynchronize(stream_) != hipSuccess) {
@@ -346,14 +384,16 @@ void Engine::server_loop() {
                     (double)(j.params.spp_end - j.params.spp_begin);
       if (j.d_recs) k::gather_records(j.d_recs, (uint32_t)nspp, j.cam.width * j.cam.height, j.d_accum, stream_);
       k::resolve(j.d_accum, j.d_out, j.cam.width * j.cam.height, nspp, stream_);
-      hipMemcpyAsync(r->out, j.d_out, (size_t)j.cam.width * j.cam.height * 3 * 4, hipMemcpyDeviceToHost, stream_);
-      hipFreeAsync(j.d_accum, stream_);
-      hipFreeAsync(j.d_out, stream_);
-      if (j.d_recs) hipFreeAsync(j.d_recs, stream_);
+      if (hipEventCreateWithFlags(&r->ev_copy, hipEventDisableTiming) != hipSuccess ||
+          hipEventRecord(r->ev_copy, stream_) != hipSuccess) {
+        dev.unlock();
+        fail_all("result event");
+        return;
+      }
+      copying.push_back(r);
       finished.push_back(r);
     }
     if (!finished.empty()) {
-      hipStreamSynchronize(stream_);
       while (!live.empty() && live.front()->run.done) {
         live.pop_front();
         lo++;
@@ -362,30 +402,22 @@ void Engine::server_loop() {
       h_stage->hi = (uint32_t)hi;
       hipMemcpyAsync(&d_ctrl_->job_lo, &h_stage->lo, 8, hipMemcpyHostToDevice, stream_);
     }
+    // nothing left to overlap with: copy out now
+    if (live.empty()) flush_copies();
     dev.unlock();
     {
       std::lock_guard<std::mutex> g(smu);
       srv_segments_ = sum_shards(h_ctrl_->segments);
       srv_shadow_ = sum_shards(h_ctrl_->shadow_rays);
-      for (auto &r : finished) {
-        used_bytes_ -= r->bytes;
-        if (SceneHost *s = scene(r->scene)) s->users--;
-        if (r->cancel_req) {
-          r->state = 3;
-          n_cancelled_++;
-        } else {
-          r->state = 2;
-          n_done_++;
-          latencies_.push_back(ms_since(r->t_submit));
-        }
-      }
       if (live.empty()) busy_ms_ += ms_since(busy_t0);
-      if (!finished.empty()) scv.notify_all();
     }
   }
   {
     std::lock_guard<std::mutex> g(mu);
     hipStreamSynchronize(stream_);
+    flush_copies();
+    hipStreamSynchronize(stream_);
+    hipStreamDestroy(cs);
     for (auto &r : live) {
       hipFree(r->run.d_accum);
       if (r->run.d_recs) hipFree(r->run.d_recs);
