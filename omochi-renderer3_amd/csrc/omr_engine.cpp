@@ -164,6 +164,7 @@ KernelArgs Engine::args() const {
   a.p.df_budget = tuning.fetch_budget;
   a.p.df_fetch_min = tuning.fetch_min;
   a.p.stack_short = tuning.stack_short;
+  a.solo = solo_;
   return a;
 }
 
@@ -532,8 +533,8 @@ int Engine::launch_steps(uint32_t steps, bool with_shadow) {
     return OMR_OK;
   }
   int gi = with_shadow ? 1 : 0;
-  if (!graph_[gi] || graph_steps_ != steps) {
-    if (graph_steps_ != steps) drop_graphs();
+  if (!graph_[gi] || graph_steps_ != steps || graph_solo_ != a.solo) {
+    if (graph_steps_ != steps || graph_solo_ != a.solo) drop_graphs();
     hipGraph_t g = nullptr;
     OMR_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     hipError_t e = enqueue();
@@ -546,6 +547,7 @@ int Engine::launch_steps(uint32_t steps, bool with_shadow) {
     if (e != hipSuccess) return hipfail(e, "hipGraphInstantiate");
     graph_[gi] = ex;
     graph_steps_ = steps;
+    graph_solo_ = a.solo;
   }
   OMR_HIP(hipGraphLaunch(graph_[gi], stream_));
   return OMR_OK;
@@ -575,6 +577,7 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
 
   std::vector<JobDev> jd(jobs.size());
   unsigned long long total = 0;
+  solo_ = jobs.size() == 1 ? 0u : kNoSolo;  // job i takes ring slot i
   for (size_t i = 0; i < jobs.size(); i++) {
     JobRun &j = jobs[i];
     JobDev &D = jd[i];

@@ -137,6 +137,8 @@ class Engine {
   // graphs: [with shadow kernel]
   hipGraphExec_t graph_[2] = {nullptr, nullptr};
   uint32_t graph_steps_ = 0;
+  uint32_t graph_solo_ = kNoSolo;  // KernelArgs::solo the graphs were captured with
+  uint32_t solo_ = kNoSolo;        // ring slot of the current run's only job, or kNoSolo
 
   int ensure_pool();
   int ensure_gstack(uint32_t need_entries, size_t threads);

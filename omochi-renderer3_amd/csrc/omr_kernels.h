@@ -33,12 +33,16 @@ struct PoolPtrs {
   uint32_t stack_short;   // LDS traversal stack entries per lane (8 or 16)
 };
 
+// KernelArgs::solo when a run may hold several jobs
+constexpr uint32_t kNoSolo = 0xFFFFFFFFu;
+
 struct KernelArgs {
   Ctrl *ctrl;
   JobDev *jobs;          // ring of kJobRing entries
   unsigned long long *done;  // completed-sample counters, done_index(shard, ring)
   const SceneDev *scenes;
   PoolPtrs p;
+  uint32_t solo;         // ring slot of the run's only job, or kNoSolo
 };
 
 struct LaunchShape {

@@ -699,25 +699,26 @@ __device__ inline unsigned long long fix(float v) {
 // lanes ever hold the same sample (its paths and pending branches run one at
 // a time in one slot, and a shadow record exists only for the step that made
 // it).
-__device__ inline void deposit(const JobDev &J, uint32_t sidx, uint32_t pixel, unsigned long long a,
-                               unsigned long long b, unsigned long long c) {
+__device__ inline void deposit(unsigned long long *recs, unsigned long long *accum, uint32_t sidx, uint32_t pixel,
+                               unsigned long long a, unsigned long long b, unsigned long long c) {
   if ((a | b | c) == 0ull) return;
-  if (J.recs) {
+  if (recs) {
     __attribute__((address_space(1))) unsigned long long *r =
-        (__attribute__((address_space(1))) unsigned long long *)J.recs + 3ull * sidx;
+        (__attribute__((address_space(1))) unsigned long long *)recs + 3ull * sidx;
     const unsigned long long ra = r[0], rb = r[1], rc = r[2];
     r[0] = ra + a;
     r[1] = rb + b;
     r[2] = rc + c;
     return;
   }
-  unsigned long long *acc = J.accum + 3ull * pixel;
+  unsigned long long *acc = accum + 3ull * pixel;
   if (a) gadd(acc + 0, a);
   if (b) gadd(acc + 1, b);
   if (c) gadd(acc + 2, c);
 }
-__device__ inline void add_px(const JobDev &J, uint32_t sidx, uint32_t pixel, f3 beta, f3 L, float s) {
-  deposit(J, sidx, pixel, fix(beta.x * L.x * s), fix(beta.y * L.y * s), fix(beta.z * L.z * s));
+__device__ inline void add_px(unsigned long long *recs, unsigned long long *accum, uint32_t sidx, uint32_t pixel,
+                              f3 beta, f3 L, float s) {
+  deposit(recs, accum, sidx, pixel, fix(beta.x * L.x * s), fix(beta.y * L.y * s), fix(beta.z * L.z * s));
 }
 
 // ------------------------------------------------------------ job lookup
@@ -1002,8 +1003,23 @@ struct ShadowRay {
 // stored (null: st.pdf).  Returns true while the path, or one of its pending
 // glass branches, continues; `shadow` is set when an environment sample needs
 // a visibility ray.
+// Copy of a job or scene record read with scalar loads (p wave-uniform, in
+// uniform control flow): the fields then sit in scalar registers.
+template <class T>
+__device__ inline T cload(const T *p) {
+  static_assert(sizeof(T) % 4 == 0, "records are whole words");
+  const __attribute__((address_space(4))) uint32_t *q = (const __attribute__((address_space(4))) uint32_t *)p;
+  uint32_t w[sizeof(T) / 4];
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); i++) w[i] = q[i];
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+
 __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathState &st, float4 hr,
-                                     const float *pdf_mem, bool &shadow, ShadowRay &sh) {
+                                     const float *pdf_mem, bool &shadow, ShadowRay &sh, const JobDev &J,
+                                     const SceneDev &S) {
   float4 *pend = reinterpret_cast<float4 *>(A.p.pend);
   const uint32_t P = A.p.pool;
   bool cont = false;
@@ -1012,12 +1028,10 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
   uint32_t word = st.word;
   const uint32_t sidx = st.sidx;
   float pdf_next = 0.0f;
-  const JobDev &J = A.jobs[st.js];
-  const SceneDev &S = A.scenes[J.scene];
   uint32_t depth = ps_depth(word), branch = ps_branch(word), npend = ps_pending(word);
-  bool env_on = S.env != nullptr;
+  const bool env_on = S.env != nullptr;
   bool nee = env_on && (J.flags & 1u) && gld(S.env_marg + S.env_h) > 0ull;
-  uint32_t pixel = decompose(J, sidx).pixel;
+  const uint32_t pixel = sidx / (J.ss * J.ss * J.nspp);
   int32_t prim = __float_as_int(hr.y);
   uint32_t mis_next = 0;
   if (prim < 0) {
@@ -1030,7 +1044,7 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
         float pe = env_pdf_at(S, ei, d);
         w = (pp * pp) / (pp * pp + pe * pe);
       }
-      add_px(J, sidx, pixel, beta, env_at(S, ei), w);
+      add_px(J.recs, J.accum, sidx, pixel, beta, env_at(S, ei), w);
     }
   } else {
     uint32_t pid = (uint32_t)prim;
@@ -1077,7 +1091,7 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
     } else {
       prr = 1.0f;
     }
-    add_px(J, sidx, pixel, beta, em, 1.0f);
+    add_px(J.recs, J.accum, sidx, pixel, beta, em, 1.0f);
     uint32_t dmax = J.max_depth == 0u ? 255u : J.max_depth;
     bool can_continue = !killed && depth + 1u < dmax;
     if (!killed) {
@@ -1230,8 +1244,11 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
 // regenerate finished slots (regen) and compact the continuing paths into the
 // other queue.
 // At most 96 VGPRs (five waves per SIMD, no spills): shade waits on memory
-// most of the time, and the extra wave hides more of it.
-template <int WG>
+// most of the time, and the extra wave hides more of it.  SOLO: the run has
+// one job (A.solo), whose job and scene records are read once, into scalar
+// registers, instead of per lane and segment (two dependent loads ahead of
+// every other load of the segment).
+template <int WG, bool SOLO>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5))) void k_shade(KernelArgs A, int parity,
                                                                                       int regen) {
   __shared__ uint32_t s_pre[kShards + 1];
@@ -1245,6 +1262,12 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5))) void k_
   uint4 *rngs = reinterpret_cast<uint4 *>(A.p.rng);
   SampleCursor sc = sample_cursor(A.ctrl);
   DoneAcc dacc;
+  JobDev Js{};
+  SceneDev Ss{};
+  if (SOLO) {
+    Js = cload(A.jobs + A.solo);
+    Ss = cload(A.scenes + Js.scene);
+  }
   for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     bool valid = i < n;
@@ -1265,7 +1288,12 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5))) void k_
       st.sidx = ms.z;
       st.js = ms.w;
       st.pdf = 0.0f;
-      cont = shade_segment(A, slot, st, hr, A.p.pdf, shadow, sh);
+      if (SOLO) {
+        cont = shade_segment(A, slot, st, hr, A.p.pdf, shadow, sh, Js, Ss);
+      } else {
+        const JobDev &J = A.jobs[st.js];
+        cont = shade_segment(A, slot, st, hr, A.p.pdf, shadow, sh, J, A.scenes[J.scene]);
+      }
       fin = !cont;
     }
     count_done(A.done, dacc, fin, st.js);
@@ -1304,7 +1332,7 @@ __device__ inline void add_shadow(const KernelArgs &A, uint32_t r) {
   const float4 *sq = reinterpret_cast<const float4 *>(A.p.shadow) + (size_t)r * 3;
   float4 a = sq[0], b = sq[1], c = sq[2];
   const JobDev &J = A.jobs[__float_as_uint(a.w)];
-  deposit(J, __float_as_uint(c.w), __float_as_uint(b.w), fix(c.x), fix(c.y), fix(c.z));
+  deposit(J.recs, J.accum, __float_as_uint(c.w), __float_as_uint(b.w), fix(c.x), fix(c.y), fix(c.z));
 }
 
 template <int WG, int SHORT, bool DF>
@@ -1462,7 +1490,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     bool shadow = false;
     ShadowRay sh;
     bool cont = shade_segment(A, slot, st, make_float4(tr.best.t, __int_as_float(tr.best.prim), tr.best.u, tr.best.v),
-                              nullptr, shadow, sh);
+                              nullptr, shadow, sh, J, S);
     if (shadow) {
       Hit hs;
       hs.t = kInf;
@@ -1471,7 +1499,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
       uint32_t snv = 0, snt = 0;
       traverse<WG, SHORT, true, false>(S, sh.o, sh.d, hs, stk, snv, snt);
       shadows++;
-      if (hs.prim < 0) deposit(J, st.sidx, sh.px, fix(sh.c.x), fix(sh.c.y), fix(sh.c.z));
+      if (hs.prim < 0) deposit(J.recs, J.accum, st.sidx, sh.px, fix(sh.c.x), fix(sh.c.y), fix(sh.c.z));
     }
     if (cont) {
       tracing = true;
@@ -1808,10 +1836,20 @@ hipError_t extend(const KernelArgs &a, const LaunchShape &s, int parity, bool st
 hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool regen, hipStream_t st) {
   hipLaunchKernelGGL(k_clear_shadow, dim3(1), dim3(kShards), 0, st, a.ctrl);
   int r = regen ? 1 : 0;
+  const bool solo = a.solo != kNoSolo;
   switch (s.wg) {
-    case 64: hipLaunchKernelGGL(k_shade<64>, dim3(s.shade_grid), dim3(64), 0, st, a, parity, r); break;
-    case 128: hipLaunchKernelGGL(k_shade<128>, dim3(s.shade_grid), dim3(128), 0, st, a, parity, r); break;
-    default: hipLaunchKernelGGL(k_shade<256>, dim3(s.shade_grid), dim3(256), 0, st, a, parity, r); break;
+    case 64:
+      if (solo) hipLaunchKernelGGL((k_shade<64, true>), dim3(s.shade_grid), dim3(64), 0, st, a, parity, r);
+      else hipLaunchKernelGGL((k_shade<64, false>), dim3(s.shade_grid), dim3(64), 0, st, a, parity, r);
+      break;
+    case 128:
+      if (solo) hipLaunchKernelGGL((k_shade<128, true>), dim3(s.shade_grid), dim3(128), 0, st, a, parity, r);
+      else hipLaunchKernelGGL((k_shade<128, false>), dim3(s.shade_grid), dim3(128), 0, st, a, parity, r);
+      break;
+    default:
+      if (solo) hipLaunchKernelGGL((k_shade<256, true>), dim3(s.shade_grid), dim3(256), 0, st, a, parity, r);
+      else hipLaunchKernelGGL((k_shade<256, false>), dim3(s.shade_grid), dim3(256), 0, st, a, parity, r);
+      break;
   }
   return hipGetLastError();
 }

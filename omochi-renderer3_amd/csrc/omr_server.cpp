@@ -175,6 +175,10 @@ int Engine::server_stats(omr_server_stats *s) {
 
 void Engine::server_loop() {
   hipSetDevice(device_);
+  {
+    std::lock_guard<std::mutex> g(mu);
+    solo_ = kNoSolo;  // requests share the run
+  }
   std::deque<std::shared_ptr<Req>> live;
   uint64_t lo = 0, hi = 0;          // logical ring window [lo, hi)
   unsigned long long total = 0;     // samples admitted
