@@ -70,6 +70,20 @@ __device__ inline uint2_t gld(const uint2_t *p) {
   v2u v = *(__attribute__((address_space(1))) const v2u *)p;
   return uint2_t{v.x, v.y};
 }
+// Copy of a job or scene record read with scalar loads (p wave-uniform, in
+// uniform control flow): the fields then sit in scalar registers.
+template <class T>
+__device__ inline T cload(const T *p) {
+  static_assert(sizeof(T) % 4 == 0, "records are whole words");
+  const __attribute__((address_space(4))) uint32_t *q = (const __attribute__((address_space(4))) uint32_t *)p;
+  uint32_t w[sizeof(T) / 4];
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); i++) w[i] = q[i];
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+
 __device__ inline void gadd(unsigned long long *p, unsigned long long v) {
   __hip_atomic_fetch_add((__attribute__((address_space(1))) unsigned long long *)p, v, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
@@ -918,7 +932,7 @@ __global__ __launch_bounds__(WG) void k_refill(KernelArgs A, int parity) {
   }
 }
 
-template <int WG, int SHORT, bool STATS, bool DF>
+template <int WG, int SHORT, bool STATS, bool DF, bool SOLO>
 __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
   __shared__ uint2 lds[SHORT * WG];
   __shared__ uint32_t s_pre[kShards + 1];
@@ -938,14 +952,16 @@ __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
   stk.lds = lds;
   stk.g = A.p.gstack + (size_t)gtid * A.p.gcap;
   uint32_t segs = 0, nv = 0, nt = 0;
+  SceneDev Ss{};  // SOLO: the only job's scene, in scalar registers
+  if (SOLO) Ss = cload(A.scenes + cload(A.jobs + A.solo).scene);
   if (DF) {
     auto count = [&](uint32_t k) { return s_pre[k + 1] - s_pre[k]; };
     auto load = [&](uint32_t k, uint32_t off, const SceneDev *&S, f3 &o, f3 &d, float &tmax, uint32_t &tag) {
       uint32_t slot = queue[k * qcap + off];
       float4 ro = reinterpret_cast<const float4 *>(A.p.ray_o)[slot];
       float4 rd = reinterpret_cast<const float4 *>(A.p.ray_d)[slot];
-      uint32_t js = A.p.misc[4 * (size_t)slot + 3];
-      S = &A.scenes[A.jobs[js].scene];
+      if (SOLO) S = &Ss;
+      else S = &A.scenes[A.jobs[A.p.misc[4 * (size_t)slot + 3]].scene];
       o = xyz(ro);
       d = xyz(rd);
       tmax = kInf;
@@ -963,8 +979,7 @@ __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
       uint32_t slot = queue[flat_slot(s_pre, i, qcap)];
       float4 ro = reinterpret_cast<const float4 *>(A.p.ray_o)[slot];
       float4 rd = reinterpret_cast<const float4 *>(A.p.ray_d)[slot];
-      uint32_t js = A.p.misc[4 * (size_t)slot + 3];
-      const SceneDev &S = A.scenes[A.jobs[js].scene];
+      const SceneDev &S = SOLO ? Ss : A.scenes[A.jobs[A.p.misc[4 * (size_t)slot + 3]].scene];
       Hit h;
       h.t = kInf;
       h.u = h.v = 0.0f;
@@ -1003,20 +1018,6 @@ struct ShadowRay {
 // stored (null: st.pdf).  Returns true while the path, or one of its pending
 // glass branches, continues; `shadow` is set when an environment sample needs
 // a visibility ray.
-// Copy of a job or scene record read with scalar loads (p wave-uniform, in
-// uniform control flow): the fields then sit in scalar registers.
-template <class T>
-__device__ inline T cload(const T *p) {
-  static_assert(sizeof(T) % 4 == 0, "records are whole words");
-  const __attribute__((address_space(4))) uint32_t *q = (const __attribute__((address_space(4))) uint32_t *)p;
-  uint32_t w[sizeof(T) / 4];
-#pragma unroll
-  for (int i = 0; i < (int)(sizeof(T) / 4); i++) w[i] = q[i];
-  T r;
-  __builtin_memcpy(&r, w, sizeof(T));
-  return r;
-}
-
 __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathState &st, float4 hr,
                                      const float *pdf_mem, bool &shadow, ShadowRay &sh, const JobDev &J,
                                      const SceneDev &S) {
@@ -1399,7 +1400,7 @@ __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
 // Held to 128 VGPRs (four waves per SIMD; nine registers spill, which costs
 // less than the fourth wave gains: 6.8 -> 5.9 ms on the Cornell + icosphere
 // render).
-template <int WG, int SHORT, bool STATS>
+template <int WG, int SHORT, bool STATS, bool SOLO>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_drain(KernelArgs A, int parity) {
   __shared__ uint2 lds[SHORT * WG];
   __shared__ uint32_t s_pre[kShards + 1];
@@ -1432,6 +1433,13 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
   uint32_t slot = 0;
   PathState st;
   st.js = 0;
+  // SOLO: the run's only job and its scene, in scalar registers
+  JobDev Js{};
+  SceneDev Ss{};
+  if (SOLO) {
+    Js = cload(A.jobs + A.solo);
+    Ss = cload(A.scenes + Js.scene);
+  }
   for (;;) {
     unsigned long long m = __ballot(!have);
     uint32_t idle = (uint32_t)__popcll(m);
@@ -1456,7 +1464,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
         st.pdf = ps_mis(st.word) ? A.p.pdf[slot] : 0.0f;
         have = true;
         tracing = true;
-        tr.init(A.scenes[A.jobs[st.js].scene], st.o, st.d, kInf, stk);
+        tr.init(SOLO ? Ss : A.scenes[A.jobs[st.js].scene], st.o, st.d, kInf, stk);
       }
       if (base + idle >= cnt) {  // this sub-queue is used up: find one with entries left
         const uint32_t lane = lane_id();
@@ -1484,8 +1492,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     if (rm == 0ull || (__ballot(tracing) != 0ull && 2u * (uint32_t)__popcll(rm) < (uint32_t)__popcll(hm))) continue;
     if (!ready) continue;
     ready = false;
-    const JobDev &J = A.jobs[st.js];
-    const SceneDev &S = A.scenes[J.scene];
+    const JobDev &J = SOLO ? Js : A.jobs[st.js];
+    const SceneDev &S = SOLO ? Ss : A.scenes[J.scene];
     segs++;
     bool shadow = false;
     ShadowRay sh;
@@ -1755,10 +1763,15 @@ __global__ void k_env_guides(const unsigned long long *row, const unsigned long 
 template <int WG, int SHORT>
 hipError_t launch_extend_ws(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
   bool df = a.p.df_budget > 0;
-  if (stats && df) hipLaunchKernelGGL((k_extend<WG, SHORT, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else if (stats) hipLaunchKernelGGL((k_extend<WG, SHORT, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else if (df) hipLaunchKernelGGL((k_extend<WG, SHORT, false, true>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else hipLaunchKernelGGL((k_extend<WG, SHORT, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  if (WG == 64 && df && a.solo != kNoSolo) {  // single-job runs at the default group size and fetch mode
+    if (stats) hipLaunchKernelGGL((k_extend<WG, SHORT, true, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+    else hipLaunchKernelGGL((k_extend<WG, SHORT, false, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+    return hipGetLastError();
+  }
+  if (stats && df) hipLaunchKernelGGL((k_extend<WG, SHORT, true, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else if (stats) hipLaunchKernelGGL((k_extend<WG, SHORT, true, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else if (df) hipLaunchKernelGGL((k_extend<WG, SHORT, false, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else hipLaunchKernelGGL((k_extend<WG, SHORT, false, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
   return hipGetLastError();
 }
 template <int WG>
@@ -1782,10 +1795,18 @@ hipError_t launch_shadow_wg(const KernelArgs &a, uint32_t grid, hipStream_t st) 
 template <int WG>
 hipError_t launch_drain_wg(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
   const bool s8 = a.p.stack_short == kShortStackMin;
-  if (stats && s8) hipLaunchKernelGGL((k_drain<WG, (int)kShortStackMin, true>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else if (stats) hipLaunchKernelGGL((k_drain<WG, (int)kShortStackMax, true>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else if (s8) hipLaunchKernelGGL((k_drain<WG, (int)kShortStackMin, false>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else hipLaunchKernelGGL((k_drain<WG, (int)kShortStackMax, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  constexpr int S8 = (int)kShortStackMin, S16 = (int)kShortStackMax;
+  if (WG == 64 && a.solo != kNoSolo) {  // single-job runs at the default group size
+    if (stats && s8) hipLaunchKernelGGL((k_drain<WG, S8, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+    else if (stats) hipLaunchKernelGGL((k_drain<WG, S16, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+    else if (s8) hipLaunchKernelGGL((k_drain<WG, S8, false, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+    else hipLaunchKernelGGL((k_drain<WG, S16, false, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+    return hipGetLastError();
+  }
+  if (stats && s8) hipLaunchKernelGGL((k_drain<WG, S8, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else if (stats) hipLaunchKernelGGL((k_drain<WG, S16, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else if (s8) hipLaunchKernelGGL((k_drain<WG, S8, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else hipLaunchKernelGGL((k_drain<WG, S16, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
   return hipGetLastError();
 }
 
