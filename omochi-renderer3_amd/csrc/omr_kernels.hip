@@ -1329,14 +1329,16 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5))) void k_
 }
 
 // Accumulates the contribution of shadow record r if its ray was unoccluded.
-__device__ inline void add_shadow(const KernelArgs &A, uint32_t r) {
+// SOLO: Js is the run's only job.
+template <bool SOLO>
+__device__ inline void add_shadow(const KernelArgs &A, uint32_t r, const JobDev &Js) {
   const float4 *sq = reinterpret_cast<const float4 *>(A.p.shadow) + (size_t)r * 3;
   float4 a = sq[0], b = sq[1], c = sq[2];
-  const JobDev &J = A.jobs[__float_as_uint(a.w)];
+  const JobDev &J = SOLO ? Js : A.jobs[__float_as_uint(a.w)];
   deposit(J.recs, J.accum, __float_as_uint(c.w), __float_as_uint(b.w), fix(c.x), fix(c.y), fix(c.z));
 }
 
-template <int WG, int SHORT, bool DF>
+template <int WG, int SHORT, bool DF, bool SOLO>
 __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
   __shared__ uint2 lds[SHORT * WG];
   __shared__ uint32_t s_pre[kShards + 1];
@@ -1349,11 +1351,18 @@ __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
   stk.lds = lds;
   stk.g = A.p.gstack + (size_t)gtid * A.p.gcap;
   uint32_t cnt = 0;
+  JobDev Js{};  // SOLO: the only job and its scene, in scalar registers
+  SceneDev Ss{};
+  if (SOLO) {
+    Js = cload(A.jobs + A.solo);
+    Ss = cload(A.scenes + Js.scene);
+  }
   auto load = [&](uint32_t k, uint32_t off, const SceneDev *&S, f3 &o, f3 &d, float &tmax, uint32_t &tag) {
     uint32_t r = k * qcap + off;
     const float4 *sq = reinterpret_cast<const float4 *>(A.p.shadow) + (size_t)r * 3;
     float4 a = sq[0], b = sq[1];
-    S = &A.scenes[A.jobs[__float_as_uint(a.w)].scene];
+    if (SOLO) S = &Ss;
+    else S = &A.scenes[A.jobs[__float_as_uint(a.w)].scene];
     o = xyz(a);
     d = xyz(b);
     tmax = kInf;
@@ -1362,7 +1371,7 @@ __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
   if (DF) {
     auto count = [&](uint32_t k) { return s_pre[k + 1] - s_pre[k]; };
     auto store = [&](uint32_t r, const Hit &h) {
-      if (h.prim < 0) add_shadow(A, r);
+      if (h.prim < 0) add_shadow<SOLO>(A, r, Js);
     };
     uint32_t nv = 0, nt = 0;
     cnt = dynamic_fetch<WG, SHORT, true, false>(A.ctrl->shead, count, stk, (int)A.p.df_budget, A.p.df_fetch_min,
@@ -1383,7 +1392,7 @@ __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
       h.prim = -1;
       uint32_t nv = 0, nt = 0;
       traverse<WG, SHORT, true, false>(*S, o, d, h, stk, nv, nt);
-      if (h.prim < 0) add_shadow(A, r);
+      if (h.prim < 0) add_shadow<SOLO>(A, r, Js);
       cnt++;
     }
   }
@@ -1782,8 +1791,10 @@ hipError_t launch_extend_wg(const KernelArgs &a, uint32_t grid, int parity, bool
 
 template <int WG, int SHORT>
 hipError_t launch_shadow_ws(const KernelArgs &a, uint32_t grid, hipStream_t st) {
-  if (a.p.df_budget > 0) hipLaunchKernelGGL((k_shadow<WG, SHORT, true>), dim3(grid), dim3(WG), 0, st, a);
-  else hipLaunchKernelGGL((k_shadow<WG, SHORT, false>), dim3(grid), dim3(WG), 0, st, a);
+  if (WG == 64 && a.p.df_budget > 0 && a.solo != kNoSolo)  // single-job runs, default group size and fetch mode
+    hipLaunchKernelGGL((k_shadow<WG, SHORT, true, true>), dim3(grid), dim3(WG), 0, st, a);
+  else if (a.p.df_budget > 0) hipLaunchKernelGGL((k_shadow<WG, SHORT, true, false>), dim3(grid), dim3(WG), 0, st, a);
+  else hipLaunchKernelGGL((k_shadow<WG, SHORT, false, false>), dim3(grid), dim3(WG), 0, st, a);
   return hipGetLastError();
 }
 template <int WG>
