@@ -612,17 +612,35 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
   c.total_samples = total;
   c.job_lo = 0;
   c.job_hi = (uint32_t)jobs.size();
+  // First fill of the cleared pool: slot i takes sample i for i < n0, with
+  // the cursors and sub-queue counts set here to what the cursor grabs would
+  // have left (shard k: the positions below n0); falls back to the refill
+  // kernel if a sub-queue could not hold its share.
+  const uint32_t n0 = (uint32_t)std::min<uint64_t>(pool_cap_, total);
+  bool fresh = true;
+  for (uint32_t k = 0; k < kShards; k++)
+    if (shard_limit(n0, k) > pool_.qcap) fresh = false;
+  if (fresh) {
+    for (uint32_t k = 0; k < kShards; k++) {
+      c.next_sample[k] = shard_limit(n0, k);
+      c.qcount[0][k] = (uint32_t)shard_limit(n0, k);
+    }
+  }
   OMR_HIP(hipMemcpyAsync(d_jobs_, jd.data(), jd.size() * sizeof(JobDev), hipMemcpyHostToDevice, stream_));
   OMR_HIP(hipMemcpyAsync(d_ctrl_, &c, sizeof(Ctrl), hipMemcpyHostToDevice, stream_));
   OMR_HIP(hipMemsetAsync(d_done_, 0, kJobRing * kShards * sizeof(unsigned long long), stream_));
-  OMR_HIP(hipMemsetAsync(pool_.misc, 0, (size_t)pool_cap_ * 16, stream_));
+  if (fresh) {
+    if (n0 < pool_cap_) OMR_HIP(hipMemsetAsync(pool_.misc + 4ull * n0, 0, (size_t)(pool_cap_ - n0) * 16, stream_));
+  } else {
+    OMR_HIP(hipMemsetAsync(pool_.misc, 0, (size_t)pool_cap_ * 16, stream_));
+  }
   for (auto &j : jobs) {
     OMR_HIP(hipMemsetAsync(j.d_accum, 0, (size_t)j.cam.width * j.cam.height * 3 * 8, stream_));
     if (j.d_recs) OMR_HIP(hipMemsetAsync(j.d_recs, 0, (size_t)j.n_samples * 24, stream_));
   }
   OMR_HIP(hipEventRecord(ev_a_, stream_));
   KernelArgs a = args();
-  hipError_t e = k::refill(a, sh, 0, stream_);
+  hipError_t e = fresh ? k::fill_fresh(a, n0, stream_) : k::refill(a, sh, 0, stream_);
   if (e != hipSuccess) return hipfail(e, "refill");
   uint64_t steps_done = 0;
   const uint32_t chunk = tuning.steps_per_sync;

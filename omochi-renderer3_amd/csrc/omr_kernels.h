@@ -52,6 +52,10 @@ struct LaunchShape {
 };
 
 namespace k {
+// First fill of a cleared pool (queue 0): slot i takes sample i for i < n0,
+// without atomics; the caller has set the cursors and queue-0 counts of
+// shard k to shard_limit(n0, k).
+hipError_t fill_fresh(const KernelArgs &a, uint32_t n0, hipStream_t st);
 hipError_t refill(const KernelArgs &a, const LaunchShape &s, int parity, hipStream_t st);
 // stats: count node visits / primitive tests into Ctrl (profiling runs)
 hipError_t extend(const KernelArgs &a, const LaunchShape &s, int parity, bool stats, hipStream_t st);

@@ -932,6 +932,20 @@ __global__ __launch_bounds__(WG) void k_refill(KernelArgs A, int parity) {
   }
 }
 
+// First fill of a cleared pool: slot i < n0 takes sample i.  Sample i sits
+// at cursor position (r / kShards) * 64 + i % 64 of shard r % kShards, r =
+// i / 64, so run r goes to that sub-queue at the same entries, and the host
+// sets the cursors and counts to match: the same state the cursor grabs
+// would leave, without an atomic.
+__global__ __launch_bounds__(256) void k_fill_fresh(KernelArgs A, uint32_t n0) {
+  const uint32_t stride = gridDim.x * 256u;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n0; i += stride) {
+    start_sample(A, i, (unsigned long long)i);
+    const uint32_t r = i / 64u;
+    A.p.queue[0][(r % kShards) * A.p.qcap + (r / kShards) * 64u + i % 64u] = i;
+  }
+}
+
 template <int WG, int SHORT, bool STATS, bool DF, bool SOLO>
 __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
   __shared__ uint2 lds[SHORT * WG];
@@ -1847,6 +1861,13 @@ hipError_t launch_trace_wg(const SceneDev *scene, const float4 *r4, uint32_t n, 
 }  // namespace
 
 namespace k {
+
+hipError_t fill_fresh(const KernelArgs &a, uint32_t n0, hipStream_t st) {
+  if (n0 == 0) return hipSuccess;
+  const uint32_t need = (n0 + 255u) / 256u;
+  hipLaunchKernelGGL(k_fill_fresh, dim3(need < 16384u ? need : 16384u), dim3(256), 0, st, a, n0);
+  return hipGetLastError();
+}
 
 hipError_t refill(const KernelArgs &a, const LaunchShape &s, int parity, hipStream_t st) {
   switch (s.wg) {
