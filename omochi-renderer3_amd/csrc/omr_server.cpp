@@ -229,11 +229,13 @@ void Engine::server_loop() {
   std::vector<std::shared_ptr<Req>> copying;
   auto flush_copies = [&]() {
     if (copying.empty()) return;
+    bool copied = true;
     for (auto &r : copying) {
-      hipStreamWaitEvent(cs, r->ev_copy, 0);
-      hipMemcpyAsync(r->out, r->run.d_out, (size_t)r->cam.width * r->cam.height * 3 * 4, hipMemcpyDeviceToHost, cs);
+      copied = copied && hipStreamWaitEvent(cs, r->ev_copy, 0) == hipSuccess &&
+               hipMemcpyAsync(r->out, r->run.d_out, (size_t)r->cam.width * r->cam.height * 3 * 4,
+                              hipMemcpyDeviceToHost, cs) == hipSuccess;
     }
-    hipStreamSynchronize(cs);
+    copied = hipStreamSynchronize(cs) == hipSuccess && copied;
     for (auto &r : copying) {
       hipEventDestroy(r->ev_copy);
       r->ev_copy = nullptr;
@@ -245,7 +247,10 @@ void Engine::server_loop() {
     for (auto &r : copying) {
       used_bytes_ -= r->bytes;
       if (SceneHost *s = scene(r->scene)) s->users--;
-      if (r->cancel_req) {
+      if (!copied) {
+        r->state = 4;  // its image did not reach the caller
+        err = "result copy failed";
+      } else if (r->cancel_req) {
         r->state = 3;
         n_cancelled_++;
       } else {
