@@ -4,9 +4,12 @@
 // (1) admits queued requests while the job ring and the HBM budget allow,
 //     appending their samples to the global sample range and waking idle
 //     path slots (refill), so new work overlaps with work already in flight;
-// (2) runs one chunk of bounce steps (a hipGraph);
+// (2) runs one chunk of bounce steps (a hipGraph), or, once every admitted
+//     sample is handed out and few paths remain, one drain launch that ends
+//     them all;
 // (3) detects finished jobs from their device completion counters, resolves
-//     them into the caller's buffer and frees their memory.
+//     them and copies them to the caller's buffer on a second stream while
+//     the next chunk runs, then frees their memory.
 // Results are identical to running each job alone: per-path seeds depend only
 // on (job seed, pixel, sub-pixel, sample, branch) and accumulation is integer.
 #include <algorithm>
@@ -383,6 +386,23 @@ void Engine::server_loop() {
       return;
     }
     for (uint32_t k = 0; k < kShards; k++) cursor[k] = h_ctrl_->next_sample[k];
+    // ---- 2b. drain: every admitted sample is handed out and few paths are
+    // left (no request could be admitted above): finish them in one launch
+    // instead of many short steps, then start the next run on empty queues.
+    const uint64_t queued = queue_len(*h_ctrl_, 0);
+    if (tuning.drain_below && queued > 0 && queued <= tuning.drain_below && samples_exhausted(*h_ctrl_)) {
+      bool ok2 = k::drain(args(), shape(), 0, false, stream_) == hipSuccess &&
+                 hipMemsetAsync(&d_ctrl_->qcount[0], 0, sizeof(d_ctrl_->qcount[0]), stream_) == hipSuccess &&
+                 hipMemsetAsync(&d_ctrl_->qhead[0], 0, sizeof(d_ctrl_->qhead[0]), stream_) == hipSuccess;
+      hipMemcpyAsync(h_ctrl_, d_ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_);
+      hipMemcpyAsync(h_done_, d_done_, kJobRing * kShards * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                     stream_);
+      if (!ok2 || hipStreamSynchronize(stream_) != hipSuccess) {
+        dev.unlock();
+        fail_all("device error in the drain");
+        return;
+      }
+    }
     // ---- 3. completions
     std::vector<std::shared_ptr<Req>> finished;
     for (auto &r : live) {
