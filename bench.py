@@ -190,8 +190,9 @@ def cpu_baseline(scene, cam, prm, rows, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed renders (default: 20 for the plain workloads, 3 batches for c3)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed renders first (default 2; 1 for c3)")
     ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS) + ["c3", "c4"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU baseline sample (0 = all)")
@@ -204,6 +205,11 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device three times (hardware-counter calibration)")
     args = ap.parse_args()
+    plain = args.workload not in ("c3", "c4")
+    if args.steps is None:
+        args.steps = 20 if plain else 3
+    if args.warmup is None:
+        args.warmup = 2 if plain else 1
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is design-only.",
               file=sys.stderr)
