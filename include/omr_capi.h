@@ -145,7 +145,7 @@ typedef struct {
 typedef struct {
   uint32_t node_width;        /* 4 (128-byte nodes) */
   uint32_t wg_size;           /* 64, 128 or 256 */
-  uint32_t blocks_per_cu;     /* persistent grid = CUs x blocks_per_cu (shade runs 20 waves per CU) */
+  uint32_t blocks_per_cu;     /* persistent grid = CUs x blocks_per_cu (shade runs 20 waves per CU, 16 for one job) */
   uint32_t pool_paths;        /* path slots in flight (max) */
   uint32_t steps_per_sync;    /* bounce steps per host check (graph length) */
   uint32_t use_graph;         /* capture bounce steps in a hipGraph */

@@ -1258,13 +1258,16 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
 // One shading step over the ray queue `parity`: shade each hit, then
 // regenerate finished slots (regen) and compact the continuing paths into the
 // other queue.
-// At most 96 VGPRs (five waves per SIMD, no spills): shade waits on memory
-// most of the time, and the extra wave hides more of it.  SOLO: the run has
+// Five waves per SIMD (96 VGPRs, no spills); the single-job form, which
+// spills at five, runs four (up to 128 VGPRs) on a grid of its own.  Each
+// lane reads the queue entry of its next item one item ahead, so that load
+// is off the front of the next item's chain of dependent loads: shade waits
+// on memory most of the time.  SOLO: the run has
 // one job (A.solo), whose job and scene records are read once, into scalar
 // registers, instead of per lane and segment (two dependent loads ahead of
 // every other load of the segment).
 template <int WG, bool SOLO>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5))) void k_shade(KernelArgs A, int parity,
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO ? 4 : 5))) void k_shade(KernelArgs A, int parity,
                                                                                       int regen) {
   __shared__ uint32_t s_pre[kShards + 1];
   load_prefix(s_pre, A.ctrl->qcount[parity]);
@@ -1283,10 +1286,14 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5))) void k_
     Js = cload(A.jobs + A.solo);
     Ss = cload(A.scenes + Js.scene);
   }
+  const uint32_t i0 = blockIdx.x * WG + threadIdx.x;
+  uint32_t next = i0 < n ? A.p.queue[parity][flat_slot(s_pre, i0, qcap)] : 0u;
   for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     bool valid = i < n;
-    uint32_t slot = valid ? A.p.queue[parity][flat_slot(s_pre, i, qcap)] : 0u;
+    const uint32_t slot = next;
+    const uint32_t i2 = i + stride;
+    next = i2 < n ? A.p.queue[parity][flat_slot(s_pre, i2, qcap)] : 0u;
     bool cont = false, fin = false, shadow = false;
     PathState st;
     st.js = 0;
@@ -1892,15 +1899,15 @@ hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool reg
   const bool solo = a.solo != kNoSolo;
   switch (s.wg) {
     case 64:
-      if (solo) hipLaunchKernelGGL((k_shade<64, true>), dim3(s.shade_grid), dim3(64), 0, st, a, parity, r);
+      if (solo) hipLaunchKernelGGL((k_shade<64, true>), dim3(s.solo_shade_grid), dim3(64), 0, st, a, parity, r);
       else hipLaunchKernelGGL((k_shade<64, false>), dim3(s.shade_grid), dim3(64), 0, st, a, parity, r);
       break;
     case 128:
-      if (solo) hipLaunchKernelGGL((k_shade<128, true>), dim3(s.shade_grid), dim3(128), 0, st, a, parity, r);
+      if (solo) hipLaunchKernelGGL((k_shade<128, true>), dim3(s.solo_shade_grid), dim3(128), 0, st, a, parity, r);
       else hipLaunchKernelGGL((k_shade<128, false>), dim3(s.shade_grid), dim3(128), 0, st, a, parity, r);
       break;
     default:
-      if (solo) hipLaunchKernelGGL((k_shade<256, true>), dim3(s.shade_grid), dim3(256), 0, st, a, parity, r);
+      if (solo) hipLaunchKernelGGL((k_shade<256, true>), dim3(s.solo_shade_grid), dim3(256), 0, st, a, parity, r);
       else hipLaunchKernelGGL((k_shade<256, false>), dim3(s.shade_grid), dim3(256), 0, st, a, parity, r);
       break;
   }
