@@ -179,6 +179,7 @@ class Engine {
   std::deque<std::shared_ptr<Req>> queue_;
   std::vector<std::shared_ptr<Req>> all_;  // by id-1
   omr_server_config scfg_{};
+  uint64_t srv_record_bytes_ = 0;  // per-request record limit, fixed at server start
   size_t used_bytes_ = 0;
   uint64_t n_submitted_ = 0, n_done_ = 0, n_rejected_ = 0, n_cancelled_ = 0;
   uint64_t srv_segments_ = 0, srv_shadow_ = 0;

@@ -35,6 +35,10 @@ int Engine::server_start(const omr_server_config &cfg) {
   scfg_ = cfg;
   if (scfg_.hbm_budget_bytes == 0) scfg_.hbm_budget_bytes = 64ull << 30;
   if (scfg_.max_live_jobs == 0 || scfg_.max_live_jobs > kJobRing) scfg_.max_live_jobs = 512;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    srv_record_bytes_ = std::min<uint64_t>(scfg_.hbm_budget_bytes / 4, (uint64_t)tuning.record_mb << 20);
+  }
   if (scfg_.pool_paths) {
     if (scfg_.pool_paths < 64 || scfg_.pool_paths > (1u << 26)) return fail(OMR_ERR_INVALID, "pool_paths out of range");
     std::lock_guard<std::mutex> g(mu);
@@ -103,7 +107,7 @@ int Engine::submit(omr_scene_id scene_id, const omr_camera &cam, const omr_rende
   // per-sample records (no atomics) for requests whose records take at most a
   // quarter of the budget and fit in record_mb; larger ones use pixel atomics
   const size_t rec = (size_t)samples_of(cam, p) * 24;
-  r->records = rec <= scfg_.hbm_budget_bytes / 4 && rec <= ((size_t)tuning.record_mb << 20);
+  r->records = rec <= srv_record_bytes_;
   if (r->records) r->bytes += rec;
   {
     std::lock_guard<std::mutex> g(smu);
