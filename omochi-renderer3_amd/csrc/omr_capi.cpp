@@ -355,7 +355,7 @@ int omr_autotune(omr_ctx *ctx, size_t n, const omr_scene_id *scenes, const omr_c
       {&omr_tuning::node_width, {4}},
       {&omr_tuning::wg_size, {64, 128, 256}},
       {&omr_tuning::blocks_per_cu, {4, 8, 16, 19}},
-      {&omr_tuning::pool_paths, {1u << 22, 1u << 24, 1u << 25}},
+      {&omr_tuning::pool_paths, {1u << 22, 1u << 24, 1u << 25, 1u << 26}},
       {&omr_tuning::refill_every, {0, 1, 2, 4}},
       {&omr_tuning::fetch_budget, {0, 4, 8, 16}},
       {&omr_tuning::fetch_min, {8, 16, 32}},

@@ -30,7 +30,7 @@ Engine::Engine(int device) : device_(device) {
   tuning.node_width = 4;
   tuning.wg_size = 64;
   tuning.blocks_per_cu = 19;  // the most 64-thread groups whose 16-entry LDS stacks fit in one CU
-  tuning.pool_paths = 1u << 25;
+  tuning.pool_paths = 1u << 26;  // 15 GB of path state: more paths in flight, fewer steps (c2 +4 %)
   tuning.steps_per_sync = 2;
   tuning.use_graph = 1;
   tuning.refill_every = 0;
@@ -636,7 +636,10 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
   OMR_HIP(hipMemcpyAsync(d_ctrl_, &c, sizeof(Ctrl), hipMemcpyHostToDevice, stream_));
   OMR_HIP(hipMemsetAsync(d_done_, 0, kJobRing * kShards * sizeof(unsigned long long), stream_));
   if (fresh) {
-    if (n0 < pool_cap_) OMR_HIP(hipMemsetAsync(pool_.misc + 4ull * n0, 0, (size_t)(pool_cap_ - n0) * 16, stream_));
+    // slots past n0 are read only by the bulk refill (with regeneration they
+    // never enter a queue)
+    if (n0 < pool_cap_ && tuning.refill_every != 0)
+      OMR_HIP(hipMemsetAsync(pool_.misc + 4ull * n0, 0, (size_t)(pool_cap_ - n0) * 16, stream_));
   } else {
     OMR_HIP(hipMemsetAsync(pool_.misc, 0, (size_t)pool_cap_ * 16, stream_));
   }
