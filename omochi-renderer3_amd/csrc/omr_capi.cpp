@@ -79,17 +79,24 @@ int render_impl(omr_ctx *c, size_t n, const omr_scene_id *scenes, const omr_came
     bytes += (px * 3 * 8 + 255) & ~(size_t)255;
     off_out[i] = bytes;
     bytes += (px * 3 * 4 + 255) & ~(size_t)255;
-    // per-sample records while the render's records fit in record_mb
-    const size_t rb = ((size_t)ns * 24 + 255) & ~(size_t)255;
+  }
+  // per-sample records, after every job's sums and image, while the
+  // render's records fit in record_mb
+  for (size_t i = 0; i < n; i++) {
+    const size_t rb = ((size_t)jobs[i].n_samples * 24 + 255) & ~(size_t)255;
     if (rec_bytes + rb <= rec_cap) {
-      off_rec[i] = bytes;
-      bytes += rb;
+      off_rec[i] = bytes + rec_bytes;
       rec_bytes += rb;
     }
   }
   std::lock_guard<std::mutex> g(c->eng->mu);
   hipSetDevice(c->eng->device_id());
-  int rc = ensure_scratch(c, bytes);
+  int rc = ensure_scratch(c, bytes + rec_bytes);
+  if (rc && rec_bytes) {  // no room for the records: pixel atomics instead (the same results)
+    (void)hipGetLastError();  // the failed allocation must not surface at the next launch check
+    std::fill(off_rec.begin(), off_rec.end(), SIZE_MAX);
+    rc = ensure_scratch(c, bytes);
+  }
   if (rc) return rc;
   for (size_t i = 0; i < n; i++) {
     jobs[i].d_accum = (unsigned long long *)((char *)c->scratch + off_acc[i]);
