@@ -68,6 +68,11 @@ def batched_jobs(n_jobs=64, seed=0x6F6D7233):
     return jobs
 
 
+# Config 4's eight resident scenes: (procedural kind, seed offset).  Kind 4 is
+# a 2000x1000 terrain (3,996,002 triangles) + 200 spheres with its own sky.
+SERVED_SCENES = [(0, 1), (1, 1), (1, 2), (2, 1), (3, 1), (3, 2), (4, 1), (0, 2)]
+
+
 def served_requests(n, kinds, seed):
     """Config 4 requests: pixel count log-uniform in [64^2, 1920x1080] at 16:9,
     spp in {1, 4, 16, 64, 256} (ss = sqrt(spp)), random seeds."""
@@ -126,7 +131,7 @@ def run_batched(ctx, omr, args):
 
 
 def run_served(ctx, omr, args):
-    kinds = [(0, 1), (1, 1), (1, 2), (2, 1), (3, 1), (3, 2), (4, 1), (0, 2)]
+    kinds = SERVED_SCENES
     scenes = [omr.procedural(k, seed=0x6F6D7233 + s, width=64, height=64) for k, s in kinds]
     sids = [ctx.upload(s) for s in scenes]
     n_tri = sum(s.tri.shape[0] for s in scenes)

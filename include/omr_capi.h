@@ -168,8 +168,13 @@ typedef struct {
 } omr_tuning;
 
 typedef struct {
-  uint64_t pool_paths;        /* path slots (0 = default) */
-  uint64_t hbm_budget_bytes;  /* admission budget for per-job buffers (0 = 64 GiB) */
+  uint64_t pool_paths;        /* path slots (0 = the tuning's pool_paths; restored at omr_server_stop) */
+  uint64_t hbm_budget_bytes;  /* device memory the server may hold (0 = 64 GiB): the path pool, its
+                                 queues and traversal stacks, the resident scenes and the admitted
+                                 requests' buffers.  omr_server_start fails with OMR_ERR_BUDGET when
+                                 the pool and scenes alone exceed it; a request whose sums and image
+                                 (12 bytes per pixel) cannot fit is rejected at submission; per-sample
+                                 records (24 bytes per sample) are added only where they fit */
   uint32_t max_live_jobs;     /* 0 = 512 */
 } omr_server_config;
 
@@ -178,6 +183,8 @@ typedef struct {
   uint64_t segments, shadow_rays;
   double busy_ms;             /* wall time with work on the device */
   double lat_p50_ms, lat_p99_ms, lat_max_ms;  /* submit -> result on host */
+  uint64_t fixed_bytes;       /* budgeted device memory of the path pool, queues, stacks and scenes */
+  uint64_t used_bytes;        /* budgeted device memory of the admitted requests */
 } omr_server_stats;
 
 /* ---- context ---- */
@@ -248,9 +255,15 @@ int omr_server_start(omr_ctx *ctx, const omr_server_config *cfg);
 int omr_server_stop(omr_ctx *ctx);
 int omr_job_submit(omr_ctx *ctx, omr_scene_id scene, const omr_camera *cam, const omr_render_params *params,
                    float *out_rgb, omr_job_id *job);
-/* state: 0 queued, 1 running, 2 done, 3 cancelled, 4 failed */
+/* state: 0 queued, 1 running, 2 done, 3 cancelled, 4 failed (device or HIP error; the
+ * server stops serving), 5 rejected (does not fit the HBM budget) */
 int omr_job_poll(omr_ctx *ctx, omr_job_id job, int *state);
+/* OMR_OK when done; OMR_ERR_CANCELLED, OMR_ERR_TIMEOUT, or the failure's own code
+ * (OMR_ERR_BUDGET for a rejection, OMR_ERR_HIP / OMR_ERR_NOMEM / OMR_ERR_STATE
+ * for a failure). */
 int omr_job_wait(omr_ctx *ctx, omr_job_id job, int64_t timeout_ms);
+/* The error text of a failed or rejected job (empty otherwise). */
+int omr_job_error(omr_ctx *ctx, omr_job_id job, char *msg, size_t len);
 int omr_job_cancel(omr_ctx *ctx, omr_job_id job);
 int omr_server_stats_get(omr_ctx *ctx, omr_server_stats *stats);
 

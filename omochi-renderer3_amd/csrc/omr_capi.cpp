@@ -28,7 +28,7 @@ int set_err(omr_ctx *c, int code, const std::string &m) {
   return code;
 }
 int from_engine(omr_ctx *c, int code) {
-  if (code != OMR_OK && c) c->err = c->eng->err;
+  if (code != OMR_OK && c) c->err = c->eng->get_err();
   return code;
 }
 
@@ -416,6 +416,15 @@ int omr_job_poll(omr_ctx *ctx, omr_job_id job, int *state) {
 int omr_job_wait(omr_ctx *ctx, omr_job_id job, int64_t timeout_ms) {
   if (!ctx) return OMR_ERR_INVALID;
   return ctx->eng->wait(job, timeout_ms);
+}
+int omr_job_error(omr_ctx *ctx, omr_job_id job, char *msg, size_t len) {
+  if (!ctx || !msg || len == 0) return OMR_ERR_INVALID;
+  std::string m;
+  int rc = ctx->eng->job_error(job, &m);
+  if (rc) return rc;
+  std::strncpy(msg, m.c_str(), len - 1);
+  msg[len - 1] = 0;
+  return OMR_OK;
 }
 int omr_job_cancel(omr_ctx *ctx, omr_job_id job) {
   if (!ctx) return OMR_ERR_INVALID;

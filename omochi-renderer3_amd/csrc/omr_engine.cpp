@@ -15,12 +15,12 @@ namespace omr {
   } while (0)
 
 int Engine::fail(int code, const std::string &msg) {
-  err = msg;
+  set_err(msg);
   return code;
 }
 
 int Engine::hipfail(hipError_t e, const char *what) {
-  err = std::string(what) + ": " + hipGetErrorString(e);
+  set_err(std::string(what) + ": " + hipGetErrorString(e));
   return e == hipErrorOutOfMemory ? OMR_ERR_NOMEM : OMR_ERR_HIP;
 }
 
@@ -41,29 +41,29 @@ Engine::Engine(int device) : device_(device) {
   tuning.record_mb = 32768;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
-    err = "no HIP device " + std::to_string(device);
+    set_err("no HIP device " + std::to_string(device));
     return;
   }
-  if (hipSetDevice(device) != hipSuccess) { err = "hipSetDevice failed"; return; }
+  if (hipSetDevice(device) != hipSuccess) { set_err("hipSetDevice failed"); return; }
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) != hipSuccess) { err = "hipGetDeviceProperties failed"; return; }
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) { set_err("hipGetDeviceProperties failed"); return; }
   n_cu_ = prop.multiProcessorCount;
   hbm_ = prop.totalGlobalMem;
   name_ = prop.gcnArchName;
-  if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) { err = "stream"; return; }
+  if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) { set_err("stream"); return; }
   if (hipEventCreate(&ev_a_) != hipSuccess || hipEventCreate(&ev_b_) != hipSuccess ||
       hipEventCreate(&ev_c_) != hipSuccess || hipEventCreate(&ev_d_) != hipSuccess) {
-    err = "event";
+    set_err("event");
     return;
   }
-  if (hipMalloc(&d_scenes_, kMaxScenes * sizeof(SceneDev)) != hipSuccess) { err = "scene table"; return; }
-  if (hipMemset(d_scenes_, 0, kMaxScenes * sizeof(SceneDev)) != hipSuccess) { err = "scene table"; return; }
-  if (hipMalloc(&d_ctrl_, sizeof(Ctrl)) != hipSuccess) { err = "ctrl"; return; }
-  if (hipHostMalloc(&h_ctrl_, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess) { err = "ctrl pinned"; return; }
-  if (hipMalloc(&d_jobs_, kJobRing * sizeof(JobDev)) != hipSuccess) { err = "jobs"; return; }
-  if (hipMalloc(&d_done_, kJobRing * kShards * sizeof(unsigned long long)) != hipSuccess) { err = "done"; return; }
+  if (hipMalloc(&d_scenes_, kMaxScenes * sizeof(SceneDev)) != hipSuccess) { set_err("scene table"); return; }
+  if (hipMemset(d_scenes_, 0, kMaxScenes * sizeof(SceneDev)) != hipSuccess) { set_err("scene table"); return; }
+  if (hipMalloc(&d_ctrl_, sizeof(Ctrl)) != hipSuccess) { set_err("ctrl"); return; }
+  if (hipHostMalloc(&h_ctrl_, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess) { set_err("ctrl pinned"); return; }
+  if (hipMalloc(&d_jobs_, kJobRing * sizeof(JobDev)) != hipSuccess) { set_err("jobs"); return; }
+  if (hipMalloc(&d_done_, kJobRing * kShards * sizeof(unsigned long long)) != hipSuccess) { set_err("done"); return; }
   if (hipHostMalloc(&h_done_, kJobRing * kShards * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
-    err = "done pinned";
+    set_err("done pinned");
     return;
   }
   std::memset(h_ctrl_, 0, sizeof(Ctrl));
@@ -225,8 +225,24 @@ void Engine::free_scene_mem(SceneHost &s) {
 }
 
 SceneHost *Engine::scene(omr_scene_id id) {
-  if (id < 0 || (size_t)id >= scenes_.size()) return nullptr;
+  if (id < 0 || (size_t)id >= n_scenes_.load()) return nullptr;
   return scenes_[(size_t)id].get();
+}
+
+uint64_t Engine::scene_bytes() {
+  std::lock_guard<std::mutex> g(scene_mu);
+  uint64_t b = 0;
+  for (size_t i = 0; i < n_scenes_.load(); i++)
+    if (scenes_[i]) b += scenes_[i]->bytes;
+  return b;
+}
+
+uint64_t Engine::pool_bytes() const {
+  const uint64_t P = pool_cap_;
+  // ray_o, ray_d, misc, rng, hit (16 B each), pdf (4 B), pending branches (144 B)
+  const uint64_t slots = P * (5 * 16 + 4 + 3 * 48);
+  const uint64_t queues = (uint64_t)kShards * pool_.qcap * (4 + 4 + 48);
+  return slots + queues + (uint64_t)gstack_threads_ * gcap_ * sizeof(uint2);
 }
 
 int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
@@ -328,8 +344,8 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
 
   std::lock_guard<std::mutex> g(mu);
   hipSetDevice(device_);
-  size_t slot = scenes_.size();
-  if (slot >= kMaxScenes) return fail(OMR_ERR_INVALID, "too many scenes (max 64)");
+  size_t slot = n_scenes_.load();
+  if (slot >= kMaxScenes) return fail(OMR_ERR_INVALID, "too many scenes (max 64 uploads per context)");
   SceneHost &S = *sh;
   auto dup = [&](const void *src, size_t bytes, void **dst) -> int {
     *dst = nullptr;
@@ -408,22 +424,32 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
     S.env_h = h;
   }
   OMR_HIP(hipMemcpy(d_scenes_ + slot, &D, sizeof(SceneDev), hipMemcpyHostToDevice));
-  scenes_.push_back(std::move(sh));
+  {
+    std::lock_guard<std::mutex> sg(scene_mu);
+    scenes_[slot] = std::move(sh);
+    n_scenes_.store(slot + 1);
+  }
   *id = (omr_scene_id)slot;
   return OMR_OK;
 }
 
 int Engine::release_scene(omr_scene_id id) {
   std::lock_guard<std::mutex> g(mu);
-  SceneHost *s = scene(id);
-  if (!s) return fail(OMR_ERR_NOTFOUND, "unknown scene");
-  if (s->users.load() > 0) return fail(OMR_ERR_STATE, "scene is in use by live jobs");
+  std::unique_ptr<SceneHost> s;
+  {
+    // the check and the removal under the lock a served submission takes
+    // its scene under: no request can take the scene in between
+    std::lock_guard<std::mutex> sg(scene_mu);
+    SceneHost *p = scene(id);
+    if (!p) return fail(OMR_ERR_NOTFOUND, "unknown scene");
+    if (p->users.load() > 0) return fail(OMR_ERR_STATE, "scene is held by queued or running served jobs");
+    s = std::move(scenes_[(size_t)id]);
+  }
   hipSetDevice(device_);
   hipStreamSynchronize(stream_);
   free_scene_mem(*s);
   SceneDev z{};
   hipMemcpy(d_scenes_ + id, &z, sizeof(SceneDev), hipMemcpyHostToDevice);
-  scenes_[(size_t)id].reset();
   return OMR_OK;
 }
 

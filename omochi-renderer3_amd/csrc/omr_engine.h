@@ -53,7 +53,7 @@ struct SceneHost {
   size_t bytes = 0;
   bool has_env = false;
   uint32_t env_w = 0, env_h = 0;
-  std::atomic<int> users{0};  // live jobs referencing the scene
+  std::atomic<int> users{0};  // queued or live served jobs holding the scene (guarded by Engine::scene_mu)
 };
 
 // One job as the runtime sees it (plain, batched or served).
@@ -76,8 +76,21 @@ class Engine {
   ~Engine();
   bool ok() const { return init_ok_; }
 
-  std::string err;
+  // Last error text.  The server thread and the caller's thread both report
+  // errors, so the text sits behind its own lock.
+  void set_err(const std::string &m) {
+    std::lock_guard<std::mutex> g(err_mu_);
+    err_ = m;
+  }
+  std::string get_err() const {
+    std::lock_guard<std::mutex> g(err_mu_);
+    return err_;
+  }
   std::mutex mu;  // guards the stream and all device state
+  // Guards the scene table entries and SceneHost::users: a served request
+  // takes its scene (lookup + users++) under this lock, and release checks
+  // users and removes the entry under it, so neither can interleave.
+  std::mutex scene_mu;
 
   int device_id() const { return device_; }
   int n_cu() const { return n_cu_; }
@@ -86,7 +99,14 @@ class Engine {
 
   int upload_scene(const omr_scene_desc *desc, omr_scene_id *id);
   int release_scene(omr_scene_id id);
+  // The scene of `id` or null.  Entries never move (fixed table); an entry
+  // is removed only by release_scene, under `mu` and `scene_mu`.
   SceneHost *scene(omr_scene_id id);
+  // Device bytes of every resident scene.
+  uint64_t scene_bytes();
+  // Device bytes of the path pool, its queues and the traversal overflow
+  // stacks as allocated now.
+  uint64_t pool_bytes() const;
 
   int trace(omr_scene_id id, const float *d_rays, size_t n, int32_t *d_prim, float *d_t, uint64_t *stats2);
 
@@ -108,8 +128,11 @@ class Engine {
   bool server_running() const { return server_on_.load(); }
 
   int env_tables_host(omr_scene_id id, uint64_t *row, uint64_t *marg);
+  int job_error(omr_job_id id, std::string *msg);
 
  private:
+  mutable std::mutex err_mu_;
+  std::string err_;
   bool init_ok_ = false;
   int device_ = 0;
   int n_cu_ = 0;
@@ -118,7 +141,8 @@ class Engine {
   hipStream_t stream_ = nullptr;
   hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr, ev_c_ = nullptr, ev_d_ = nullptr;
 
-  std::vector<std::unique_ptr<SceneHost>> scenes_;
+  std::unique_ptr<SceneHost> scenes_[kMaxScenes];
+  std::atomic<size_t> n_scenes_{0};  // slots handed out so far (released slots are not reused)
   SceneDev *d_scenes_ = nullptr;
 
   // pool
@@ -164,8 +188,11 @@ class Engine {
     omr_render_params params;
     float *out;
     std::chrono::steady_clock::time_point t_submit;
-    int state = 0;  // 0 queued 1 running 2 done 3 cancelled 4 failed
-    size_t bytes = 0;
+    SceneHost *sh = nullptr;  // its scene, held (users) from submission to completion
+    int state = 0;  // 0 queued 1 running 2 done 3 cancelled 4 failed 5 rejected (HBM budget)
+    int code = OMR_OK;        // error code of a failed or rejected request
+    std::string msg;          // its error text
+    size_t bytes = 0;         // device bytes while live: sums, image and (when `records`) per-sample records
     bool records = false;  // per-sample records instead of pixel atomics
     JobRun run;
     bool cancel_req = false;
@@ -180,7 +207,11 @@ class Engine {
   std::vector<std::shared_ptr<Req>> all_;  // by id-1
   omr_server_config scfg_{};
   uint64_t srv_record_bytes_ = 0;  // per-request record limit, fixed at server start
-  size_t used_bytes_ = 0;
+  uint32_t saved_pool_paths_ = 0;  // tuning.pool_paths before server_start changed it
+  size_t used_bytes_ = 0;          // device bytes of the admitted requests
+  std::atomic<uint64_t> fixed_bytes_{0};  // device bytes of the pool, queues and stacks while serving
+  bool server_failed_ = false;     // the server thread stopped after an error (srv_err_)
+  std::string srv_err_;
   uint64_t n_submitted_ = 0, n_done_ = 0, n_rejected_ = 0, n_cancelled_ = 0;
   uint64_t srv_segments_ = 0, srv_shadow_ = 0;
   double busy_ms_ = 0.0;

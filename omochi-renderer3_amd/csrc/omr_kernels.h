@@ -69,6 +69,17 @@ hipError_t shadow(const KernelArgs &a, const LaunchShape &s, hipStream_t st);
 hipError_t drain(const KernelArgs &a, const LaunchShape &s, int parity, bool stats, hipStream_t st);
 // Zeroes the completed-sample counters of ring slot `ring` (every shard).
 hipError_t clear_done(unsigned long long *done, uint32_t ring, hipStream_t st);
+// Served mode, between chunks of bounce steps (stream-ordered, no kernel of
+// the run in flight): admits the samples [old_total, new_total) -- a cursor
+// that ran past its shard's end of the old range moves back to that end, so
+// the new samples start exactly at old_total -- and sets the live job window
+// [lo, hi).
+hipError_t admit(Ctrl *ctrl, unsigned long long old_total, unsigned long long new_total, uint32_t lo, uint32_t hi,
+                 hipStream_t st);
+// Served mode: the ServeStatus of the run after the work before it on the
+// stream, for the n live jobs starting at logical ring index lo.
+hipError_t serve_status(const Ctrl *ctrl, const unsigned long long *done, uint32_t lo, uint32_t n, ServeStatus *out,
+                        hipStream_t st);
 // accum[3p..3p+2] = sums of the records of pixel p's spp_px samples
 hipError_t gather_records(const unsigned long long *recs, uint32_t spp_px, uint32_t n_px, unsigned long long *accum,
                           hipStream_t st);

@@ -227,7 +227,7 @@ __device__ inline float acos_f(float x) {
 }
 
 // ------------------------------------------------------------ primitives
-// Polygon::CheckIntersection (src/renderer/Polygon.h:147-190), single-sided.
+// Polygon::CheckIntersection (src/renderer/Polygon.h:63-106), single-sided.
 // u_out, v_out are the unnormalized barycentrics and det_out their divisor:
 // the traversal divides once, for the closest hit only (u/det is the same
 // value whenever it is computed).
@@ -1588,6 +1588,57 @@ __global__ void k_clear_done(unsigned long long *done, uint32_t ring) {
   if (threadIdx.x < kShards) done[done_index(threadIdx.x, ring)] = 0ull;
 }
 
+// One wave: lane k clamps cursor shard k to the end of the old range.
+__global__ void k_admit(Ctrl *ctrl, unsigned long long old_total, unsigned long long new_total, uint32_t lo,
+                        uint32_t hi) {
+  const uint32_t k = threadIdx.x;
+  if (k < kShards) {
+    const unsigned long long lim = shard_limit(old_total, k);
+    if (ctrl->next_sample[k] > lim) ctrl->next_sample[k] = lim;
+  }
+  if (k == 0) {
+    ctrl->total_samples = new_total;
+    ctrl->job_lo = lo;
+    ctrl->job_hi = hi;
+  }
+}
+
+// First wave: the queue length, cursor state and ray totals (lane k reads
+// shard k); every thread: the completion sums of live jobs t, t + 1024, ...
+__global__ __launch_bounds__(1024) void k_serve_status(const Ctrl *ctrl, const unsigned long long *done, uint32_t lo,
+                                                        uint32_t n, ServeStatus *out) {
+  const uint32_t t = threadIdx.x;
+  if (t < 64u) {
+    unsigned long long q = 0ull, seg = 0ull, sh = 0ull;
+    bool left = false;
+    if (t < kShards) {
+      q = ctrl->qcount[0][t];
+      seg = ctrl->segments[t];
+      sh = ctrl->shadow_rays[t];
+      left = ctrl->next_sample[t] < shard_limit(ctrl->total_samples, t);
+    }
+    const unsigned long long any_left = __ballot(left);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      q += __shfl_xor(q, off);
+      seg += __shfl_xor(seg, off);
+      sh += __shfl_xor(sh, off);
+    }
+    if (t == 0u) {
+      out->queued = q;
+      out->exhausted = any_left == 0ull ? 1ull : 0ull;
+      out->segments = seg;
+      out->shadow_rays = sh;
+    }
+  }
+  for (uint32_t i = t; i < n; i += 1024u) {
+    const uint32_t ring = (lo + i) % kJobRing;
+    unsigned long long s = 0ull;
+    for (uint32_t k = 0; k < kShards; k++) s += done[done_index(k, ring)];
+    out->done[i] = s;
+  }
+}
+
 // Pixel sums from per-sample records: sample s belongs to pixel s / spp_px,
 // so each pixel's records are contiguous; one wave sums a pixel's records
 // (lanes stride through them) and writes its three sums.
@@ -1869,6 +1920,7 @@ hipError_t launch_trace_wg(const SceneDev *scene, const float4 *r4, uint32_t n, 
 
 namespace k {
 
+
 hipError_t fill_fresh(const KernelArgs &a, uint32_t n0, hipStream_t st) {
   if (n0 == 0) return hipSuccess;
   const uint32_t need = (n0 + 255u) / 256u;
@@ -1932,6 +1984,16 @@ hipError_t drain(const KernelArgs &a, const LaunchShape &s, int parity, bool sta
 
 hipError_t clear_done(unsigned long long *done, uint32_t ring, hipStream_t st) {
   hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(kShards), 0, st, done, ring);
+  return hipGetLastError();
+}
+hipError_t admit(Ctrl *ctrl, unsigned long long old_total, unsigned long long new_total, uint32_t lo, uint32_t hi,
+                 hipStream_t st) {
+  hipLaunchKernelGGL(k_admit, dim3(1), dim3(kShards), 0, st, ctrl, old_total, new_total, lo, hi);
+  return hipGetLastError();
+}
+hipError_t serve_status(const Ctrl *ctrl, const unsigned long long *done, uint32_t lo, uint32_t n, ServeStatus *out,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_serve_status, dim3(1), dim3(1024), 0, st, ctrl, done, lo, n, out);
   return hipGetLastError();
 }
 hipError_t gather_records(const unsigned long long *recs, uint32_t spp_px, uint32_t n_px, unsigned long long *accum,

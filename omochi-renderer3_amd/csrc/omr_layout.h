@@ -183,6 +183,19 @@ constexpr uint32_t kJobRing = 1024;
 // one shard that finish samples of one job update one line.
 OMR_HD inline size_t done_index(uint32_t shard, uint32_t ring) { return (size_t)shard * kJobRing + ring; }
 
+// What the served loop reads back after each chunk of bounce steps: the
+// length of ray queue 0, whether every admitted sample has been handed out,
+// the traced-ray totals, and the completed samples of each live job (entry i:
+// ring slot (lo + i) % kJobRing).  Written by one small kernel, so a chunk
+// costs one copy of 32 + 8 x live-jobs bytes instead of the whole control
+// block and counter array.
+struct ServeStatus {
+  unsigned long long queued;
+  unsigned long long exhausted;
+  unsigned long long segments, shadow_rays;
+  unsigned long long done[kJobRing];
+};
+
 // Per-slot packed path state word (misc.y).
 //  bits 0-7 depth, 8-15 branch, 16-17 pending count, 18 prev-lambert (MIS), 19 active
 OMR_HD inline uint32_t ps_depth(uint32_t w) { return w & 0xFFu; }

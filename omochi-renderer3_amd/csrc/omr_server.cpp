@@ -1,15 +1,18 @@
 // omr_server.cpp -- served mode: continuous batching on the shared queues.
 //
-// A server thread owns the device stream.  Between bounce-step chunks it
-// (1) admits queued requests while the job ring and the HBM budget allow,
-//     appending their samples to the global sample range and waking idle
-//     path slots (refill), so new work overlaps with work already in flight;
-// (2) runs one chunk of bounce steps (a hipGraph), or, once every admitted
-//     sample is handed out and few paths remain, one drain launch that ends
-//     them all;
-// (3) detects finished jobs from their device completion counters, resolves
-//     them and copies them to the caller's buffer on a second stream while
-//     the next chunk runs, then frees their memory.
+// A server thread owns the device stream.  It keeps two chunks of bounce
+// steps in flight: after it enqueues chunk N+1 it reads the status of chunk N
+// (one small kernel writes the queue length, cursor state and the live jobs'
+// completion counts, one copy brings them back), so the device never waits
+// for the host.  From each status it
+// (1) admits queued requests while the job ring and the HBM budget allow:
+//     their samples are appended to the global sample range on the device
+//     (k::admit, stream-ordered after the chunks in flight) and idle path
+//     slots take them (refill), so new work overlaps with work in flight;
+// (2) once every admitted sample is handed out and few paths remain, ends
+//     them in one drain launch instead of many short steps;
+// (3) resolves finished jobs and copies them to the caller's buffer on a
+//     second stream while the next chunks run, then frees their memory.
 // Results are identical to running each job alone: per-path seeds depend only
 // on (job seed, pixel, sub-pixel, sample, branch) and accumulation is integer.
 #include <algorithm>
@@ -28,23 +31,46 @@ double ms_since(std::chrono::steady_clock::time_point t) {
 uint64_t samples_of(const omr_camera &c, const omr_render_params &p) {
   return (uint64_t)c.width * c.height * p.supersamples * p.supersamples * (uint64_t)(p.spp_end - p.spp_begin);
 }
+// Device bytes of a request's fixed-point sums and resolved image.
+uint64_t image_bytes(const omr_camera &c) { return (uint64_t)c.width * c.height * 3 * (8 + 4); }
 }  // namespace
 
 int Engine::server_start(const omr_server_config &cfg) {
   if (server_on_.load()) return fail(OMR_ERR_STATE, "server already running");
-  scfg_ = cfg;
-  if (scfg_.hbm_budget_bytes == 0) scfg_.hbm_budget_bytes = 64ull << 30;
-  if (scfg_.max_live_jobs == 0 || scfg_.max_live_jobs > kJobRing) scfg_.max_live_jobs = 512;
+  omr_server_config c = cfg;
+  if (c.hbm_budget_bytes == 0) c.hbm_budget_bytes = 64ull << 30;
+  if (c.max_live_jobs == 0 || c.max_live_jobs > kJobRing) c.max_live_jobs = 512;
+  if (c.pool_paths && (c.pool_paths < 64 || c.pool_paths > (1u << 26)))
+    return fail(OMR_ERR_INVALID, "pool_paths out of range");
   {
     std::lock_guard<std::mutex> g(mu);
-    srv_record_bytes_ = std::min<uint64_t>(scfg_.hbm_budget_bytes / 4, (uint64_t)tuning.record_mb << 20);
+    hipSetDevice(device_);
+    saved_pool_paths_ = tuning.pool_paths;
+    if (c.pool_paths && c.pool_paths != tuning.pool_paths) {
+      drop_graphs();
+      tuning.pool_paths = (uint32_t)c.pool_paths;
+    }
+    // the pool is part of the budget: allocate it now so its size is known
+    int rc = ensure_pool();
+    if (rc) {
+      tuning.pool_paths = saved_pool_paths_;
+      return rc;
+    }
+    srv_record_bytes_ = std::min<uint64_t>(c.hbm_budget_bytes / 4, (uint64_t)tuning.record_mb << 20);
+    fixed_bytes_ = pool_bytes();
   }
-  if (scfg_.pool_paths) {
-    if (scfg_.pool_paths < 64 || scfg_.pool_paths > (1u << 26)) return fail(OMR_ERR_INVALID, "pool_paths out of range");
+  const uint64_t fixed = fixed_bytes_.load() + scene_bytes();
+  if (fixed >= c.hbm_budget_bytes) {
     std::lock_guard<std::mutex> g(mu);
-    drop_graphs();
-    tuning.pool_paths = (uint32_t)scfg_.pool_paths;
+    if (tuning.pool_paths != saved_pool_paths_) {
+      drop_graphs();
+      tuning.pool_paths = saved_pool_paths_;
+    }
+    return fail(OMR_ERR_BUDGET, "the path pool and resident scenes (" + std::to_string(fixed) +
+                                    " bytes) leave nothing of the HBM budget (" +
+                                    std::to_string(c.hbm_budget_bytes) + " bytes)");
   }
+  scfg_ = c;
   {
     std::lock_guard<std::mutex> g(smu);
     queue_.clear();
@@ -54,13 +80,13 @@ int Engine::server_start(const omr_server_config &cfg) {
     srv_segments_ = srv_shadow_ = 0;
     busy_ms_ = 0;
     latencies_.clear();
+    server_failed_ = false;
   }
   // Request buffers come from the device's default pool (stream-ordered);
   // keep freed blocks in the pool while serving instead of handing them back
   // to the driver at every synchronization, so admissions reuse mapped memory.
   {
     std::lock_guard<std::mutex> g(mu);
-    hipSetDevice(device_);
     hipMemPool_t pool = nullptr;
     uint64_t keep = UINT64_MAX;
     if (hipDeviceGetDefaultMemPool(&pool, device_) == hipSuccess && pool)
@@ -81,6 +107,11 @@ int Engine::server_stop() {
   {
     std::lock_guard<std::mutex> g(mu);
     hipSetDevice(device_);
+    // the pool size a server_start override chose does not outlive the server
+    if (tuning.pool_paths != saved_pool_paths_) {
+      drop_graphs();
+      tuning.pool_paths = saved_pool_paths_;
+    }
     hipMemPool_t pool = nullptr;
     uint64_t keep = 0;
     if (hipDeviceGetDefaultMemPool(&pool, device_) == hipSuccess && pool) {
@@ -94,36 +125,56 @@ int Engine::server_stop() {
 int Engine::submit(omr_scene_id scene_id, const omr_camera &cam, const omr_render_params &p, float *out,
                    omr_job_id *id) {
   if (!server_on_.load()) return OMR_ERR_STATE;
-  SceneHost *s = scene(scene_id);
-  if (!s) return OMR_ERR_NOTFOUND;
   auto r = std::make_shared<Req>();
+  {
+    // lookup and hold under the lock release_scene checks the holders under
+    std::lock_guard<std::mutex> g(scene_mu);
+    SceneHost *s = scene(scene_id);
+    if (!s) return OMR_ERR_NOTFOUND;
+    s->users++;
+    r->sh = s;
+  }
   r->scene = scene_id;
   r->cam = cam;
   r->params = p;
   r->out = out;
   r->t_submit = std::chrono::steady_clock::now();
-  size_t px = (size_t)cam.width * cam.height;
-  r->bytes = px * 3 * 8 + px * 3 * 4;
-  // per-sample records (no atomics) for requests whose records take at most a
-  // quarter of the budget and fit in record_mb; larger ones use pixel atomics
-  const size_t rec = (size_t)samples_of(cam, p) * 24;
-  r->records = rec <= srv_record_bytes_;
-  if (r->records) r->bytes += rec;
+  // The budget covers the pool, the resident scenes and the admitted
+  // requests.  A request needs its sums and image; per-sample records (no
+  // atomics, the same results) are added when they fit in the record limit
+  // and, with the image, in what the pool and scenes leave of the budget.
+  const uint64_t fixed = fixed_bytes_.load() + scene_bytes();
+  const uint64_t room = scfg_.hbm_budget_bytes > fixed ? scfg_.hbm_budget_bytes - fixed : 0;
+  const uint64_t need = image_bytes(cam);
+  const uint64_t rec = samples_of(cam, p) * 24;
+  r->records = rec <= srv_record_bytes_ && need + rec <= room;
+  r->bytes = need + (r->records ? rec : 0);
   {
     std::lock_guard<std::mutex> g(smu);
     all_.push_back(r);
     r->id = (omr_job_id)all_.size();
-    n_submitted_++;
-    if (r->bytes > scfg_.hbm_budget_bytes) {
-      r->state = 4;
-      n_rejected_++;
-      *id = r->id;
-      scv.notify_all();
-      return OMR_ERR_BUDGET;
-    }
-    s->users++;
-    queue_.push_back(r);
     *id = r->id;
+    n_submitted_++;
+    int code = OMR_OK;
+    if (server_failed_) {
+      code = OMR_ERR_STATE;
+      r->msg = "the server stopped after an error: " + srv_err_;
+      r->state = 4;
+    } else if (need > room) {
+      code = OMR_ERR_BUDGET;
+      r->msg = "the request needs " + std::to_string(need) + " bytes; the path pool and resident scenes leave " +
+               std::to_string(room) + " of the HBM budget";
+      r->state = 5;
+      n_rejected_++;
+    }
+    if (code != OMR_OK) {
+      r->code = code;
+      r->sh->users--;
+      r->sh = nullptr;
+      scv.notify_all();
+      return code;
+    }
+    queue_.push_back(r);
   }
   scv.notify_all();
   return OMR_OK;
@@ -144,7 +195,14 @@ int Engine::wait(omr_job_id id, int64_t timeout_ms) {
   if (timeout_ms < 0) scv.wait(g, done);
   else if (!scv.wait_for(g, std::chrono::milliseconds(timeout_ms), done)) return OMR_ERR_TIMEOUT;
   if (r->state == 3) return OMR_ERR_CANCELLED;
-  if (r->state == 4) return OMR_ERR_BUDGET;
+  if (r->state >= 4) return r->code;
+  return OMR_OK;
+}
+
+int Engine::job_error(omr_job_id id, std::string *msg) {
+  std::lock_guard<std::mutex> g(smu);
+  if (id < 1 || (size_t)id > all_.size()) return OMR_ERR_NOTFOUND;
+  *msg = all_[(size_t)id - 1]->msg;
   return OMR_OK;
 }
 
@@ -157,8 +215,11 @@ int Engine::cancel(omr_job_id id) {
 }
 
 int Engine::server_stats(omr_server_stats *s) {
+  const uint64_t fixed = fixed_bytes_.load() + scene_bytes();
   std::lock_guard<std::mutex> g(smu);
   std::memset(s, 0, sizeof(*s));
+  s->fixed_bytes = fixed;
+  s->used_bytes = used_bytes_;
   s->jobs_submitted = n_submitted_;
   s->jobs_done = n_done_;
   s->jobs_rejected = n_rejected_;
@@ -186,29 +247,23 @@ void Engine::server_loop() {
     std::lock_guard<std::mutex> g(mu);
     solo_ = kNoSolo;  // requests share the run
   }
-  std::deque<std::shared_ptr<Req>> live;
-  uint64_t lo = 0, hi = 0;          // logical ring window [lo, hi)
-  unsigned long long total = 0;     // samples admitted
-  unsigned long long cursor[kShards] = {};  // device sample cursors as of the last readback
-  JobDev *h_jobs = nullptr;         // pinned staging for job records
-  struct Stage {  // same layout as Ctrl::next_sample, total_samples (one copy updates both)
-    Sharded<unsigned long long> cursor;
-    alignas(kLine) unsigned long long total;
-    uint32_t lo, hi;
-  } *h_stage = nullptr;
-  static_assert(offsetof(Stage, total) - offsetof(Stage, cursor) ==
-                    offsetof(Ctrl, total_samples) - offsetof(Ctrl, next_sample),
-                "staged cursors must mirror the control block");
-  static_assert(offsetof(Ctrl, job_hi) == offsetof(Ctrl, job_lo) + 4 && offsetof(Stage, hi) == offsetof(Stage, lo) + 4,
-                "the job window is copied as one 8-byte pair");
-  bool ok = true;
+  std::deque<std::shared_ptr<Req>> live;  // admitted, by logical ring index
+  uint64_t lo = 0, hi = 0;                // logical ring window [lo, hi)
+  unsigned long long total = 0;           // samples admitted
+  JobDev *h_jobs = nullptr;               // pinned staging for job records
+  ServeStatus *d_st = nullptr;            // two status slots on the device ...
+  ServeStatus *h_st = nullptr;            // ... and their pinned copies
+  hipEvent_t ev_st[2] = {nullptr, nullptr};
   hipStream_t cs = nullptr;  // result copies, beside the compute stream
+  bool ok = true;
   {
     std::lock_guard<std::mutex> g(mu);
-    if (ensure_pool() != OMR_OK) ok = false;
-    if (ok && hipStreamCreate(&cs) != hipSuccess) ok = false;
-    if (ok && hipHostMalloc(&h_jobs, kJobRing * sizeof(JobDev), hipHostMallocDefault) != hipSuccess) ok = false;
-    if (ok && hipHostMalloc(&h_stage, sizeof(Stage), hipHostMallocDefault) != hipSuccess) ok = false;
+    ok = hipStreamCreate(&cs) == hipSuccess &&
+         hipHostMalloc(&h_jobs, kJobRing * sizeof(JobDev), hipHostMallocDefault) == hipSuccess &&
+         hipMalloc(&d_st, 2 * sizeof(ServeStatus)) == hipSuccess &&
+         hipHostMalloc(&h_st, 2 * sizeof(ServeStatus), hipHostMallocDefault) == hipSuccess &&
+         hipEventCreateWithFlags(&ev_st[0], hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&ev_st[1], hipEventDisableTiming) == hipSuccess;
     if (ok) {
       Ctrl c;
       std::memset(&c, 0, sizeof(c));
@@ -217,22 +272,52 @@ void Engine::server_loop() {
            hipMemset(pool_.misc, 0, (size_t)pool_cap_ * 16) == hipSuccess;
     }
   }
-  auto fail_all = [&](const std::string &why) {
+  // Marks every live and queued request failed (the server stops serving).
+  auto fail_all = [&](int code, const std::string &why) {
+    set_err(why);
     std::lock_guard<std::mutex> g(smu);
-    err = why;
-    for (auto &r : live) r->state = 4;
-    for (auto &r : queue_) r->state = 4;
+    srv_err_ = why;
+    server_failed_ = true;
+    auto fail_one = [&](const std::shared_ptr<Req> &r) {
+      r->state = 4;
+      r->code = code;
+      r->msg = why;
+      if (r->sh) r->sh->users--;
+      r->sh = nullptr;
+    };
+    for (auto &r : live) fail_one(r);
+    for (auto &r : queue_) fail_one(r);
     queue_.clear();
     scv.notify_all();
   };
+  auto release_all = [&]() {
+    std::lock_guard<std::mutex> g(mu);
+    hipStreamSynchronize(stream_);
+    if (cs) {
+      hipStreamSynchronize(cs);
+      hipStreamDestroy(cs);
+    }
+    for (auto &r : live) {
+      if (r->run.d_accum) hipFree(r->run.d_accum);
+      if (r->run.d_recs) hipFree(r->run.d_recs);
+      if (r->run.d_out) hipFree(r->run.d_out);
+      if (r->ev_copy) hipEventDestroy(r->ev_copy);
+    }
+    for (hipEvent_t e : ev_st)
+      if (e) hipEventDestroy(e);
+    if (d_st) hipFree(d_st);
+    if (h_st) hipHostFree(h_st);
+    if (h_jobs) hipHostFree(h_jobs);
+  };
   if (!ok) {
-    fail_all("server start: device allocation failed");
+    fail_all(OMR_ERR_NOMEM, "server start: device allocation failed");
+    release_all();
     return;
   }
   // Requests whose image is resolved on the compute stream but not yet copied
   // out.  Their copies run on `cs` while the compute stream works on the next
-  // chunk of steps (the copy to a caller's pageable buffer holds this thread,
-  // not the GPU); then their device buffers go back and they complete.
+  // chunks of steps (the copy to a caller's pageable buffer holds this
+  // thread, not the GPU); then their device buffers go back and they complete.
   std::vector<std::shared_ptr<Req>> copying;
   auto flush_copies = [&]() {
     if (copying.empty()) return;
@@ -249,14 +334,18 @@ void Engine::server_loop() {
       hipFreeAsync(r->run.d_accum, stream_);
       hipFreeAsync(r->run.d_out, stream_);
       if (r->run.d_recs) hipFreeAsync(r->run.d_recs, stream_);
+      r->run.d_accum = r->run.d_recs = nullptr;
+      r->run.d_out = nullptr;
     }
     std::lock_guard<std::mutex> g(smu);
     for (auto &r : copying) {
       used_bytes_ -= r->bytes;
-      if (SceneHost *s = scene(r->scene)) s->users--;
+      if (r->sh) r->sh->users--;
+      r->sh = nullptr;
       if (!copied) {
         r->state = 4;  // its image did not reach the caller
-        err = "result copy failed";
+        r->code = OMR_ERR_HIP;
+        r->msg = "result copy to the caller's buffer failed";
       } else if (r->cancel_req) {
         r->state = 3;
         n_cancelled_++;
@@ -269,32 +358,57 @@ void Engine::server_loop() {
     copying.clear();
     scv.notify_all();
   };
+  // Status reads in flight, oldest first: the status slot, the job window
+  // and the admitted total when it was enqueued, and its sequence number.
+  struct Pending {
+    int slot;
+    uint64_t lo;
+    uint32_t n;
+    unsigned long long total;
+    uint64_t seq;
+  };
+  std::deque<Pending> pend;
+  uint64_t seq = 0, drain_seq = 0;  // statuses enqueued; the first status after the last drain
+  int slot_next = 0;
+  auto enqueue_status = [&]() -> bool {
+    const int s = slot_next;
+    slot_next ^= 1;
+    const uint32_t n = (uint32_t)(hi - lo);
+    const bool okq = k::serve_status(d_ctrl_, d_done_, (uint32_t)(lo % kJobRing), n, d_st + s, stream_) == hipSuccess &&
+                     hipMemcpyAsync(h_st + s, d_st + s, offsetof(ServeStatus, done) + 8ull * n,
+                                    hipMemcpyDeviceToHost, stream_) == hipSuccess &&
+                     hipEventRecord(ev_st[s], stream_) == hipSuccess;
+    pend.push_back(Pending{s, lo, n, total, seq++});
+    return okq;
+  };
   auto busy_t0 = std::chrono::steady_clock::now();
   while (!server_stop_.load()) {
     // ---- 1. admission
     std::vector<std::shared_ptr<Req>> admit;
     {
       std::unique_lock<std::mutex> g(smu);
-      if (live.empty() && queue_.empty()) {
+      if (live.empty() && queue_.empty() && pend.empty()) {
         scv.wait_for(g, std::chrono::milliseconds(20));
         if (queue_.empty()) continue;
       }
-      while (!queue_.empty() && (hi - lo) < scfg_.max_live_jobs) {
+      const uint64_t fixed = fixed_bytes_.load();
+      while (!queue_.empty() && (hi + admit.size() - lo) < scfg_.max_live_jobs) {
         auto r = queue_.front();
         if (r->cancel_req) {
           queue_.pop_front();
           r->state = 3;
           n_cancelled_++;
-          if (SceneHost *s = scene(r->scene)) s->users--;
+          if (r->sh) r->sh->users--;
+          r->sh = nullptr;
           scv.notify_all();
           continue;
         }
-        if (used_bytes_ + r->bytes > scfg_.hbm_budget_bytes) break;  // wait for memory
+        // scene bytes are read under scene_mu; smu is never taken inside it
+        if (fixed + scene_bytes() + used_bytes_ + r->bytes > scfg_.hbm_budget_bytes) break;  // wait for memory
         queue_.pop_front();
         used_bytes_ += r->bytes;
         r->state = 1;
         admit.push_back(r);
-        hi++;
       }
     }
     std::unique_lock<std::mutex> dev(mu);
@@ -302,28 +416,26 @@ void Engine::server_loop() {
       const unsigned long long total_before = total;
       if (live.empty()) busy_t0 = std::chrono::steady_clock::now();
       uint32_t need_stack = 1;
+      bool okd = true;
       for (auto &r : admit) {
-        SceneHost *s = scene(r->scene);
-        need_stack = std::max(need_stack, s ? s->bvh.stack_bound : 1u);
+        need_stack = std::max(need_stack, r->sh->bvh.stack_bound);
         JobRun &j = r->run;
         j.scene = (uint32_t)r->scene;
         j.cam = r->cam;
         j.params = r->params;
         j.n_samples = (uint32_t)samples_of(r->cam, r->params);
-        j.logical = hi - admit.size() + (&r - &admit[0]);
+        j.logical = hi++;
         j.ring = (uint32_t)(j.logical % kJobRing);
-        size_t px = (size_t)r->cam.width * r->cam.height;
+        const size_t px = (size_t)r->cam.width * r->cam.height;
         j.d_recs = nullptr;
-        if (hipMallocAsync((void **)&j.d_accum, px * 3 * 8, stream_) != hipSuccess ||
-            hipMallocAsync((void **)&j.d_out, px * 3 * 4, stream_) != hipSuccess ||
-            (r->records && hipMallocAsync((void **)&j.d_recs, (size_t)j.n_samples * 24, stream_) != hipSuccess)) {
-          dev.unlock();
-          fail_all("device allocation for a job failed");
-          return;
-        }
-        hipMemsetAsync(j.d_accum, 0, px * 3 * 8, stream_);
-        if (j.d_recs) hipMemsetAsync(j.d_recs, 0, (size_t)j.n_samples * 24, stream_);
-        k::clear_done(d_done_, j.ring, stream_);
+        live.push_back(r);
+        okd = okd && hipMallocAsync((void **)&j.d_accum, px * 3 * 8, stream_) == hipSuccess &&
+              hipMallocAsync((void **)&j.d_out, px * 3 * 4, stream_) == hipSuccess &&
+              (!r->records || hipMallocAsync((void **)&j.d_recs, (size_t)j.n_samples * 24, stream_) == hipSuccess);
+        if (!okd) break;
+        okd = hipMemsetAsync(j.d_accum, 0, px * 3 * 8, stream_) == hipSuccess &&
+              (!j.d_recs || hipMemsetAsync(j.d_recs, 0, (size_t)j.n_samples * 24, stream_) == hipSuccess) &&
+              k::clear_done(d_done_, j.ring, stream_) == hipSuccess;
         JobDev &D = h_jobs[j.ring];
         std::memset(&D, 0, sizeof(D));
         D.sample_base = total;
@@ -348,122 +460,146 @@ void Engine::server_loop() {
         D.focal = j.cam.focal_distance;
         D.accum = j.d_accum;
         D.recs = j.d_recs;
-        hipMemcpyAsync(d_jobs_ + j.ring, &D, sizeof(JobDev), hipMemcpyHostToDevice, stream_);
+        okd = okd && hipMemcpyAsync(d_jobs_ + j.ring, &D, sizeof(JobDev), hipMemcpyHostToDevice, stream_) == hipSuccess;
         total += j.n_samples;
-        live.push_back(r);
+        if (!okd) break;
+      }
+      if (!okd) {
+        dev.unlock();
+        fail_all(OMR_ERR_NOMEM, "device allocation for a request failed");
+        release_all();
+        return;
       }
       LaunchShape sh = shape();
       if (ensure_gstack(need_stack, (size_t)sh.grid * sh.wg) != OMR_OK) {
         dev.unlock();
-        fail_all("traversal stack allocation failed");
+        fail_all(OMR_ERR_NOMEM, "traversal stack allocation failed");
+        release_all();
         return;
       }
-      // the cursor may have run past the old total once work ran out: new
-      // samples start exactly at the old total
-      for (uint32_t k = 0; k < kShards; k++) h_stage->cursor[k] = std::min(cursor[k], shard_limit(total_before, k));
-      h_stage->total = total;
-      h_stage->lo = (uint32_t)lo;
-      h_stage->hi = (uint32_t)hi;
-      hipMemcpyAsync(&d_ctrl_->next_sample, &h_stage->cursor, offsetof(Stage, total) + 8, hipMemcpyHostToDevice,
-                     stream_);
-      hipMemcpyAsync(&d_ctrl_->job_lo, &h_stage->lo, 8, hipMemcpyHostToDevice, stream_);
-      k::refill(args(), sh, 0, stream_);
-    }
-    if (live.empty()) continue;
-    // ---- 2. one chunk of bounce steps
-    bool env_is = false;
-    for (auto &r : live) {
-      SceneHost *s = scene(r->scene);
-      if (r->params.env_is && s && s->has_env) env_is = true;
-    }
-    if (launch_steps(tuning.steps_per_sync, env_is) != OMR_OK) {
-      dev.unlock();
-      fail_all(err);
-      return;
-    }
-    flush_copies();
-    hipMemcpyAsync(h_ctrl_, d_ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_);
-    hipMemcpyAsync(h_done_, d_done_, kJobRing * kShards * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_);
-    if (hipStreamSynchronize(stream_) != hipSuccess) {
-      dev.unlock();
-      fail_all("device error in the bounce loop");
-      return;
-    }
-    for (uint32_t k = 0; k < kShards; k++) cursor[k] = h_ctrl_->next_sample[k];
-    // ---- 2b. drain: every admitted sample is handed out and few paths are
-    // left (no request could be admitted above): finish them in one launch
-    // instead of many short steps, then start the next run on empty queues.
-    const uint64_t queued = queue_len(*h_ctrl_, 0);
-    if (tuning.drain_below && queued > 0 && queued <= tuning.drain_below && samples_exhausted(*h_ctrl_)) {
-      bool ok2 = k::drain(args(), shape(), 0, false, stream_) == hipSuccess &&
-                 hipMemsetAsync(&d_ctrl_->qcount[0], 0, sizeof(d_ctrl_->qcount[0]), stream_) == hipSuccess &&
-                 hipMemsetAsync(&d_ctrl_->qhead[0], 0, sizeof(d_ctrl_->qhead[0]), stream_) == hipSuccess;
-      hipMemcpyAsync(h_ctrl_, d_ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_);
-      hipMemcpyAsync(h_done_, d_done_, kJobRing * kShards * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                     stream_);
-      if (!ok2 || hipStreamSynchronize(stream_) != hipSuccess) {
+      fixed_bytes_ = pool_bytes();
+      if (k::admit(d_ctrl_, total_before, total, (uint32_t)lo, (uint32_t)hi, stream_) != hipSuccess ||
+          k::refill(args(), sh, 0, stream_) != hipSuccess) {
         dev.unlock();
-        fail_all("device error in the drain");
+        fail_all(OMR_ERR_HIP, "admission launch failed");
+        release_all();
         return;
       }
     }
-    // ---- 3. completions
-    std::vector<std::shared_ptr<Req>> finished;
+    // ---- 2. keep two chunks of bounce steps (and their statuses) in flight
+    if (!live.empty()) {
+      bool env_is = false;
+      for (auto &r : live)
+        if (r->params.env_is && r->sh->has_env) env_is = true;
+      if (launch_steps(tuning.steps_per_sync, env_is) != OMR_OK || !enqueue_status()) {
+        dev.unlock();
+        fail_all(OMR_ERR_HIP, "bounce step launch failed: " + get_err());
+        release_all();
+        return;
+      }
+      if (pend.size() < 2) continue;  // fill the pipeline before waiting
+    }
+    if (pend.empty()) continue;
+    // ---- 3. the oldest status
+    const Pending P = pend.front();
+    pend.pop_front();
+    dev.unlock();
+    const bool st_ok = hipEventSynchronize(ev_st[P.slot]) == hipSuccess;
+    dev.lock();
+    flush_copies();  // results resolved before this status
+    if (!st_ok) {
+      dev.unlock();
+      fail_all(OMR_ERR_HIP, "device error in the bounce loop");
+      release_all();
+      return;
+    }
+    const ServeStatus &S = h_st[P.slot];
+    // drain: every admitted sample is handed out and few paths are left; only
+    // from a status taken after the last drain with nothing admitted since
+    // (the drain ends the queued paths; newly admitted samples would wait)
+    if (tuning.drain_below && S.exhausted && S.queued > 0 && S.queued <= tuning.drain_below && P.total == total &&
+        P.seq >= drain_seq) {
+      const bool okd = k::drain(args(), shape(), 0, false, stream_) == hipSuccess &&
+                       hipMemsetAsync(&d_ctrl_->qcount[0], 0, sizeof(d_ctrl_->qcount[0]), stream_) == hipSuccess &&
+                       hipMemsetAsync(&d_ctrl_->qhead[0], 0, sizeof(d_ctrl_->qhead[0]), stream_) == hipSuccess;
+      drain_seq = seq;
+      if (!okd || !enqueue_status()) {
+        dev.unlock();
+        fail_all(OMR_ERR_HIP, "drain launch failed");
+        release_all();
+        return;
+      }
+    }
+    // completions
+    bool retired = false;
     for (auto &r : live) {
       JobRun &j = r->run;
-      if (j.done || job_done(h_done_, j.ring) != j.n_samples) continue;
+      if (j.done || j.logical < P.lo || j.logical >= P.lo + P.n) continue;
+      if (S.done[j.logical - P.lo] != j.n_samples) continue;
       j.done = true;
-      double nspp = (double)j.params.supersamples * j.params.supersamples *
-                    (double)(j.params.spp_end - j.params.spp_begin);
-      if (j.d_recs) k::gather_records(j.d_recs, (uint32_t)nspp, j.cam.width * j.cam.height, j.d_accum, stream_);
-      k::resolve(j.d_accum, j.d_out, j.cam.width * j.cam.height, nspp, stream_);
-      if (hipEventCreateWithFlags(&r->ev_copy, hipEventDisableTiming) != hipSuccess ||
-          hipEventRecord(r->ev_copy, stream_) != hipSuccess) {
+      const double nspp =
+          (double)j.params.supersamples * j.params.supersamples * (double)(j.params.spp_end - j.params.spp_begin);
+      bool okr = (!j.d_recs ||
+                  k::gather_records(j.d_recs, (uint32_t)nspp, j.cam.width * j.cam.height, j.d_accum, stream_) ==
+                      hipSuccess) &&
+                 k::resolve(j.d_accum, j.d_out, j.cam.width * j.cam.height, nspp, stream_) == hipSuccess &&
+                 hipEventCreateWithFlags(&r->ev_copy, hipEventDisableTiming) == hipSuccess &&
+                 hipEventRecord(r->ev_copy, stream_) == hipSuccess;
+      if (!okr) {
         dev.unlock();
-        fail_all("result event");
+        fail_all(OMR_ERR_HIP, "result resolve failed");
+        release_all();
         return;
       }
       copying.push_back(r);
-      finished.push_back(r);
+      retired = true;
     }
-    if (!finished.empty()) {
+    if (retired) {
       while (!live.empty() && live.front()->run.done) {
         live.pop_front();
         lo++;
       }
-      h_stage->lo = (uint32_t)lo;
-      h_stage->hi = (uint32_t)hi;
-      hipMemcpyAsync(&d_ctrl_->job_lo, &h_stage->lo, 8, hipMemcpyHostToDevice, stream_);
+      if (k::admit(d_ctrl_, total, total, (uint32_t)lo, (uint32_t)hi, stream_) != hipSuccess) {
+        dev.unlock();
+        fail_all(OMR_ERR_HIP, "job window update failed");
+        release_all();
+        return;
+      }
     }
-    // nothing left to overlap with: copy out now
-    if (live.empty()) flush_copies();
     dev.unlock();
     {
       std::lock_guard<std::mutex> g(smu);
-      srv_segments_ = sum_shards(h_ctrl_->segments);
-      srv_shadow_ = sum_shards(h_ctrl_->shadow_rays);
-      if (live.empty()) busy_ms_ += ms_since(busy_t0);
+      srv_segments_ = S.segments;
+      srv_shadow_ = S.shadow_rays;
+      if (retired && live.empty()) busy_ms_ += ms_since(busy_t0);
+    }
+    // nothing left to overlap with: copy out now
+    if (live.empty() && pend.empty()) {
+      std::lock_guard<std::mutex> g(mu);
+      flush_copies();
     }
   }
   {
     std::lock_guard<std::mutex> g(mu);
     hipStreamSynchronize(stream_);
     flush_copies();
-    hipStreamSynchronize(stream_);
-    hipStreamDestroy(cs);
-    for (auto &r : live) {
-      hipFree(r->run.d_accum);
-      if (r->run.d_recs) hipFree(r->run.d_recs);
-      hipFree(r->run.d_out);
-    }
-    hipHostFree(h_jobs);
-    hipHostFree(h_stage);
   }
-  std::lock_guard<std::mutex> g(smu);
-  for (auto &r : live) r->state = 3;
-  for (auto &r : queue_) r->state = 3;
-  queue_.clear();
-  scv.notify_all();
+  {
+    std::lock_guard<std::mutex> g(smu);
+    for (auto &r : live) {
+      r->state = 3;
+      if (r->sh) r->sh->users--;
+      r->sh = nullptr;
+    }
+    for (auto &r : queue_) {
+      r->state = 3;
+      if (r->sh) r->sh->users--;
+      r->sh = nullptr;
+    }
+    queue_.clear();
+    scv.notify_all();
+  }
+  release_all();
 }
 
 }  // namespace omr

@@ -13,6 +13,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libomr.so")
 
 OMR_OK = 0
 OMR_ERR_INVALID = -1
+ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_BUDGET, ERR_NOTFOUND, ERR_STATE, ERR_NODEVICE, ERR_TIMEOUT, ERR_CANCELLED = \
+    -1, -2, -3, -4, -5, -6, -7, -8, -9
 ERRORS = {
     -1: "invalid argument", -2: "out of memory", -3: "HIP error", -4: "over HBM budget",
     -5: "not found", -6: "bad state", -7: "no device", -8: "timeout", -9: "cancelled",
@@ -87,7 +89,7 @@ class ServerStats(C.Structure):
     _fields_ = [("jobs_submitted", C.c_uint64), ("jobs_done", C.c_uint64), ("jobs_rejected", C.c_uint64),
                 ("jobs_cancelled", C.c_uint64), ("segments", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("busy_ms", C.c_double), ("lat_p50_ms", C.c_double), ("lat_p99_ms", C.c_double),
-                ("lat_max_ms", C.c_double)]
+                ("lat_max_ms", C.c_double), ("fixed_bytes", C.c_uint64), ("used_bytes", C.c_uint64)]
 
 
 class Procedural(C.Structure):
@@ -99,7 +101,7 @@ EXPORTS = [
     "omr_scene_info_get", "omr_scene_bvh", "omr_scene_env_tables", "omr_trace_closest", "omr_trace_closest_device",
     "omr_trace_stats_device", "omr_render", "omr_render_sums", "omr_last_render_stats", "omr_render_batch",
     "omr_get_tuning", "omr_set_tuning", "omr_server_start", "omr_server_stop", "omr_job_submit", "omr_job_poll",
-    "omr_job_wait", "omr_job_cancel", "omr_server_stats_get", "omr_procedural_scene", "omr_procedural_free",
+    "omr_job_wait", "omr_job_error", "omr_job_cancel", "omr_server_stats_get", "omr_procedural_scene", "omr_procedural_free",
     "omr_camera_from_lookat", "omr_bvh_build", "omr_bvh_data", "omr_bvh_free", "omr_set_profiling", "omr_autotune",
     "omr_procedural_camera", "omr_io_last_error", "omr_free", "omr_hdr_read", "omr_hdr_write", "omr_scene_file_load",
     "omr_obj_load", "omr_loaded_scene_free", "omr_image_read", "omr_texture_read",
@@ -144,6 +146,7 @@ def load():
     lib.omr_job_submit.argtypes = [vp, i32, C.POINTER(Camera), C.POINTER(RenderParams), vp, C.POINTER(C.c_int64)]
     lib.omr_job_poll.argtypes = [vp, C.c_int64, C.POINTER(C.c_int)]
     lib.omr_job_wait.argtypes = [vp, C.c_int64, C.c_int64]
+    lib.omr_job_error.argtypes = [vp, C.c_int64, C.c_char_p, sz]
     lib.omr_job_cancel.argtypes = [vp, C.c_int64]
     lib.omr_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
     lib.omr_procedural_scene.argtypes = [C.c_int, u64, C.c_uint32, C.c_uint32, C.POINTER(Procedural)]
@@ -542,6 +545,11 @@ class Context:
 
     def wait(self, job, timeout_ms=-1):
         return self.lib.omr_job_wait(self.h, job, timeout_ms)
+
+    def job_error(self, job):
+        buf = C.create_string_buffer(512)
+        self._check(self.lib.omr_job_error(self.h, job, buf, 512))
+        return buf.value.decode()
 
     def cancel(self, job):
         return self.lib.omr_job_cancel(self.h, job)

@@ -159,7 +159,7 @@ float orc_acos(float x) {
 }
 
 /* ------------------------------------------------------- intersections */
-/* Polygon::CheckIntersection (src/renderer/Polygon.h:147-190): single-sided
+/* Polygon::CheckIntersection (src/renderer/Polygon.h:63-106): single-sided
  * Moller-Trumbore, det > EPS, 0 <= u <= det, v >= 0, u+v <= det, t >= EPS. */
 int orc_intersect_triangle(const float ray[8], const float tri[9], float *t_out, float *u_out, float *v_out) {
   v3 o = ld3(ray), d = ld3(ray + 4);

@@ -89,25 +89,65 @@ def test_served_equals_alone(record_mb):
         jobs.append((sids[si], _cam(omr, scenes[si], w, h),
                      omr.params(ss=ss, spp_begin=b, spp_end=e, max_depth=d, rr_depth=rr, seed=seed)))
     alone = [ctx.render(*j) for j in jobs]
+    pool_before = ctx.get_tuning()["pool_paths"]
     ctx.server_start(pool_paths=1 << 16)
     outs = [np.zeros_like(a) for a in alone]
     ids = []
     for k, (j, o) in enumerate(zip(jobs, outs)):
         ids.append(ctx.submit(j[0], j[1], j[2], o))
         if k == 2:
-            time.sleep(0.01)   # later requests arrive while earlier ones run
+            # later requests arrive while the first ones are admitted or running
+            while ctx.poll(ids[0]) == 0:
+                time.sleep(0.001)
     print("submitted", len(ids), flush=True)
     for i in ids:
         assert ctx.wait(i, 60000) == 0
     st = ctx.server_stats()
     assert st["jobs_done"] == len(jobs) and st["lat_p99_ms"] > 0
-    # a request above the HBM budget is rejected at submission
+    assert st["used_bytes"] == 0 and st["fixed_bytes"] > 0
     ctx.server_stop()
+    # the server's pool size does not outlive it
+    assert ctx.get_tuning()["pool_paths"] == pool_before
     for a, o in zip(alone, outs):
         assert np.array_equal(a, o)
-    ctx.server_start(hbm_budget_bytes=1024)
-    big = np.zeros((48, 64, 3), np.float32)
-    with pytest.raises(omr.OmrError):
-        ctx.submit(jobs[0][0], jobs[0][1], jobs[0][2], big)
+    ctx.close()
+
+
+def test_served_budget_and_scene_holds():
+    """The HBM budget covers the pool and scenes: a start whose pool alone
+    exceeds it fails; a request whose image cannot fit in what is left is
+    rejected (state 5, OMR_ERR_BUDGET, with a message).  A scene held by a
+    queued or running request cannot be released until the request ends."""
+    import omr
+
+    ctx = omr.Context(0)
+    scenes, spec = _jobs(omr)
+    sids = [ctx.upload(s) for s in scenes]
+    with pytest.raises(omr.OmrError) as e:
+        ctx.server_start(pool_paths=1 << 12, hbm_budget_bytes=1024)
+    assert e.value.code == omr.ERR_BUDGET
+    ctx.server_start(pool_paths=1 << 12)
+    fixed = ctx.server_stats()["fixed_bytes"]
     ctx.server_stop()
+    ctx.server_start(pool_paths=1 << 12, hbm_budget_bytes=fixed + 4096)
+    big = np.zeros((48, 64, 3), np.float32)
+    cam = _cam(omr, scenes[0], 64, 48)
+    prm = omr.params(ss=1, spp_end=1, seed=3)
+    with pytest.raises(omr.OmrError) as e:
+        ctx.submit(sids[0], cam, prm, big)
+    assert e.value.code == omr.ERR_BUDGET
+    st = ctx.server_stats()
+    assert st["jobs_rejected"] == 1
+    ctx.server_stop()
+    # a held scene: release is refused while its request is queued or running
+    ctx.server_start(pool_paths=1 << 16)
+    out = np.zeros((64, 80, 3), np.float32)
+    cam1 = _cam(omr, scenes[1], 80, 64)
+    job = ctx.submit(sids[1], cam1, omr.params(ss=4, spp_end=8, max_depth=16, seed=8), out)
+    with pytest.raises(omr.OmrError) as e:
+        ctx.release(sids[1])
+    assert e.value.code == omr.ERR_STATE
+    assert ctx.wait(job, 60000) == 0
+    ctx.server_stop()
+    ctx.release(sids[1])
     ctx.close()

@@ -46,7 +46,7 @@ def _tri(ray, tri):
 
 def test_triangle_single_sided_closed_form():
     """Polygon::CheckIntersection: det > EPS (front face only), barycentric
-    bounds, t >= EPS (src/renderer/Polygon.h:147-190)."""
+    bounds, t >= EPS (src/renderer/Polygon.h:63-106)."""
     inf = helpers.INF
     tri = [0, 0, 0, 1, 0, 0, 0, 1, 0]          # normal e1 x e2 = +z
     ok, t, u, v = _tri([0.25, 0.25, 5, inf, 0, 0, -1, 0], tri)   # from +z side
