@@ -217,6 +217,18 @@ int omr_scene_env_tables(omr_ctx *ctx, omr_scene_id id, uint64_t *row /* h x (w+
 int omr_trace_closest(omr_ctx *ctx, omr_scene_id scene, const float *rays, size_t n, int32_t *prim_id, float *t);
 int omr_trace_closest_device(omr_ctx *ctx, omr_scene_id scene, const float *d_rays, size_t n, int32_t *d_prim_id,
                              float *d_t);
+/* omr_trace_closest_device without any device allocation (no hipMalloc,
+ * hipFree or allocating library call).  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, serving loops and harnesses that forbid hipMalloc.  Its working
+ * memory (fetch heads and traversal overflow stacks) is one fixed 32 MiB
+ * device array per GPU, held until the process ends and shared by every call
+ * on that GPU: calls from one process take turns, calls from several
+ * processes on one GPU race.  The traversal grid shrinks to what the array
+ * holds; a scene whose stack bound leaves no room for one workgroup is
+ * refused with OMR_ERR_INVALID.  Same results as omr_trace_closest_device. */
+int omr_trace_closest_device_noalloc(omr_ctx *ctx, omr_scene_id scene, const float *d_rays, size_t n,
+                                     int32_t *d_prim_id, float *d_t);
 /* Node visits / primitive tests for a ray set (device pointers; stats pass). */
 int omr_trace_stats_device(omr_ctx *ctx, omr_scene_id scene, const float *d_rays, size_t n, int32_t *d_prim_id,
                            float *d_t, uint64_t *node_visits, uint64_t *prim_tests);

@@ -234,6 +234,19 @@ int omr_trace_closest_device(omr_ctx *ctx, omr_scene_id scene, const float *d_ra
   return from_engine(ctx, ctx->eng->trace(scene, d_rays, n, d_prim_id, d_t, nullptr));
 }
 
+int omr_trace_closest_device_noalloc(omr_ctx *ctx, omr_scene_id scene, const float *d_rays, size_t n,
+                                     int32_t *d_prim_id, float *d_t) {
+  if (!ctx || (n && (!d_rays || !d_prim_id || !d_t))) return OMR_ERR_INVALID;
+  // every context on a GPU shares that GPU's scratch array: calls from this
+  // process take turns per device
+  static std::mutex per_device[64];
+  const int dev = ctx->eng->device_id();
+  std::unique_lock<std::mutex> d;
+  if (dev >= 0 && dev < 64) d = std::unique_lock<std::mutex>(per_device[dev]);
+  std::lock_guard<std::mutex> g(ctx->eng->mu);
+  return from_engine(ctx, ctx->eng->trace_noalloc(scene, d_rays, n, d_prim_id, d_t));
+}
+
 int omr_trace_stats_device(omr_ctx *ctx, omr_scene_id scene, const float *d_rays, size_t n, int32_t *d_prim_id,
                            float *d_t, uint64_t *node_visits, uint64_t *prim_tests) {
   if (!ctx || (n && (!d_rays || !d_prim_id || !d_t))) return OMR_ERR_INVALID;

@@ -67,6 +67,14 @@ Engine::Engine(int device) : device_(device) {
     return;
   }
   std::memset(h_ctrl_, 0, sizeof(Ctrl));
+  // the allocation-free entry point's scratch array: its address on this
+  // device (each GPU holds its own copy), looked up once with the device current
+  hipError_t se = k::noalloc_scratch(&noalloc_scratch_);
+  if (se != hipSuccess) {
+    noalloc_scratch_ = nullptr;
+    noalloc_err_ = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se);
+    (void)hipGetLastError();
+  }
   init_ok_ = true;
 }
 
@@ -492,6 +500,36 @@ int Engine::trace(omr_scene_id id, const float *d_rays, size_t n, int32_t *d_pri
     stats2[0] = c[0];
     stats2[1] = c[1];
   }
+  return OMR_OK;
+}
+
+int Engine::trace_noalloc(omr_scene_id id, const float *d_rays, size_t n, int32_t *d_prim, float *d_t) {
+  if (!noalloc_scratch_) return fail(OMR_ERR_HIP, noalloc_err_);
+  SceneHost *s = scene(id);
+  if (!s) return fail(OMR_ERR_NOTFOUND, "unknown scene");
+  if (n > 0xFFFFFFFFull) return fail(OMR_ERR_INVALID, "too many rays");
+  if (n == 0) return OMR_OK;
+  hipSetDevice(device_);
+  // overflow entries per thread: the scene's exact stack bound past the LDS part
+  const uint32_t wg = 128, shrt = tuning.stack_short;
+  const uint32_t gcap = s->bvh.stack_bound > shrt ? s->bvh.stack_bound - shrt : 1u;
+  const size_t per_group = (size_t)wg * gcap * sizeof(uint2);
+  const size_t max_groups = (kNoAllocScratchBytes - kNoAllocHeadBytes) / per_group;
+  if (max_groups == 0)
+    return fail(OMR_ERR_INVALID, "the scene's traversal stack bound (" + std::to_string(s->bvh.stack_bound) +
+                                     " entries) does not fit the allocation-free scratch");
+  // persistent grid: every ray is reached whatever the grid (dynamic fetch or
+  // grid-stride), so the grid is cut to what the scratch holds
+  uint32_t grid = (uint32_t)std::min<size_t>({(n + wg - 1) / wg, (size_t)n_cu_ * 16, max_groups});
+  char *base = static_cast<char *>(noalloc_scratch_);
+  Sharded<uint32_t> *heads = reinterpret_cast<Sharded<uint32_t> *>(base);
+  uint2 *stacks = reinterpret_cast<uint2 *>(base + kNoAllocHeadBytes);
+  hipError_t e = k::zero_words(reinterpret_cast<uint32_t *>(base), (uint32_t)(kNoAllocHeadBytes / 4), stream_);
+  if (e == hipSuccess)
+    e = k::trace_rays(d_scenes_ + id, d_rays, (uint32_t)n, d_prim, d_t, nullptr, stacks, gcap, shrt, grid, wg, heads,
+                      (int)tuning.fetch_budget, tuning.fetch_min, stream_);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  if (e != hipSuccess) return hipfail(e, "trace_rays (allocation-free)");
   return OMR_OK;
 }
 

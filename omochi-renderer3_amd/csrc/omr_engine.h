@@ -109,6 +109,10 @@ class Engine {
   uint64_t pool_bytes() const;
 
   int trace(omr_scene_id id, const float *d_rays, size_t n, int32_t *d_prim, float *d_t, uint64_t *stats2);
+  // trace() without any device allocation: its fetch heads and overflow
+  // stacks live in the fixed scratch array of this GPU (k::noalloc_scratch),
+  // whose address is looked up once, at construction.
+  int trace_noalloc(omr_scene_id id, const float *d_rays, size_t n, int32_t *d_prim, float *d_t);
 
   // Runs jobs to completion on the shared queues (plain = one job).
   int run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st);
@@ -141,6 +145,8 @@ class Engine {
   hipStream_t stream_ = nullptr;
   hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr, ev_c_ = nullptr, ev_d_ = nullptr;
 
+  void *noalloc_scratch_ = nullptr;  // device address of this GPU's allocation-free scratch
+  std::string noalloc_err_;
   std::unique_ptr<SceneHost> scenes_[kMaxScenes];
   std::atomic<size_t> n_scenes_{0};  // slots handed out so far (released slots are not reused)
   SceneDev *d_scenes_ = nullptr;

@@ -52,7 +52,19 @@ struct LaunchShape {
   uint32_t solo_shade_grid;  // ... of its single-job form: four waves per SIMD
 };
 
+// Working memory of the allocation-free closest-hit entry point: a fixed
+// device array per GPU (held until the process ends).  The first
+// kNoAllocHeadBytes hold the dynamic-fetch heads (cleared by a fill kernel at
+// every call), the rest the per-thread overflow traversal stacks.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+constexpr size_t kNoAllocHeadBytes = sizeof(Sharded<uint32_t>);
+static_assert(kNoAllocHeadBytes % 256 == 0, "the stacks start on a 256-byte boundary");
+
 namespace k {
+// Device address of the allocation-free scratch array on the current device.
+hipError_t noalloc_scratch(void **ptr);
+// p[0..n) = 0 (a kernel, not a memset: callers may capture it in a graph).
+hipError_t zero_words(uint32_t *p, uint32_t n, hipStream_t st);
 // First fill of a cleared pool (queue 0): slot i takes sample i for i < n0,
 // without atomics; the caller has set the cursors and queue-0 counts of
 // shard k to shard_limit(n0, k).

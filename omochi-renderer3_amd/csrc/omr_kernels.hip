@@ -1916,10 +1916,27 @@ hipError_t launch_trace_wg(const SceneDev *scene, const float4 *r4, uint32_t n, 
                                                           budget, fetch_min, st);
 }
 
+__global__ void k_zero_words(uint32_t *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
 }  // namespace
+
+// The allocation-free entry point's scratch (fetch heads, then overflow
+// stacks).  Host code reaches it only through its device address
+// (k::noalloc_scratch); the name itself is a host-side stand-in.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 namespace k {
 
+hipError_t noalloc_scratch(void **ptr) { return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch)); }
+
+hipError_t zero_words(uint32_t *p, uint32_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const uint32_t blocks = (n + 255u) / 256u;
+  hipLaunchKernelGGL(k_zero_words, dim3(blocks < 1024u ? blocks : 1024u), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
 
 hipError_t fill_fresh(const KernelArgs &a, uint32_t n0, hipStream_t st) {
   if (n0 == 0) return hipSuccess;

@@ -99,7 +99,7 @@ class Procedural(C.Structure):
 EXPORTS = [
     "omr_create", "omr_destroy", "omr_last_error", "omr_device_info", "omr_scene_upload", "omr_scene_release",
     "omr_scene_info_get", "omr_scene_bvh", "omr_scene_env_tables", "omr_trace_closest", "omr_trace_closest_device",
-    "omr_trace_stats_device", "omr_render", "omr_render_sums", "omr_last_render_stats", "omr_render_batch",
+    "omr_trace_stats_device", "omr_trace_closest_device_noalloc", "omr_render", "omr_render_sums", "omr_last_render_stats", "omr_render_batch",
     "omr_get_tuning", "omr_set_tuning", "omr_server_start", "omr_server_stop", "omr_job_submit", "omr_job_poll",
     "omr_job_wait", "omr_job_error", "omr_job_cancel", "omr_server_stats_get", "omr_procedural_scene", "omr_procedural_free",
     "omr_camera_from_lookat", "omr_bvh_build", "omr_bvh_data", "omr_bvh_free", "omr_set_profiling", "omr_autotune",
@@ -132,6 +132,7 @@ def load():
     lib.omr_scene_env_tables.argtypes = [vp, i32, vp, vp]
     lib.omr_trace_closest.argtypes = [vp, i32, vp, sz, vp, vp]
     lib.omr_trace_closest_device.argtypes = [vp, i32, vp, sz, vp, vp]
+    lib.omr_trace_closest_device_noalloc.argtypes = [vp, i32, vp, sz, vp, vp]
     lib.omr_trace_stats_device.argtypes = [vp, i32, vp, sz, vp, vp, C.POINTER(u64), C.POINTER(u64)]
     lib.omr_render.argtypes = [vp, i32, C.POINTER(Camera), C.POINTER(RenderParams), vp]
     lib.omr_render_sums.argtypes = [vp, i32, C.POINTER(Camera), C.POINTER(RenderParams), vp]
@@ -476,6 +477,10 @@ class Context:
             return nv.value, nt.value
         self._check(self.lib.omr_trace_closest_device(self.h, sid, d_rays, n, d_prim, d_t))
         return None
+
+    def trace_device_noalloc(self, sid, d_rays, n, d_prim, d_t):
+        """Closest hit on device buffers through the allocation-free entry point."""
+        self._check(self.lib.omr_trace_closest_device_noalloc(self.h, sid, d_rays, n, d_prim, d_t))
 
     def render(self, sid, cam, prm):
         out = np.empty((cam.height, cam.width, 3), dtype=np.float32)

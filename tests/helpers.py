@@ -53,3 +53,52 @@ def hit_rays(scene, n, seed, box_scale=1.5, focus=None):
         o2 = (c + e * (2 * rng.random((n2, 3)) - 1)).astype(np.float32)
         d2 = unit_dirs(rng, n2)
     return make_rays(np.concatenate([o1, o2]), np.concatenate([d1, d2]))
+
+
+class DeviceArray:
+    """A device copy of a numpy array through the HIP runtime libomr.so links
+    (libamdhip64.so.7).  Tests use it for the entry points that take device
+    pointers; torch bundles a HIP runtime of its own, which cannot share the
+    device with the one already loaded."""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        import ctypes as C
+
+        if cls._hip is None:
+            h = C.CDLL("libamdhip64.so.7")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, a):
+        import ctypes as C
+
+        self.a = np.ascontiguousarray(a)
+        self.ptr = C.c_void_p()
+        h = self.hip()
+        if h.hipMalloc(C.byref(self.ptr), max(1, self.a.nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+        if self.a.nbytes and h.hipMemcpy(self.ptr, self.a.ctypes.data, self.a.nbytes, 1) != 0:
+            raise RuntimeError("hipMemcpy to the device failed")
+
+    @property
+    def address(self):
+        return self.ptr.value
+
+    def get(self):
+        out = np.empty_like(self.a)
+        self.hip().hipDeviceSynchronize()
+        if out.nbytes and self.hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) != 0:
+            raise RuntimeError("hipMemcpy to the host failed")
+        return out
+
+    def free(self):
+        if self.ptr.value:
+            self.hip().hipFree(self.ptr)
+            self.ptr = type(self.ptr)()
