@@ -1,16 +1,34 @@
-"""Benchmark of the path-tracing hot path on one MI355X.
+"""Benchmark of the path-tracing hot path on one MI355X.  Prints one JSON line.
 
-One step = one full render of the workload (every sample of every pixel
-through the wavefront bounce loop).  Default workload: the Cornell box with a
-noise-displaced level-6 icosphere (81,934 primitives), 512x512, 8x8
-sub-pixels x 1 sample = 64 spp, max depth 16, Russian roulette after depth 3,
-FP32.  Prints one JSON line.
+Default workload (the configuration the headline metric is quoted on): c4,
+served rendering.  Eight procedural scenes (5.75M triangles, two HDR skies)
+stay resident; a seeded generator draws render requests (pixel count
+log-uniform in 64^2..1920x1080, spp in {1,4,16,64,256}, depth 8, random
+seeds).  One step is one Poisson stream of --requests requests offered at
+--load (default 2.0) times the plain path's capacity, so the server, not the
+arrival process, bounds the rate; served by continuous batching (requests
+admitted while others run, rays of all live requests in shared queues).
 
-value = closest-hit segments per second (Mrays/s) over the timed steps, with
-the scene resident in HBM before timing starts.  The roofline object prices
-the dominant kernel (extend: BVH traversal) with its algorithmic bytes per
-launch over its event-timed launch durations; the cpu_baseline object times
-the CPU restatement (oracle/) on a band of rows of the same workload.
+  value            closest-hit segments + shadow rays per second over the K
+                   timed streams (first submission to last result on the
+                   host), scenes resident before timing starts
+  plain_mrays      the same requests rendered one after another, each alone
+                   (omr_render), wall clock
+  latency          p50 / p99 submission-to-result latency at 0.5 and 0.8 of
+                   the plain capacity and at --load (over all K streams)
+  batched          config 3 in the same run: 64 uneven jobs over four scenes
+                   in one persistent launch vs the same jobs alone
+  by_size          where packing wins: rays and plain Mrays/s per request size
+  roofline         the dominant kernel of a profiled plain pass over the first
+                   --roofline-requests requests: algorithmic bytes (the
+                   survey's per-unit figures) over its event-timed launches,
+                   measured HBM traffic (rocprof, scaled by a copy
+                   calibration), and the VALU issue share that bounds it
+  cpu_baseline     the CPU restatement of the path (oracle/) on all usable
+                   host cores over a bounded sample of the same requests
+
+--workload c0|c1|c2 measure one plain render configuration (20 timed renders);
+--workload c3 the batched configuration alone.
 """
 import argparse
 import json
@@ -25,17 +43,16 @@ sys.path.insert(0, os.path.join(ROOT, "omochi-renderer3_amd"))
 
 METRIC = "Mrays/s at matched RMSE, served/batched vs plain GPU; p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes of the extend kernel: per segment the queue entry (4),
-# ray origin+direction (32), job slot (4) and hit record (16); per node visit
-# the six bound vectors and the child vector (112); per primitive test one
-# 48-byte primitive record.
-B_SEG, B_NODE, B_PRIM = 56, 112, 48
-# shade: queue index 4 + path state and hit 80 read, state 64 + queue 4 written, material 48 + shading-normal
-# record 48 read per segment; a shadow record (48) per environment sample
-B_SHADE_SEG, B_SHADOW_REC = 248, 48
-# drain: queue index 4 + path state 68 read and the state word 4 written per path; material 48 + shading-normal
-# record 48 per segment, plus the traversal's node and primitive bytes
-B_DRAIN_PATH, B_DRAIN_SEG = 76, 96
+# Algorithmic bytes per unit (the survey's roofline model): a closest-hit
+# segment moves its ray in (32) and hit out (16) in the traversal kernel and
+# the path state in and out (96) in shading; a 4-wide node visit reads 128; a
+# primitive test reads 48 (a sphere's 16-byte record is priced as a triangle:
+# sphere tests are < 0.1 % of the tests here).  The drain does both halves.
+B_SEG_TRACE, B_SEG_SHADE, B_NODE, B_PRIM = 48, 96, 128, 48
+B_SHADOW = 48  # a shadow ray: its record in and the visibility result
+# MI355X: 256 CUs x 4 SIMDs, one wave64 VALU instruction per SIMD every 4
+# cycles at 2.4 GHz
+VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
 
 WORKLOADS = {
     "c0": dict(kind=0, w=256, h=256, ss=4, spp=1, depth=8, rr=3, env_is=0,
@@ -47,6 +64,11 @@ WORKLOADS = {
                desc="heightfield terrain (997,002 triangles) + 200 spheres, 2048x1024 HDR sky with importance "
                     "sampling, 1280x720, 256 spp, depth 12"),
 }
+
+# Config 4's eight resident scenes: (procedural kind, seed offset).  Kind 4 is
+# a 2000x1000 terrain (3,996,002 triangles) + 200 spheres with its own sky.
+SERVED_SCENES = [(0, 1), (1, 1), (1, 2), (2, 1), (3, 1), (3, 2), (4, 1), (0, 2)]
+SIZE_BUCKETS = [0, 1 << 20, 1 << 23, 1 << 26, 1 << 40]  # samples per job
 
 
 def batched_jobs(n_jobs=64, seed=0x6F6D7233):
@@ -68,11 +90,6 @@ def batched_jobs(n_jobs=64, seed=0x6F6D7233):
     return jobs
 
 
-# Config 4's eight resident scenes: (procedural kind, seed offset).  Kind 4 is
-# a 2000x1000 terrain (3,996,002 triangles) + 200 spheres with its own sky.
-SERVED_SCENES = [(0, 1), (1, 1), (1, 2), (2, 1), (3, 1), (3, 2), (4, 1), (0, 2)]
-
-
 def served_requests(n, kinds, seed):
     """Config 4 requests: pixel count log-uniform in [64^2, 1920x1080] at 16:9,
     spp in {1, 4, 16, 64, 256} (ss = sqrt(spp)), random seeds."""
@@ -89,11 +106,225 @@ def served_requests(n, kinds, seed):
     return reqs
 
 
+def log(msg):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def _prm(omr, j):
     return omr.params(ss=j["ss"], spp_end=j["spp"], max_depth=j["depth"], rr_depth=3, env_is=j["env_is"],
                       seed=j["seed"])
 
 
+def _samples(j):
+    return j["w"] * j["h"] * j["ss"] * j["ss"] * j["spp"]
+
+
+def by_size(jobs, rays, wall_s, extra=None):
+    """Per request-size bucket: count, share of all rays, and Mrays/s of the
+    jobs rendered alone (small jobs cannot fill the GPU alone: that is where
+    packing them with others pays)."""
+    out = []
+    tot = float(sum(rays))
+    for lo, hi in zip(SIZE_BUCKETS[:-1], SIZE_BUCKETS[1:]):
+        idx = [i for i, j in enumerate(jobs) if lo <= _samples(j) < hi]
+        if not idx:
+            continue
+        r = sum(rays[i] for i in idx)
+        t = sum(wall_s[i] for i in idx)
+        b = {"samples": f"[{lo}, {hi})" if hi < SIZE_BUCKETS[-1] else f">= {lo}", "jobs": len(idx),
+             "share_of_rays": round(r / tot, 4), "alone_mrays": round(r / t / 1e6, 1),
+             "alone_ms_mean": round(t / len(idx) * 1e3, 3)}
+        if extra:
+            b.update(extra(idx))
+        out.append(b)
+    return out
+
+
+def render_alone(ctx, args_list):
+    """Each job alone, back to back: images, rays and wall seconds per job."""
+    imgs, rays, walls = [], [], []
+    for a in args_list:
+        t0 = time.perf_counter()
+        imgs.append(ctx.render(*a))
+        walls.append(time.perf_counter() - t0)
+        st = ctx.last_stats()
+        rays.append(st["segments"] + st["shadow_rays"])
+    return imgs, rays, walls
+
+
+def serve_stream(ctx, reqs, job_args, lam, rng, alone):
+    """One Poisson stream at rate lam (jobs/s): returns rays, wall seconds
+    (first submission to last result), per-request latencies (ms) and
+    whether every result equals the request rendered alone."""
+    ctx.server_start()
+    try:
+        outs = [np.empty((r["h"], r["w"], 3), np.float32) for r in reqs]
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, len(reqs)))
+        arrivals -= arrivals[0]
+        ids = []
+        t0 = time.perf_counter()
+        for r, o, a in zip(reqs, outs, arrivals):
+            now = time.perf_counter() - t0
+            if a > now:
+                time.sleep(a - now)
+            ids.append(ctx.submit(*job_args(r), o))
+        for i in ids:
+            rc = ctx.wait(i, 600000)
+            if rc != 0:
+                raise RuntimeError(f"served request failed ({rc}): {ctx.job_error(i)}")
+        wall = time.perf_counter() - t0
+        lat = [ctx.job_latency(i) for i in ids]
+        st = ctx.server_stats()
+    finally:
+        ctx.server_stop()
+    same = all(np.array_equal(a, b) for a, b in zip(alone, outs))
+    return st["segments"] + st["shadow_rays"], wall, lat, same
+
+
+def pct(v, q):
+    v = sorted(v)
+    return float(v[min(len(v) - 1, int(round(q * (len(v) - 1))))])
+
+
+# ---------------------------------------------------------------- roofline
+def profiled_pass(ctx, args_list):
+    """Renders with per-launch HIP events on the engine's stream (profiling
+    on: every bounce-step kernel launched directly, node visits and primitive
+    tests counted by the extend and drain kernels' counting variants)."""
+    keys = ("segments", "shadow_rays", "extend_ms", "shade_ms", "shadow_ms", "extend_launches", "node_visits",
+            "prim_tests", "drain_segments", "drained_paths", "drain_ms", "drain_node_visits", "drain_prim_tests")
+    acc = dict.fromkeys(keys, 0)
+    drains = 0
+    ctx.set_profiling(True)
+    try:
+        for a in args_list:
+            ctx.render(*a)
+            st = ctx.last_stats()
+            for k in keys:
+                acc[k] += st[k]
+            drains += 1 if st["drained_paths"] else 0
+    finally:
+        ctx.set_profiling(False)
+    acc["drain_launches"] = drains
+    return acc
+
+
+def roofline_from(acc, workload):
+    ext_seg = acc["segments"] - acc["drain_segments"]
+    kernels = {
+        # name: (algorithmic bytes, summed event ms, launches, rocprof symbol of the counting variant)
+        "k_extend": (B_SEG_TRACE * ext_seg + B_NODE * acc["node_visits"] + B_PRIM * acc["prim_tests"],
+                     acc["extend_ms"], acc["extend_launches"], "k_extend<64, 16, true, true, true>"),
+        "k_shade": (B_SEG_SHADE * ext_seg, acc["shade_ms"], acc["extend_launches"], "k_shade<64, true, true>"),
+        "k_shadow": (B_SHADOW * acc["shadow_rays"], acc["shadow_ms"], acc["extend_launches"],
+                     "k_shadow<64, 16, true, true, true>"),
+        "k_drain": ((B_SEG_TRACE + B_SEG_SHADE) * acc["drain_segments"] + B_NODE * acc["drain_node_visits"]
+                    + B_PRIM * acc["drain_prim_tests"], acc["drain_ms"], acc["drain_launches"],
+                    "k_drain<64, 16, true, true>"),
+    }
+    table = {}
+    for name, (b, ms, n, sym) in kernels.items():
+        if ms > 0 and n > 0:
+            a = b / (ms / 1e3) / 1e9
+            table[name] = {"ms_total": round(ms, 3), "launches": n, "avg_launch_ms": round(ms / n, 4),
+                           "alg_bytes_per_launch": int(b / n), "achieved_GBs": round(a, 1),
+                           "frac": round(a / HBM_PEAK_GBS, 4), "symbol": sym}
+    dom = max(table, key=lambda k: table[k]["ms_total"])
+    kd = table[dom]
+    # measured HBM traffic and VALU issue of the same kernel (rocprof counters
+    # of the same pass, committed under profiles/)
+    traffic, valu = None, None
+    tpath = os.path.join(ROOT, "profiles", f"{dom[2:]}_{workload}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    vpath = os.path.join(ROOT, "profiles", f"{dom[2:]}_{workload}_valu.json")
+    if os.path.exists(vpath):
+        with open(vpath) as f:
+            v = json.load(f)
+        insts = v.get("valu_insts_per_launch")
+        if insts:
+            valu = {"valu_insts_per_launch": insts,
+                    "issue_frac": round(insts / (kd["avg_launch_ms"] / 1e3) / VALU_ISSUE_PER_S, 4),
+                    "peak_wave_insts_per_s": VALU_ISSUE_PER_S, "source": os.path.relpath(vpath, ROOT)}
+    roof = {"bound": "hbm", "achieved": kd["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": kd["frac"], "traffic": traffic, "kernel": dom, "symbol": kd["symbol"],
+            "avg_launch_ms": kd["avg_launch_ms"], "alg_bytes_per_launch": kd["alg_bytes_per_launch"]}
+    if traffic:
+        hbm = traffic / (kd["avg_launch_ms"] / 1e3) / 1e9
+        roof["hbm_measured_GBs"] = round(hbm, 1)
+        roof["hbm_measured_frac"] = round(hbm / HBM_PEAK_GBS, 4)
+    if valu:
+        roof["valu_issue"] = valu
+    roof["kernels"] = table
+    seg = max(1, acc["segments"])
+    roof["nodes_per_segment"] = round((acc["node_visits"] + acc["drain_node_visits"]) / seg, 3)
+    roof["prims_per_segment"] = round((acc["prim_tests"] + acc["drain_prim_tests"]) / seg, 3)
+    return roof
+
+
+# ---------------------------------------------------------------- CPU baseline
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model, os.cpu_count() or 1, usable
+
+
+def cpu_threads():
+    # the GPU host gives one GPU's job a share of 16 cores, whatever the
+    # machine has in all
+    _, _, usable = host_cpu()
+    return max(1, min(16, usable))
+
+
+def cpu_sample(items, seconds, threads):
+    """items: (scene, camera, params, gpu image) per request.  Renders bands
+    of 8 rows of each request in turn on the CPU restatement until `seconds`
+    of wall time are spent; returns Mrays/s, the sample, and the RMSE of the
+    CPU rows against the GPU rows."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    oracles, rays, t_cpu, sq, n_px = {}, 0, 0.0, 0.0, 0
+    cursor = [0] * len(items)
+    done_rows = 0
+    used = set()
+    while t_cpu < seconds:
+        progressed = False
+        for k, (scene, cam, prm, gimg) in enumerate(items):
+            if t_cpu >= seconds or cursor[k] >= cam.height:
+                continue
+            if id(scene) not in oracles:
+                oracles[id(scene)] = pyoracle.Oracle(scene)  # preparation (own BVH build) is not timed
+            r0, r1 = cursor[k], min(cam.height, cursor[k] + 8)
+            t0 = time.perf_counter()
+            img, seg, sh = oracles[id(scene)].render(cam, prm, accel=pyoracle.OWN_BVH, threads=threads, rows=(r0, r1))
+            t_cpu += time.perf_counter() - t0
+            rays += seg + sh
+            d = img.astype(np.float64) - gimg[r0:r1].astype(np.float64)
+            sq += float((d * d).sum())
+            n_px += d.size
+            cursor[k] = r1
+            done_rows += r1 - r0
+            used.add(k)
+            progressed = True
+        if not progressed:
+            break
+    for o in oracles.values():
+        o.close()
+    return rays / t_cpu / 1e6, t_cpu, done_rows, len(used), float(np.sqrt(sq / max(1, n_px)))
+
+
+# ---------------------------------------------------------------- workloads
 def run_batched(ctx, omr, args):
     kinds = [0, 1, 2, 3]
     scenes = {k: omr.procedural(k, width=64, height=64) for k in kinds}
@@ -102,253 +333,210 @@ def run_batched(ctx, omr, args):
     cams = [omr.procedural_camera(j["kind"], j["w"], j["h"]) for j in jobs]
     prms = [_prm(omr, j) for j in jobs]
     ids = [sids[j["kind"]] for j in jobs]
+    alist = list(zip(ids, cams, prms))
+    saved = ctx.get_tuning()
     tune = None
-    if not args.no_tune:
-        # tune on the workload itself: a subset of the jobs favours smaller
-        # pools than the whole batch wants
+    log("c3: scenes resident")
+    if args.tune_ms > 0:
+        # the autotuner picks launch shape, pool, cadence and stack depth on
+        # the batch itself; the same settings then serve both forms
         tune = ctx.autotune(ids, cams, prms, budget_ms=args.tune_ms)
-    # plain: each job alone, one after another
-    t0 = time.perf_counter()
-    plain, rays_plain = [], 0
-    for i in range(len(jobs)):
-        plain.append(ctx.render(ids[i], cams[i], prms[i]))
-        st = ctx.last_stats()
-        rays_plain += st["segments"] + st["shadow_rays"]
-    t_plain = time.perf_counter() - t0
-    # batched: all jobs in one persistent wavefront run
-    for _ in range(args.warmup):
-        ctx.render_batch(ids, cams, prms)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    log(f"c3: tuning {tune}")
+    render_alone(ctx, alist[:4])  # warm-up
+    plain, rays_alone, walls = render_alone(ctx, alist)
+    t_plain = sum(walls)
+    ctx.render_batch(ids, cams, prms)  # warm-up
+    t_batch, reps = 0.0, max(1, args.batch_reps)
+    for _ in range(reps):
+        t0 = time.perf_counter()
         batch = ctx.render_batch(ids, cams, prms)
-    t_batch = (time.perf_counter() - t0) / args.steps
+        t_batch += time.perf_counter() - t0
+    t_batch /= reps
     st = ctx.last_stats()
     rays_batch = st["segments"] + st["shadow_rays"]
     identical = all(np.array_equal(a, b) for a, b in zip(plain, batch))
-    return dict(jobs=jobs, plain_mrays=rays_plain / t_plain / 1e6, batch_mrays=rays_batch / t_batch / 1e6,
-                t_plain=t_plain, t_batch=t_batch, rays=rays_batch, identical=identical, tuning=tune,
-                n_prims=sum(s.n_prims for s in scenes.values()))
+    ctx.set_tuning(**saved)
+    for s in sids.values():
+        ctx.release(s)
+    return {
+        "workload": "c3: 64 uneven jobs over 4 resident scenes (Cornell, Cornell + icosphere, terrain + sky, "
+                    "torus knot), one persistent wavefront run",
+        "plain_mrays": round(sum(rays_alone) / t_plain / 1e6, 2), "batch_mrays": round(rays_batch / t_batch / 1e6, 2),
+        "speedup_vs_plain": round((rays_batch / t_batch) / (sum(rays_alone) / t_plain), 3),
+        "plain_ms": round(t_plain * 1e3, 1), "batch_ms": round(t_batch * 1e3, 1), "batch_reps": reps,
+        "rays": rays_batch, "rays_equal_alone": rays_batch == sum(rays_alone), "identical_to_alone": identical,
+        "tuning": tune, "primitives": sum(s.n_prims for s in scenes.values()),
+        "by_size": by_size(jobs, rays_alone, walls),
+    }
 
 
-def run_served(ctx, omr, args):
+def run_served(ctx, omr, args, dev):
     kinds = SERVED_SCENES
+    log("c4: building the eight scenes")
     scenes = [omr.procedural(k, seed=0x6F6D7233 + s, width=64, height=64) for k, s in kinds]
     sids = [ctx.upload(s) for s in scenes]
     n_tri = sum(s.tri.shape[0] for s in scenes)
-    cal = served_requests(args.requests, kinds, seed=7)
+    reqs = served_requests(args.requests, kinds, seed=7)
 
     def job_args(r):
-        cam = omr.procedural_camera(kinds[r["scene"]][0], r["w"], r["h"])
-        return sids[r["scene"]], cam, _prm(omr, r)
+        return sids[r["scene"]], omr.procedural_camera(kinds[r["scene"]][0], r["w"], r["h"]), _prm(omr, r)
 
-    # plain: each request alone, back to back (also gives the service rate)
-    t0 = time.perf_counter()
-    rays_plain = 0
-    alone = []
-    for r in cal:
-        alone.append(ctx.render(*job_args(r)))
-        st = ctx.last_stats()
-        rays_plain += st["segments"] + st["shadow_rays"]
-    t_plain = time.perf_counter() - t0
-    capacity = len(cal) / t_plain
-    rates = []
+    alist = [job_args(r) for r in reqs]
+    # warm-up: a few requests alone and one short served stream (allocations,
+    # graph captures)
+    render_alone(ctx, alist[:max(1, args.warmup)])
     rng = np.random.default_rng(11)
-    for f in (0.5, 0.8, 1.1):
-        lam = f * capacity
-        ctx.server_start()
-        outs = [np.zeros((r["h"], r["w"], 3), np.float32) for r in cal]
-        arrivals = np.cumsum(rng.exponential(1.0 / lam, len(cal)))
-        t_start = time.perf_counter()
-        ids = []
-        for r, o, a in zip(cal, outs, arrivals):
-            now = time.perf_counter() - t_start
-            if a > now:
-                time.sleep(a - now)
-            ids.append(ctx.submit(*job_args(r), o))
-        for i in ids:
-            if ctx.wait(i, 600000) != 0:
-                raise RuntimeError("served job failed")
-        t_all = time.perf_counter() - t_start
-        st = ctx.server_stats()
-        ctx.server_stop()
-        same = all(np.array_equal(a, b) for a, b in zip(alone, outs))
-        rays = st["segments"] + st["shadow_rays"]
-        rates.append(dict(load=f, rate_jobs_per_s=round(lam, 3), p50_ms=round(st["lat_p50_ms"], 2),
-                          p99_ms=round(st["lat_p99_ms"], 2), mrays=round(rays / st["busy_ms"] / 1e3, 2),
-                          wall_s=round(t_all, 2), identical_to_alone=same))
-    return dict(plain_mrays=rays_plain / t_plain / 1e6, capacity=capacity, rates=rates, n_tri=n_tri,
-                n_requests=len(cal))
+    # plain: each request alone, back to back; also the service rate
+    log(f"c4: {len(reqs)} requests alone")
+    alone, rays_alone, walls = render_alone(ctx, alist)
+    t_plain = sum(walls)
+    rays_plain = sum(rays_alone)
+    capacity = len(reqs) / t_plain
+    log(f"c4: plain {rays_plain / t_plain / 1e6:.1f} Mrays/s, {t_plain:.2f} s")
+    serve_stream(ctx, reqs[:16], job_args, 4 * capacity, rng, alone[:16])
+    # timed: K streams offered at `load` x the plain capacity
+    lam = args.load * capacity
+    rays_s, wall_s, lat_s, same_all = 0, 0.0, [], True
+    for k in range(args.steps):
+        rays, wall, lat, same = serve_stream(ctx, reqs, job_args, lam, rng, alone)
+        log(f"c4: served stream {k + 1}/{args.steps}: {rays / wall / 1e6:.1f} Mrays/s, identical {same}")
+        rays_s += rays
+        wall_s += wall
+        lat_s += lat
+        same_all = same_all and same and rays == rays_plain
+    served_mrays = rays_s / wall_s / 1e6
+    latency = []
+    for f in (0.5, 0.8):
+        rays, wall, lat, same = serve_stream(ctx, reqs, job_args, f * capacity, rng, alone)
+        log(f"c4: load {f}: p50 {pct(lat, 0.5):.1f} ms, p99 {pct(lat, 0.99):.1f} ms")
+        same_all = same_all and same and rays == rays_plain
+        latency.append({"load": f, "rate_jobs_per_s": round(f * capacity, 3), "requests": len(lat),
+                        "p50_ms": round(pct(lat, 0.5), 2), "p99_ms": round(pct(lat, 0.99), 2),
+                        "max_ms": round(max(lat), 2), "mrays": round(rays / wall / 1e6, 1)})
+    latency.append({"load": args.load, "rate_jobs_per_s": round(lam, 3), "requests": len(lat_s),
+                    "p50_ms": round(pct(lat_s, 0.5), 2), "p99_ms": round(pct(lat_s, 0.99), 2),
+                    "max_ms": round(max(lat_s), 2), "mrays": round(served_mrays, 1)})
+    # latency per size bucket in the K timed streams
+    lat_by_req = np.array(lat_s).reshape(args.steps, len(reqs))
+
+    def extra(idx):
+        l = lat_by_req[:, idx].ravel()
+        return {"served_p50_ms": round(float(np.median(l)), 2)}
+
+    sizes = by_size(reqs, rays_alone, walls, extra)
+    # roofline: a profiled plain pass over the first requests
+    nroof = min(len(reqs), args.roofline_requests)
+    log(f"c4: profiled pass over {nroof} requests")
+    acc = profiled_pass(ctx, alist[:nroof])
+    roof = roofline_from(acc, "c4")
+    roof["pass"] = (f"the first {nroof} requests rendered alone with per-launch HIP events on the engine's "
+                    "stream (counting kernel variants)")
+    out = {
+        "metric": METRIC, "value": round(served_mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall_s / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"c4: served, Poisson arrivals offered at {args.load}x plain capacity, {len(reqs)} "
+                               "requests per step "
+                               f"over 8 resident scenes ({n_tri} triangles, 2 HDR skies); pixels log-uniform in "
+                               "64^2..1920x1080, spp in {1,4,16,64,256}, depth 8",
+                   "mode": "served (continuous batching, HBM-budget admission)", "requests_per_step": len(reqs),
+                   "rays_per_step": rays_plain, "device": dev["name"], "n_cu": dev["n_cu"]},
+        "plain_mrays": round(rays_plain / t_plain / 1e6, 2), "plain_ms": round(t_plain * 1e3, 1),
+        "speedup_vs_plain": round(served_mrays / (rays_plain / t_plain / 1e6), 3),
+        "identical_to_alone": same_all, "capacity_jobs_per_s": round(capacity, 3),
+        "p50_ms": latency[-1]["p50_ms"], "p99_ms": latency[-1]["p99_ms"], "latency": latency,
+        "by_size": sizes, "roofline": roof,
+    }
+    items = [(scenes[r["scene"]], alist[i][1], alist[i][2], alone[i]) for i, r in enumerate(reqs[:8])]
+    return out, items
 
 
-def cpu_baseline(scene, cam, prm, rows, threads):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-
-    o = pyoracle.Oracle(scene)
-    t0 = time.perf_counter()
-    img, seg, sh = o.render(cam, prm, accel=pyoracle.OWN_BVH, threads=threads, rows=rows)
-    dt = time.perf_counter() - t0
-    o.close()
-    return img, (seg + sh) / dt / 1e6, dt
+def run_plain(ctx, omr, args, dev):
+    wl = WORKLOADS[args.workload]
+    scene = omr.procedural(wl["kind"], width=wl["w"], height=wl["h"])
+    sid = ctx.upload(scene)
+    info = ctx.scene_info(sid)
+    prm = omr.params(ss=wl["ss"], spp_end=wl["spp"], max_depth=wl["depth"], rr_depth=wl["rr"], env_is=wl["env_is"],
+                     seed=0x6F6D7233)
+    a = (sid, scene.camera, prm)
+    for _ in range(args.warmup):
+        ctx.render(*a)
+    imgs, rays, walls = render_alone(ctx, [a] * args.steps)
+    dt = sum(walls)
+    acc = profiled_pass(ctx, [a] * min(args.steps, 5))
+    roof = roofline_from(acc, args.workload)
+    roof["pass"] = f"{min(args.steps, 5)} renders with per-launch HIP events on the engine's stream"
+    out = {
+        "metric": METRIC, "value": round(sum(rays) / dt / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {wl['desc']}", "mode": "plain (one job per render)",
+                   "resolution": f"{wl['w']}x{wl['h']}", "spp": wl["ss"] ** 2 * wl["spp"], "max_depth": wl["depth"],
+                   "rr_depth": wl["rr"], "env_is": wl["env_is"], "primitives": scene.n_prims,
+                   "bvh_nodes": info["n_nodes"], "device": dev["name"], "n_cu": dev["n_cu"],
+                   "tuning": ctx.get_tuning()},
+        "rays_per_step": rays[0], "roofline": roof,
+    }
+    return out, [(scene, scene.camera, prm, imgs[-1])]
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed renders (default: 20 for the plain workloads, 3 batches for c3)")
-    ap.add_argument("--warmup", type=int, default=None, help="untimed renders first (default 2; 1 for c3)")
-    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS) + ["c3", "c4"])
+                    help="timed steps: served streams (c4, default 20) or renders (c0-c2, default 20)")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed renders first")
+    ap.add_argument("--workload", default="c4", choices=sorted(WORKLOADS) + ["c3", "c4"])
+    ap.add_argument("--requests", type=int, default=128, help="c4: requests per served stream")
+    ap.add_argument("--load", type=float, default=2.0, help="c4: offered load of the timed streams (x plain capacity)")
+    ap.add_argument("--roofline-requests", type=int, default=32, help="c4: requests of the profiled pass")
+    ap.add_argument("--no-c3", action="store_true", help="c4: leave out the batched configuration")
+    ap.add_argument("--batch-reps", type=int, default=3, help="c3: timed batches")
+    ap.add_argument("--tune-ms", type=int, default=10000, help="c3: autotuner budget (0 = default tuning)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU baseline sample (0 = all)")
-    ap.add_argument("--no-tune", action="store_true", help="c3: skip the autotuner")
-    ap.add_argument("--tune-ms", type=int, default=20000)
-    ap.add_argument("--requests", type=int, default=32, help="c4: requests per arrival rate")
-    ap.add_argument("--no-profile", action="store_true",
-                    help="time the renders without per-launch events (no roofline timing)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="wall seconds of the CPU baseline sample")
     ap.add_argument("--tune", default="", help="launch tuning overrides, e.g. wg_size=64,blocks_per_cu=16")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device three times (hardware-counter calibration)")
     args = ap.parse_args()
-    plain = args.workload not in ("c3", "c4")
-    if args.steps is None:
-        args.steps = 20 if plain else 3
-    if args.warmup is None:
-        args.warmup = 2 if plain else 1
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is design-only.",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU (asked for --gpus {args.gpus}): every box here has one GPU, and "
+              "sharding over several is a design only.", file=sys.stderr)
         sys.exit(2)
+    if args.steps is None:
+        args.steps = 20
     import omr
 
-    if args.workload in ("c3", "c4"):
-        ctx = omr.Context(0)
-        dev = ctx.device_info()
-        if args.workload == "c3":
-            r = run_batched(ctx, omr, args)
-            out = {
-                "metric": METRIC, "value": round(r["batch_mrays"], 2), "unit": "Mrays/s", "n_gpus": 1,
-                "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r["t_batch"] * 1e3, 3),
-                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-                "data": "synthetic",
-                "config": {"workload": "c3: 64 uneven jobs over 4 resident scenes (Cornell, Cornell + icosphere, "
-                                       "terrain + sky, torus knot), one persistent wavefront run",
-                           "mode": "batched", "primitives": r["n_prims"], "device": dev["name"]},
-                "plain_mrays": round(r["plain_mrays"], 2), "plain_ms": round(r["t_plain"] * 1e3, 1),
-                "speedup_vs_plain": round(r["batch_mrays"] / r["plain_mrays"], 3),
-                "images_identical_to_plain": r["identical"], "rays_per_step": r["rays"], "tuning": r["tuning"],
-            }
-        else:
-            r = run_served(ctx, omr, args)
-            mid = r["rates"][1]
-            out = {
-                "metric": METRIC, "value": mid["mrays"], "unit": "Mrays/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-                "ms_per_step": round(mid["wall_s"] * 1e3, 1), "higher_is_better": True, "scaling": "weak",
-                "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-                "config": {"workload": f"c4: Poisson arrivals over 8 resident scenes ({r['n_tri']} triangles), "
-                                       f"{r['n_requests']} requests per rate", "mode": "served",
-                           "device": dev["name"]},
-                "plain_mrays": round(r["plain_mrays"], 2), "capacity_jobs_per_s": round(r["capacity"], 3),
-                "rates": r["rates"], "p50_ms": mid["p50_ms"], "p99_ms": mid["p99_ms"],
-            }
-        print(json.dumps(out), flush=True)
-        ctx.close()
-        return
-
-    wl = WORKLOADS[args.workload]
-    scene = omr.procedural(wl["kind"], width=wl["w"], height=wl["h"])
     ctx = omr.Context(0)
     dev = ctx.device_info()
-    sid = ctx.upload(scene)
-    info = ctx.scene_info(sid)
     if args.tune:
         ctx.set_tuning(**{k: int(v) for k, v in (kv.split("=") for kv in args.tune.split(","))})
-    prm = omr.params(ss=wl["ss"], spp_end=wl["spp"], max_depth=wl["depth"], rr_depth=wl["rr"], env_is=wl["env_is"],
-                     seed=0x6F6D7233)
-    ctx.set_profiling(not args.no_profile)
-    for _ in range(args.warmup):
-        ctx.render(sid, scene.camera, prm)
-    seg = shadow = drain_seg = drained = 0
-    ext_ms = shade_ms = shadow_ms = drain_ms = 0.0
-    launches = nodes = prims = dnodes = dprims = 0
-    img = None
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        img = ctx.render(sid, scene.camera, prm)  # returns after the device finished and copied back
-        st = ctx.last_stats()
-        seg += st["segments"]
-        shadow += st["shadow_rays"]
-        ext_ms += st["extend_ms"]
-        shade_ms += st["shade_ms"]
-        shadow_ms += st["shadow_ms"]
-        launches += st["extend_launches"]
-        nodes += st["node_visits"]
-        prims += st["prim_tests"]
-        drain_seg += st["drain_segments"]
-        drained += st["drained_paths"]
-        drain_ms += st["drain_ms"]
-        dnodes += st["drain_node_visits"]
-        dprims += st["drain_prim_tests"]
-    dt = time.perf_counter() - t0
-    rays = seg + shadow
-    value = rays / dt / 1e6
-    ext_seg = seg - drain_seg  # segments of the extend launches
-    drains = args.steps if drained else 0  # one drain launch per render
-    # algorithmic bytes of each kernel over the timed renders (per-unit figures in DESIGN.md section 3)
-    alg = {
-        "k_extend": (B_SEG * ext_seg + B_NODE * nodes + B_PRIM * prims, ext_ms, launches),
-        "k_shade": (B_SHADE_SEG * ext_seg + B_SHADOW_REC * shadow, shade_ms, launches),
-        "k_drain": (B_DRAIN_PATH * drained + B_DRAIN_SEG * drain_seg + B_NODE * dnodes + B_PRIM * dprims, drain_ms,
-                    drains),
-    }
-    kernels = {}
-    for name, (b, ms, n) in alg.items():
-        if ms > 0 and n > 0:
-            a = b / (ms / 1e3) / 1e9
-            kernels[name] = {"ms_per_step": round(ms / args.steps, 3), "avg_launch_ms": round(ms / n, 4),
-                             "alg_bytes_per_launch": int(b / n), "achieved": round(a, 1),
-                             "frac": round(a / HBM_PEAK_GBS, 4)}
-    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"]) if kernels else "k_extend"
-    kd = kernels.get(dom, {"achieved": 0.0, "frac": 0.0, "avg_launch_ms": 0.0, "alg_bytes_per_launch": 0})
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"{dom[2:]}_{args.workload}_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-    roofline = {
-        "bound": "hbm", "achieved": kd["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": kd["frac"], "traffic": traffic,
-        "kernel": dom, "avg_launch_ms": kd["avg_launch_ms"], "alg_bytes_per_launch": kd["alg_bytes_per_launch"],
-        "kernels": kernels,
-        "nodes_per_segment": round((nodes + dnodes) / max(1, seg), 3),
-        "prims_per_segment": round((prims + dprims) / max(1, seg), 3),
-        "kernel_ms_per_step": {"extend": round(ext_ms / args.steps, 3), "shade": round(shade_ms / args.steps, 3),
-                               "shadow": round(shadow_ms / args.steps, 3), "drain": round(drain_ms / args.steps, 3)},
-        "drain": {"paths_per_step": drained // args.steps, "segments_per_step": drain_seg // args.steps},
-    }
-    out = {
-        "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {wl['desc']}", "mode": "plain (one job per render)",
-                   "resolution": f"{wl['w']}x{wl['h']}", "spp": wl["ss"] ** 2 * wl["spp"],
-                   "max_depth": wl["depth"], "rr_depth": wl["rr"], "env_is": wl["env_is"],
-                   "primitives": scene.n_prims, "bvh_nodes": info["n_nodes"], "device": dev["name"],
-                   "n_cu": dev["n_cu"], "tuning": ctx.get_tuning()},
-        "segments_per_step": seg // args.steps, "shadow_rays_per_step": shadow // args.steps,
-        "roofline": roofline,
-    }
-    if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        nrows = args.cpu_rows if 0 < args.cpu_rows < wl["h"] else wl["h"]
-        r0 = max(0, wl["h"] // 2 - nrows // 2)
-        rows = (r0, r0 + nrows)
-        cimg, cmrays, cdt = cpu_baseline(scene, scene.camera, prm, rows, threads)
-        diff = img[rows[0]:rows[1]].astype(np.float64) - cimg.astype(np.float64)
-        out["rmse_vs_cpu"] = float(np.sqrt(np.mean(diff * diff)))
-        out["cpu_baseline"] = {"value": round(cmrays, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-                               "sample": f"rows {rows[0]}-{rows[1] - 1} of the same render ({nrows} x "
-                                         f"{wl['w']} px, all {wl['ss'] ** 2 * wl['spp']} spp), {cdt:.1f} s wall"}
+    if args.workload == "c3":
+        r = run_batched(ctx, omr, args)
+        out = {"metric": METRIC, "value": r["batch_mrays"], "unit": "Mrays/s", "n_gpus": 1,
+               "steps": r["batch_reps"], "warmup": 1, "ms_per_step": r["batch_ms"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+               "config": {"workload": r["workload"], "mode": "batched", "device": dev["name"]}}
+        out.update({k: v for k, v in r.items() if k != "workload"})
+        items = []
+    elif args.workload == "c4":
+        out, items = run_served(ctx, omr, args, dev)
+        if not args.no_c3:
+            out["batched"] = run_batched(ctx, omr, args)
+    else:
+        out, items = run_plain(ctx, omr, args, dev)
+    if not args.no_cpu and items:
+        threads = cpu_threads()
+        model, logical, usable = host_cpu()
+        log(f"cpu baseline: {threads} threads")
+        mrays, secs, rows, nreq, rmse = cpu_sample(items, args.cpu_seconds, threads)
+        out["cpu_baseline"] = {
+            "value": round(mrays, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"bands of 8 rows of the first {nreq} requests' images at their full spp, {rows} rows, "
+                      f"{secs:.1f} s wall; binary32 restatement of the reference path with its own binary BVH",
+            "cpu_model": model, "host_logical_cpus": logical, "usable_cpus": usable,
+            "rmse_vs_gpu": rmse,
+        }
     print(json.dumps(out), flush=True)
     ctx.close()
     if args.calib_copy > 0:
@@ -361,7 +549,7 @@ def copy_calibration(nbytes):
     nbytes once)."""
     import ctypes as C
 
-    hip = C.CDLL("libamdhip64.so")
+    hip = C.CDLL("libamdhip64.so.7")
     src, dst = C.c_void_p(), C.c_void_p()
     for p in (src, dst):
         if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:

@@ -276,6 +276,8 @@ int omr_job_poll(omr_ctx *ctx, omr_job_id job, int *state);
 int omr_job_wait(omr_ctx *ctx, omr_job_id job, int64_t timeout_ms);
 /* The error text of a failed or rejected job (empty otherwise). */
 int omr_job_error(omr_ctx *ctx, omr_job_id job, char *msg, size_t len);
+/* Submission-to-result latency of a done job (OMR_ERR_STATE otherwise). */
+int omr_job_latency(omr_ctx *ctx, omr_job_id job, double *ms);
 int omr_job_cancel(omr_ctx *ctx, omr_job_id job);
 int omr_server_stats_get(omr_ctx *ctx, omr_server_stats *stats);
 

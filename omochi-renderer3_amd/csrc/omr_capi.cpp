@@ -439,6 +439,10 @@ int omr_job_error(omr_ctx *ctx, omr_job_id job, char *msg, size_t len) {
   msg[len - 1] = 0;
   return OMR_OK;
 }
+int omr_job_latency(omr_ctx *ctx, omr_job_id job, double *ms) {
+  if (!ctx || !ms) return OMR_ERR_INVALID;
+  return ctx->eng->job_latency(job, ms);
+}
 int omr_job_cancel(omr_ctx *ctx, omr_job_id job) {
   if (!ctx) return OMR_ERR_INVALID;
   return ctx->eng->cancel(job);

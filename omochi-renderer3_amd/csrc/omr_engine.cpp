@@ -572,8 +572,8 @@ int Engine::launch_steps(uint32_t steps, bool with_shadow) {
     for (uint32_t s = 0; s < steps; s++) {
       int parity = (int)(s & 1u);
       hipError_t e = timed(0, [&] { return k::extend(a, sh, parity, true, stream_); });
-      if (e == hipSuccess) e = timed(1, [&] { return k::shade(a, sh, parity, regen, stream_); });
-      if (e == hipSuccess && with_shadow) e = timed(2, [&] { return k::shadow(a, sh, stream_); });
+      if (e == hipSuccess) e = timed(1, [&] { return k::shade(a, sh, parity, regen, true, stream_); });
+      if (e == hipSuccess && with_shadow) e = timed(2, [&] { return k::shadow(a, sh, true, stream_); });
       if (e == hipSuccess && refill_now(s)) e = k::refill(a, sh, parity ^ 1, stream_);
       if (e != hipSuccess) return hipfail(e, "profiled bounce step launch");
     }
@@ -584,10 +584,10 @@ int Engine::launch_steps(uint32_t steps, bool with_shadow) {
       int parity = (int)(s & 1u);
       hipError_t e = k::extend(a, sh, parity, false, stream_);
       if (e != hipSuccess) return e;
-      e = k::shade(a, sh, parity, regen, stream_);
+      e = k::shade(a, sh, parity, regen, false, stream_);
       if (e != hipSuccess) return e;
       if (with_shadow) {
-        e = k::shadow(a, sh, stream_);
+        e = k::shadow(a, sh, false, stream_);
         if (e != hipSuccess) return e;
       }
       if (refill_now(s)) {

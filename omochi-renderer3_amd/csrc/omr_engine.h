@@ -7,6 +7,7 @@
 #include <condition_variable>
 #include <cstdint>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -133,6 +134,7 @@ class Engine {
 
   int env_tables_host(omr_scene_id id, uint64_t *row, uint64_t *marg);
   int job_error(omr_job_id id, std::string *msg);
+  int job_latency(omr_job_id id, double *ms);
 
  private:
   mutable std::mutex err_mu_;
@@ -196,9 +198,11 @@ class Engine {
     std::chrono::steady_clock::time_point t_submit;
     SceneHost *sh = nullptr;  // its scene, held (users) from submission to completion
     int state = 0;  // 0 queued 1 running 2 done 3 cancelled 4 failed 5 rejected (HBM budget)
+    double lat_ms = 0.0;      // submission to result in the caller's buffer (done requests)
     int code = OMR_OK;        // error code of a failed or rejected request
     std::string msg;          // its error text
     size_t bytes = 0;         // device bytes while live: sums, image and (when `records`) per-sample records
+    uint64_t arena_off = 0;   // its block in the server's request arena
     bool records = false;  // per-sample records instead of pixel atomics
     JobRun run;
     bool cancel_req = false;
@@ -215,6 +219,16 @@ class Engine {
   uint64_t srv_record_bytes_ = 0;  // per-request record limit, fixed at server start
   uint32_t saved_pool_paths_ = 0;  // tuning.pool_paths before server_start changed it
   size_t used_bytes_ = 0;          // device bytes of the admitted requests
+  // Request arena: one device allocation made at server start (what the pool
+  // and scenes leave of the budget), sub-allocated first-fit per request, so
+  // serving allocates and frees no device memory.  Blocks are returned by the
+  // copier thread, taken by the server thread: own lock.
+  char *arena_ = nullptr;
+  uint64_t arena_bytes_ = 0;
+  std::mutex arena_mu_;
+  std::map<uint64_t, uint64_t> arena_free_;  // free blocks: offset -> length
+  bool arena_take(uint64_t bytes, uint64_t *off);
+  void arena_give(uint64_t off, uint64_t bytes);
   std::atomic<uint64_t> fixed_bytes_{0};  // device bytes of the pool, queues and stacks while serving
   bool server_failed_ = false;     // the server thread stopped after an error (srv_err_)
   std::string srv_err_;

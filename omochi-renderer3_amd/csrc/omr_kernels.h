@@ -74,8 +74,10 @@ hipError_t refill(const KernelArgs &a, const LaunchShape &s, int parity, hipStre
 hipError_t extend(const KernelArgs &a, const LaunchShape &s, int parity, bool stats, hipStream_t st);
 // regen: finished slots take the next sample inline; otherwise they go idle
 // until the next refill.
-hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool regen, hipStream_t st);
-hipError_t shadow(const KernelArgs &a, const LaunchShape &s, hipStream_t st);
+// tag: a profiled step (single-job runs at 64-thread groups launch a
+// separately named copy of the kernel, so profiler traces tell them apart)
+hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool regen, bool tag, hipStream_t st);
+hipError_t shadow(const KernelArgs &a, const LaunchShape &s, bool tag, hipStream_t st);
 // Runs every path of ray queue `parity` to its end in one launch (closest
 // hit, shading, visibility, pending branches); only once no samples are left.
 hipError_t drain(const KernelArgs &a, const LaunchShape &s, int parity, bool stats, hipStream_t st);

@@ -1265,8 +1265,9 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
 // on memory most of the time.  SOLO: the run has
 // one job (A.solo), whose job and scene records are read once, into scalar
 // registers, instead of per lane and segment (two dependent loads ahead of
-// every other load of the segment).
-template <int WG, bool SOLO>
+// every other load of the segment).  TAG: the launch of a profiled step (the
+// same code under its own name, so a profiler can tell those launches apart).
+template <int WG, bool SOLO, bool TAG = false>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO ? 4 : 5))) void k_shade(KernelArgs A, int parity,
                                                                                       int regen) {
   __shared__ uint32_t s_pre[kShards + 1];
@@ -1359,7 +1360,7 @@ __device__ inline void add_shadow(const KernelArgs &A, uint32_t r, const JobDev 
   deposit(J.recs, J.accum, __float_as_uint(c.w), __float_as_uint(b.w), fix(c.x), fix(c.y), fix(c.z));
 }
 
-template <int WG, int SHORT, bool DF, bool SOLO>
+template <int WG, int SHORT, bool DF, bool SOLO, bool TAG = false>
 __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
   __shared__ uint2 lds[SHORT * WG];
   __shared__ uint32_t s_pre[kShards + 1];
@@ -1862,17 +1863,19 @@ hipError_t launch_extend_wg(const KernelArgs &a, uint32_t grid, int parity, bool
 }
 
 template <int WG, int SHORT>
-hipError_t launch_shadow_ws(const KernelArgs &a, uint32_t grid, hipStream_t st) {
-  if (WG == 64 && a.p.df_budget > 0 && a.solo != kNoSolo)  // single-job runs, default group size and fetch mode
+hipError_t launch_shadow_ws(const KernelArgs &a, uint32_t grid, bool tag, hipStream_t st) {
+  if (WG == 64 && a.p.df_budget > 0 && a.solo != kNoSolo && tag)  // profiled single-job step
+    hipLaunchKernelGGL((k_shadow<WG, SHORT, true, true, true>), dim3(grid), dim3(WG), 0, st, a);
+  else if (WG == 64 && a.p.df_budget > 0 && a.solo != kNoSolo)  // single-job runs, default group size and fetch mode
     hipLaunchKernelGGL((k_shadow<WG, SHORT, true, true>), dim3(grid), dim3(WG), 0, st, a);
   else if (a.p.df_budget > 0) hipLaunchKernelGGL((k_shadow<WG, SHORT, true, false>), dim3(grid), dim3(WG), 0, st, a);
   else hipLaunchKernelGGL((k_shadow<WG, SHORT, false, false>), dim3(grid), dim3(WG), 0, st, a);
   return hipGetLastError();
 }
 template <int WG>
-hipError_t launch_shadow_wg(const KernelArgs &a, uint32_t grid, hipStream_t st) {
-  if (a.p.stack_short == kShortStackMin) return launch_shadow_ws<WG, (int)kShortStackMin>(a, grid, st);
-  return launch_shadow_ws<WG, (int)kShortStackMax>(a, grid, st);
+hipError_t launch_shadow_wg(const KernelArgs &a, uint32_t grid, bool tag, hipStream_t st) {
+  if (a.p.stack_short == kShortStackMin) return launch_shadow_ws<WG, (int)kShortStackMin>(a, grid, tag, st);
+  return launch_shadow_ws<WG, (int)kShortStackMax>(a, grid, tag, st);
 }
 
 template <int WG>
@@ -1962,13 +1965,15 @@ hipError_t extend(const KernelArgs &a, const LaunchShape &s, int parity, bool st
   }
 }
 
-hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool regen, hipStream_t st) {
+hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool regen, bool tag, hipStream_t st) {
   hipLaunchKernelGGL(k_clear_shadow, dim3(1), dim3(kShards), 0, st, a.ctrl);
   int r = regen ? 1 : 0;
   const bool solo = a.solo != kNoSolo;
   switch (s.wg) {
     case 64:
-      if (solo) hipLaunchKernelGGL((k_shade<64, true>), dim3(s.solo_shade_grid), dim3(64), 0, st, a, parity, r);
+      if (solo && tag)
+        hipLaunchKernelGGL((k_shade<64, true, true>), dim3(s.solo_shade_grid), dim3(64), 0, st, a, parity, r);
+      else if (solo) hipLaunchKernelGGL((k_shade<64, true>), dim3(s.solo_shade_grid), dim3(64), 0, st, a, parity, r);
       else hipLaunchKernelGGL((k_shade<64, false>), dim3(s.shade_grid), dim3(64), 0, st, a, parity, r);
       break;
     case 128:
@@ -1983,11 +1988,11 @@ hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool reg
   return hipGetLastError();
 }
 
-hipError_t shadow(const KernelArgs &a, const LaunchShape &s, hipStream_t st) {
+hipError_t shadow(const KernelArgs &a, const LaunchShape &s, bool tag, hipStream_t st) {
   switch (s.wg) {
-    case 64: return launch_shadow_wg<64>(a, s.grid, st);
-    case 128: return launch_shadow_wg<128>(a, s.grid, st);
-    default: return launch_shadow_wg<256>(a, s.grid, st);
+    case 64: return launch_shadow_wg<64>(a, s.grid, tag, st);
+    case 128: return launch_shadow_wg<128>(a, s.grid, false, st);
+    default: return launch_shadow_wg<256>(a, s.grid, false, st);
   }
 }
 

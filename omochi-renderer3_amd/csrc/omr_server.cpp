@@ -31,9 +31,46 @@ double ms_since(std::chrono::steady_clock::time_point t) {
 uint64_t samples_of(const omr_camera &c, const omr_render_params &p) {
   return (uint64_t)c.width * c.height * p.supersamples * p.supersamples * (uint64_t)(p.spp_end - p.spp_begin);
 }
-// Device bytes of a request's fixed-point sums and resolved image.
-uint64_t image_bytes(const omr_camera &c) { return (uint64_t)c.width * c.height * 3 * (8 + 4); }
+uint64_t align256(uint64_t b) { return (b + 255) & ~uint64_t(255); }
+// Device bytes of a request's fixed-point sums and resolved image, each
+// section 256-byte aligned in its arena block.
+uint64_t image_bytes(const omr_camera &c) {
+  const uint64_t px = (uint64_t)c.width * c.height;
+  return align256(px * 3 * 8) + align256(px * 3 * 4);
+}
 }  // namespace
+
+bool Engine::arena_take(uint64_t bytes, uint64_t *off) {
+  bytes = align256(bytes);
+  std::lock_guard<std::mutex> g(arena_mu_);
+  for (auto it = arena_free_.begin(); it != arena_free_.end(); ++it) {
+    if (it->second < bytes) continue;
+    *off = it->first;
+    const uint64_t rest = it->second - bytes, at = it->first + bytes;
+    arena_free_.erase(it);
+    if (rest) arena_free_[at] = rest;
+    return true;
+  }
+  return false;
+}
+
+void Engine::arena_give(uint64_t off, uint64_t bytes) {
+  bytes = align256(bytes);
+  std::lock_guard<std::mutex> g(arena_mu_);
+  auto it = arena_free_.emplace(off, bytes).first;
+  auto next = std::next(it);
+  if (next != arena_free_.end() && it->first + it->second == next->first) {  // merge with the block after
+    it->second += next->second;
+    arena_free_.erase(next);
+  }
+  if (it != arena_free_.begin()) {  // merge with the block before
+    auto prev = std::prev(it);
+    if (prev->first + prev->second == it->first) {
+      prev->second += it->second;
+      arena_free_.erase(it);
+    }
+  }
+}
 
 int Engine::server_start(const omr_server_config &cfg) {
   if (server_on_.load()) return fail(OMR_ERR_STATE, "server already running");
@@ -82,15 +119,24 @@ int Engine::server_start(const omr_server_config &cfg) {
     latencies_.clear();
     server_failed_ = false;
   }
-  // Request buffers come from the device's default pool (stream-ordered);
-  // keep freed blocks in the pool while serving instead of handing them back
-  // to the driver at every synchronization, so admissions reuse mapped memory.
+  // Request buffers: one arena of what the pool and scenes leave of the
+  // budget, allocated now; serving itself allocates nothing
   {
     std::lock_guard<std::mutex> g(mu);
-    hipMemPool_t pool = nullptr;
-    uint64_t keep = UINT64_MAX;
-    if (hipDeviceGetDefaultMemPool(&pool, device_) == hipSuccess && pool)
-      hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    const uint64_t room = (c.hbm_budget_bytes - fixed) & ~uint64_t(255);
+    if (room == 0 || hipMalloc((void **)&arena_, room) != hipSuccess) {
+      (void)hipGetLastError();
+      arena_ = nullptr;
+      if (tuning.pool_paths != saved_pool_paths_) {
+        drop_graphs();
+        tuning.pool_paths = saved_pool_paths_;
+      }
+      return fail(OMR_ERR_NOMEM, "request arena of " + std::to_string(room) + " bytes: device allocation failed");
+    }
+    arena_bytes_ = room;
+    std::lock_guard<std::mutex> ag(arena_mu_);
+    arena_free_.clear();
+    arena_free_[0] = room;
   }
   server_stop_ = false;
   server_on_ = true;
@@ -112,12 +158,9 @@ int Engine::server_stop() {
       drop_graphs();
       tuning.pool_paths = saved_pool_paths_;
     }
-    hipMemPool_t pool = nullptr;
-    uint64_t keep = 0;
-    if (hipDeviceGetDefaultMemPool(&pool, device_) == hipSuccess && pool) {
-      hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-      hipMemPoolTrimTo(pool, 0);
-    }
+    if (arena_) hipFree(arena_);
+    arena_ = nullptr;
+    arena_bytes_ = 0;
   }
   return OMR_OK;
 }
@@ -143,10 +186,9 @@ int Engine::submit(omr_scene_id scene_id, const omr_camera &cam, const omr_rende
   // requests.  A request needs its sums and image; per-sample records (no
   // atomics, the same results) are added when they fit in the record limit
   // and, with the image, in what the pool and scenes leave of the budget.
-  const uint64_t fixed = fixed_bytes_.load() + scene_bytes();
-  const uint64_t room = scfg_.hbm_budget_bytes > fixed ? scfg_.hbm_budget_bytes - fixed : 0;
+  const uint64_t room = arena_bytes_;
   const uint64_t need = image_bytes(cam);
-  const uint64_t rec = samples_of(cam, p) * 24;
+  const uint64_t rec = align256(samples_of(cam, p) * 24);
   r->records = rec <= srv_record_bytes_ && need + rec <= room;
   r->bytes = need + (r->records ? rec : 0);
   {
@@ -163,7 +205,7 @@ int Engine::submit(omr_scene_id scene_id, const omr_camera &cam, const omr_rende
     } else if (need > room) {
       code = OMR_ERR_BUDGET;
       r->msg = "the request needs " + std::to_string(need) + " bytes; the path pool and resident scenes leave " +
-               std::to_string(room) + " of the HBM budget";
+               std::to_string(room) + " bytes of the HBM budget for requests";
       r->state = 5;
       n_rejected_++;
     }
@@ -203,6 +245,15 @@ int Engine::job_error(omr_job_id id, std::string *msg) {
   std::lock_guard<std::mutex> g(smu);
   if (id < 1 || (size_t)id > all_.size()) return OMR_ERR_NOTFOUND;
   *msg = all_[(size_t)id - 1]->msg;
+  return OMR_OK;
+}
+
+int Engine::job_latency(omr_job_id id, double *ms) {
+  std::lock_guard<std::mutex> g(smu);
+  if (id < 1 || (size_t)id > all_.size()) return OMR_ERR_NOTFOUND;
+  const auto &r = all_[(size_t)id - 1];
+  if (r->state != 2) return OMR_ERR_STATE;
+  *ms = r->lat_ms;
   return OMR_OK;
 }
 
@@ -251,19 +302,22 @@ void Engine::server_loop() {
   uint64_t lo = 0, hi = 0;                // logical ring window [lo, hi)
   unsigned long long total = 0;           // samples admitted
   JobDev *h_jobs = nullptr;               // pinned staging for job records
-  ServeStatus *d_st = nullptr;            // two status slots on the device ...
+  // Status slots: two chunks' statuses and a drain's can be in flight at
+  // once; a slot is reused only kStatusSlots enqueues later.
+  constexpr int kStatusSlots = 4;
+  ServeStatus *d_st = nullptr;            // status slots on the device ...
   ServeStatus *h_st = nullptr;            // ... and their pinned copies
-  hipEvent_t ev_st[2] = {nullptr, nullptr};
+  hipEvent_t ev_st[kStatusSlots] = {};
   hipStream_t cs = nullptr;  // result copies, beside the compute stream
   bool ok = true;
   {
     std::lock_guard<std::mutex> g(mu);
     ok = hipStreamCreate(&cs) == hipSuccess &&
          hipHostMalloc(&h_jobs, kJobRing * sizeof(JobDev), hipHostMallocDefault) == hipSuccess &&
-         hipMalloc(&d_st, 2 * sizeof(ServeStatus)) == hipSuccess &&
-         hipHostMalloc(&h_st, 2 * sizeof(ServeStatus), hipHostMallocDefault) == hipSuccess &&
-         hipEventCreateWithFlags(&ev_st[0], hipEventDisableTiming) == hipSuccess &&
-         hipEventCreateWithFlags(&ev_st[1], hipEventDisableTiming) == hipSuccess;
+         hipMalloc(&d_st, kStatusSlots * sizeof(ServeStatus)) == hipSuccess &&
+         hipHostMalloc(&h_st, kStatusSlots * sizeof(ServeStatus), hipHostMallocDefault) == hipSuccess;
+    for (int i = 0; ok && i < kStatusSlots; i++)
+      ok = hipEventCreateWithFlags(&ev_st[i], hipEventDisableTiming) == hipSuccess;
     if (ok) {
       Ctrl c;
       std::memset(&c, 0, sizeof(c));
@@ -298,9 +352,9 @@ void Engine::server_loop() {
       hipStreamDestroy(cs);
     }
     for (auto &r : live) {
-      if (r->run.d_accum) hipFree(r->run.d_accum);
-      if (r->run.d_recs) hipFree(r->run.d_recs);
-      if (r->run.d_out) hipFree(r->run.d_out);
+      if (r->run.d_accum) arena_give(r->arena_off, r->bytes);  // not yet returned by the copier
+      r->run.d_accum = r->run.d_recs = nullptr;
+      r->run.d_out = nullptr;
       if (r->ev_copy) hipEventDestroy(r->ev_copy);
     }
     for (hipEvent_t e : ev_st)
@@ -314,31 +368,38 @@ void Engine::server_loop() {
     release_all();
     return;
   }
-  // Requests whose image is resolved on the compute stream but not yet copied
-  // out.  Their copies run on `cs` while the compute stream works on the next
-  // chunks of steps (the copy to a caller's pageable buffer holds this
-  // thread, not the GPU); then their device buffers go back and they complete.
-  std::vector<std::shared_ptr<Req>> copying;
-  auto flush_copies = [&]() {
-    if (copying.empty()) return;
-    bool copied = true;
-    for (auto &r : copying) {
-      copied = copied && hipStreamWaitEvent(cs, r->ev_copy, 0) == hipSuccess &&
-               hipMemcpyAsync(r->out, r->run.d_out, (size_t)r->cam.width * r->cam.height * 3 * 4,
-                              hipMemcpyDeviceToHost, cs) == hipSuccess;
-    }
-    copied = hipStreamSynchronize(cs) == hipSuccess && copied;
-    for (auto &r : copying) {
+  // Result copies run on their own host thread and stream: for each finished
+  // request the copier waits for its resolve (an event on the compute
+  // stream), copies the image into the caller's (pageable) buffer, returns
+  // its device buffers and completes it.  The copy to pageable memory holds
+  // the copier, never the server thread, so the chunks of steps keep coming.
+  std::deque<std::shared_ptr<Req>> to_copy;
+  std::mutex cmu;
+  std::condition_variable ccv;
+  bool copier_stop = false;
+  std::thread copier([&]() {
+    hipSetDevice(device_);
+    for (;;) {
+      std::shared_ptr<Req> r;
+      {
+        std::unique_lock<std::mutex> g(cmu);
+        ccv.wait(g, [&] { return copier_stop || !to_copy.empty(); });
+        if (to_copy.empty()) return;  // stopped and drained
+        r = to_copy.front();
+        to_copy.pop_front();
+      }
+      bool copied = hipStreamWaitEvent(cs, r->ev_copy, 0) == hipSuccess &&
+                    hipMemcpyAsync(r->out, r->run.d_out, (size_t)r->cam.width * r->cam.height * 3 * 4,
+                                   hipMemcpyDeviceToHost, cs) == hipSuccess;
+      copied = hipStreamSynchronize(cs) == hipSuccess && copied;
       hipEventDestroy(r->ev_copy);
       r->ev_copy = nullptr;
-      hipFreeAsync(r->run.d_accum, stream_);
-      hipFreeAsync(r->run.d_out, stream_);
-      if (r->run.d_recs) hipFreeAsync(r->run.d_recs, stream_);
+      // the copy is complete and so is all device work of the request (the
+      // copy waited for its resolve): its block goes back to the arena
+      arena_give(r->arena_off, r->bytes);
       r->run.d_accum = r->run.d_recs = nullptr;
       r->run.d_out = nullptr;
-    }
-    std::lock_guard<std::mutex> g(smu);
-    for (auto &r : copying) {
+      std::lock_guard<std::mutex> g(smu);
       used_bytes_ -= r->bytes;
       if (r->sh) r->sh->users--;
       r->sh = nullptr;
@@ -352,11 +413,26 @@ void Engine::server_loop() {
       } else {
         r->state = 2;
         n_done_++;
-        latencies_.push_back(ms_since(r->t_submit));
+        r->lat_ms = ms_since(r->t_submit);
+        latencies_.push_back(r->lat_ms);
       }
+      scv.notify_all();
     }
-    copying.clear();
-    scv.notify_all();
+  });
+  auto stop_copier = [&]() {
+    {
+      std::lock_guard<std::mutex> g(cmu);
+      copier_stop = true;
+    }
+    ccv.notify_all();
+    if (copier.joinable()) copier.join();
+  };
+  // A device or HIP failure: every live and queued request fails with the
+  // code, the copies already queued finish, then everything is released.
+  auto abort_with = [&](int code, const std::string &why) {
+    fail_all(code, why);
+    stop_copier();
+    release_all();
   };
   // Status reads in flight, oldest first: the status slot, the job window
   // and the admitted total when it was enqueued, and its sequence number.
@@ -372,7 +448,7 @@ void Engine::server_loop() {
   int slot_next = 0;
   auto enqueue_status = [&]() -> bool {
     const int s = slot_next;
-    slot_next ^= 1;
+    slot_next = (slot_next + 1) % kStatusSlots;
     const uint32_t n = (uint32_t)(hi - lo);
     const bool okq = k::serve_status(d_ctrl_, d_done_, (uint32_t)(lo % kJobRing), n, d_st + s, stream_) == hipSuccess &&
                      hipMemcpyAsync(h_st + s, d_st + s, offsetof(ServeStatus, done) + 8ull * n,
@@ -391,7 +467,6 @@ void Engine::server_loop() {
         scv.wait_for(g, std::chrono::milliseconds(20));
         if (queue_.empty()) continue;
       }
-      const uint64_t fixed = fixed_bytes_.load();
       while (!queue_.empty() && (hi + admit.size() - lo) < scfg_.max_live_jobs) {
         auto r = queue_.front();
         if (r->cancel_req) {
@@ -403,8 +478,13 @@ void Engine::server_loop() {
           scv.notify_all();
           continue;
         }
-        // scene bytes are read under scene_mu; smu is never taken inside it
-        if (fixed + scene_bytes() + used_bytes_ + r->bytes > scfg_.hbm_budget_bytes) break;  // wait for memory
+        // a block of the arena, without the per-sample records if only that fits
+        // (pixel atomics instead: the same results); else wait for memory
+        if (!arena_take(r->bytes, &r->arena_off)) {
+          if (!r->records || !arena_take(image_bytes(r->cam), &r->arena_off)) break;
+          r->records = false;
+          r->bytes = image_bytes(r->cam);
+        }
         queue_.pop_front();
         used_bytes_ += r->bytes;
         r->state = 1;
@@ -429,11 +509,11 @@ void Engine::server_loop() {
         const size_t px = (size_t)r->cam.width * r->cam.height;
         j.d_recs = nullptr;
         live.push_back(r);
-        okd = okd && hipMallocAsync((void **)&j.d_accum, px * 3 * 8, stream_) == hipSuccess &&
-              hipMallocAsync((void **)&j.d_out, px * 3 * 4, stream_) == hipSuccess &&
-              (!r->records || hipMallocAsync((void **)&j.d_recs, (size_t)j.n_samples * 24, stream_) == hipSuccess);
-        if (!okd) break;
-        okd = hipMemsetAsync(j.d_accum, 0, px * 3 * 8, stream_) == hipSuccess &&
+        char *blk = arena_ + r->arena_off;
+        j.d_accum = reinterpret_cast<unsigned long long *>(blk);
+        j.d_out = reinterpret_cast<float *>(blk + align256(px * 3 * 8));
+        if (r->records) j.d_recs = reinterpret_cast<unsigned long long *>(blk + image_bytes(r->cam));
+        okd = okd && hipMemsetAsync(j.d_accum, 0, px * 3 * 8, stream_) == hipSuccess &&
               (!j.d_recs || hipMemsetAsync(j.d_recs, 0, (size_t)j.n_samples * 24, stream_) == hipSuccess) &&
               k::clear_done(d_done_, j.ring, stream_) == hipSuccess;
         JobDev &D = h_jobs[j.ring];
@@ -466,35 +546,33 @@ void Engine::server_loop() {
       }
       if (!okd) {
         dev.unlock();
-        fail_all(OMR_ERR_NOMEM, "device allocation for a request failed");
-        release_all();
+        abort_with(OMR_ERR_HIP, "request set-up failed");
         return;
       }
       LaunchShape sh = shape();
       if (ensure_gstack(need_stack, (size_t)sh.grid * sh.wg) != OMR_OK) {
         dev.unlock();
-        fail_all(OMR_ERR_NOMEM, "traversal stack allocation failed");
-        release_all();
+        abort_with(OMR_ERR_NOMEM, "traversal stack allocation failed");
         return;
       }
       fixed_bytes_ = pool_bytes();
       if (k::admit(d_ctrl_, total_before, total, (uint32_t)lo, (uint32_t)hi, stream_) != hipSuccess ||
           k::refill(args(), sh, 0, stream_) != hipSuccess) {
         dev.unlock();
-        fail_all(OMR_ERR_HIP, "admission launch failed");
-        release_all();
+        abort_with(OMR_ERR_HIP, "admission launch failed");
         return;
       }
     }
     // ---- 2. keep two chunks of bounce steps (and their statuses) in flight
     if (!live.empty()) {
       bool env_is = false;
+      // (requests that finished behind an unfinished one stay in `live`
+      // until the window passes them; their scene hold is already released)
       for (auto &r : live)
-        if (r->params.env_is && r->sh->has_env) env_is = true;
+        if (!r->run.done && r->params.env_is && r->sh && r->sh->has_env) env_is = true;
       if (launch_steps(tuning.steps_per_sync, env_is) != OMR_OK || !enqueue_status()) {
         dev.unlock();
-        fail_all(OMR_ERR_HIP, "bounce step launch failed: " + get_err());
-        release_all();
+        abort_with(OMR_ERR_HIP, "bounce step launch failed: " + get_err());
         return;
       }
       if (pend.size() < 2) continue;  // fill the pipeline before waiting
@@ -506,11 +584,9 @@ void Engine::server_loop() {
     dev.unlock();
     const bool st_ok = hipEventSynchronize(ev_st[P.slot]) == hipSuccess;
     dev.lock();
-    flush_copies();  // results resolved before this status
     if (!st_ok) {
       dev.unlock();
-      fail_all(OMR_ERR_HIP, "device error in the bounce loop");
-      release_all();
+      abort_with(OMR_ERR_HIP, "device error in the bounce loop");
       return;
     }
     const ServeStatus &S = h_st[P.slot];
@@ -525,8 +601,7 @@ void Engine::server_loop() {
       drain_seq = seq;
       if (!okd || !enqueue_status()) {
         dev.unlock();
-        fail_all(OMR_ERR_HIP, "drain launch failed");
-        release_all();
+        abort_with(OMR_ERR_HIP, "drain launch failed");
         return;
       }
     }
@@ -547,11 +622,14 @@ void Engine::server_loop() {
                  hipEventRecord(r->ev_copy, stream_) == hipSuccess;
       if (!okr) {
         dev.unlock();
-        fail_all(OMR_ERR_HIP, "result resolve failed");
-        release_all();
+        abort_with(OMR_ERR_HIP, "result resolve failed");
         return;
       }
-      copying.push_back(r);
+      {
+        std::lock_guard<std::mutex> g(cmu);
+        to_copy.push_back(r);
+      }
+      ccv.notify_one();
       retired = true;
     }
     if (retired) {
@@ -561,8 +639,7 @@ void Engine::server_loop() {
       }
       if (k::admit(d_ctrl_, total, total, (uint32_t)lo, (uint32_t)hi, stream_) != hipSuccess) {
         dev.unlock();
-        fail_all(OMR_ERR_HIP, "job window update failed");
-        release_all();
+        abort_with(OMR_ERR_HIP, "job window update failed");
         return;
       }
     }
@@ -573,20 +650,16 @@ void Engine::server_loop() {
       srv_shadow_ = S.shadow_rays;
       if (retired && live.empty()) busy_ms_ += ms_since(busy_t0);
     }
-    // nothing left to overlap with: copy out now
-    if (live.empty() && pend.empty()) {
-      std::lock_guard<std::mutex> g(mu);
-      flush_copies();
-    }
   }
   {
     std::lock_guard<std::mutex> g(mu);
     hipStreamSynchronize(stream_);
-    flush_copies();
   }
+  stop_copier();
   {
     std::lock_guard<std::mutex> g(smu);
     for (auto &r : live) {
+      if (r->run.done) continue;  // finished: completed by the copier
       r->state = 3;
       if (r->sh) r->sh->users--;
       r->sh = nullptr;

@@ -101,7 +101,7 @@ EXPORTS = [
     "omr_scene_info_get", "omr_scene_bvh", "omr_scene_env_tables", "omr_trace_closest", "omr_trace_closest_device",
     "omr_trace_stats_device", "omr_trace_closest_device_noalloc", "omr_render", "omr_render_sums", "omr_last_render_stats", "omr_render_batch",
     "omr_get_tuning", "omr_set_tuning", "omr_server_start", "omr_server_stop", "omr_job_submit", "omr_job_poll",
-    "omr_job_wait", "omr_job_error", "omr_job_cancel", "omr_server_stats_get", "omr_procedural_scene", "omr_procedural_free",
+    "omr_job_wait", "omr_job_error", "omr_job_latency", "omr_job_cancel", "omr_server_stats_get", "omr_procedural_scene", "omr_procedural_free",
     "omr_camera_from_lookat", "omr_bvh_build", "omr_bvh_data", "omr_bvh_free", "omr_set_profiling", "omr_autotune",
     "omr_procedural_camera", "omr_io_last_error", "omr_free", "omr_hdr_read", "omr_hdr_write", "omr_scene_file_load",
     "omr_obj_load", "omr_loaded_scene_free", "omr_image_read", "omr_texture_read",
@@ -148,6 +148,7 @@ def load():
     lib.omr_job_poll.argtypes = [vp, C.c_int64, C.POINTER(C.c_int)]
     lib.omr_job_wait.argtypes = [vp, C.c_int64, C.c_int64]
     lib.omr_job_error.argtypes = [vp, C.c_int64, C.c_char_p, sz]
+    lib.omr_job_latency.argtypes = [vp, C.c_int64, C.POINTER(C.c_double)]
     lib.omr_job_cancel.argtypes = [vp, C.c_int64]
     lib.omr_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
     lib.omr_procedural_scene.argtypes = [C.c_int, u64, C.c_uint32, C.c_uint32, C.POINTER(Procedural)]
@@ -555,6 +556,11 @@ class Context:
         buf = C.create_string_buffer(512)
         self._check(self.lib.omr_job_error(self.h, job, buf, 512))
         return buf.value.decode()
+
+    def job_latency(self, job):
+        ms = C.c_double()
+        self._check(self.lib.omr_job_latency(self.h, job, C.byref(ms)))
+        return ms.value
 
     def cancel(self, job):
         return self.lib.omr_job_cancel(self.h, job)

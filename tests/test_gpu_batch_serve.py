@@ -151,3 +151,31 @@ def test_served_budget_and_scene_holds():
     ctx.server_stop()
     ctx.release(sids[1])
     ctx.close()
+
+
+def test_served_out_of_order_completion():
+    """Requests that finish behind an unfinished earlier one (short
+    environment-lit jobs queued after a long one whose deepest paths outlast
+    them) stay in the live window until it passes them; the loop keeps
+    serving and every result equals the job alone."""
+    import omr
+
+    ctx = omr.Context(0)
+    scene = omr.procedural(3, width=64, height=48)
+    sid = ctx.upload(scene)
+    spec = [(320, 240, 4, 4, 16), (24, 16, 1, 1, 2), (32, 18, 2, 1, 2), (16, 9, 1, 1, 1), (48, 27, 2, 2, 2)]
+    jobs = [(sid, _cam(omr, scene, w, h), omr.params(ss=ss, spp_end=spp, max_depth=d, env_is=1, seed=40 + k))
+            for k, (w, h, ss, spp, d) in enumerate(spec)]
+    alone = [ctx.render(*j) for j in jobs]
+    ctx.server_start(pool_paths=1 << 16)
+    outs = [np.zeros_like(a) for a in alone]
+    ids = [ctx.submit(j[0], j[1], j[2], o) for j, o in zip(jobs, outs)]
+    print("submitted", len(ids), flush=True)
+    for i in ids:
+        assert ctx.wait(i, 60000) == 0, ctx.job_error(i)
+    lat = [ctx.job_latency(i) for i in ids]
+    print("latencies", lat, flush=True)
+    ctx.server_stop()
+    for a, o in zip(alone, outs):
+        assert np.array_equal(a, o)
+    ctx.close()
