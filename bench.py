@@ -203,7 +203,7 @@ def profiled_pass(ctx, args_list):
             st = ctx.last_stats()
             for k in keys:
                 acc[k] += st[k]
-            drains += 1 if st["drained_paths"] else 0
+            drains += st["drain_launches"]
     finally:
         ctx.set_profiling(False)
     acc["drain_launches"] = drains

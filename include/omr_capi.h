@@ -130,6 +130,7 @@ typedef struct {
   uint64_t drained_paths, drain_segments;
   double drain_ms;
   uint64_t drain_node_visits, drain_prim_tests;  /* drain launch, when profiling */
+  uint64_t drain_launches;
 } omr_render_stats;
 
 typedef struct {
@@ -165,6 +166,13 @@ typedef struct {
                                  without atomics and a gather sums them per pixel at the end; the
                                  rest add to their pixel sums atomically (0 = atomics only).  The
                                  results are the same bits either way */
+  uint32_t drain_rounds;      /* 1: the run is a sequence of rounds, each filling every idle path
+                                 slot with a new sample and running all of them to their end in one
+                                 drain launch (no path state through memory between segments);
+                                 0: extend/shade bounce steps until the drain threshold */
+  uint32_t round_steps;       /* drain rounds: extend/shade bounce steps (even, 0..16) before each
+                                 round's drain -- the first bounces of fresh camera paths run as
+                                 wide, coherent steps */
 } omr_tuning;
 
 typedef struct {

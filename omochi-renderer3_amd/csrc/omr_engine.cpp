@@ -39,6 +39,8 @@ Engine::Engine(int device) : device_(device) {
   tuning.stack_short = kShortStackMax;
   tuning.drain_below = 1u << 24;
   tuning.record_mb = 32768;
+  tuning.drain_rounds = 1;
+  tuning.round_steps = 2;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
     set_err("no HIP device " + std::to_string(device));
@@ -124,6 +126,8 @@ int Engine::set_tuning(const omr_tuning &t) {
   if (t.stack_short != kShortStackMin && t.stack_short != kShortStackMax)
     return fail(OMR_ERR_INVALID, "stack_short must be 8 or 16");
   if ((uint64_t)t.record_mb << 20 > hbm_) return fail(OMR_ERR_INVALID, "record_mb exceeds the device memory");
+  if (t.drain_rounds > 1) return fail(OMR_ERR_INVALID, "drain_rounds must be 0 or 1");
+  if (t.round_steps > 16 || (t.round_steps & 1u)) return fail(OMR_ERR_INVALID, "round_steps must be even, 0..16");
   std::lock_guard<std::mutex> g(mu);
   hipSetDevice(device_);
   hipStreamSynchronize(stream_);
@@ -533,6 +537,30 @@ int Engine::trace_noalloc(omr_scene_id id, const float *d_rays, size_t n, int32_
   return OMR_OK;
 }
 
+int Engine::launch_round(bool first, bool with_shadow) {
+  const KernelArgs a = args();
+  const LaunchShape sh = shape();
+  hipError_t e = hipSuccess;
+  if (!first) {
+    // the last round's drain left queue 0 stale: clear it, then every idle slot takes a sample
+    e = k::zero_words(reinterpret_cast<uint32_t *>(&d_ctrl_->qcount[0]), sizeof(d_ctrl_->qcount[0]) / 4, stream_);
+    if (e == hipSuccess)
+      e = k::zero_words(reinterpret_cast<uint32_t *>(&d_ctrl_->qhead[0]), sizeof(d_ctrl_->qhead[0]) / 4, stream_);
+    if (e == hipSuccess) e = k::refill(a, sh, 0, stream_);
+    if (e != hipSuccess) return hipfail(e, "round refill");
+  }
+  if (tuning.round_steps) {
+    int rc = launch_steps(tuning.round_steps, with_shadow, true);
+    if (rc) return rc;
+  }
+  // an even number of steps leaves the live paths in queue 0
+  if (profile) OMR_HIP(hipEventRecord(ev_c_, stream_));
+  e = k::drain(a, sh, 0, profile, stream_);
+  if (e != hipSuccess) return hipfail(e, "drain");
+  if (profile) OMR_HIP(hipEventRecord(ev_d_, stream_));
+  return OMR_OK;
+}
+
 int Engine::collect_profile() {
   for (size_t i = 0; i < prof_kind_.size(); i++) {
     float ms = 0.0f;
@@ -544,13 +572,16 @@ int Engine::collect_profile() {
   return OMR_OK;
 }
 
-int Engine::launch_steps(uint32_t steps, bool with_shadow) {
+int Engine::launch_steps(uint32_t steps, bool with_shadow, bool round) {
   KernelArgs a = args();
   LaunchShape sh = shape();
   // refill_every = 0: finished slots regenerate inside shade; k > 0: they go
   // idle and a bulk refill every k steps hands out samples in pixel order.
-  const bool regen = tuning.refill_every == 0;
-  auto refill_now = [&](uint32_t s) { return !regen && ((s + 1) % tuning.refill_every == 0 || s + 1 == steps); };
+  // In a drain round finished slots go idle until the next round.
+  const bool regen = !round && tuning.refill_every == 0;
+  auto refill_now = [&](uint32_t s) {
+    return !round && !regen && ((s + 1) % tuning.refill_every == 0 || s + 1 == steps);
+  };
   if (profile) {
     // one event pair per launch, every kernel directly on the stream
     size_t need = (size_t)steps * 3 * 2;
@@ -603,8 +634,8 @@ int Engine::launch_steps(uint32_t steps, bool with_shadow) {
     return OMR_OK;
   }
   int gi = with_shadow ? 1 : 0;
-  if (!graph_[gi] || graph_steps_ != steps || graph_solo_ != a.solo) {
-    if (graph_steps_ != steps || graph_solo_ != a.solo) drop_graphs();
+  if (!graph_[gi] || graph_steps_ != steps || graph_solo_ != a.solo || graph_round_ != round) {
+    if (graph_steps_ != steps || graph_solo_ != a.solo || graph_round_ != round) drop_graphs();
     hipGraph_t g = nullptr;
     OMR_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     hipError_t e = enqueue();
@@ -618,6 +649,7 @@ int Engine::launch_steps(uint32_t steps, bool with_shadow) {
     graph_[gi] = ex;
     graph_steps_ = steps;
     graph_solo_ = a.solo;
+    graph_round_ = round;
   }
   OMR_HIP(hipGraphLaunch(graph_[gi], stream_));
   return OMR_OK;
@@ -702,7 +734,7 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
   if (fresh) {
     // slots past n0 are read only by the bulk refill (with regeneration they
     // never enter a queue)
-    if (n0 < pool_cap_ && tuning.refill_every != 0)
+    if (n0 < pool_cap_ && (tuning.refill_every != 0 || tuning.drain_rounds))
       OMR_HIP(hipMemsetAsync(pool_.misc + 4ull * n0, 0, (size_t)(pool_cap_ - n0) * 16, stream_));
   } else {
     OMR_HIP(hipMemsetAsync(pool_.misc, 0, (size_t)pool_cap_ * 16, stream_));
@@ -717,11 +749,35 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
   if (e != hipSuccess) return hipfail(e, "refill");
   uint64_t steps_done = 0;
   const uint32_t chunk = tuning.steps_per_sync;
+  uint64_t drained = 0, drain_launches = 0;
+  double drain_ms = 0.0;
   prof_ms_[0] = prof_ms_[1] = prof_ms_[2] = 0.0;
   prof_extend_launches_ = 0;
-  uint64_t drained = 0;
-  double drain_ms = 0.0;
-  for (uint64_t it = 0;; it++) {
+  if (tuning.drain_rounds) {
+    // Rounds: every slot holds a sample after the fill, the drain runs them
+    // all to their end (and marks the slots idle), the refill gives every
+    // idle slot the next sample.  Each round hands out min(pool, remaining)
+    // samples, so the number of rounds is known and nothing is read back
+    // until the end.
+    const uint64_t rounds = (total + pool_cap_ - 1) / pool_cap_;
+    for (uint64_t r = 0; r < rounds; r++) {
+      rc = launch_round(r == 0, any_env_is);
+      if (rc) return rc;
+      steps_done += tuning.round_steps;
+      drain_launches++;
+      if (profile) {
+        OMR_HIP(hipEventSynchronize(ev_d_));
+        if ((rc = collect_profile())) return rc;
+        float ms = 0.0f;
+        OMR_HIP(hipEventElapsedTime(&ms, ev_c_, ev_d_));
+        drain_ms += ms;
+      }
+    }
+    OMR_HIP(hipMemcpyAsync(h_ctrl_, d_ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_));
+    OMR_HIP(hipStreamSynchronize(stream_));
+    drained = sum_shards(h_ctrl_->drain_paths);
+  }
+  for (uint64_t it = 0; !tuning.drain_rounds; it++) {
     double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (it > 2000000ull || el > 600.0) return fail(OMR_ERR_STATE, "bounce loop did not terminate");
     rc = launch_steps(chunk, any_env_is);
@@ -739,6 +795,7 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
       if (profile) OMR_HIP(hipEventRecord(ev_c_, stream_));
       e = k::drain(a, sh, 0, profile, stream_);
       if (e != hipSuccess) return hipfail(e, "drain");
+      drain_launches++;
       if (profile) OMR_HIP(hipEventRecord(ev_d_, stream_));
       OMR_HIP(hipMemcpyAsync(h_ctrl_, d_ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_));
       OMR_HIP(hipStreamSynchronize(stream_));
@@ -793,6 +850,7 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     st->drain_ms = drain_ms;
     st->drain_node_visits = sum_shards(h_ctrl_->drain_node_visits);
     st->drain_prim_tests = sum_shards(h_ctrl_->drain_prim_tests);
+    st->drain_launches = drain_launches;
   }
   return OMR_OK;
 }

@@ -170,12 +170,18 @@ class Engine {
   hipGraphExec_t graph_[2] = {nullptr, nullptr};
   uint32_t graph_steps_ = 0;
   uint32_t graph_solo_ = kNoSolo;  // KernelArgs::solo the graphs were captured with
+  bool graph_round_ = false;       // the graphs hold drain-round steps (no regeneration, no refill)
   uint32_t solo_ = kNoSolo;        // ring slot of the current run's only job, or kNoSolo
 
   int ensure_pool();
   int ensure_gstack(uint32_t need_entries, size_t threads);
   void drop_graphs();
-  int launch_steps(uint32_t steps, bool with_shadow);
+  // `steps` bounce steps; round: the steps of a drain round (finished slots
+  // go idle, no regeneration or bulk refill)
+  int launch_steps(uint32_t steps, bool with_shadow, bool round = false);
+  // One drain round on queue 0: (unless first) every idle slot takes a new
+  // sample, tuning.round_steps bounce steps, then one drain launch.
+  int launch_round(bool first, bool with_shadow);
   // profiling: event pairs around each launch of the last chunk
   std::vector<hipEvent_t> prof_ev_;
   std::vector<int> prof_kind_;  // 0 extend, 1 shade, 2 shadow

@@ -1446,7 +1446,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const float4 *ray_d = reinterpret_cast<const float4 *>(A.p.ray_d);
   uint4 *misc = reinterpret_cast<uint4 *>(A.p.misc);
   const uint4 *rngs = reinterpret_cast<const uint4 *>(A.p.rng);
-  uint32_t segs = 0, shadows = 0, nv = 0, nt = 0;  // node/primitive counts of the closest-hit traversals
+  uint32_t segs = 0, shadows = 0, nv = 0, nt = 0, taken = 0;  // node/primitive counts of the closest-hit traversals
   uint32_t djob = ~0u, dcnt = 0;  // this lane's completions of job djob not yet added
   // Lanes take a new path as soon as theirs ends: a wave-level fetch on the
   // sub-queue heads (zero here: the extend before the last shade cleared
@@ -1495,6 +1495,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
         st.pdf = ps_mis(st.word) ? A.p.pdf[slot] : 0.0f;
         have = true;
         tracing = true;
+        taken++;
         tr.init(SOLO ? Ss : A.scenes[A.jobs[st.js].scene], st.o, st.d, kInf, stk);
       }
       if (base + idle >= cnt) {  // this sub-queue is used up: find one with entries left
@@ -1570,6 +1571,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
   }
   wave_count(A.ctrl->segments, segs);
   wave_count(A.ctrl->drain_segments, segs);
+  wave_count(A.ctrl->drain_paths, taken);
   wave_count(A.ctrl->shadow_rays, shadows);
   if (STATS) {
     wave_count(A.ctrl->drain_node_visits, nv);

@@ -170,6 +170,7 @@ struct Ctrl {
   Sharded<unsigned long long> node_visits;  // stats kernels only
   Sharded<unsigned long long> prim_tests;   // stats kernels only
   Sharded<unsigned long long> drain_segments;  // closest-hit queries traced by the drain kernel
+  Sharded<unsigned long long> drain_paths;     // paths the drain kernel took from its queue
   Sharded<unsigned long long> drain_node_visits;  // drain, stats variant only
   Sharded<unsigned long long> drain_prim_tests;   // drain, stats variant only
   Sharded<uint32_t> qcount[2];              // ping-pong ray sub-queue counts

@@ -556,21 +556,30 @@ void Engine::server_loop() {
         return;
       }
       fixed_bytes_ = pool_bytes();
+      // (in drain rounds the next round's refill hands the new samples out)
       if (k::admit(d_ctrl_, total_before, total, (uint32_t)lo, (uint32_t)hi, stream_) != hipSuccess ||
-          k::refill(args(), sh, 0, stream_) != hipSuccess) {
+          (!tuning.drain_rounds && k::refill(args(), sh, 0, stream_) != hipSuccess)) {
         dev.unlock();
         abort_with(OMR_ERR_HIP, "admission launch failed");
         return;
       }
     }
-    // ---- 2. keep two chunks of bounce steps (and their statuses) in flight
+    // ---- 2. keep two chunks of work (and their statuses) in flight: a drain
+    // round (every idle slot takes a sample, all run to their end) or a chunk
+    // of bounce steps
     if (!live.empty()) {
       bool env_is = false;
       // (requests that finished behind an unfinished one stay in `live`
       // until the window passes them; their scene hold is already released)
       for (auto &r : live)
         if (!r->run.done && r->params.env_is && r->sh && r->sh->has_env) env_is = true;
-      if (launch_steps(tuning.steps_per_sync, env_is) != OMR_OK || !enqueue_status()) {
+      bool okc;
+      if (tuning.drain_rounds) {
+        okc = launch_round(false, env_is) == OMR_OK;
+      } else {
+        okc = launch_steps(tuning.steps_per_sync, env_is) == OMR_OK;
+      }
+      if (!okc || !enqueue_status()) {
         dev.unlock();
         abort_with(OMR_ERR_HIP, "bounce step launch failed: " + get_err());
         return;
@@ -593,7 +602,7 @@ void Engine::server_loop() {
     // drain: every admitted sample is handed out and few paths are left; only
     // from a status taken after the last drain with nothing admitted since
     // (the drain ends the queued paths; newly admitted samples would wait)
-    if (tuning.drain_below && S.exhausted && S.queued > 0 && S.queued <= tuning.drain_below && P.total == total &&
+    if (!tuning.drain_rounds && tuning.drain_below && S.exhausted && S.queued > 0 && S.queued <= tuning.drain_below && P.total == total &&
         P.seq >= drain_seq) {
       const bool okd = k::drain(args(), shape(), 0, false, stream_) == hipSuccess &&
                        hipMemsetAsync(&d_ctrl_->qcount[0], 0, sizeof(d_ctrl_->qcount[0]), stream_) == hipSuccess &&

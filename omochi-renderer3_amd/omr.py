@@ -66,7 +66,7 @@ class RenderStats(C.Structure):
                 ("shade_ms", C.c_double), ("shadow_ms", C.c_double), ("extend_launches", C.c_uint64),
                 ("node_visits", C.c_uint64), ("prim_tests", C.c_uint64), ("drained_paths", C.c_uint64),
                 ("drain_segments", C.c_uint64), ("drain_ms", C.c_double), ("drain_node_visits", C.c_uint64),
-                ("drain_prim_tests", C.c_uint64)]
+                ("drain_prim_tests", C.c_uint64), ("drain_launches", C.c_uint64)]
 
 
 class SceneInfo(C.Structure):
@@ -78,7 +78,8 @@ class Tuning(C.Structure):
     _fields_ = [("node_width", C.c_uint32), ("wg_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("pool_paths", C.c_uint32), ("steps_per_sync", C.c_uint32), ("use_graph", C.c_uint32),
                 ("refill_every", C.c_uint32), ("fetch_budget", C.c_uint32), ("fetch_min", C.c_uint32),
-                ("stack_short", C.c_uint32), ("drain_below", C.c_uint32), ("record_mb", C.c_uint32)]
+                ("stack_short", C.c_uint32), ("drain_below", C.c_uint32), ("record_mb", C.c_uint32),
+                ("drain_rounds", C.c_uint32), ("round_steps", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):

@@ -120,18 +120,24 @@ def test_render_bit_exact(ctx, kind, ss, spp, depth, rr):
     ctx.release(sid)
 
 
-@pytest.mark.parametrize("tune", [dict(fetch_budget=8, fetch_min=16), dict(refill_every=2),
-                                  dict(fetch_budget=4, fetch_min=32, refill_every=1, use_graph=0, wg_size=256),
-                                  dict(wg_size=64, blocks_per_cu=2, pool_paths=4096, steps_per_sync=4),
+@pytest.mark.parametrize("tune", [dict(fetch_budget=8, fetch_min=16), dict(drain_rounds=0),
+                                  dict(drain_rounds=0, refill_every=2),
+                                  dict(drain_rounds=0, fetch_budget=4, fetch_min=32, refill_every=1, use_graph=0,
+                                       wg_size=256),
+                                  dict(drain_rounds=0, wg_size=64, blocks_per_cu=2, pool_paths=4096, steps_per_sync=4),
+                                  dict(wg_size=64, blocks_per_cu=2, pool_paths=4096),
+                                  dict(pool_paths=1000, wg_size=128, blocks_per_cu=1),
                                   dict(stack_short=8), dict(stack_short=8, fetch_budget=8, wg_size=64,
                                                             blocks_per_cu=16),
-                                  dict(drain_below=1 << 20), dict(drain_below=2000, pool_paths=8192),
-                                  dict(record_mb=0), dict(record_mb=0, refill_every=2)])
+                                  dict(drain_rounds=0, drain_below=1 << 20),
+                                  dict(drain_rounds=0, drain_below=2000, pool_paths=8192),
+                                  dict(record_mb=0), dict(drain_rounds=0, record_mb=0, refill_every=2)])
 def test_tuning_modes_bit_exact(ctx, tune):
     """Every launch configuration the autotuner can pick gives the same bits:
-    dynamic-fetch traversal, bulk refill cadence, graph off, small pools,
-    the 8-entry LDS stack (more entries spill to global memory), the drain
-    launch that finishes the last paths in one go."""
+    drain rounds (one or many per render, small pools) and bounce steps with
+    dynamic-fetch traversal, bulk refill cadence, graph off, small pools, the
+    drain launch that finishes the last paths in one go; the 8-entry LDS
+    stack (more entries spill to global memory)."""
     import omr
     import pyoracle
 
@@ -154,6 +160,8 @@ def test_tuning_modes_bit_exact(ctx, tune):
         assert st["segments"] == seg
         if tune.get("drain_below"):
             assert st["drained_paths"] > 0 and 0 < st["drain_segments"] <= seg
+        if tune.get("drain_rounds", 1) and tune.get("pool_paths"):
+            assert st["drain_launches"] == -(-40 * 32 * 4 * 2 // tune["pool_paths"])  # one per round
         ctx.release(sid)
     finally:
         ctx.set_tuning(**saved)
