@@ -208,3 +208,48 @@ def texel_xy(w, h, uv):
     xy = np.empty((uv.shape[0], 2), np.int32)
     lib.ref_texel_xy(w, h, _p(uv), uv.shape[0], _p(xy))
     return xy
+
+
+class RefCornell:
+    """The reference's own CornellBoxScene (src/scenes/CornellBoxScene.cpp:10-36)."""
+
+    def __init__(self):
+        self.lib = load()
+        vp, sz, u32 = C.c_void_p, C.c_size_t, C.c_uint32
+        self.lib.ref_cornell_new.argtypes = []
+        self.lib.ref_cornell_new.restype = vp
+        self.lib.ref_cornell_free.argtypes = [vp]
+        self.lib.ref_cornell_free.restype = None
+        self.lib.ref_cornell_objects.argtypes = [vp]
+        self.lib.ref_cornell_objects.restype = sz
+        self.lib.ref_cornell_trace.argtypes = [vp, vp, sz, vp, vp, vp, vp, vp, vp]
+        self.lib.ref_cornell_trace.restype = None
+        self.lib.ref_cornell_render.argtypes = [vp, u32, u32, vp, vp, vp, C.c_double, C.c_double, u32, u32, vp, vp]
+        self.h = self.lib.ref_cornell_new()
+
+    def objects(self):
+        return int(self.lib.ref_cornell_objects(self.h))
+
+    def close(self):
+        if self.h:
+            self.lib.ref_cornell_free(self.h)
+            self.h = None
+
+    def trace(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        t = np.empty(n, np.float64)
+        nrm = np.zeros((n, 3), np.float64)
+        mat = np.zeros((n, 4, 9), np.float32)
+        nmat = np.zeros(n, np.int32)
+        obj = np.empty(n, np.int64)
+        uv = np.zeros((n, 2), np.float64)
+        self.lib.ref_cornell_trace(self.h, _p(rays), n, _p(t), _p(nrm), _p(mat), _p(nmat), _p(obj), _p(uv))
+        return dict(t=t, normal=nrm, mat=mat, nmat=nmat, obj=obj, uv=uv)
+
+    def render(self, w, h, pos, direction, up, screen_h, dist, ss, batches):
+        mean = np.empty((h, w, 3), dtype=np.float64)
+        var = np.empty((h, w, 3), dtype=np.float64)
+        P, D, U = (np.asarray(v, dtype=np.float64) for v in (pos, direction, up))
+        self.lib.ref_cornell_render(self.h, w, h, _p(P), _p(D), _p(U), screen_h, dist, ss, batches, _p(mean), _p(var))
+        return mean, var

@@ -16,6 +16,7 @@
 #include "renderer/PathTracer.h"
 #include "renderer/Polygon.h"
 #include "renderer/Sphere.h"
+#include "scenes/CornellBoxScene.h"
 #include "scenes/Scene.h"
 #include "scenes/SceneFromExternalFile.h"
 #include "tools/HDRImage.h"
@@ -84,17 +85,95 @@ class HarnessScene : public Scene {
   }
 };
 
-// A scene the reference reads from its .scene format, with its object list.
-class FileScene : public SceneFromExternalFile {
+// A reference scene class with its object list readable: the .scene reader
+// (SceneFromExternalFile) or the reference's own CornellBoxScene.
+template <class Base>
+class Listed : public Base {
  public:
-  explicit FileScene(const std::string &f) : SceneFromExternalFile(f) {}
-  size_t Count() const { return m_objects.size(); }
+  using Base::Base;
+  size_t Count() const { return this->m_objects.size(); }
   size_t IndexOf(const SceneObject *o) const {
-    for (size_t k = 0; k < m_objects.size(); k++)
-      if (m_objects[k].object == o) return k;
+    for (size_t k = 0; k < this->m_objects.size(); k++)
+      if (this->m_objects[k].object == o) return k;
     return (size_t)-1;
   }
 };
+using FileScene = Listed<SceneFromExternalFile>;
+using CornellScene = Listed<CornellBoxScene>;
+
+// Scene::CheckIntersection of a listed scene on n rays: FP64 distance,
+// normal of the hit record, the object's material count, the object's index
+// in the scene's object list (-1 on a miss), and its first four materials as
+// floats (type, color, emission, ior, rate; 9 per material, 36 per ray).
+template <class S>
+void trace_listed(S *s, const float *rays, size_t n, double *t, double *normal, float *mats, int32_t *nmat,
+                  int64_t *obj, double *uv) {
+  for (size_t i = 0; i < n; i++) {
+    const float *r = rays + 8 * i;
+    Ray ray(Vector3(r[0], r[1], r[2]), Vector3(r[4], r[5], r[6]));
+    Scene::IntersectionInformation info;
+    bool hit = s->CheckIntersection(ray, info);
+    obj[i] = -1;
+    t[i] = info.hit.distance;
+    if (!hit) continue;
+    normal[3 * i] = info.hit.normal.x;
+    normal[3 * i + 1] = info.hit.normal.y;
+    normal[3 * i + 2] = info.hit.normal.z;
+    uv[2 * i] = info.hit.uv.x;
+    uv[2 * i + 1] = info.hit.uv.y;
+    const size_t cnt = info.object->GetMaterialCount();
+    for (size_t k = 0; k < cnt && k < 4; k++) {
+      const SceneObject::MultiMaterialElem *e = info.object->GetMaterial(k);
+      const Material &m = e->material_;
+      float *o = mats + 36 * i + 9 * k;
+      o[0] = (float)m.reflection_type;
+      o[1] = (float)m.color.x; o[2] = (float)m.color.y; o[3] = (float)m.color.z;
+      o[4] = (float)m.emission.x; o[5] = (float)m.emission.y; o[6] = (float)m.emission.z;
+      o[7] = (float)m.refraction_rate;
+      o[8] = (float)e->rate_;
+    }
+    nmat[i] = (int32_t)cnt;
+    obj[i] = (int64_t)s->IndexOf(info.object);
+  }
+}
+
+// PathTracer::RenderScene (src/renderer/PathTracer.cpp:55-85) with one
+// thread: `batches` sample batches of ss x ss samples each, pinhole camera.
+// Writes the per-pixel mean and variance of the batch means (top-down rows,
+// W x H x 3).
+int render_scene(const Scene &scene, uint32_t W, uint32_t H, const double *pos, const double *dir, const double *up,
+                 double screen_h, double dist, uint32_t ss, uint32_t batches, double *mean, double *var) {
+  Camera cam(W, H, Vector3(pos[0], pos[1], pos[2]), Vector3(dir[0], dir[1], dir[2]), Vector3(up[0], up[1], up[2]),
+             screen_h, dist, 200.0);
+  std::vector<double> prev((size_t)W * H * 3, 0.0), m1((size_t)W * H * 3, 0.0), m2((size_t)W * H * 3, 0.0);
+  PathTracer::RenderingFinishCallbackFunction cb = [&](int samples, const Color *img, size_t w2, size_t h, double) {
+    // left W x H half of the 2W x H buffer; stored row (H-1-y) is screen row y
+    for (size_t r = 0; r < h; r++)
+      for (size_t x = 0; x < W; x++) {
+        const Color &c = img[x + r * w2];
+        double cur[3] = {c.x, c.y, c.z};
+        for (int k = 0; k < 3; k++) {
+          size_t i = (r * W + x) * 3 + k;
+          double batch = cur[k] * samples - prev[i] * (samples - 1);
+          prev[i] = cur[k];
+          m1[i] += batch;
+          m2[i] += batch * batch;
+        }
+      }
+  };
+  // Heap-allocated and never destroyed: ~PathTracer applies delete[] to the
+  // ScreenPixels it allocated with new (src/renderer/PathTracer.cpp:50-53,62),
+  // which aborts under glibc.
+  PathTracer *pt = new PathTracer(cam, 1, (int)batches, 1, (int)ss, cb);
+  pt->SetMaxThreadCount(1);
+  pt->RenderScene(scene);
+  for (size_t i = 0; i < m1.size(); i++) {
+    double mu = m1[i] / batches;
+    mean[i] = mu;
+    var[i] = batches > 1 ? (m2[i] - batches * mu * mu) / (batches - 1) : 0.0;
+  }
+  return 0;
+}
 
 }  // namespace
 
@@ -143,34 +222,22 @@ size_t ref_file_scene_objects(void *s) { return ((FileScene *)s)->Count(); }
 // floats (type, color, emission, ior, rate; 9 per material, 36 per ray).
 void ref_file_trace(void *sp, const float *rays, size_t n, double *t, double *normal, float *mats, int32_t *nmat,
                     int64_t *obj, double *uv) {
-  FileScene *s = (FileScene *)sp;
-  for (size_t i = 0; i < n; i++) {
-    const float *r = rays + 8 * i;
-    Ray ray(Vector3(r[0], r[1], r[2]), Vector3(r[4], r[5], r[6]));
-    Scene::IntersectionInformation info;
-    bool hit = s->CheckIntersection(ray, info);
-    obj[i] = -1;
-    t[i] = info.hit.distance;
-    if (!hit) continue;
-    normal[3 * i] = info.hit.normal.x;
-    normal[3 * i + 1] = info.hit.normal.y;
-    normal[3 * i + 2] = info.hit.normal.z;
-    uv[2 * i] = info.hit.uv.x;
-    uv[2 * i + 1] = info.hit.uv.y;
-    const size_t cnt = info.object->GetMaterialCount();
-    for (size_t k = 0; k < cnt && k < 4; k++) {
-      const SceneObject::MultiMaterialElem *e = info.object->GetMaterial(k);
-      const Material &m = e->material_;
-      float *o = mats + 36 * i + 9 * k;
-      o[0] = (float)m.reflection_type;
-      o[1] = (float)m.color.x; o[2] = (float)m.color.y; o[3] = (float)m.color.z;
-      o[4] = (float)m.emission.x; o[5] = (float)m.emission.y; o[6] = (float)m.emission.z;
-      o[7] = (float)m.refraction_rate;
-      o[8] = (float)e->rate_;
-    }
-    nmat[i] = (int32_t)cnt;
-    obj[i] = (int64_t)s->IndexOf(info.object);
-  }
+  trace_listed((FileScene *)sp, rays, n, t, normal, mats, nmat, obj, uv);
+}
+
+// The reference's own CornellBoxScene (src/scenes/CornellBoxScene.cpp:10-36):
+// floors, a two-material Lambert/mirror sphere, mirror and glass spheres and
+// a SphereLight, no BVH (brute-force closest hit over its object list).
+void *ref_cornell_new() { return new CornellScene(); }
+void ref_cornell_free(void *s) { delete (CornellScene *)s; }
+size_t ref_cornell_objects(void *s) { return ((CornellScene *)s)->Count(); }
+void ref_cornell_trace(void *sp, const float *rays, size_t n, double *t, double *normal, float *mats, int32_t *nmat,
+                       int64_t *obj, double *uv) {
+  trace_listed((CornellScene *)sp, rays, n, t, normal, mats, nmat, obj, uv);
+}
+int ref_cornell_render(void *sp, uint32_t W, uint32_t H, const double *pos, const double *dir, const double *up,
+                       double screen_h, double dist, uint32_t ss, uint32_t batches, double *mean, double *var) {
+  return render_scene(*(CornellScene *)sp, W, H, pos, dir, up, screen_h, dist, ss, batches, mean, var);
 }
 
 // HDRImage::ReadFromRadianceFile: dims into wh (w, h); pixels (h x w x 3,
@@ -305,42 +372,10 @@ int ref_env_texel(const char *hdr, size_t n, const double *dirs, int32_t *ix, in
   return 0;
 }
 
-// PathTracer::RenderScene (src/renderer/PathTracer.cpp:55-85) with one
-// thread: `batches` sample batches of ss x ss samples each.  Writes the
-// per-pixel mean and variance of the batch means (top-down rows, W x H x 3).
+// PathTracer::RenderScene over a harness scene (see render_scene).
 int ref_render(void *sp, uint32_t W, uint32_t H, const double *pos, const double *dir, const double *up,
                double screen_h, double dist, uint32_t ss, uint32_t batches, double *mean, double *var) {
-  HarnessScene *s = (HarnessScene *)sp;
-  Camera cam(W, H, Vector3(pos[0], pos[1], pos[2]), Vector3(dir[0], dir[1], dir[2]), Vector3(up[0], up[1], up[2]),
-             screen_h, dist, 200.0);
-  std::vector<double> prev((size_t)W * H * 3, 0.0), m1((size_t)W * H * 3, 0.0), m2((size_t)W * H * 3, 0.0);
-  PathTracer::RenderingFinishCallbackFunction cb = [&](int samples, const Color *img, size_t w2, size_t h, double) {
-    // left W x H half of the 2W x H buffer; stored row (H-1-y) is screen row y
-    for (size_t r = 0; r < h; r++)
-      for (size_t x = 0; x < W; x++) {
-        const Color &c = img[x + r * w2];
-        double cur[3] = {c.x, c.y, c.z};
-        for (int k = 0; k < 3; k++) {
-          size_t i = (r * W + x) * 3 + k;
-          double batch = cur[k] * samples - prev[i] * (samples - 1);
-          prev[i] = cur[k];
-          m1[i] += batch;
-          m2[i] += batch * batch;
-        }
-      }
-  };
-  // Heap-allocated and never destroyed: ~PathTracer applies delete[] to the
-  // ScreenPixels it allocated with new (src/renderer/PathTracer.cpp:50-53,62),
-  // which aborts under glibc.
-  PathTracer *pt = new PathTracer(cam, 1, (int)batches, 1, (int)ss, cb);
-  pt->SetMaxThreadCount(1);
-  pt->RenderScene(*s);
-  for (size_t i = 0; i < m1.size(); i++) {
-    double mu = m1[i] / batches;
-    mean[i] = mu;
-    var[i] = batches > 1 ? (m2[i] - batches * mu * mu) / (batches - 1) : 0.0;
-  }
-  return 0;
+  return render_scene(*(HarnessScene *)sp, W, H, pos, dir, up, screen_h, dist, ss, batches, mean, var);
 }
 
 }  // extern "C"

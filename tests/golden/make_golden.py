@@ -177,7 +177,58 @@ def textures():
     print("textures", sorted(out))
 
 
+def cornell_ref():
+    """The reference's own CornellBoxScene: closest hits on a ray set inside
+    the box, and its PathTracer render (1 thread, reference mode).  The same
+    scene restated as scenes/cornell_ref.scene is what the tests load."""
+    cwd = os.getcwd()
+    os.chdir(SCENES)
+    try:
+        scene = omr.load_scene_file("cornell_ref.scene")
+    finally:
+        os.chdir(cwd)
+    rays = gi.cornell_rays(scene)
+    r = pyref.RefCornell()
+    res = r.trace(rays)
+    nobj = r.objects()
+    c = gi.CORNELL_RENDER
+    mean, var = r.render(c["w"], c["h"], c["pos"], c["dir"], (0, 1, 0), c["screen_h"], c["dist"], c["ss"],
+                         c["batches"])
+    r.close()
+    np.savez_compressed(os.path.join(HERE, "hits_cornell_ref.npz"), rays_sha256=sha(rays), n_objects=nobj,
+                        t=res["t"], obj=res["obj"].astype(np.int16), nmat=res["nmat"].astype(np.int8),
+                        mat=res["mat"][:, :2], normal=res["normal"].astype(np.float32))
+    np.savez_compressed(os.path.join(HERE, "ref_render_cornell.npz"), mean=mean.astype(np.float32),
+                        var=var.astype(np.float32), **{k: np.asarray(v) for k, v in c.items()})
+    print("cornell hits", int((res["obj"] >= 0).sum()), "of", len(rays), "render mean", float(mean.mean()))
+
+
+def ibl_sphere():
+    """A Lambert sphere lit only by the reference's Barce HDR: the reference's
+    PathTracer render.  The HDR's texels (as the reference decodes them) are
+    kept as barce.hdr, written by the project's RGBE writer."""
+    env = pyref.hdr_read(REF_HDR)
+    path = os.path.join(HERE, "barce.hdr")
+    omr.hdr_write(path, env)
+    assert np.array_equal(omr.hdr_read(path), env), "RGBE round trip must be exact"
+    scene = gi.ibl_sphere_scene(env)
+    r = pyref.RefScene(scene, bvh=False, hdr=REF_HDR)
+    c = gi.IBL_RENDER
+    mean, var = r.render(c["w"], c["h"], c["pos"], c["dir"], (0, 1, 0), c["screen_h"], c["dist"], c["ss"],
+                         c["batches"])
+    r.close()
+    np.savez_compressed(os.path.join(HERE, "ref_render_ibl_sphere.npz"), mean=mean.astype(np.float32),
+                        var=var.astype(np.float32), **{k: np.asarray(v) for k, v in c.items()})
+    print("ibl sphere render mean", float(mean.mean()))
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1:
+        for name in sys.argv[1:]:
+            globals()[name]()
+        sys.exit(0)
+    cornell_ref()
+    ibl_sphere()
     textures()
     texture_lookup()
     hdr_files()

@@ -40,6 +40,39 @@ PLANE_HITS = (0, 20000, 303)
 
 LOADER_RAYS = (30000, 404)
 
+# The reference's own CornellBoxScene (src/scenes/CornellBoxScene.cpp:10-36),
+# restated as tests/golden/scenes/cornell_ref.scene: a ray set inside the box
+# and a render with the camera of the reference's settings.txt for it.
+CORNELL_RAYS = (20000, 505)
+CORNELL_BOX = ((0.0, 0.0, 0.0), (100.0, 81.6, 250.0))
+CORNELL_RENDER = dict(w=64, h=36, pos=(50.0, 55.0, 230.0), dir=(0.00001, -0.15, -1.0), screen_h=30.0, dist=40.0,
+                      ss=2, batches=256)
+
+# A Lambert sphere lit only by the reference's Barce_Rooftop_C_Env.hdr
+# (tests/golden/barce.hdr: the same texels, written by the RGBE writer).
+IBL_SPHERE = dict(center=(0.0, 0.0, 0.0), radius=20.0, albedo=(0.8, 0.8, 0.8))
+IBL_RENDER = dict(w=32, h=32, pos=(0.0, 0.0, 90.0), dir=(0.0, 0.0, -1.0), screen_h=30.0, dist=40.0, ss=2,
+                  batches=256)
+
+
+def cornell_rays(scene):
+    n, seed = CORNELL_RAYS
+    lo, hi = (np.asarray(v, np.float32) for v in CORNELL_BOX)
+    return helpers.hit_rays(scene, n, seed, box_scale=1.0, focus=(lo, hi))
+
+
+def ibl_sphere_scene(env):
+    """The Lambert sphere as an omr.Scene (no triangles) with `env` as sky."""
+    import omr
+
+    mats = np.zeros(1, dtype=omr.MATERIAL_DTYPE)
+    mats["type"] = 0
+    mats["color"] = IBL_SPHERE["albedo"]
+    mats["rate"] = 1.0
+    sph = np.array([list(IBL_SPHERE["center"]) + [IBL_SPHERE["radius"]]], np.float32)
+    return omr.Scene(np.zeros((0, 9), np.float32), np.zeros((0, 9), np.float32), sph, np.zeros(1, np.uint32), mats,
+                     env=env)
+
 
 def hdr_image(w, h, seed):
     """Non-negative float image with constant runs (RLE runs of every

@@ -185,7 +185,8 @@ def test_trace_planes_bit_exact(ctx):
     ctx.release(sid)
 
 
-@pytest.mark.parametrize("name,env_is", [("loader", 0), ("loader", 1), ("textured", 0), ("textured", 1)])
+@pytest.mark.parametrize("name,env_is", [("loader", 0), ("loader", 1), ("textured", 0), ("textured", 1),
+                                         ("cornell_ref", 0)])
 def test_render_loaded_scene_bit_exact(ctx, name, env_is):
     """Scenes read from the .scene format render to the same bits as the
     restatement: loader.scene (OBJ mesh with MTL materials, multi-material
@@ -207,6 +208,8 @@ def test_render_loaded_scene_bit_exact(ctx, name, env_is):
     if name == "textured":
         assert len(scene.textures) == 6
         cam = omr.camera_lookat(48, 32, (0.0, 25.0, 80.0), (0.0, -0.3, -1.0), screen_height=30.0, dist=40.0)
+    elif name == "cornell_ref":  # the reference's CornellBoxScene, its settings.txt camera
+        cam = omr.camera_lookat(64, 36, (50.0, 55.0, 230.0), (0.00001, -0.15, -1.0), screen_height=30.0, dist=40.0)
     else:
         cam = omr.camera_lookat(48, 32, (0.0, 30.0, 110.0), (0.0, -0.25, -1.0), screen_height=30.0, dist=40.0)
     sid = ctx.upload(scene)
@@ -218,4 +221,31 @@ def test_render_loaded_scene_bit_exact(ctx, name, env_is):
     assert not (g != c).any(axis=2).any(), f"{(g != c).any(axis=2).sum()} pixels differ"
     assert st["segments"] == seg and st["shadow_rays"] == sh
     assert g.any()
+    ctx.release(sid)
+
+
+@pytest.mark.parametrize("env_is", [0, 1])
+def test_render_lambert_sphere_under_reference_hdr_bit_exact(ctx, env_is):
+    """The Lambert sphere under the reference's Barce HDR (the scene whose
+    restatement render is checked against the reference's PathTracer),
+    reference mode (RR after depth 5, no depth cap), with and without
+    environment importance sampling."""
+    import os
+
+    import golden_inputs as gi
+    import omr
+    import pyoracle
+
+    env = omr.hdr_read(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "barce.hdr"))
+    scene = gi.ibl_sphere_scene(env)
+    c = gi.IBL_RENDER
+    cam = omr.camera_lookat(c["w"], c["h"], c["pos"], c["dir"], screen_height=c["screen_h"], dist=c["dist"])
+    sid = ctx.upload(scene)
+    prm = omr.params(ss=2, spp_end=4, max_depth=0, rr_depth=5, env_is=env_is, seed=2025)
+    print(f"render ibl sphere env_is={env_is}", flush=True)
+    g = ctx.render_sums(sid, cam, prm)
+    st = ctx.last_stats()
+    c_, seg, sh = _oracle(scene, ctx, sid).render_sums(cam, prm, pyoracle.WIDE_BVH)
+    assert not (g != c_).any(axis=2).any(), f"{(g != c_).any(axis=2).sum()} pixels differ"
+    assert st["segments"] == seg and st["shadow_rays"] == sh
     ctx.release(sid)

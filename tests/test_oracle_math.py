@@ -182,3 +182,61 @@ def test_env_cdf_chi_square():
     dirs = helpers.unit_dirs(rng, 200000)
     pdfs = np.array([o.env_pdf(v) for v in dirs[:20000]])
     assert abs(pdfs.mean() * 4 * np.pi - 1.0) < 0.05
+
+
+def _seed_fn():
+    import pyoracle
+
+    lib = pyoracle.load()
+    return lambda *a: int(lib.orc_path_seed(*a))
+
+
+def test_path_seed_depends_on_every_argument():
+    """Per-path seeding hash(seed, pixel, sub-pixel, sample, branch): changing
+    any one argument (the 64-bit seed's low or high word included) changes
+    the stream, and swapping two arguments does too -- a hash that dropped or
+    transposed an argument would reuse streams across samples."""
+    seed_of = _seed_fn()
+    rng = np.random.default_rng(5)
+    for _ in range(2000):
+        s = int(rng.integers(0, 2 ** 63))
+        p, u, k = (int(x) for x in rng.integers(0, 1 << 20, 3))
+        b = int(rng.integers(0, 8))
+        h = seed_of(s, p, u, k, b)
+        for alt in ((s ^ 1, p, u, k, b), (s ^ (1 << 40), p, u, k, b), (s, p + 1, u, k, b), (s, p, u + 1, k, b),
+                    (s, p, u, k + 1, b), (s, p, u, k, b ^ 1), (s, u, p, k, b), (s, p, k, u, b), (s, k, u, p, b)):
+            if alt[1:] != (p, u, k, b):
+                assert seed_of(*alt) != h, (s, p, u, k, b, alt)
+    # the lens stream (branch ~0) differs from every path stream of the sample
+    assert seed_of(7, 3, 2, 1, 0xFFFFFFFF) not in {seed_of(7, 3, 2, 1, b) for b in range(64)}
+
+
+def test_path_seed_streams_are_distinct_and_decorrelated():
+    """Over a grid of 32 pixels x 16 sub-pixels x 4 samples x 4 branches no
+    two paths share a seed; one input bit flips half the output bits on
+    average; the first draws of neighbouring pixels' streams are uniform on
+    [0, 1] and uncorrelated."""
+    import itertools
+
+    import pyoracle
+
+    seed_of = _seed_fn()
+    grid = [seed_of(0x6F6D7233, p, s, k, b)
+            for p, s, k, b in itertools.product(range(32), range(16), range(1, 5), (0, 1, 2, 5))]
+    assert len(set(grid)) == len(grid)
+    rng = np.random.default_rng(9)
+    flips = []
+    for _ in range(400):
+        a = [int(rng.integers(0, 2 ** 63))] + [int(x) for x in rng.integers(0, 1 << 24, 4)]
+        h = seed_of(*a)
+        i = int(rng.integers(0, 5))
+        bit = int(rng.integers(0, 24))
+        b = list(a)
+        b[i] ^= 1 << bit
+        flips.append(bin(h ^ seed_of(*b)).count("1"))
+    assert 14.5 < np.mean(flips) < 17.5
+    first = np.array([pyoracle.rng_floats(seed_of(11, p, 0, 1, 0), 1)[0] for p in range(8001)], np.float64)
+    counts, _ = np.histogram(first, bins=20, range=(0.0, 1.0))
+    chi2 = float(((counts - 400.05) ** 2 / 400.05).sum())
+    assert chi2 < 43.8  # 19 degrees of freedom, p = 0.001
+    assert abs(np.corrcoef(first[:-1], first[1:])[0, 1]) < 0.05
