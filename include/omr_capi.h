@@ -112,6 +112,13 @@ typedef struct {
   uint32_t rr_depth;       /* Russian roulette when depth > rr_depth (reference: 5) */
   uint32_t env_is;         /* 1 = environment importance sampling with shadow rays */
   uint64_t seed;           /* per-path seeds = hash(seed, pixel, sub-pixel, sample, branch) */
+  uint32_t nee;            /* 1 = the reference's next-event estimation mode
+                              (PathTracer::EnableNextEventEstimation, PathTracer.cpp:406-439):
+                              surface emission counts only where the camera ray hits, and the
+                              direct-light term it adds is zero (:312-314).  Scenes whose mirror
+                              or glass materials emit are refused (OMR_ERR_INVALID): the reference
+                              adds that emission when their continuation ray hits (:513-516) */
+  uint32_t pad;
 } omr_render_params;
 
 typedef struct {

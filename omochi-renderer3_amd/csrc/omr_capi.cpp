@@ -38,6 +38,7 @@ int check_job(omr_ctx *c, const omr_camera *cam, const omr_render_params *p, uin
   if (p->supersamples == 0 || p->supersamples > 64) return set_err(c, OMR_ERR_INVALID, "supersamples must be 1..64");
   if (p->spp_end <= p->spp_begin) return set_err(c, OMR_ERR_INVALID, "spp_end must exceed spp_begin");
   if (p->max_depth > 255) return set_err(c, OMR_ERR_INVALID, "max_depth must be <= 255 (0 = unlimited)");
+  if (p->nee > 1) return set_err(c, OMR_ERR_INVALID, "nee must be 0 or 1");
   uint64_t n = (uint64_t)cam->width * cam->height * p->supersamples * p->supersamples *
                (uint64_t)(p->spp_end - p->spp_begin);
   if (n > 0xFFFFFFFFull) return set_err(c, OMR_ERR_INVALID, "job exceeds 2^32-1 samples; split spp into batches");

@@ -301,6 +301,11 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
   if (d->env_rgb && (d->env_width == 0 || d->env_height == 0)) return fail(OMR_ERR_INVALID, "env size is zero");
 
   auto sh = std::make_unique<SceneHost>();
+  for (size_t i = 0; i < d->n_materials; i++) {
+    const omr_material &m = d->materials[i];
+    if (m.type != 0 && (m.emission[0] > 0.0f || m.emission[1] > 0.0f || m.emission[2] > 0.0f))
+      sh->emissive_specular = true;
+  }
   BuildInput bi;
   bi.n_tri = nt;
   bi.tri = d->triangles;
@@ -672,6 +677,9 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     if (!s) return fail(OMR_ERR_NOTFOUND, "unknown scene");
     need_stack = std::max(need_stack, s->bvh.stack_bound);
     if (j.params.env_is && s->has_env) any_env_is = true;
+    if (j.params.nee && s->emissive_specular)
+      return fail(OMR_ERR_INVALID, "next-event estimation on a scene whose mirror or glass materials emit is not "
+                                   "supported (the reference adds that emission when their continuation ray hits)");
   }
   LaunchShape sh = shape();
   rc = ensure_gstack(need_stack, (size_t)sh.grid * sh.wg);
@@ -693,7 +701,7 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     D.spp_begin = j.params.spp_begin;
     D.max_depth = j.params.max_depth;
     D.rr_depth = j.params.rr_depth;
-    D.flags = j.params.env_is ? 1u : 0u;
+    D.flags = (j.params.env_is ? 1u : 0u) | (j.params.nee ? 2u : 0u);
     D.scene = j.scene;
     D.seed = j.params.seed;
     for (int c = 0; c < 3; c++) {

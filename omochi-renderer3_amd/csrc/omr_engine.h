@@ -53,6 +53,7 @@ struct SceneHost {
   std::vector<void *> allocs;
   size_t bytes = 0;
   bool has_env = false;
+  bool emissive_specular = false;  // a mirror or glass material emits (refused with next-event estimation)
   uint32_t env_w = 0, env_h = 0;
   std::atomic<int> users{0};  // queued or live served jobs holding the scene (guarded by Engine::scene_mu)
 };

@@ -1106,7 +1106,10 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
     } else {
       prr = 1.0f;
     }
-    add_px(J.recs, J.accum, sidx, pixel, beta, em, 1.0f);
+    // with the reference's next-event estimation (flags bit 1) emission counts
+    // only where the camera ray hits (PathTracer.cpp:406-439); its direct
+    // term is zero (:312-314)
+    if (!(J.flags & 2u) || depth == 0u) add_px(J.recs, J.accum, sidx, pixel, beta, em, 1.0f);
     uint32_t dmax = J.max_depth == 0u ? 255u : J.max_depth;
     bool can_continue = !killed && depth + 1u < dmax;
     if (!killed) {

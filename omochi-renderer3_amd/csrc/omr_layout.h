@@ -110,7 +110,7 @@ struct JobDev {
   uint32_t width, height, ss;
   uint32_t nspp, spp_begin;
   uint32_t max_depth, rr_depth;
-  uint32_t flags;                  // bit0: environment importance sampling
+  uint32_t flags;                  // bit0: environment importance sampling; bit1: next-event estimation mode
   uint32_t scene;
   unsigned long long seed;
   float cam_pos[3], cam_c[3], cam_x[3], cam_y[3];

@@ -174,6 +174,7 @@ int Engine::submit(omr_scene_id scene_id, const omr_camera &cam, const omr_rende
     std::lock_guard<std::mutex> g(scene_mu);
     SceneHost *s = scene(scene_id);
     if (!s) return OMR_ERR_NOTFOUND;
+    if (p.nee && s->emissive_specular) return OMR_ERR_INVALID;  // as run_jobs refuses it
     s->users++;
     r->sh = s;
   }
@@ -527,7 +528,7 @@ void Engine::server_loop() {
         D.spp_begin = j.params.spp_begin;
         D.max_depth = j.params.max_depth;
         D.rr_depth = j.params.rr_depth;
-        D.flags = j.params.env_is ? 1u : 0u;
+        D.flags = (j.params.env_is ? 1u : 0u) | (j.params.nee ? 2u : 0u);
         D.scene = j.scene;
         D.seed = j.params.seed;
         for (int c = 0; c < 3; c++) {

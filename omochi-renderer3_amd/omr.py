@@ -57,7 +57,8 @@ class Camera(C.Structure):
 
 class RenderParams(C.Structure):
     _fields_ = [("supersamples", C.c_uint32), ("spp_begin", C.c_uint32), ("spp_end", C.c_uint32),
-                ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32), ("env_is", C.c_uint32), ("seed", C.c_uint64)]
+                ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32), ("env_is", C.c_uint32), ("seed", C.c_uint64),
+                ("nee", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class RenderStats(C.Structure):
@@ -397,8 +398,8 @@ def camera_lookat(width, height, pos, direction, up=(0.0, 1.0, 0.0), screen_heig
     return cam
 
 
-def params(ss=1, spp_begin=0, spp_end=1, max_depth=8, rr_depth=3, env_is=0, seed=0x6F6D7233):
-    return RenderParams(ss, spp_begin, spp_end, max_depth, rr_depth, env_is, seed)
+def params(ss=1, spp_begin=0, spp_end=1, max_depth=8, rr_depth=3, env_is=0, seed=0x6F6D7233, nee=0):
+    return RenderParams(ss, spp_begin, spp_end, max_depth, rr_depth, env_is, seed, nee, 0)
 
 
 class OmrError(RuntimeError):

@@ -885,8 +885,11 @@ static void trace_sample(const render_job *J, uint32_t pixel, uint32_t subpx, ui
       } else {
         prr = 1.0f;
       }
-      /* PathTracer.cpp:433-439 (and :410 on a roulette kill): += emission */
-      add_contrib(acc, beta, em, 1.0f);
+      /* PathTracer.cpp:433-439 (and :406-410 on a roulette kill): += emission,
+       * with next-event estimation only where the eye ray itself hits
+       * (depth 0); its direct term is always 0 (:312-314 multiply by the
+       * non-emissive surface's own emission) */
+      if (!p->nee || depth == 0) add_contrib(acc, beta, em, 1.0f);
       /* unlimited depth (reference mode) is capped at 255 segments */
       int can_continue = !killed && depth + 1 < (p->max_depth == 0 ? 255u : p->max_depth);
       prev_mis = 0;

@@ -78,6 +78,8 @@ typedef struct {
   uint32_t rr_depth;
   uint32_t env_is;
   uint64_t seed;
+  uint32_t nee;         /* the reference's next-event estimation mode (emission at depth 0 only) */
+  uint32_t pad;
 } orc_params;
 
 typedef struct orc_handle orc_handle;

@@ -34,7 +34,8 @@ class OCamera(C.Structure):
 
 class OParams(C.Structure):
     _fields_ = [("supersamples", C.c_uint32), ("spp_begin", C.c_uint32), ("spp_end", C.c_uint32),
-                ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32), ("env_is", C.c_uint32), ("seed", C.c_uint64)]
+                ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32), ("env_is", C.c_uint32), ("seed", C.c_uint64),
+                ("nee", C.c_uint32), ("pad", C.c_uint32)]
 
 
 _lib = None

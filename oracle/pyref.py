@@ -224,7 +224,8 @@ class RefCornell:
         self.lib.ref_cornell_objects.restype = sz
         self.lib.ref_cornell_trace.argtypes = [vp, vp, sz, vp, vp, vp, vp, vp, vp]
         self.lib.ref_cornell_trace.restype = None
-        self.lib.ref_cornell_render.argtypes = [vp, u32, u32, vp, vp, vp, C.c_double, C.c_double, u32, u32, vp, vp]
+        self.lib.ref_cornell_render.argtypes = [vp, u32, u32, vp, vp, vp, C.c_double, C.c_double, u32, u32, vp, vp,
+                                                C.c_int]
         self.h = self.lib.ref_cornell_new()
 
     def objects(self):
@@ -247,9 +248,10 @@ class RefCornell:
         self.lib.ref_cornell_trace(self.h, _p(rays), n, _p(t), _p(nrm), _p(mat), _p(nmat), _p(obj), _p(uv))
         return dict(t=t, normal=nrm, mat=mat, nmat=nmat, obj=obj, uv=uv)
 
-    def render(self, w, h, pos, direction, up, screen_h, dist, ss, batches):
+    def render(self, w, h, pos, direction, up, screen_h, dist, ss, batches, nee=False):
         mean = np.empty((h, w, 3), dtype=np.float64)
         var = np.empty((h, w, 3), dtype=np.float64)
         P, D, U = (np.asarray(v, dtype=np.float64) for v in (pos, direction, up))
-        self.lib.ref_cornell_render(self.h, w, h, _p(P), _p(D), _p(U), screen_h, dist, ss, batches, _p(mean), _p(var))
+        self.lib.ref_cornell_render(self.h, w, h, _p(P), _p(D), _p(U), screen_h, dist, ss, batches, _p(mean), _p(var),
+                                    1 if nee else 0)
         return mean, var

@@ -142,7 +142,7 @@ void trace_listed(S *s, const float *rays, size_t n, double *t, double *normal, 
 // Writes the per-pixel mean and variance of the batch means (top-down rows,
 // W x H x 3).
 int render_scene(const Scene &scene, uint32_t W, uint32_t H, const double *pos, const double *dir, const double *up,
-                 double screen_h, double dist, uint32_t ss, uint32_t batches, double *mean, double *var) {
+                 double screen_h, double dist, uint32_t ss, uint32_t batches, double *mean, double *var, int nee = 0) {
   Camera cam(W, H, Vector3(pos[0], pos[1], pos[2]), Vector3(dir[0], dir[1], dir[2]), Vector3(up[0], up[1], up[2]),
              screen_h, dist, 200.0);
   std::vector<double> prev((size_t)W * H * 3, 0.0), m1((size_t)W * H * 3, 0.0), m2((size_t)W * H * 3, 0.0);
@@ -166,6 +166,7 @@ int render_scene(const Scene &scene, uint32_t W, uint32_t H, const double *pos, 
   // which aborts under glibc.
   PathTracer *pt = new PathTracer(cam, 1, (int)batches, 1, (int)ss, cb);
   pt->SetMaxThreadCount(1);
+  pt->EnableNextEventEstimation(nee != 0);
   pt->RenderScene(scene);
   for (size_t i = 0; i < m1.size(); i++) {
     double mu = m1[i] / batches;
@@ -235,9 +236,11 @@ void ref_cornell_trace(void *sp, const float *rays, size_t n, double *t, double 
                        int64_t *obj, double *uv) {
   trace_listed((CornellScene *)sp, rays, n, t, normal, mats, nmat, obj, uv);
 }
+// nee: PathTracer::EnableNextEventEstimation (PathTracer.h:30-33)
 int ref_cornell_render(void *sp, uint32_t W, uint32_t H, const double *pos, const double *dir, const double *up,
-                       double screen_h, double dist, uint32_t ss, uint32_t batches, double *mean, double *var) {
-  return render_scene(*(CornellScene *)sp, W, H, pos, dir, up, screen_h, dist, ss, batches, mean, var);
+                       double screen_h, double dist, uint32_t ss, uint32_t batches, double *mean, double *var,
+                       int nee) {
+  return render_scene(*(CornellScene *)sp, W, H, pos, dir, up, screen_h, dist, ss, batches, mean, var, nee);
 }
 
 // HDRImage::ReadFromRadianceFile: dims into wh (w, h); pixels (h x w x 3,

@@ -194,7 +194,11 @@ def cornell_ref():
     c = gi.CORNELL_RENDER
     mean, var = r.render(c["w"], c["h"], c["pos"], c["dir"], (0, 1, 0), c["screen_h"], c["dist"], c["ss"],
                          c["batches"])
+    nmean, nvar = r.render(c["w"], c["h"], c["pos"], c["dir"], (0, 1, 0), c["screen_h"], c["dist"], c["ss"],
+                           c["batches"], nee=True)
     r.close()
+    np.savez_compressed(os.path.join(HERE, "ref_render_cornell_nee.npz"), mean=nmean.astype(np.float32),
+                        var=nvar.astype(np.float32), **{k: np.asarray(v) for k, v in c.items()})
     np.savez_compressed(os.path.join(HERE, "hits_cornell_ref.npz"), rays_sha256=sha(rays), n_objects=nobj,
                         t=res["t"], obj=res["obj"].astype(np.int16), nmat=res["nmat"].astype(np.int8),
                         mat=res["mat"][:, :2], normal=res["normal"].astype(np.float32))

@@ -249,3 +249,37 @@ def test_render_lambert_sphere_under_reference_hdr_bit_exact(ctx, env_is):
     assert not (g != c_).any(axis=2).any(), f"{(g != c_).any(axis=2).sum()} pixels differ"
     assert st["segments"] == seg and st["shadow_rays"] == sh
     ctx.release(sid)
+
+
+def test_render_next_event_estimation_mode_bit_exact(ctx):
+    """The reference's next-event estimation mode (emission counted where the
+    camera ray hits only) on its CornellBoxScene: GPU equals the restatement;
+    a scene whose mirror material emits is refused in that mode."""
+    import os
+
+    import omr
+    import pyoracle
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes")
+    cwd = os.getcwd()
+    os.chdir(here)
+    try:
+        scene = omr.load_scene_file("cornell_ref.scene")
+    finally:
+        os.chdir(cwd)
+    cam = omr.camera_lookat(64, 36, (50.0, 55.0, 230.0), (0.00001, -0.15, -1.0), screen_height=30.0, dist=40.0)
+    sid = ctx.upload(scene)
+    prm = omr.params(ss=2, spp_end=2, max_depth=0, rr_depth=5, seed=4, nee=1)
+    print("render cornell_ref nee", flush=True)
+    g = ctx.render_sums(sid, cam, prm)
+    c, seg, _ = _oracle(scene, ctx, sid).render_sums(cam, prm, pyoracle.WIDE_BVH)
+    assert not (g != c).any(axis=2).any()
+    assert ctx.last_stats()["segments"] == seg
+    ctx.release(sid)
+    mats = scene.mats.copy()
+    mats["emission"][mats["type"] == 1] = (1.0, 1.0, 1.0)
+    bad = omr.Scene(scene.tri, scene.tri_n, scene.sph, scene.prim_mat, mats, prim_mat_count=scene.prim_mat_count)
+    sid = ctx.upload(bad)
+    with pytest.raises(omr.OmrError):
+        ctx.render_sums(sid, cam, prm)
+    ctx.release(sid)
