@@ -60,6 +60,98 @@ Vector3 Polygon::CalculateNormal(const Vector3 &v0, const Vector3 &v1, const Vec
   return n;
 }
 
+bool Polygon::CheckIntersection(const Ray &ray, HitInformation &hit) const {
+  const Vector3 &e1 = m_posAndEdges[1], &e2 = m_posAndEdges[2];
+  const Vector3 P(ray.dir.cross(e2));
+  const double det = P.dot(e1);
+  if (!(det > EPS)) return false;  // single-sided
+  const Vector3 T(ray.orig - (m_posAndEdges[0] + position_));
+  const double u = P.dot(T);
+  if (!(u >= 0 && u <= det)) return false;
+  const Vector3 Q(T.cross(e1));
+  const double v = Q.dot(ray.dir);
+  if (!(v >= 0 && u + v <= det)) return false;
+  const double t = Q.dot(e2) / det;
+  if (!(t >= EPS)) return false;
+  const double ur = u / det, vr = v / det;
+  hit.distance = t;
+  hit.position = ray.orig + ray.dir * t;
+  hit.normal = m_normalAndDiffs[1] * ur + m_normalAndDiffs[2] * vr + m_normalAndDiffs[0];
+  hit.normal.normalize();
+  hit.uv = m_uvOrigAndEdges[1] * ur + m_uvOrigAndEdges[2] * vr + m_uvOrigAndEdges[0];
+  return true;
+}
+
+bool Sphere::CheckIntersection(const Ray &ray, HitInformation &hit) const {
+  const Vector3 &x = ray.orig, &v = ray.dir, &c = position_;
+  const Vector3 cx = c - x;
+  const double b = v.dot(cx);
+  const double D = b * b - v.lengthSq() * (cx.lengthSq() - m_radius * m_radius);
+  if (D < 0) return false;
+  const double sD = std::sqrt(D), t1 = sD + b, t2 = -sD + b;
+  if (t1 <= EPS && t2 <= EPS) return false;
+  hit.distance = t2 > EPS ? t2 : t1;
+  hit.position = x + v * hit.distance;
+  hit.normal = hit.position - position_;
+  hit.normal.normalize();
+  bool textured = false;
+  for (const auto &m : materials_) textured = textured || m.material_.texture_id != ImageHandler::INVALID_IMAGE_ID;
+  if (textured) {  // latitude-longitude uv, origin at the top left of the image
+    double tv = (std::asin(hit.normal.y) + PI * 0.5) / PI;
+    while (tv > 1) tv -= 1;
+    while (tv < 0) tv += 1;
+    double tu = (std::atan2(hit.normal.z, hit.normal.x) + PI) / (2 * PI);
+    while (tu > 1) tu -= 1;
+    while (tu < 0) tu += 1;
+    hit.uv.x = tu;
+    hit.uv.y = 1 - tv;
+  }
+  return true;
+}
+
+void SphereLight::SampleOnePoint(Vector3 &point, Vector3 &normal, double &pdf, const Random &rnd) const {
+  pdf = 1.0 / (4 * PI * m_radius * m_radius);
+  const double phi = 2 * PI * rnd.nextDouble();
+  const double ct = 1 - 2 * rnd.nextDouble(), st = std::sqrt(1 - ct * ct);
+  const Vector3 dir(st * std::cos(phi), ct, st * std::sin(phi));
+  point = position_ + dir * (m_radius + EPS);
+  normal = dir;
+}
+
+bool SphereLight::SampleOnePointWithTargetPoint(Vector3 &sampledPoint, Vector3 &sampledPointNormal, double &pdf,
+                                                const Vector3 &targetPoint, const Random &rnd) const {
+  Vector3 axis = targetPoint - position_;
+  const double dist = axis.length();
+  const double max_ct = m_radius / dist;
+  if (max_ct > 1) return false;  // the target is inside the sphere
+  axis /= dist;
+  Vector3 a1, a2;
+  Utils::GetCrossAxes(axis, a1, a2);
+  pdf = 1.0 / (2 * PI * m_radius * m_radius * (1 - max_ct));
+  const double phi = 2 * PI * rnd.nextDouble();
+  const double ct = 1 - (1 - max_ct) * rnd.nextDouble(), st = std::sqrt(1 - ct * ct);
+  const Vector3 dir = axis * ct + a1 * st * std::sin(phi) + a2 * st * std::cos(phi);
+  sampledPoint = position_ + dir * (m_radius + EPS);
+  sampledPointNormal = dir;
+  return true;
+}
+
+double SphereLight::TotalPower() const {
+  double e = 0;
+  for (const auto &m : materials_) e += m.material_.emission.length() * m.rate_;
+  return 4 * PI * m_radius * m_radius * e;
+}
+
+bool InfiniteFloor::CheckIntersection(const Ray &ray, HitInformation &hit) const {
+  if (std::fabs(ray.dir.y) < EPS) return false;
+  const double t = (m_y - ray.orig.y) / ray.dir.y;
+  if (t < EPS) return false;
+  hit.distance = t;
+  hit.position = ray.orig + ray.dir * t;
+  hit.normal = Vector3(0, ray.dir.y < 0 ? 1 : -1, 0);
+  return true;
+}
+
 Camera::Camera(size_t w, size_t h, const Vector3 &pos, const Vector3 &dir, const Vector3 &up, double screenH,
                double dist, double focal)
     : m_width(w), m_height(h), m_camPos(pos), m_camDir(dir), m_camUp(up), m_nearScreenHeight(screenH),
@@ -77,6 +169,21 @@ Camera::Camera(size_t w, size_t h, const Vector3 &pos, const Vector3 &dir, const
   m_screenYaxis *= m_nearScreenHeight;
 }
 
+Ray Camera::SampleRayForPixel(double x, double y, const Random &rnd) const {
+  Vector3 target = m_screenCenter + m_screenXaxis * (x / m_width - 0.5) + m_screenYaxis * (y / m_height - 0.5);
+  Vector3 dir = target - m_camPos;
+  dir.normalize();
+  Ray ray(m_camPos, dir);
+  if (m_aperture) {
+    double dx, dy;
+    m_aperture->SampleOnePoint(dx, dy, rnd);
+    const Vector3 focus = ray(m_focalDistance / std::fabs(dir.z));  // focal plane of the centre ray
+    ray.orig = m_camPos + Vector3(dx, dy, 0);
+    ray.dir = (focus - ray.orig).normalize();
+  }
+  return ray;
+}
+
 omr_camera Camera::ToDevice() const {
   omr_camera c;
   std::memset(&c, 0, sizeof(c));
@@ -89,7 +196,7 @@ omr_camera Camera::ToDevice() const {
     dst[i][1] = (float)v[i]->y;
     dst[i][2] = (float)v[i]->z;
   }
-  c.aperture_radius = m_aperture ? (float)m_aperture->Radius() : 0.0f;
+  c.aperture_radius = m_aperture ? (m_aperture->Radius() < 0 ? -1.0f : (float)m_aperture->Radius()) : 0.0f;
   c.focal_distance = (float)m_focalDistance;
   return c;
 }
@@ -102,6 +209,7 @@ Scene::~Scene() {
 
 void Scene::AddObject(SceneObject *obj, bool doDelete, bool) {
   m_objects.push_back({obj, doDelete});
+  if (LightBase *l = dynamic_cast<LightBase *>(obj)) m_lights.push_back(l);
   if (m_ctx && m_id >= 0) {  // contents changed: re-upload on next use
     omr_scene_release(m_ctx, m_id);
     m_id = -1;
@@ -252,19 +360,36 @@ omr_scene_id Scene::Upload(omr_ctx *ctx) const {
 }
 
 bool Scene::CheckIntersection(const Ray &ray, IntersectionInformation &info) const {
-  info.distance = (double)0x1.fffffep+126f;
+  info.hit = HitInformation();
+  info.hit.distance = INF;
   info.object = nullptr;
   omr_ctx *c = m_ctx ? m_ctx : DefaultContext();
-  if (!c || Upload(c) < 0) return false;
-  float r[8] = {(float)ray.orig.x, (float)ray.orig.y, (float)ray.orig.z, 0x1.fffffep+126f,
-                (float)ray.dir.x, (float)ray.dir.y, (float)ray.dir.z, 0.0f};
-  int32_t prim = -1;
-  float t = 0;
-  if (omr_trace_closest(c, m_id, r, 1, &prim, &t) != OMR_OK || prim < 0) return false;
-  info.distance = t;
-  info.position = ray(t);
-  info.object = m_prims[(size_t)prim];
-  return true;
+  if (c && Upload(c) >= 0) {
+    float r[8] = {(float)ray.orig.x, (float)ray.orig.y, (float)ray.orig.z, 0x1.fffffep+126f,
+                  (float)ray.dir.x, (float)ray.dir.y, (float)ray.dir.z, 0.0f};
+    int32_t prim = -1;
+    float t = 0;
+    if (omr_trace_closest(c, m_id, r, 1, &prim, &t) == OMR_OK) {
+      if (prim < 0) return false;
+      // the object the GPU found computes its own FP64 hit record
+      SceneObject *o = m_prims[(size_t)prim];
+      if (o->CheckIntersection(ray, info.hit)) {
+        info.object = o;
+        return true;
+      }
+    }
+  }
+  // no device, or a hit the FP64 test rejects (a grazing binary32 hit): test
+  // every object in FP64, the first of equal distances winning as in
+  // Scene.cpp:51-85
+  for (const auto &ow : m_objects) {
+    HitInformation h;
+    if (ow.obj->GetMaterialCount() && ow.obj->CheckIntersection(ray, h) && h.distance < info.hit.distance) {
+      info.hit = h;
+      info.object = ow.obj;
+    }
+  }
+  return info.object != nullptr;
 }
 
 // Floor helpers: two polygons per quad, same vertices, uvs and normals as
@@ -315,12 +440,13 @@ void PathTracer::RenderScene(const Scene &scene) {
   m_error.clear();
   const size_t W = m_camera.GetScreenWidth(), H = m_camera.GetScreenHeight();
   m_result.reset(new ScreenPixels());
-  m_result->InitializePixels(W, H);
+  m_result->InitializePixels(W, H, 2, 1);  // PathTracer.cpp:62-63
   m_ctx = DefaultContext();
   if (!m_ctx) { m_error = "no GPU context"; return; }
   omr_scene_id id = scene.Upload(m_ctx);
   if (id < 0) { m_error = omr_last_error(m_ctx); return; }
   omr_camera cam = m_camera.ToDevice();
+  if (cam.aperture_radius < 0) { m_error = "only circular apertures (CircleAperture) render on the GPU"; return; }
   std::vector<float> batch(W * H * 3);
   m_previousSamples = 0;
   double minutes = 0;
@@ -333,7 +459,8 @@ void PathTracer::RenderScene(const Scene &scene) {
     p.spp_end = (uint32_t)m_currentSamples;
     p.max_depth = m_maxDepth;
     p.rr_depth = m_rrDepth;
-    p.env_is = m_nee ? 1u : 0u;
+    p.env_is = m_envIS ? 1u : 0u;
+    p.nee = m_nee ? 1u : 0u;
     p.seed = m_seed;
     if (omr_render(m_ctx, id, &cam, &p, batch.data()) != OMR_OK) {
       m_error = omr_last_error(m_ctx);

@@ -8,7 +8,11 @@
 //   Material               src/renderer/Material.h:11-36
 //   SceneObject            src/renderer/SceneObject.h:17-95 (multi-material rates)
 //   Sphere, Polygon        src/renderer/Sphere.h:11-82, src/renderer/Polygon.h:11-125
-//   SphereLight            src/renderer/SphereLight.h:8-79 (an emissive sphere; NEE is off)
+//   SphereLight, LightBase src/renderer/SphereLight.h:8-79, src/renderer/LightBase.h (light sampling)
+//   HitInformation         src/renderer/HitInformation.h:7-20
+//   Random                 src/tools/Random.h:9-29
+//   Utils::GetCrossAxes    src/tools/Utils.h:123-131
+//   INF, PI, EPS, ...      src/tools/Constant.h:4-8
 //   Scene                  src/scenes/Scene.h:18-124 (AddObject, AddFloor*, ConstructQBVH)
 //   Camera, CircleAperture src/renderer/Camera.h:10-117, src/renderer/Aperture.h:9-39
 //   ScreenPixels           src/renderer/ScreenPixels.hpp / .cpp (2W x H, row-flipped)
@@ -24,6 +28,8 @@
 // device closest-hit kernel for single rays (slow; for compatibility).
 #pragma once
 #include <atomic>
+#include <cfloat>
+#include <climits>
 #include <cmath>
 #include <functional>
 #include <map>
@@ -32,6 +38,23 @@
 #include <vector>
 
 #include "../../include/omr_capi.h"
+
+// Constants of the reference (src/tools/Constant.h:4-8), as macros as there.
+#ifndef INF
+#define INF ((FLT_MAX - 1) / 2)
+#endif
+#ifndef PI
+#define PI 3.14159265358979323846264338327950288419716939
+#endif
+#ifndef EPS
+#define EPS 0.0005
+#endif
+#ifndef REFRACTIVE_INDEX_VACUUM
+#define REFRACTIVE_INDEX_VACUUM 1.0
+#endif
+#ifndef REFRACTIVE_INDEX_OBJECT
+#define REFRACTIVE_INDEX_OBJECT 1.5
+#endif
 
 namespace OmochiRenderer {
 
@@ -64,6 +87,46 @@ class Ray {
   Ray(const Vector3 &o, const Vector3 &d) : orig(o), dir(d) {}
   Vector3 operator()(double t) const { return orig + dir * t; }
   Vector3 orig, dir;
+};
+
+// Random (src/tools/Random.h:9-29): xorshift128 with the reference's seeding
+// recurrence; nextDouble() in [0, 1].
+class Random {
+ public:
+  explicit Random(unsigned int seed) {
+    unsigned int s = seed;
+    for (unsigned int i = 1; i <= 4; i++) m_seed[i - 1] = s = 1812433253U * (s ^ (s >> 30)) + i;
+  }
+  unsigned int next() const {
+    const unsigned int t = m_seed[0] ^ (m_seed[0] << 11);
+    m_seed[0] = m_seed[1];
+    m_seed[1] = m_seed[2];
+    m_seed[2] = m_seed[3];
+    return m_seed[3] = (m_seed[3] ^ (m_seed[3] >> 19)) ^ (t ^ (t >> 8));
+  }
+  double nextDouble() const { return static_cast<double>(next()) / UINT_MAX; }
+
+ private:
+  mutable unsigned int m_seed[4];
+};
+
+// Utils::GetCrossAxes (src/tools/Utils.h:123-131): the two axes the
+// reference builds around a normal (unnormalized, as there).
+class Utils {
+ public:
+  static void GetCrossAxes(const Vector3 &up, Vector3 &axis1, Vector3 &axis2) {
+    axis1 = std::fabs(up.x) > EPS ? Vector3(0, 1, 0).cross(up) : Vector3(1, 0, 0).cross(up);
+    axis2 = up.cross(axis1);
+  }
+};
+
+// HitInformation (src/renderer/HitInformation.h:7-20).
+class HitInformation {
+ public:
+  double distance = 0;
+  Vector3 position;
+  Vector3 normal;
+  Vector3 uv;
 };
 
 // Texture registry (ImageHandler, src/tools/ImageHandler.h): file name -> id.
@@ -150,6 +213,9 @@ class SceneObject {
   explicit SceneObject(const Material &m) { AddMaterial(m, 1.0, false); }
   SceneObject() {}
   virtual ~SceneObject() {}
+  // SceneObject::CheckIntersection (src/renderer/SceneObject.h:46), in FP64
+  // on the host as the reference computes it; rendering uses the GPU path.
+  virtual bool CheckIntersection(const Ray &ray, HitInformation &hit) const = 0;
   // SceneObject::AddMaterial (src/renderer/SceneObject.h:48-67): rates renormalized
   void AddMaterial(const Material &m, double rate, bool doNormalizeRate = true);
   size_t GetMaterialCount() const { return materials_.size(); }
@@ -167,20 +233,35 @@ class Sphere : public SceneObject {
  public:
   Sphere(double r, const Vector3 &pos, const Material &m) : SceneObject(m), m_radius(r) { position_ = pos; }
   double GetRadius() const { return m_radius; }
+  // Sphere::CheckIntersection (src/renderer/Sphere.h:21-78), uv when textured
+  bool CheckIntersection(const Ray &ray, HitInformation &hit) const override;
 
  protected:
   double m_radius;
 };
 
+// LightBase (src/renderer/LightBase.h): light sampling for next-event
+// estimation.
 class LightBase {
  public:
   virtual ~LightBase() {}
+  virtual void SampleOnePoint(Vector3 &point, Vector3 &normal, double &pdf, const Random &rnd) const = 0;
+  virtual bool SampleOnePointWithTargetPoint(Vector3 &sampledPoint, Vector3 &sampledPointNormal, double &pdf,
+                                             const Vector3 &targetPoint, const Random &rnd) const = 0;
+  virtual double TotalPower() const = 0;
 };
-// With next-event estimation off (the reference default) a sphere light is an
-// emissive sphere.
+// SphereLight (src/renderer/SphereLight.h:8-79): an emissive sphere the
+// renderer sees as a light.
 class SphereLight : public Sphere, public LightBase {
  public:
   SphereLight(double r, const Vector3 &pos, const Material &m) : Sphere(r, pos, m) {}
+  // uniform point on the sphere (:17-35)
+  void SampleOnePoint(Vector3 &point, Vector3 &normal, double &pdf, const Random &rnd) const override;
+  // uniform over the cone of the sphere seen from targetPoint (:38-66)
+  bool SampleOnePointWithTargetPoint(Vector3 &sampledPoint, Vector3 &sampledPointNormal, double &pdf,
+                                     const Vector3 &targetPoint, const Random &rnd) const override;
+  // 4 pi r^2 x the rate-weighted emission length (:68-75)
+  double TotalPower() const override;
 };
 
 class Polygon : public SceneObject {
@@ -196,6 +277,8 @@ class Polygon : public SceneObject {
   const Vector3 *NormalAndDiffs() const { return m_normalAndDiffs; }
   // Polygon::CalculateNormal (src/renderer/Polygon.h:45-49)
   static Vector3 CalculateNormal(const Vector3 &v0, const Vector3 &v1, const Vector3 &v2);
+  // Polygon::CheckIntersection (src/renderer/Polygon.h:63-106): single-sided
+  bool CheckIntersection(const Ray &ray, HitInformation &hit) const override;
 
  private:
   Vector3 m_posAndEdges[3];
@@ -203,14 +286,25 @@ class Polygon : public SceneObject {
   Vector3 m_normalAndDiffs[3];
 };
 
+// Aperture (src/renderer/Aperture.h:9-13).  The GPU path renders circular
+// apertures: Radius() is the circle's radius, or negative for an aperture it
+// cannot render (PathTracer then reports an error).
 class Aperture {
  public:
   virtual ~Aperture() {}
-  virtual double Radius() const = 0;
+  virtual void SampleOnePoint(double &sampledX, double &sampledY, const Random &rnd) const = 0;
+  virtual double Radius() const { return -1.0; }
 };
+// CircleAperture (src/renderer/Aperture.h:16-37): uniform on the disc.
 class CircleAperture : public Aperture {
  public:
   explicit CircleAperture(double radius = 0.0) : m_radius(radius) {}
+  void SampleOnePoint(double &sampledX, double &sampledY, const Random &rnd) const override {
+    const double theta = 2 * PI * rnd.nextDouble();
+    const double r = m_radius * std::sqrt(rnd.nextDouble());
+    sampledX = r * std::cos(theta);
+    sampledY = r * std::sin(theta);
+  }
   double Radius() const override { return m_radius; }
 
  private:
@@ -229,6 +323,10 @@ class Camera {
   const Vector3 &GetScreenCenterPosition() const { return m_screenCenter; }
   const Vector3 &GetScreenAxisXvector() const { return m_screenXaxis; }
   const Vector3 &GetScreenAxisYvector() const { return m_screenYaxis; }
+  // Camera::SampleRayForPixel (src/renderer/Camera.h:65-93): the ray through
+  // screen point (x, y), through a lens point when an aperture is set
+  Ray SampleRayForPixel(double x, double y, const Random &rnd) const;
+  // the device form; aperture_radius < 0 when the aperture is not circular
   omr_camera ToDevice() const;
 
  private:
@@ -245,6 +343,7 @@ class InfiniteFloor : public SceneObject {
  public:
   InfiniteFloor(double y, const Material &m) : SceneObject(m), m_y(y) {}
   double GetY() const { return m_y; }
+  bool CheckIntersection(const Ray &ray, HitInformation &hit) const override;
 
  private:
   double m_y;
@@ -303,16 +402,22 @@ struct SceneArrays {
 
 class Scene {
  public:
+  // Scene::IntersectionInformation (src/scenes/Scene.h:20-24)
   struct IntersectionInformation {
-    double distance = 0;
-    Vector3 position, normal;
+    HitInformation hit;
     SceneObject *object = nullptr;
+    Color texturedHitpointColor;
   };
   virtual ~Scene();
   void ConstructBVH() { ConstructQBVH(); }
   // Builds the wide BVH and keeps the scene resident on the GPU.
   void ConstructQBVH();
+  // Scene::CheckIntersection (src/scenes/Scene.cpp:51-85): the closest object
+  // (found by the GPU traversal), its hit record computed by the object in
+  // FP64 as the reference does.  Slow per ray; rendering does not use it.
   bool CheckIntersection(const Ray &ray, IntersectionInformation &info) const;
+  // Scene::GetLights (src/scenes/Scene.h:36): the objects added that are lights
+  const std::vector<LightBase *> GetLights() const { return m_lights; }
   const IBL *GetIBL() const { return m_ibl.get(); }
   virtual Color Background() const { return Color(0, 0, 0); }
   virtual bool IsValid() const { return true; }
@@ -344,6 +449,7 @@ class Scene {
     bool doDelete;
   };
   std::vector<Owned> m_objects;
+  std::vector<LightBase *> m_lights;
   std::vector<std::unique_ptr<Model>> m_models;
   std::unique_ptr<IBL> m_ibl;
   mutable omr_ctx *m_ctx = nullptr;
@@ -400,22 +506,56 @@ class SceneFromExternalFile : public Scene {
   std::string m_baseDir, m_iblFileName, m_spacePartitioningMethod;
 };
 
+// ScreenPixels (src/renderer/ScreenPixels.hpp:18-78, ScreenPixels.cpp:37-130):
+// xCount x yCount viewports of w x h pixels in one buffer; viewport (x, y)
+// coordinates map to buffer index X + (Height - Y - 1) x Width with X, Y the
+// screen coordinates, so rows are stored bottom-up.  PathTracer fills
+// viewport (0, 0) of a 2 x 1 layout, as the reference does.
 class ScreenPixels {
  public:
-  // InitializePixels(w, h, 2, 1) layout: 2W x H, row y stored at (H-y-1)
-  void InitializePixels(size_t w, size_t h) {
-    m_w = w;
-    m_h = h;
-    m_pixels.assign(2 * w * h, Color());
+  class ScreenViewport {
+   public:
+    void SetColor(size_t x, size_t y, const Color &color) { m_screen->m_pixels[calcIndex(x, y)] = color; }
+    const Color &GetColor(size_t x, size_t y) const { return m_screen->m_pixels[calcIndex(x, y)]; }
+
+   private:
+    friend class ScreenPixels;
+    size_t calcIndex(size_t x, size_t y) const {
+      const size_t W = m_screen->CalcScreenWidth(), H = m_screen->CalcScreenHeight();
+      return (x + m_xOffset) + (H - (y + m_yOffset) - 1) * W;
+    }
+    size_t m_xOffset = 0, m_yOffset = 0;
+    ScreenPixels *m_screen = nullptr;
+  };
+  void InitializePixels(size_t widthPerViewport, size_t heightPerViewport, size_t xViewportCount,
+                        size_t yViewportCount) {
+    m_w = widthPerViewport;
+    m_h = heightPerViewport;
+    m_xCount = xViewportCount;
+    m_yCount = yViewportCount;
+    m_pixels.assign(m_w * m_h * m_xCount * m_yCount, Color());
+    m_views.assign(m_xCount * m_yCount, ScreenViewport());
+    for (size_t y = 0; y < m_yCount; y++)
+      for (size_t x = 0; x < m_xCount; x++) {
+        ScreenViewport &v = m_views[x + y * m_xCount];
+        v.m_screen = this;
+        v.m_xOffset = x * m_w;
+        v.m_yOffset = y * m_h;
+      }
   }
-  size_t CalcScreenWidth() const { return 2 * m_w; }
-  size_t CalcScreenHeight() const { return m_h; }
+  ScreenViewport *GetViewport(size_t xViewport, size_t yViewport) {
+    if (xViewport >= m_xCount || yViewport >= m_yCount) return nullptr;
+    return &m_views[xViewport + yViewport * m_xCount];
+  }
+  size_t CalcScreenWidth() const { return m_xCount * m_w; }
+  size_t CalcScreenHeight() const { return m_yCount * m_h; }
   const Color *GetScreen() const { return m_pixels.data(); }
   Color *GetScreen() { return m_pixels.data(); }
 
  private:
-  size_t m_w = 0, m_h = 0;
+  size_t m_w = 0, m_h = 0, m_xCount = 0, m_yCount = 0;
   std::vector<Color> m_pixels;
+  std::vector<ScreenViewport> m_views;
 };
 
 class Renderer {
@@ -441,9 +581,13 @@ class PathTracer : public Renderer {
   PathTracer(const Camera &camera, int min_samples, int max_samples, int step, int supersamples,
              RenderingFinishCallbackFunction callback);
   virtual ~PathTracer();
-  // Next-event estimation is off in the reference's working path; enabling it
-  // turns on environment importance sampling (shadow rays) here.
+  // PathTracer::EnableNextEventEstimation (PathTracer.h:30-33), with the
+  // reference's semantics (PathTracer.cpp:406-439): surface emission counts
+  // only where the camera ray hits, and the direct-light term is zero.
   void EnableNextEventEstimation(bool enable = true) { m_nee = enable; }
+  // Environment importance sampling with shadow rays and MIS (no reference
+  // counterpart: the reference's IBL importance map is never built).
+  void EnableEnvironmentImportanceSampling(bool enable = true) { m_envIS = enable; }
   void SetMaxThreadCount(size_t) {}  // threads are GPU lanes
   // Depth parameters (reference constants MinDepth = 5, MaxDepth = 64).
   void SetDepth(uint32_t rr_depth, uint32_t max_depth) { m_rrDepth = rr_depth; m_maxDepth = max_depth; }
@@ -461,7 +605,7 @@ class PathTracer : public Renderer {
   int m_currentSamples = 0, m_previousSamples = 0;
   RenderingFinishCallbackFunction m_callback;
   std::unique_ptr<ScreenPixels> m_result;
-  bool m_nee = false;
+  bool m_nee = false, m_envIS = false;
   uint32_t m_rrDepth = 5, m_maxDepth = 0;
   uint64_t m_seed = 0x6F6D7233ull;
   uint64_t m_segments = 0;

@@ -62,6 +62,6 @@ int main(int argc, char **argv) {
   Scene::IntersectionInformation info;
   bool hit = scene.CheckIntersection(Ray(Vector3(50, 40, 200), Vector3(0, 0, -1)), info);
   std::printf("{\"batches\": %d, \"mean\": %.6f, \"probe_hit\": %d, \"probe_t\": %.4f}\n", calls, last_mean,
-              hit ? 1 : 0, info.distance);
+              hit ? 1 : 0, info.hit.distance);
   return (calls == batches && std::isfinite(last_mean) && last_mean > 0 && hit) ? 0 : 2;
 }

@@ -24,3 +24,53 @@ def test_dropin_cornell_box_renders():
     # the probe ray from (50,40,200) along -z meets the back wall plane z = 0
     assert abs(summary["probe_t"] - 200.0) < 1e-3 or summary["probe_t"] < 200.0
     assert 0.01 < summary["mean"] < 50.0
+
+
+COMPAT = os.path.join(ROOT, "oracle", "_ref", "dropin_compat")
+
+
+@pytest.mark.skipif(not os.path.exists(COMPAT), reason="dropin_compat builds only beside the reference checkout")
+def test_reference_scene_code_renders_on_gpu():
+    """The reference's own CornellBoxScene.cpp, built against the drop-in,
+    rendered through PathTracer on the GPU: both sample batches reach the
+    finish callback, and the result holds two camera-sized viewports side by
+    side, as the reference's PathTracer sets up its ScreenPixels
+    (PathTracer.cpp:62-63, ScreenPixels.hpp:57)."""
+    print("running", COMPAT, "render", flush=True)
+    p = subprocess.run([COMPAT, "render"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert p.stdout.split() == ["render", "2", "128", "36"]
+
+
+@pytest.mark.skipif(not os.path.exists(COMPAT), reason="dropin_compat builds only beside the reference checkout")
+def test_reference_scene_code_gpu_hits(tmp_path):
+    """Scene::CheckIntersection on the GPU (closest primitive in binary32,
+    then the object's own FP64 hit record) over the reference's scene code
+    against the reference's hits on 20,000 rays: the same object wherever
+    the reference's closest hit is not within 1e-4 of another surface, and
+    the reference's FP64 distance on those rays."""
+    import numpy as np
+
+    import golden_inputs as gi
+    import omr
+
+    here = os.getcwd()
+    os.chdir(os.path.join(ROOT, "tests", "golden", "scenes"))
+    try:
+        scene = omr.load_scene_file("cornell_ref.scene")
+    finally:
+        os.chdir(here)
+    rays = gi.cornell_rays(scene)
+    rays.astype(np.float32).tofile(tmp_path / "rays.bin")
+    print("running", COMPAT, "hits", rays.shape, flush=True)
+    p = subprocess.run([COMPAT, str(tmp_path / "rays.bin"), str(tmp_path / "hits.bin")], capture_output=True,
+                       text=True, timeout=300)
+    assert p.returncode == 0, p.stdout + p.stderr
+    rec = np.fromfile(tmp_path / "hits.bin", np.float64).reshape(-1, 9)
+    g = np.load(os.path.join(ROOT, "tests", "golden", "hits_cornell_ref.npz"))
+    hit = g["obj"] >= 0
+    same = np.where(hit, np.isclose(rec[:, 0], g["t"], rtol=1e-12, atol=0), rec[:, 0] < 0)
+    assert same.sum() > 0.999 * len(same), f"{int((~same).sum())} of {len(same)} rays differ"
+    both = same & hit
+    assert both.sum() > 0.5 * len(same)
+    assert np.allclose(rec[both, 1:4], g["normal"][both], rtol=0, atol=1e-6)
