@@ -324,6 +324,23 @@ def cpu_sample(items, seconds, threads):
     return rays / t_cpu / 1e6, t_cpu, done_rows, len(used), float(np.sqrt(sq / max(1, n_px)))
 
 
+def cpu_calibration():
+    """The restatement's speed over the reference's compiled PathTracer on
+    one thread (oracle/cpu_calibration.json, written where the reference
+    checkout is present): the geometric mean over its workloads, the per
+    workload ratios and the CPU they were measured on."""
+    path = os.path.join(ROOT, "oracle", "cpu_calibration.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        doc = json.load(f)
+    per = {r["workload"]: r["restatement_over_reference_bvh"]["t1"] for r in doc["results"]}
+    gm = float(np.exp(np.mean(np.log(list(per.values())))))
+    return {"restatement_over_reference": round(gm, 3), "per_workload_1_thread": per, "measured_on": doc["cpu_model"],
+            "source": "oracle/cpu_calibration.json (oracle/calibrate_cpu.py: the reference's PathTracer compiled "
+                      "from its sources, binary BVH, vs the restatement, same scenes, cameras and sample counts)"}
+
+
 # ---------------------------------------------------------------- workloads
 def run_batched(ctx, omr, args):
     kinds = [0, 1, 2, 3]
@@ -537,6 +554,10 @@ def main():
             "cpu_model": model, "host_logical_cpus": logical, "usable_cpus": usable,
             "rmse_vs_gpu": rmse,
         }
+        cal = cpu_calibration()
+        if cal:
+            out["cpu_baseline"]["calibration"] = cal
+            out["cpu_baseline"]["reference_equivalent_estimate"] = round(mrays / cal["restatement_over_reference"], 3)
     print(json.dumps(out), flush=True)
     ctx.close()
     if args.calib_copy > 0:

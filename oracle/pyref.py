@@ -65,12 +65,35 @@ class RefScene:
         self.lib.ref_trace(self.h, _p(rays), rays.shape[0], _p(prim), _p(t))
         return prim, t
 
-    def render(self, w, h, pos, direction, up, screen_h, dist, ss, batches):
-        mean = np.empty((h, w, 3), dtype=np.float64)
-        var = np.empty((h, w, 3), dtype=np.float64)
-        P, D, U = (np.asarray(v, dtype=np.float64) for v in (pos, direction, up))
-        self.lib.ref_render(self.h, w, h, _p(P), _p(D), _p(U), screen_h, dist, ss, batches, _p(mean), _p(var))
-        return mean, var
+    def render(self, w, h, pos, direction, up, screen_h, dist, ss, batches, threads=1):
+        return _render(self.lib, self.lib.ref_render, self.h, w, h, pos, direction, up, screen_h, dist, ss, batches,
+                       threads)
+
+
+def _render(lib, fn, handle, w, h, pos, direction, up, screen_h, dist, ss, batches, threads):
+    """PathTracer::RenderScene through a harness entry: (mean, var) of the
+    batch means; lib.ref_last_seconds() is its wall time afterwards."""
+    lib.ref_set_threads.argtypes = [C.c_size_t]
+    lib.ref_set_threads.restype = C.c_int
+    lib.ref_last_seconds.argtypes = []
+    lib.ref_last_seconds.restype = C.c_double
+    if lib.ref_set_threads(threads) != 0:
+        raise ValueError("the reference's row bands support 1 or 2 threads")
+    mean = np.empty((h, w, 3), dtype=np.float64)
+    var = np.empty((h, w, 3), dtype=np.float64)
+    P, D, U = (np.asarray(v, dtype=np.float64) for v in (pos, direction, up))
+    try:
+        fn(handle, w, h, _p(P), _p(D), _p(U), screen_h, dist, ss, batches, _p(mean), _p(var))
+    finally:
+        lib.ref_set_threads(1)
+    return mean, var
+
+
+def last_seconds():
+    """Wall time of the last reference render."""
+    lib = load()
+    lib.ref_last_seconds.restype = C.c_double
+    return float(lib.ref_last_seconds())
 
 
 def env_texels(hdr, dirs):
@@ -140,6 +163,13 @@ class RefFileScene:
         if self.h:
             self.lib.ref_file_scene_free(self.h)
             self.h = None
+
+    def render(self, w, h, pos, direction, up, screen_h, dist, ss, batches, threads=1):
+        fn = self.lib.ref_file_render
+        fn.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                       C.c_double, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]
+        fn.restype = C.c_int
+        return _render(self.lib, fn, self.h, w, h, pos, direction, up, screen_h, dist, ss, batches, threads)
 
     def trace(self, rays):
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)

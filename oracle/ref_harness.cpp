@@ -6,6 +6,7 @@
 // on the GPU host).
 #include "stdafx.h"
 
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -137,8 +138,14 @@ void trace_listed(S *s, const float *rays, size_t n, double *t, double *normal, 
   }
 }
 
-// PathTracer::RenderScene (src/renderer/PathTracer.cpp:55-85) with one
-// thread: `batches` sample batches of ss x ss samples each, pinhole camera.
+// Threads PathTracer::RenderScene runs with (SetMaxThreadCount; the
+// reference's row bands write viewport threadIndex / 2, so 1 and 2 threads
+// fill the one image), and the wall time of the last RenderScene.
+size_t g_threads = 1;
+double g_seconds = 0.0;
+
+// PathTracer::RenderScene (src/renderer/PathTracer.cpp:55-85) on g_threads
+// threads: `batches` sample batches of ss x ss samples each, pinhole camera.
 // Writes the per-pixel mean and variance of the batch means (top-down rows,
 // W x H x 3).
 int render_scene(const Scene &scene, uint32_t W, uint32_t H, const double *pos, const double *dir, const double *up,
@@ -165,9 +172,11 @@ int render_scene(const Scene &scene, uint32_t W, uint32_t H, const double *pos, 
   // ScreenPixels it allocated with new (src/renderer/PathTracer.cpp:50-53,62),
   // which aborts under glibc.
   PathTracer *pt = new PathTracer(cam, 1, (int)batches, 1, (int)ss, cb);
-  pt->SetMaxThreadCount(1);
+  pt->SetMaxThreadCount(g_threads);
   pt->EnableNextEventEstimation(nee != 0);
+  auto t0 = std::chrono::steady_clock::now();
   pt->RenderScene(scene);
+  g_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   for (size_t i = 0; i < m1.size(); i++) {
     double mu = m1[i] / batches;
     mean[i] = mu;
@@ -182,6 +191,15 @@ extern "C" {
 
 // Random (src/tools/Random.h): n values of next() and, from a second
 // generator with the same seed, n values of nextDouble().
+// Thread count of the following renders (at most 2, see g_threads) and the
+// wall time of the last one.
+int ref_set_threads(size_t n) {
+  if (n < 1 || n > 2) return -1;
+  g_threads = n;
+  return 0;
+}
+double ref_last_seconds() { return g_seconds; }
+
 void ref_random(uint32_t seed, size_t n, uint32_t *out_next, double *out_double) {
   Random a(seed), b(seed);
   for (size_t i = 0; i < n; i++) {
@@ -216,6 +234,11 @@ void *ref_file_scene_new(const char *path) {
   return s;
 }
 void ref_file_scene_free(void *s) { delete (FileScene *)s; }
+// PathTracer::RenderScene over the loaded scene (see render_scene).
+int ref_file_render(void *sp, uint32_t W, uint32_t H, const double *pos, const double *dir, const double *up,
+                    double screen_h, double dist, uint32_t ss, uint32_t batches, double *mean, double *var) {
+  return render_scene(*(FileScene *)sp, W, H, pos, dir, up, screen_h, dist, ss, batches, mean, var);
+}
 size_t ref_file_scene_objects(void *s) { return ((FileScene *)s)->Count(); }
 // Scene::CheckIntersection of the loaded scene on n rays: FP64 distance,
 // normal of the hit record, the object's material count, the object's index
