@@ -296,6 +296,50 @@ int omr_job_latency(omr_ctx *ctx, omr_job_id job, double *ms);
 int omr_job_cancel(omr_ctx *ctx, omr_job_id job);
 int omr_server_stats_get(omr_ctx *ctx, omr_server_stats *stats);
 
+/* ---- served mode on a fake clock (host only, no device; no reference
+ * counterpart: tests and capacity planning).  The server's own admission,
+ * arena and latency code runs as a discrete-event simulation: requests arrive
+ * at the given times, each is planned at submission (rejected when its image
+ * exceeds the arena; per-sample records when they fit the record limit and,
+ * with the image, the arena), admitted in submission order while the live
+ * window has room and the arena a block (without records when only that
+ * fits), and a modelled device finishes samples at `samples_per_ms` in
+ * admission order; a request completes, and returns its block, when the end
+ * of its sample range is finished. */
+typedef struct omr_sim_request {
+  double arrival_ms;           /* submission time; nondecreasing over the array */
+  uint32_t width, height;      /* image */
+  uint32_t supersamples, spp;  /* samples = width x height x supersamples^2 x spp */
+  uint32_t cancel;             /* cancelled by the caller before its admission */
+  uint32_t pad;
+} omr_sim_request;
+
+typedef struct omr_sim_config {
+  uint64_t arena_bytes;         /* what the path pool and scenes leave of the HBM budget */
+  uint64_t record_limit_bytes;  /* per-request limit on per-sample records */
+  uint32_t max_live_jobs;       /* live window, 1..1024 */
+  uint32_t pad;
+  double samples_per_ms;        /* device model: samples finished per ms */
+} omr_sim_config;
+
+typedef struct omr_sim_outcome {
+  int32_t state;                       /* 2 done, 3 cancelled, 5 rejected */
+  uint32_t records;                    /* 1: per-sample records, 0: pixel atomics */
+  uint64_t off, bytes;                 /* its arena block while live */
+  double admit_ms, done_ms, latency_ms; /* -1 when it never ran */
+} omr_sim_outcome;
+
+typedef struct omr_sim_summary {
+  uint64_t jobs_done, jobs_rejected, jobs_cancelled;
+  uint64_t peak_used_bytes;  /* most arena bytes held by live requests at once */
+  uint32_t peak_live_jobs, pad;
+  double lat_p50_ms, lat_p99_ms, lat_max_ms;  /* over done requests, as omr_server_stats_get */
+  double makespan_ms;                          /* time of the last event */
+} omr_sim_summary;
+
+int omr_serve_simulate(const omr_sim_request *requests, size_t n, const omr_sim_config *cfg,
+                       omr_sim_outcome *outcomes, omr_sim_summary *summary);
+
 /* ---- procedural benchmark scenes (no reference counterpart: the inputs the
  * measurements use).  kind: 0 Cornell box, 1 Cornell box + displaced
  * icosphere (level 6), 2 heightfield terrain + 200 spheres + HDR sky,

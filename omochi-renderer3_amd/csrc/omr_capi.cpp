@@ -455,4 +455,10 @@ int omr_server_stats_get(omr_ctx *ctx, omr_server_stats *stats) {
   return ctx->eng->server_stats(stats);
 }
 
+int omr_serve_simulate(const omr_sim_request *requests, size_t n, const omr_sim_config *cfg,
+                       omr_sim_outcome *outcomes, omr_sim_summary *summary) {
+  if (!cfg) return OMR_ERR_INVALID;
+  return omr::serve_simulate(requests, n, *cfg, outcomes, summary);
+}
+
 }  // extern "C"

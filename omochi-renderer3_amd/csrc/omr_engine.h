@@ -17,6 +17,7 @@
 #include "../../include/omr_capi.h"
 #include "omr_bvh.h"
 #include "omr_kernels.h"
+#include "omr_admission.h"
 #include "omr_layout.h"
 
 namespace omr {
@@ -208,9 +209,7 @@ class Engine {
     double lat_ms = 0.0;      // submission to result in the caller's buffer (done requests)
     int code = OMR_OK;        // error code of a failed or rejected request
     std::string msg;          // its error text
-    size_t bytes = 0;         // device bytes while live: sums, image and (when `records`) per-sample records
-    uint64_t arena_off = 0;   // its block in the server's request arena
-    bool records = false;  // per-sample records instead of pixel atomics
+    Placement place;          // its block in the request arena: sums, image and per-sample records
     JobRun run;
     bool cancel_req = false;
     hipEvent_t ev_copy = nullptr;  // its image is resolved (the copy out waits on this)
@@ -233,7 +232,7 @@ class Engine {
   char *arena_ = nullptr;
   uint64_t arena_bytes_ = 0;
   std::mutex arena_mu_;
-  std::map<uint64_t, uint64_t> arena_free_;  // free blocks: offset -> length
+  Arena arena_map_;  // free ranges of arena_
   bool arena_take(uint64_t bytes, uint64_t *off);
   void arena_give(uint64_t off, uint64_t bytes);
   std::atomic<uint64_t> fixed_bytes_{0};  // device bytes of the pool, queues and stacks while serving
@@ -242,7 +241,7 @@ class Engine {
   uint64_t n_submitted_ = 0, n_done_ = 0, n_rejected_ = 0, n_cancelled_ = 0;
   uint64_t srv_segments_ = 0, srv_shadow_ = 0;
   double busy_ms_ = 0.0;
-  std::vector<double> latencies_;
+  LatencyBook latencies_;
   void server_loop();
 };
 

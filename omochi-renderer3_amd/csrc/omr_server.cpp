@@ -28,48 +28,16 @@ namespace {
 double ms_since(std::chrono::steady_clock::time_point t) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
 }
-uint64_t samples_of(const omr_camera &c, const omr_render_params &p) {
-  return (uint64_t)c.width * c.height * p.supersamples * p.supersamples * (uint64_t)(p.spp_end - p.spp_begin);
-}
-uint64_t align256(uint64_t b) { return (b + 255) & ~uint64_t(255); }
-// Device bytes of a request's fixed-point sums and resolved image, each
-// section 256-byte aligned in its arena block.
-uint64_t image_bytes(const omr_camera &c) {
-  const uint64_t px = (uint64_t)c.width * c.height;
-  return align256(px * 3 * 8) + align256(px * 3 * 4);
-}
 }  // namespace
 
 bool Engine::arena_take(uint64_t bytes, uint64_t *off) {
-  bytes = align256(bytes);
   std::lock_guard<std::mutex> g(arena_mu_);
-  for (auto it = arena_free_.begin(); it != arena_free_.end(); ++it) {
-    if (it->second < bytes) continue;
-    *off = it->first;
-    const uint64_t rest = it->second - bytes, at = it->first + bytes;
-    arena_free_.erase(it);
-    if (rest) arena_free_[at] = rest;
-    return true;
-  }
-  return false;
+  return arena_map_.take(bytes, off);
 }
 
 void Engine::arena_give(uint64_t off, uint64_t bytes) {
-  bytes = align256(bytes);
   std::lock_guard<std::mutex> g(arena_mu_);
-  auto it = arena_free_.emplace(off, bytes).first;
-  auto next = std::next(it);
-  if (next != arena_free_.end() && it->first + it->second == next->first) {  // merge with the block after
-    it->second += next->second;
-    arena_free_.erase(next);
-  }
-  if (it != arena_free_.begin()) {  // merge with the block before
-    auto prev = std::prev(it);
-    if (prev->first + prev->second == it->first) {
-      prev->second += it->second;
-      arena_free_.erase(it);
-    }
-  }
+  arena_map_.give(off, bytes);
 }
 
 int Engine::server_start(const omr_server_config &cfg) {
@@ -135,8 +103,7 @@ int Engine::server_start(const omr_server_config &cfg) {
     }
     arena_bytes_ = room;
     std::lock_guard<std::mutex> ag(arena_mu_);
-    arena_free_.clear();
-    arena_free_[0] = room;
+    arena_map_.reset(room);
   }
   server_stop_ = false;
   server_on_ = true;
@@ -188,10 +155,8 @@ int Engine::submit(omr_scene_id scene_id, const omr_camera &cam, const omr_rende
   // atomics, the same results) are added when they fit in the record limit
   // and, with the image, in what the pool and scenes leave of the budget.
   const uint64_t room = arena_bytes_;
-  const uint64_t need = image_bytes(cam);
-  const uint64_t rec = align256(samples_of(cam, p) * 24);
-  r->records = rec <= srv_record_bytes_ && need + rec <= room;
-  r->bytes = need + (r->records ? rec : 0);
+  const uint64_t need = image_bytes(cam.width, cam.height);
+  const bool fits = plan_request(need, samples_of(cam, p), srv_record_bytes_, room, &r->place);
   {
     std::lock_guard<std::mutex> g(smu);
     all_.push_back(r);
@@ -203,7 +168,7 @@ int Engine::submit(omr_scene_id scene_id, const omr_camera &cam, const omr_rende
       code = OMR_ERR_STATE;
       r->msg = "the server stopped after an error: " + srv_err_;
       r->state = 4;
-    } else if (need > room) {
+    } else if (!fits) {
       code = OMR_ERR_BUDGET;
       r->msg = "the request needs " + std::to_string(need) + " bytes; the path pool and resident scenes leave " +
                std::to_string(room) + " bytes of the HBM budget for requests";
@@ -279,17 +244,9 @@ int Engine::server_stats(omr_server_stats *s) {
   s->segments = srv_segments_;
   s->shadow_rays = srv_shadow_;
   s->busy_ms = busy_ms_;
-  if (!latencies_.empty()) {
-    std::vector<double> l = latencies_;
-    std::sort(l.begin(), l.end());
-    auto pct = [&](double q) {
-      size_t i = (size_t)std::min<double>((double)l.size() - 1, q * (double)(l.size() - 1) + 0.5);
-      return l[i];
-    };
-    s->lat_p50_ms = pct(0.50);
-    s->lat_p99_ms = pct(0.99);
-    s->lat_max_ms = l.back();
-  }
+  s->lat_p50_ms = latencies_.percentile(0.50);
+  s->lat_p99_ms = latencies_.percentile(0.99);
+  s->lat_max_ms = latencies_.max();
   return OMR_OK;
 }
 
@@ -353,7 +310,7 @@ void Engine::server_loop() {
       hipStreamDestroy(cs);
     }
     for (auto &r : live) {
-      if (r->run.d_accum) arena_give(r->arena_off, r->bytes);  // not yet returned by the copier
+      if (r->run.d_accum) arena_give(r->place.off, r->place.bytes());  // not yet returned by the copier
       r->run.d_accum = r->run.d_recs = nullptr;
       r->run.d_out = nullptr;
       if (r->ev_copy) hipEventDestroy(r->ev_copy);
@@ -397,11 +354,11 @@ void Engine::server_loop() {
       r->ev_copy = nullptr;
       // the copy is complete and so is all device work of the request (the
       // copy waited for its resolve): its block goes back to the arena
-      arena_give(r->arena_off, r->bytes);
+      arena_give(r->place.off, r->place.bytes());
       r->run.d_accum = r->run.d_recs = nullptr;
       r->run.d_out = nullptr;
       std::lock_guard<std::mutex> g(smu);
-      used_bytes_ -= r->bytes;
+      used_bytes_ -= r->place.bytes();
       if (r->sh) r->sh->users--;
       r->sh = nullptr;
       if (!copied) {
@@ -415,7 +372,7 @@ void Engine::server_loop() {
         r->state = 2;
         n_done_++;
         r->lat_ms = ms_since(r->t_submit);
-        latencies_.push_back(r->lat_ms);
+        latencies_.add(r->lat_ms);
       }
       scv.notify_all();
     }
@@ -468,29 +425,26 @@ void Engine::server_loop() {
         scv.wait_for(g, std::chrono::milliseconds(20));
         if (queue_.empty()) continue;
       }
-      while (!queue_.empty() && (hi + admit.size() - lo) < scfg_.max_live_jobs) {
-        auto r = queue_.front();
-        if (r->cancel_req) {
-          queue_.pop_front();
-          r->state = 3;
-          n_cancelled_++;
-          if (r->sh) r->sh->users--;
-          r->sh = nullptr;
-          scv.notify_all();
-          continue;
-        }
-        // a block of the arena, without the per-sample records if only that fits
-        // (pixel atomics instead: the same results); else wait for memory
-        if (!arena_take(r->bytes, &r->arena_off)) {
-          if (!r->records || !arena_take(image_bytes(r->cam), &r->arena_off)) break;
-          r->records = false;
-          r->bytes = image_bytes(r->cam);
-        }
-        queue_.pop_front();
-        used_bytes_ += r->bytes;
-        r->state = 1;
-        admit.push_back(r);
-      }
+      // in submission order while the job ring and the arena allow (a block
+      // without the per-sample records when only that fits: pixel atomics,
+      // the same results); cancelled requests leave without one
+      std::lock_guard<std::mutex> ag(arena_mu_);
+      admit_in_order(
+          queue_, arena_map_, scfg_.max_live_jobs - (size_t)(hi - lo),
+          [](const std::shared_ptr<Req> &r) { return r->cancel_req; },
+          [&](const std::shared_ptr<Req> &r) {
+            r->state = 3;
+            n_cancelled_++;
+            if (r->sh) r->sh->users--;
+            r->sh = nullptr;
+            scv.notify_all();
+          },
+          [](const std::shared_ptr<Req> &r) { return &r->place; },
+          [&](const std::shared_ptr<Req> &r) {
+            used_bytes_ += r->place.bytes();
+            r->state = 1;
+            admit.push_back(r);
+          });
     }
     std::unique_lock<std::mutex> dev(mu);
     if (!admit.empty()) {
@@ -510,10 +464,10 @@ void Engine::server_loop() {
         const size_t px = (size_t)r->cam.width * r->cam.height;
         j.d_recs = nullptr;
         live.push_back(r);
-        char *blk = arena_ + r->arena_off;
+        char *blk = arena_ + r->place.off;
         j.d_accum = reinterpret_cast<unsigned long long *>(blk);
         j.d_out = reinterpret_cast<float *>(blk + align256(px * 3 * 8));
-        if (r->records) j.d_recs = reinterpret_cast<unsigned long long *>(blk + image_bytes(r->cam));
+        if (r->place.records) j.d_recs = reinterpret_cast<unsigned long long *>(blk + r->place.image);
         okd = okd && hipMemsetAsync(j.d_accum, 0, px * 3 * 8, stream_) == hipSuccess &&
               (!j.d_recs || hipMemsetAsync(j.d_recs, 0, (size_t)j.n_samples * 24, stream_) == hipSuccess) &&
               k::clear_done(d_done_, j.ring, stream_) == hipSuccess;

@@ -94,6 +94,28 @@ class ServerStats(C.Structure):
                 ("lat_max_ms", C.c_double), ("fixed_bytes", C.c_uint64), ("used_bytes", C.c_uint64)]
 
 
+class SimRequest(C.Structure):
+    _fields_ = [("arrival_ms", C.c_double), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("supersamples", C.c_uint32), ("spp", C.c_uint32), ("cancel", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class SimConfig(C.Structure):
+    _fields_ = [("arena_bytes", C.c_uint64), ("record_limit_bytes", C.c_uint64), ("max_live_jobs", C.c_uint32),
+                ("pad", C.c_uint32), ("samples_per_ms", C.c_double)]
+
+
+class SimOutcome(C.Structure):
+    _fields_ = [("state", C.c_int32), ("records", C.c_uint32), ("off", C.c_uint64), ("bytes", C.c_uint64),
+                ("admit_ms", C.c_double), ("done_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class SimSummary(C.Structure):
+    _fields_ = [("jobs_done", C.c_uint64), ("jobs_rejected", C.c_uint64), ("jobs_cancelled", C.c_uint64),
+                ("peak_used_bytes", C.c_uint64), ("peak_live_jobs", C.c_uint32), ("pad", C.c_uint32),
+                ("lat_p50_ms", C.c_double), ("lat_p99_ms", C.c_double), ("lat_max_ms", C.c_double),
+                ("makespan_ms", C.c_double)]
+
+
 class Procedural(C.Structure):
     _fields_ = [("desc", SceneDesc), ("camera", Camera), ("internal", C.c_void_p)]
 
@@ -103,7 +125,8 @@ EXPORTS = [
     "omr_scene_info_get", "omr_scene_bvh", "omr_scene_env_tables", "omr_trace_closest", "omr_trace_closest_device",
     "omr_trace_stats_device", "omr_trace_closest_device_noalloc", "omr_render", "omr_render_sums", "omr_last_render_stats", "omr_render_batch",
     "omr_get_tuning", "omr_set_tuning", "omr_server_start", "omr_server_stop", "omr_job_submit", "omr_job_poll",
-    "omr_job_wait", "omr_job_error", "omr_job_latency", "omr_job_cancel", "omr_server_stats_get", "omr_procedural_scene", "omr_procedural_free",
+    "omr_job_wait", "omr_job_error", "omr_job_latency", "omr_job_cancel", "omr_server_stats_get",
+    "omr_serve_simulate", "omr_procedural_scene", "omr_procedural_free",
     "omr_camera_from_lookat", "omr_bvh_build", "omr_bvh_data", "omr_bvh_free", "omr_set_profiling", "omr_autotune",
     "omr_procedural_camera", "omr_io_last_error", "omr_free", "omr_hdr_read", "omr_hdr_write", "omr_scene_file_load",
     "omr_obj_load", "omr_loaded_scene_free", "omr_image_read", "omr_texture_read",
@@ -153,6 +176,8 @@ def load():
     lib.omr_job_latency.argtypes = [vp, C.c_int64, C.POINTER(C.c_double)]
     lib.omr_job_cancel.argtypes = [vp, C.c_int64]
     lib.omr_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
+    lib.omr_serve_simulate.argtypes = [C.POINTER(SimRequest), sz, C.POINTER(SimConfig), C.POINTER(SimOutcome),
+                                       C.POINTER(SimSummary)]
     lib.omr_procedural_scene.argtypes = [C.c_int, u64, C.c_uint32, C.c_uint32, C.POINTER(Procedural)]
     lib.omr_procedural_free.argtypes = [C.POINTER(Procedural)]
     lib.omr_procedural_free.restype = None
@@ -319,6 +344,28 @@ def texture_read(path, inverse_gamma=True):
         return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), shape=(n,)).copy().reshape(h.value, w.value, 3)
     finally:
         lib.omr_free(p)
+
+
+def serve_simulate(requests, arena_bytes, samples_per_ms, record_limit_bytes=None, max_live_jobs=512):
+    """Served mode's admission, arena and latency code on a fake clock (host
+    only).  requests: dicts with arrival_ms, w, h, ss, spp and optional
+    cancel.  Returns (outcomes as a numpy structured array, summary dict)."""
+    lib = load()
+    n = len(requests)
+    arr = (SimRequest * max(1, n))()
+    for i, r in enumerate(requests):
+        arr[i] = SimRequest(float(r["arrival_ms"]), r["w"], r["h"], r["ss"], r["spp"], int(r.get("cancel", 0)), 0)
+    cfg = SimConfig(int(arena_bytes), int(arena_bytes // 4 if record_limit_bytes is None else record_limit_bytes),
+                    int(max_live_jobs), 0, float(samples_per_ms))
+    out = (SimOutcome * max(1, n))()
+    sm = SimSummary()
+    rc = lib.omr_serve_simulate(arr, n, C.byref(cfg), out, C.byref(sm))
+    if rc != OMR_OK:
+        raise OmrError(rc, "omr_serve_simulate failed")
+    dt = np.dtype([("state", np.int32), ("records", np.uint32), ("off", np.uint64), ("bytes", np.uint64),
+                   ("admit_ms", np.float64), ("done_ms", np.float64), ("latency_ms", np.float64)])
+    res = np.frombuffer(C.string_at(out, C.sizeof(SimOutcome) * n), dtype=dt).copy() if n else np.zeros(0, dt)
+    return res, {k: getattr(sm, k) for k, _ in SimSummary._fields_ if k != "pad"}
 
 
 def load_scene_file(path):
