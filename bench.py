@@ -45,10 +45,12 @@ METRIC = "Mrays/s at matched RMSE, served/batched vs plain GPU; p50/p99 job late
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per unit (the survey's roofline model): a closest-hit
 # segment moves its ray in (32) and hit out (16) in the traversal kernel and
-# the path state in and out (96) in shading; a 4-wide node visit reads 128; a
-# primitive test reads 48 (a sphere's 16-byte record is priced as a triangle:
-# sphere tests are < 0.1 % of the tests here).  The drain does both halves.
-B_SEG_TRACE, B_SEG_SHADE, B_NODE, B_PRIM = 48, 96, 128, 48
+# the path state in and out (96) in shading; a node visit reads its node (128
+# bytes 4-wide, 256 bytes 8-wide); a primitive test reads 48 (a sphere's
+# 16-byte record is priced as a triangle: sphere tests are < 0.1 % of the
+# tests here).  The drain does both halves.
+B_SEG_TRACE, B_SEG_SHADE, B_PRIM = 48, 96, 48
+B_NODE = {4: 128, 8: 256}
 B_SHADOW = 48  # a shadow ray: its record in and the visibility result
 # MI355X: 256 CUs x 4 SIMDs, one wave64 VALU instruction per SIMD every 4
 # cycles at 2.4 GHz
@@ -210,18 +212,19 @@ def profiled_pass(ctx, args_list):
     return acc
 
 
-def roofline_from(acc, workload):
+def roofline_from(acc, workload, node_width=4):
     ext_seg = acc["segments"] - acc["drain_segments"]
+    bn, w = B_NODE[node_width], node_width
     kernels = {
         # name: (algorithmic bytes, summed event ms, launches, rocprof symbol of the counting variant)
-        "k_extend": (B_SEG_TRACE * ext_seg + B_NODE * acc["node_visits"] + B_PRIM * acc["prim_tests"],
-                     acc["extend_ms"], acc["extend_launches"], "k_extend<64, 16, true, true, true>"),
+        "k_extend": (B_SEG_TRACE * ext_seg + bn * acc["node_visits"] + B_PRIM * acc["prim_tests"],
+                     acc["extend_ms"], acc["extend_launches"], f"k_extend<64, 16, {w}, true, true, true>"),
         "k_shade": (B_SEG_SHADE * ext_seg, acc["shade_ms"], acc["extend_launches"], "k_shade<64, true, true>"),
         "k_shadow": (B_SHADOW * acc["shadow_rays"], acc["shadow_ms"], acc["extend_launches"],
-                     "k_shadow<64, 16, true, true, true>"),
-        "k_drain": ((B_SEG_TRACE + B_SEG_SHADE) * acc["drain_segments"] + B_NODE * acc["drain_node_visits"]
+                     f"k_shadow<64, 16, {w}, true, true, true>"),
+        "k_drain": ((B_SEG_TRACE + B_SEG_SHADE) * acc["drain_segments"] + bn * acc["drain_node_visits"]
                     + B_PRIM * acc["drain_prim_tests"], acc["drain_ms"], acc["drain_launches"],
-                    "k_drain<64, 16, true, true>"),
+                    f"k_drain<64, 16, {w}, true, true>"),
     }
     table = {}
     for name, (b, ms, n, sym) in kernels.items():
@@ -445,7 +448,7 @@ def run_served(ctx, omr, args, dev):
     nroof = min(len(reqs), args.roofline_requests)
     log(f"c4: profiled pass over {nroof} requests")
     acc = profiled_pass(ctx, alist[:nroof])
-    roof = roofline_from(acc, "c4")
+    roof = roofline_from(acc, "c4", ctx.get_tuning()["node_width"])
     roof["pass"] = (f"the first {nroof} requests rendered alone with per-launch HIP events on the engine's "
                     "stream (counting kernel variants)")
     out = {
@@ -481,7 +484,7 @@ def run_plain(ctx, omr, args, dev):
     imgs, rays, walls = render_alone(ctx, [a] * args.steps)
     dt = sum(walls)
     acc = profiled_pass(ctx, [a] * min(args.steps, 5))
-    roof = roofline_from(acc, args.workload)
+    roof = roofline_from(acc, args.workload, ctx.get_tuning()["node_width"])
     roof["pass"] = f"{min(args.steps, 5)} renders with per-launch HIP events on the engine's stream"
     out = {
         "metric": METRIC, "value": round(sum(rays) / dt / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,

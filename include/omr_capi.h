@@ -141,17 +141,21 @@ typedef struct {
 } omr_render_stats;
 
 typedef struct {
-  uint32_t n_nodes;
+  uint32_t n_nodes;      /* 4-wide nodes */
   uint32_t depth;
   uint32_t stack_bound;
   uint32_t n_prims;
   uint64_t device_bytes;
   double build_ms;
+  uint32_t n_nodes8;     /* the same tree as 8-wide nodes (same leaves) */
+  uint32_t depth8;
+  uint32_t stack_bound8;
+  uint32_t pad;
 } omr_scene_info;
 
 /* Launch tuning (chosen by omr_autotune or set by hand). */
 typedef struct {
-  uint32_t node_width;        /* 4 (128-byte nodes) */
+  uint32_t node_width;        /* 4 (128-byte nodes) or 8 (256-byte nodes): the BVH the traversal reads */
   uint32_t wg_size;           /* 64, 128 or 256 */
   uint32_t blocks_per_cu;     /* persistent grid = CUs x blocks_per_cu (shade runs 20 waves per CU, 16 for one job) */
   uint32_t pool_paths;        /* path slots in flight (max) */
@@ -222,6 +226,9 @@ typedef struct omr_bvh omr_bvh;
 int omr_bvh_build(const omr_scene_desc *desc, omr_bvh **out);
 int omr_bvh_data(const omr_bvh *b, const void **nodes, size_t *n_nodes, const uint32_t **prim_order, size_t *n_order,
                  uint32_t *depth, uint32_t *stack_bound);
+/* The 8-wide collapse of the same tree (256-byte nodes, the same leaf codes
+ * and prim_order as the 4-wide nodes). */
+int omr_bvh_data8(const omr_bvh *b, const void **nodes, size_t *n_nodes, uint32_t *depth, uint32_t *stack_bound);
 void omr_bvh_free(omr_bvh *b);
 /* Environment sampling tables as built on the device (uint64 prefix sums). */
 int omr_scene_env_tables(omr_ctx *ctx, omr_scene_id id, uint64_t *row /* h x (w+1) */, uint64_t *marg /* h+1 */);

@@ -154,6 +154,86 @@ struct Builder {
 
 };
 
+template <int W> struct NodeOf;
+template <> struct NodeOf<4> { using T = WNode4; };
+template <> struct NodeOf<8> { using T = WNode8; };
+
+// Collapses the binary tree into W-wide nodes: each node gathers up to W
+// children by repeatedly opening its inner child of largest surface area;
+// leaf(bi) gives the leaf code of binary leaf bi.  Child boxes grow by `pad`.
+// Reports the depth in W-wide levels and the traversal stack bound (a visit
+// pushes at most W - 1 entries).
+template <int W, class Leaf>
+void collapse(const std::vector<BNode> &bn, float pad, Leaf leaf, std::vector<typename NodeOf<W>::T> &nodes,
+              uint32_t &depth, uint32_t &stack_bound) {
+  using Node = typename NodeOf<W>::T;
+  struct Pending {
+    int32_t bnode;
+    uint32_t wnode;
+    uint32_t depth;
+    uint32_t stack_above;
+  };
+  std::vector<Pending> work;
+  nodes.clear();
+  nodes.reserve(bn.size() / (W / 2) + 2);
+  nodes.emplace_back();
+  work.push_back({0, 0, 1, 0});
+  uint32_t max_depth = 0, max_stack = 0;
+  while (!work.empty()) {
+    Pending pw = work.back();
+    work.pop_back();
+    max_depth = std::max(max_depth, pw.depth);
+    int32_t kids[W];
+    int nk = 0;
+    const BNode &root = bn[(size_t)pw.bnode];
+    if (root.left < 0) {
+      kids[nk++] = pw.bnode;  // single-leaf scene: root node holds one leaf
+    } else {
+      kids[nk++] = root.left;
+      kids[nk++] = root.right;
+      while (nk < W) {
+        int best = -1;
+        float ba = -1.0f;
+        for (int k = 0; k < nk; k++) {
+          const BNode &c = bn[(size_t)kids[k]];
+          if (c.left >= 0 && c.box.area() > ba) { ba = c.box.area(); best = k; }
+        }
+        if (best < 0) break;
+        int32_t open = kids[best];
+        kids[best] = bn[(size_t)open].left;
+        kids[nk++] = bn[(size_t)open].right;
+      }
+    }
+    Node w;
+    std::memset(&w, 0, sizeof(w));
+    for (int k = 0; k < W; k++) {
+      w.child[k] = -1;
+      w.lox[k] = w.loy[k] = w.loz[k] = kInf;
+      w.hix[k] = w.hiy[k] = w.hiz[k] = -kInf;
+    }
+    w.meta[0] = (int32_t)pw.depth;
+    uint32_t stack_here = pw.stack_above + (uint32_t)(nk > 0 ? nk - 1 : 0);
+    max_stack = std::max(max_stack, stack_here);
+    for (int k = 0; k < nk; k++) {
+      const BNode &c = bn[(size_t)kids[k]];
+      w.lox[k] = c.box.lo[0] - pad; w.hix[k] = c.box.hi[0] + pad;
+      w.loy[k] = c.box.lo[1] - pad; w.hiy[k] = c.box.hi[1] + pad;
+      w.loz[k] = c.box.lo[2] - pad; w.hiz[k] = c.box.hi[2] + pad;
+      if (c.left < 0) {
+        w.child[k] = leaf(kids[k]);
+      } else {
+        uint32_t wi = (uint32_t)nodes.size();
+        nodes.emplace_back();
+        w.child[k] = (int32_t)wi;
+        work.push_back({kids[k], wi, pw.depth + 1, stack_here});
+      }
+    }
+    nodes[pw.wnode] = w;
+  }
+  depth = max_depth;
+  stack_bound = max_stack + 1;
+}
+
 }  // namespace
 
 bool build_wide_bvh(const BuildInput &in, BuildOutput &out, std::string &err, int threads) {
@@ -214,22 +294,15 @@ bool build_wide_bvh(const BuildInput &in, BuildOutput &out, std::string &err, in
   B.build(0, 0, (uint32_t)n, par_depth);
   std::vector<BNode> &bn = B.nodes;
 
-  // Collapse into 4-wide nodes.  Leaves are written in depth-first order so a
-  // leaf's primitives are contiguous in `prims`.
+  // Collapse into 4-wide nodes, then the same tree into 8-wide nodes.
+  // Leaves are written in the depth-first order of the 4-wide collapse, so a
+  // leaf's primitives are contiguous in `prims`; the 8-wide nodes reuse the
+  // leaf codes.
   out.prims.reserve(n);
   out.prim_order.reserve(n);
-  struct Pending {
-    int32_t bnode;
-    uint32_t wnode;
-    uint32_t depth;
-    uint32_t stack_above;
-  };
-  auto pad_box = [&](const Box &b, WNode4 &w, int k) {
-    w.lox[k] = b.lo[0] - pad; w.hix[k] = b.hi[0] + pad;
-    w.loy[k] = b.lo[1] - pad; w.hiy[k] = b.hi[1] + pad;
-    w.loz[k] = b.lo[2] - pad; w.hiz[k] = b.hi[2] + pad;
-  };
-  auto emit_leaf = [&](const BNode &b) -> int32_t {
+  std::vector<int32_t> leaf_code((size_t)B.used.load(), -1);
+  auto emit_leaf = [&](int32_t bi) -> int32_t {
+    const BNode &b = bn[(size_t)bi];
     uint32_t first = (uint32_t)out.prims.size();
     for (uint32_t i = b.first; i < b.first + b.count; i++) {
       uint32_t id = idx[i];
@@ -251,67 +324,11 @@ bool build_wide_bvh(const BuildInput &in, BuildOutput &out, std::string &err, in
       out.prims.push_back(r);
       out.prim_order.push_back(id);
     }
-    return make_leaf(first, b.count);
+    return leaf_code[(size_t)bi] = make_leaf(first, b.count);
   };
-
-  std::vector<Pending> work;
-  out.nodes.reserve(bn.size() / 2 + 2);
-  out.nodes.emplace_back();
-  work.push_back({0, 0, 1, 0});
-  uint32_t max_depth = 0, max_stack = 0;
-  while (!work.empty()) {
-    Pending pw = work.back();
-    work.pop_back();
-    max_depth = std::max(max_depth, pw.depth);
-    // gather up to four children by opening the largest-area inner child
-    int32_t kids[4];
-    int nk = 0;
-    const BNode &root = bn[(size_t)pw.bnode];
-    if (root.left < 0) {
-      kids[nk++] = pw.bnode;  // single-leaf scene: root node holds one leaf
-    } else {
-      kids[nk++] = root.left;
-      kids[nk++] = root.right;
-      while (nk < 4) {
-        int best = -1;
-        float ba = -1.0f;
-        for (int k = 0; k < nk; k++) {
-          const BNode &c = bn[(size_t)kids[k]];
-          if (c.left >= 0 && c.box.area() > ba) { ba = c.box.area(); best = k; }
-        }
-        if (best < 0) break;
-        int32_t open = kids[best];
-        kids[best] = bn[(size_t)open].left;
-        kids[nk++] = bn[(size_t)open].right;
-      }
-    }
-    WNode4 w;
-    std::memset(&w, 0, sizeof(w));
-    for (int k = 0; k < 4; k++) {
-      w.child[k] = -1;
-      w.lox[k] = w.loy[k] = w.loz[k] = kInf;
-      w.hix[k] = w.hiy[k] = w.hiz[k] = -kInf;
-    }
-    w.meta[0] = (int32_t)pw.depth;
-    // traversal pushes at most (children - 1) entries per level
-    uint32_t stack_here = pw.stack_above + (uint32_t)(nk > 0 ? nk - 1 : 0);
-    max_stack = std::max(max_stack, stack_here);
-    for (int k = 0; k < nk; k++) {
-      const BNode &c = bn[(size_t)kids[k]];
-      pad_box(c.box, w, k);
-      if (c.left < 0) {
-        w.child[k] = emit_leaf(c);
-      } else {
-        uint32_t wi = (uint32_t)out.nodes.size();
-        out.nodes.emplace_back();
-        w.child[k] = (int32_t)wi;
-        work.push_back({kids[k], wi, pw.depth + 1, stack_here});
-      }
-    }
-    out.nodes[pw.wnode] = w;
-  }
-  out.depth = max_depth;
-  out.stack_bound = max_stack + 1;
+  collapse<4>(bn, pad, emit_leaf, out.nodes, out.depth, out.stack_bound);
+  collapse<8>(bn, pad, [&](int32_t bi) { return leaf_code[(size_t)bi]; }, out.nodes8, out.depth8,
+              out.stack_bound8);
   out.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return true;
 }

@@ -174,6 +174,10 @@ int omr_scene_info_get(omr_ctx *ctx, omr_scene_id id, omr_scene_info *info) {
   info->n_prims = (uint32_t)s->bvh.prims.size();
   info->device_bytes = s->bytes;
   info->build_ms = s->bvh.build_ms;
+  info->n_nodes8 = (uint32_t)s->bvh.nodes8.size();
+  info->depth8 = s->bvh.depth8;
+  info->stack_bound8 = s->bvh.stack_bound8;
+  info->pad = 0;
   return OMR_OK;
 }
 
@@ -218,6 +222,15 @@ int omr_bvh_data(const omr_bvh *b, const void **nodes, size_t *n_nodes, const ui
   if (n_order) *n_order = b->out.prim_order.size();
   if (depth) *depth = b->out.depth;
   if (stack_bound) *stack_bound = b->out.stack_bound;
+  return OMR_OK;
+}
+
+int omr_bvh_data8(const omr_bvh *b, const void **nodes, size_t *n_nodes, uint32_t *depth, uint32_t *stack_bound) {
+  if (!b) return OMR_ERR_INVALID;
+  if (nodes) *nodes = b->out.nodes8.data();
+  if (n_nodes) *n_nodes = b->out.nodes8.size();
+  if (depth) *depth = b->out.depth8;
+  if (stack_bound) *stack_bound = b->out.stack_bound8;
   return OMR_OK;
 }
 
@@ -373,7 +386,7 @@ int omr_autotune(omr_ctx *ctx, size_t n, const omr_scene_id *scenes, const omr_c
     std::vector<uint32_t> values;
   };
   std::vector<Dim> dims = {
-      {&omr_tuning::node_width, {4}},
+      {&omr_tuning::node_width, {4, 8}},
       {&omr_tuning::wg_size, {64, 128, 256}},
       {&omr_tuning::blocks_per_cu, {4, 8, 16, 19}},
       {&omr_tuning::pool_paths, {1u << 22, 1u << 24, 1u << 25, 1u << 26}},

@@ -115,7 +115,7 @@ void Engine::drop_graphs() {
 }
 
 int Engine::set_tuning(const omr_tuning &t) {
-  if (t.node_width != 4) return fail(OMR_ERR_INVALID, "node_width must be 4");
+  if (t.node_width != 4 && t.node_width != 8) return fail(OMR_ERR_INVALID, "node_width must be 4 or 8");
   if (t.wg_size != 64 && t.wg_size != 128 && t.wg_size != 256) return fail(OMR_ERR_INVALID, "wg_size must be 64/128/256");
   if (t.blocks_per_cu < 1 || t.blocks_per_cu > 64) return fail(OMR_ERR_INVALID, "blocks_per_cu out of range");
   if (t.pool_paths < 64 || t.pool_paths > (1u << 26)) return fail(OMR_ERR_INVALID, "pool_paths out of range");
@@ -181,6 +181,7 @@ KernelArgs Engine::args() const {
   a.p.df_budget = tuning.fetch_budget;
   a.p.df_fetch_min = tuning.fetch_min;
   a.p.stack_short = tuning.stack_short;
+  a.p.node_width = tuning.node_width;
   a.solo = solo_;
   return a;
 }
@@ -237,14 +238,15 @@ void Engine::free_scene_mem(SceneHost &s) {
 }
 
 SceneHost *Engine::scene(omr_scene_id id) {
-  if (id < 0 || (size_t)id >= n_scenes_.load()) return nullptr;
-  return scenes_[(size_t)id].get();
+  if (id < 0) return nullptr;
+  SceneHost *s = scenes_[scene_slot(id)].get();
+  return s && s->id == id ? s : nullptr;
 }
 
 uint64_t Engine::scene_bytes() {
   std::lock_guard<std::mutex> g(scene_mu);
   uint64_t b = 0;
-  for (size_t i = 0; i < n_scenes_.load(); i++)
+  for (size_t i = 0; i < kMaxScenes; i++)
     if (scenes_[i]) b += scenes_[i]->bytes;
   return b;
 }
@@ -361,8 +363,13 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
 
   std::lock_guard<std::mutex> g(mu);
   hipSetDevice(device_);
-  size_t slot = n_scenes_.load();
-  if (slot >= kMaxScenes) return fail(OMR_ERR_INVALID, "too many scenes (max 64 uploads per context)");
+  size_t slot = 0;
+  {
+    std::lock_guard<std::mutex> sg(scene_mu);
+    while (slot < kMaxScenes && scenes_[slot]) slot++;
+  }
+  if (slot >= kMaxScenes) return fail(OMR_ERR_INVALID, "too many resident scenes (max 64 per context)");
+  if (slot_uses_[slot] >= 0x7FFFFFFFu / kMaxScenes) return fail(OMR_ERR_INVALID, "scene ids exhausted");
   SceneHost &S = *sh;
   auto dup = [&](const void *src, size_t bytes, void **dst) -> int {
     *dst = nullptr;
@@ -373,10 +380,11 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
     if (src) OMR_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
     return OMR_OK;
   };
-  void *p_nodes, *p_prims, *p_tn = nullptr, *p_sph = nullptr, *p_pm, *p_mats, *p_pl = nullptr;
+  void *p_nodes, *p_nodes8, *p_prims, *p_tn = nullptr, *p_sph = nullptr, *p_pm, *p_mats, *p_pl = nullptr;
   void *p_tex = nullptr, *p_trec = nullptr, *p_uv = nullptr;
   int rc;
   if ((rc = dup(S.bvh.nodes.data(), S.bvh.nodes.size() * sizeof(WNode4), &p_nodes)) ||
+      (rc = dup(S.bvh.nodes8.data(), S.bvh.nodes8.size() * sizeof(WNode8), &p_nodes8)) ||
       (rc = dup(S.bvh.prims.data(), S.bvh.prims.size() * sizeof(PrimRec), &p_prims)) ||
       (rc = dup(tn.data(), tn.size() * sizeof(float), &p_tn)) ||
       (rc = dup(d->spheres, ns * 4 * sizeof(float), &p_sph)) || (rc = dup(pm.data(), nall * 4, &p_pm)) ||
@@ -390,6 +398,7 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
   }
   SceneDev &D = S.dev;
   D.nodes = (const WNode4 *)p_nodes;
+  D.nodes8 = (const WNode8 *)p_nodes8;
   D.prims = (const PrimRec *)p_prims;
   D.tri_n = (const float *)p_tn;
   D.sph = (const float *)p_sph;
@@ -405,6 +414,8 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
   D.n_tex = (uint32_t)d->n_textures;
   D.n_nodes = (uint32_t)S.bvh.nodes.size();
   D.stack_bound = S.bvh.stack_bound;
+  D.n_nodes8 = (uint32_t)S.bvh.nodes8.size();
+  D.stack_bound8 = S.bvh.stack_bound8;
   if (d->env_rgb) {
     uint32_t w = d->env_width, h = d->env_height;
     void *p_env, *p_w, *p_max, *p_q, *p_row, *p_marg, *p_rg, *p_mg, *p_rs, *p_ms;
@@ -443,10 +454,10 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
   OMR_HIP(hipMemcpy(d_scenes_ + slot, &D, sizeof(SceneDev), hipMemcpyHostToDevice));
   {
     std::lock_guard<std::mutex> sg(scene_mu);
+    sh->id = (omr_scene_id)(slot_uses_[slot]++ * kMaxScenes + slot);
+    *id = sh->id;
     scenes_[slot] = std::move(sh);
-    n_scenes_.store(slot + 1);
   }
-  *id = (omr_scene_id)slot;
   return OMR_OK;
 }
 
@@ -460,13 +471,13 @@ int Engine::release_scene(omr_scene_id id) {
     SceneHost *p = scene(id);
     if (!p) return fail(OMR_ERR_NOTFOUND, "unknown scene");
     if (p->users.load() > 0) return fail(OMR_ERR_STATE, "scene is held by queued or running served jobs");
-    s = std::move(scenes_[(size_t)id]);
+    s = std::move(scenes_[scene_slot(id)]);
   }
   hipSetDevice(device_);
   hipStreamSynchronize(stream_);
   free_scene_mem(*s);
   SceneDev z{};
-  hipMemcpy(d_scenes_ + id, &z, sizeof(SceneDev), hipMemcpyHostToDevice);
+  hipMemcpy(d_scenes_ + scene_slot(id), &z, sizeof(SceneDev), hipMemcpyHostToDevice);
   return OMR_OK;
 }
 
@@ -489,7 +500,7 @@ int Engine::trace(omr_scene_id id, const float *d_rays, size_t n, int32_t *d_pri
   hipSetDevice(device_);
   const uint32_t wg = 128;
   uint32_t grid = (uint32_t)std::min<size_t>((n + wg - 1) / wg, (size_t)n_cu_ * 16);
-  int rc = ensure_gstack(s->bvh.stack_bound, (size_t)grid * wg);
+  int rc = ensure_gstack(stack_need(s), (size_t)grid * wg);
   if (rc) return rc;
   // d_cnt[0..1]: node visits, primitive tests; one line on, the dynamic-fetch heads
   const size_t cnt_bytes = kLine + sizeof(Sharded<uint32_t>);
@@ -497,8 +508,10 @@ int Engine::trace(omr_scene_id id, const float *d_rays, size_t n, int32_t *d_pri
   OMR_HIP(hipMalloc(&d_cnt, cnt_bytes));
   hipError_t e = hipMemsetAsync(d_cnt, 0, cnt_bytes, stream_);
   if (e == hipSuccess)
-    e = k::trace_rays(d_scenes_ + id, d_rays, (uint32_t)n, d_prim, d_t, stats2 ? d_cnt : nullptr, d_gstack_, gcap_,
-                      tuning.stack_short, grid, wg, reinterpret_cast<Sharded<uint32_t> *>(d_cnt + kLine / 8),
+    e = k::trace_rays(d_scenes_ + scene_slot(id), d_rays, (uint32_t)n, d_prim, d_t, stats2 ? d_cnt : nullptr, d_gstack_,
+                      gcap_,
+                      tuning.stack_short, tuning.node_width, grid, wg,
+                      reinterpret_cast<Sharded<uint32_t> *>(d_cnt + kLine / 8),
                       (int)tuning.fetch_budget, tuning.fetch_min, stream_);
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
   unsigned long long c[2] = {0, 0};
@@ -521,11 +534,12 @@ int Engine::trace_noalloc(omr_scene_id id, const float *d_rays, size_t n, int32_
   hipSetDevice(device_);
   // overflow entries per thread: the scene's exact stack bound past the LDS part
   const uint32_t wg = 128, shrt = tuning.stack_short;
-  const uint32_t gcap = s->bvh.stack_bound > shrt ? s->bvh.stack_bound - shrt : 1u;
+  const uint32_t bound = stack_need(s);
+  const uint32_t gcap = bound > shrt ? bound - shrt : 1u;
   const size_t per_group = (size_t)wg * gcap * sizeof(uint2);
   const size_t max_groups = (kNoAllocScratchBytes - kNoAllocHeadBytes) / per_group;
   if (max_groups == 0)
-    return fail(OMR_ERR_INVALID, "the scene's traversal stack bound (" + std::to_string(s->bvh.stack_bound) +
+    return fail(OMR_ERR_INVALID, "the scene's traversal stack bound (" + std::to_string(bound) +
                                      " entries) does not fit the allocation-free scratch");
   // persistent grid: every ray is reached whatever the grid (dynamic fetch or
   // grid-stride), so the grid is cut to what the scratch holds
@@ -535,8 +549,9 @@ int Engine::trace_noalloc(omr_scene_id id, const float *d_rays, size_t n, int32_
   uint2 *stacks = reinterpret_cast<uint2 *>(base + kNoAllocHeadBytes);
   hipError_t e = k::zero_words(reinterpret_cast<uint32_t *>(base), (uint32_t)(kNoAllocHeadBytes / 4), stream_);
   if (e == hipSuccess)
-    e = k::trace_rays(d_scenes_ + id, d_rays, (uint32_t)n, d_prim, d_t, nullptr, stacks, gcap, shrt, grid, wg, heads,
-                      (int)tuning.fetch_budget, tuning.fetch_min, stream_);
+    e = k::trace_rays(d_scenes_ + scene_slot(id), d_rays, (uint32_t)n, d_prim, d_t, nullptr, stacks, gcap, shrt,
+                      tuning.node_width,
+                      grid, wg, heads, (int)tuning.fetch_budget, tuning.fetch_min, stream_);
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
   if (e != hipSuccess) return hipfail(e, "trace_rays (allocation-free)");
   return OMR_OK;
@@ -675,7 +690,7 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
   for (auto &j : jobs) {
     SceneHost *s = scene((omr_scene_id)j.scene);
     if (!s) return fail(OMR_ERR_NOTFOUND, "unknown scene");
-    need_stack = std::max(need_stack, s->bvh.stack_bound);
+    need_stack = std::max(need_stack, stack_need(s));
     if (j.params.env_is && s->has_env) any_env_is = true;
     if (j.params.nee && s->emissive_specular)
       return fail(OMR_ERR_INVALID, "next-event estimation on a scene whose mirror or glass materials emit is not "
@@ -702,7 +717,7 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     D.max_depth = j.params.max_depth;
     D.rr_depth = j.params.rr_depth;
     D.flags = (j.params.env_is ? 1u : 0u) | (j.params.nee ? 2u : 0u);
-    D.scene = j.scene;
+    D.scene = scene_slot((omr_scene_id)j.scene);  // the device scene table is indexed by slot
     D.seed = j.params.seed;
     for (int c = 0; c < 3; c++) {
       D.cam_pos[c] = j.cam.position[c];

@@ -22,7 +22,11 @@
 
 namespace omr {
 
+// Resident scenes per context.  A scene id is use * kMaxScenes + slot: a
+// released slot takes the next upload, under a new id, and the device scene
+// table is indexed by the slot.
 constexpr uint32_t kMaxScenes = 64;
+inline uint32_t scene_slot(omr_scene_id id) { return (uint32_t)id % kMaxScenes; }
 
 // Summaries of the sharded control block as read back to the host.
 inline unsigned long long sum_shards(const Sharded<unsigned long long> &a) {
@@ -49,6 +53,7 @@ inline bool samples_exhausted(const Ctrl &c) {
 }
 
 struct SceneHost {
+  omr_scene_id id = -1;
   BuildOutput bvh;
   SceneDev dev{};
   std::vector<void *> allocs;
@@ -107,6 +112,10 @@ class Engine {
   SceneHost *scene(omr_scene_id id);
   // Device bytes of every resident scene.
   uint64_t scene_bytes();
+  // traversal stack entries a scene needs at the current node width
+  uint32_t stack_need(const SceneHost *s) const {
+    return tuning.node_width == 8 ? s->bvh.stack_bound8 : s->bvh.stack_bound;
+  }
   // Device bytes of the path pool, its queues and the traversal overflow
   // stacks as allocated now.
   uint64_t pool_bytes() const;
@@ -152,7 +161,7 @@ class Engine {
   void *noalloc_scratch_ = nullptr;  // device address of this GPU's allocation-free scratch
   std::string noalloc_err_;
   std::unique_ptr<SceneHost> scenes_[kMaxScenes];
-  std::atomic<size_t> n_scenes_{0};  // slots handed out so far (released slots are not reused)
+  uint32_t slot_uses_[kMaxScenes] = {};  // uploads each slot has held (the id's high part)
   SceneDev *d_scenes_ = nullptr;
 
   // pool

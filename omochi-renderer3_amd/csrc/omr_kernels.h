@@ -31,6 +31,7 @@ struct PoolPtrs {
   uint32_t df_budget;     // dynamic fetch: traversal visits between fetches (0 = grid-stride)
   uint32_t df_fetch_min;  // dynamic fetch: idle lanes of a wave that trigger a fetch
   uint32_t stack_short;   // LDS traversal stack entries per lane (8 or 16)
+  uint32_t node_width;    // BVH nodes the traversal reads: 4 (SceneDev::nodes) or 8 (SceneDev::nodes8)
 };
 
 // KernelArgs::solo when a run may hold several jobs
@@ -102,8 +103,9 @@ hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, d
 // node visits and primitive tests.
 // head: zeroed fetch heads (used when budget > 0, dynamic fetch).
 hipError_t trace_rays(const SceneDev *scene, const float *rays, uint32_t n, int32_t *prim, float *t,
-                      unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t stack_short, uint32_t grid,
-                      uint32_t wg, Sharded<uint32_t> *head, int budget, uint32_t fetch_min, hipStream_t st);
+                      unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t stack_short,
+                      uint32_t node_width, uint32_t grid, uint32_t wg, Sharded<uint32_t> *head, int budget,
+                      uint32_t fetch_min, hipStream_t st);
 // Environment importance-sampling tables: weights, per-row scan, marginal scan.
 hipError_t env_tables(const float *env, uint32_t w, uint32_t h, float *wbuf, unsigned int *wmax_bits,
                       unsigned long long *q, unsigned long long *row, unsigned long long *marg, hipStream_t st);

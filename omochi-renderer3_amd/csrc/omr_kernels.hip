@@ -321,7 +321,7 @@ __device__ inline void cswap(float &ka, int32_t &ca, float &kb, int32_t &cb) {
 // The traversal is resumable: init() sets up a ray, step(k) performs at most
 // k node or leaf visits and reports whether the ray is finished.  traverse()
 // runs one ray to the end.
-template <int WG, int SHORT, bool ANY, bool STATS>
+template <int WG, int SHORT, bool ANY, bool STATS, int NW>
 struct Trav {
   const float4 *nodes, *prims;
   f3 o, d;
@@ -360,7 +360,7 @@ struct Trav {
     nx = !(d.x >= 0.0f) ? 1u : 0u;
     ny = !(d.y >= 0.0f) ? 1u : 0u;
     nz = !(d.z >= 0.0f) ? 1u : 0u;
-    nodes = reinterpret_cast<const float4 *>(S.nodes);
+    nodes = NW == 8 ? reinterpret_cast<const float4 *>(S.nodes8) : reinterpret_cast<const float4 *>(S.nodes);
     prims = reinterpret_cast<const float4 *>(S.prims);
     planes = S.planes;
     n_planes = S.n_planes;
@@ -390,8 +390,8 @@ struct Trav {
 constexpr uint32_t kLeafBatch = 32;
 constexpr int32_t kNoEntry = -1;
 
-template <int WG, int SHORT, bool ANY, bool STATS>
-__device__ inline bool Trav<WG, SHORT, ANY, STATS>::step(Stack<WG, SHORT> &stk, int budget, uint32_t &nvis,
+template <int WG, int SHORT, bool ANY, bool STATS, int NW>
+__device__ inline bool Trav<WG, SHORT, ANY, STATS, NW>::step(Stack<WG, SHORT> &stk, int budget, uint32_t &nvis,
                                                           uint32_t &ntest) {
   const float ox = o.x, oy = o.y, oz = o.z;
   // next stack entry that can still hold a closer hit, or kNoEntry
@@ -417,52 +417,76 @@ __device__ inline bool Trav<WG, SHORT, ANY, STATS>::step(Stack<WG, SHORT> &stk, 
     const uint32_t lanes = (uint32_t)__popcll(__ballot(true));
     if (vm != 0ull && leaves * 64u < kLeafBatch * lanes) {
       if (can_visit) {
-        // node = lo.x, hi.x, lo.y, hi.y, lo.z, hi.z (4 children each), child
-        // codes: the near and far slab of each axis are picked by address
-        const float4 *np = nodes + (size_t)cur * 8;
-        float4 nX = ld4(np + nx), fX = ld4(np + (nx ^ 1u));
-        float4 nY = ld4(np + 2u + ny), fY = ld4(np + 2u + (ny ^ 1u));
-        float4 nZ = ld4(np + 4u + nz), fZ = ld4(np + 4u + (nz ^ 1u));
-        float4 chf = ld4(np + 6);
+        // node = lo.x, hi.x, lo.y, hi.y, lo.z, hi.z (NW children each), child
+        // codes: the near and far slab of each axis are picked by address;
+        // an 8-wide node holds each bound as two float4 (children 0-3, 4-7)
+        constexpr uint32_t Q = NW / 4;  // float4 per bound
+        const float4 *np = nodes + (size_t)cur * (8u * Q);
         if (STATS) nvis++;
-        float near_x[4] = {nX.x, nX.y, nX.z, nX.w}, far_x[4] = {fX.x, fX.y, fX.z, fX.w};
-        float near_y[4] = {nY.x, nY.y, nY.z, nY.w}, far_y[4] = {fY.x, fY.y, fY.z, fY.w};
-        float near_z[4] = {nZ.x, nZ.y, nZ.z, nZ.w}, far_z[4] = {fZ.x, fZ.y, fZ.z, fZ.w};
-        int32_t code[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z),
-                           __float_as_int(chf.w)};
-        float key[4];
-        // tmin = n > tmin ? n : tmin (and tmax likewise) as one v_max / v_min:
-        // equal for every input here (a NaN operand leaves the other one in
-        // both forms; n, f are never signalling NaNs; +0 and -0 compare equal
-        // in every later test)
+        float key[NW];
+        int32_t code[NW];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          float tmin = 0.0f, tmax = best.t;
-          float n = (near_x[k] - ox) * ix;
-          float f = (far_x[k] - ox) * ix;
-          tmin = __builtin_fmaxf(n, tmin);
-          tmax = __builtin_fminf(f, tmax);
-          n = (near_y[k] - oy) * iy;
-          f = (far_y[k] - oy) * iy;
-          tmin = __builtin_fmaxf(n, tmin);
-          tmax = __builtin_fminf(f, tmax);
-          n = (near_z[k] - oz) * iz;
-          f = (far_z[k] - oz) * iz;
-          tmin = __builtin_fmaxf(n, tmin);
-          tmax = __builtin_fminf(f, tmax);
-          bool h = (code[k] != -1) && (tmin <= tmax * kSlabInflate);
-          key[k] = h ? tmin : __builtin_huge_valf();
+        for (uint32_t h = 0; h < Q; h++) {
+          float4 nX = ld4(np + Q * nx + h), fX = ld4(np + Q * (nx ^ 1u) + h);
+          float4 nY = ld4(np + Q * (2u + ny) + h), fY = ld4(np + Q * (2u + (ny ^ 1u)) + h);
+          float4 nZ = ld4(np + Q * (4u + nz) + h), fZ = ld4(np + Q * (4u + (nz ^ 1u)) + h);
+          float4 chf = ld4(np + Q * 6u + h);
+          float near_x[4] = {nX.x, nX.y, nX.z, nX.w}, far_x[4] = {fX.x, fX.y, fX.z, fX.w};
+          float near_y[4] = {nY.x, nY.y, nY.z, nY.w}, far_y[4] = {fY.x, fY.y, fY.z, fY.w};
+          float near_z[4] = {nZ.x, nZ.y, nZ.z, nZ.w}, far_z[4] = {fZ.x, fZ.y, fZ.z, fZ.w};
+          int32_t c4[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z),
+                           __float_as_int(chf.w)};
+          // tmin = n > tmin ? n : tmin (and tmax likewise) as one v_max / v_min:
+          // equal for every input here (a NaN operand leaves the other one in
+          // both forms; n, f are never signalling NaNs; +0 and -0 compare equal
+          // in every later test)
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            float tmin = 0.0f, tmax = best.t;
+            float n = (near_x[k] - ox) * ix;
+            float f = (far_x[k] - ox) * ix;
+            tmin = __builtin_fmaxf(n, tmin);
+            tmax = __builtin_fminf(f, tmax);
+            n = (near_y[k] - oy) * iy;
+            f = (far_y[k] - oy) * iy;
+            tmin = __builtin_fmaxf(n, tmin);
+            tmax = __builtin_fminf(f, tmax);
+            n = (near_z[k] - oz) * iz;
+            f = (far_z[k] - oz) * iz;
+            tmin = __builtin_fmaxf(n, tmin);
+            tmax = __builtin_fminf(f, tmax);
+            bool hit = (c4[k] != -1) && (tmin <= tmax * kSlabInflate);
+            key[4 * h + k] = hit ? tmin : __builtin_huge_valf();
+            code[4 * h + k] = c4[k];
+          }
+          // sorting network, nearest first: each group of four, then (8-wide)
+          // Batcher's odd-even merge of the two sorted groups
+          float *kk = key + 4 * h;
+          int32_t *cc = code + 4 * h;
+          cswap(kk[0], cc[0], kk[1], cc[1]);
+          cswap(kk[2], cc[2], kk[3], cc[3]);
+          cswap(kk[0], cc[0], kk[2], cc[2]);
+          cswap(kk[1], cc[1], kk[3], cc[3]);
+          cswap(kk[1], cc[1], kk[2], cc[2]);
+          // 8-wide: the second group's bounds are loaded only after the
+          // first group is sorted, so one group's 28 words are live at a time
+          if (NW == 8 && h == 0) __builtin_amdgcn_sched_barrier(0);
         }
-        // sorting network, nearest first
-        cswap(key[0], code[0], key[1], code[1]);
-        cswap(key[2], code[2], key[3], code[3]);
-        cswap(key[0], code[0], key[2], code[2]);
-        cswap(key[1], code[1], key[3], code[3]);
-        cswap(key[1], code[1], key[2], code[2]);
+        if constexpr (NW == 8) {
+          cswap(key[0], code[0], key[4], code[4]);
+          cswap(key[1], code[1], key[5], code[5]);
+          cswap(key[2], code[2], key[6], code[6]);
+          cswap(key[3], code[3], key[7], code[7]);
+          cswap(key[2], code[2], key[4], code[4]);
+          cswap(key[3], code[3], key[5], code[5]);
+          cswap(key[1], code[1], key[2], code[2]);
+          cswap(key[3], code[3], key[4], code[4]);
+          cswap(key[5], code[5], key[6], code[6]);
+        }
         const float kmiss = __builtin_huge_valf();
-        if (key[3] != kmiss) stk.push(make_uint2((uint32_t)code[3], __float_as_uint(key[3])));
-        if (key[2] != kmiss) stk.push(make_uint2((uint32_t)code[2], __float_as_uint(key[2])));
-        if (key[1] != kmiss) stk.push(make_uint2((uint32_t)code[1], __float_as_uint(key[1])));
+#pragma unroll
+        for (int k = NW - 1; k >= 1; k--)
+          if (key[k] != kmiss) stk.push(make_uint2((uint32_t)code[k], __float_as_uint(key[k])));
         cur = (key[0] != kmiss && !(key[0] > best.t)) ? code[0] : pop_next();
         postpone();
       }
@@ -516,10 +540,10 @@ __device__ inline bool Trav<WG, SHORT, ANY, STATS>::step(Stack<WG, SHORT> &stk, 
   return false;
 }
 
-template <int WG, int SHORT, bool ANY, bool STATS>
+template <int WG, int SHORT, bool ANY, bool STATS, int NW>
 __device__ inline void traverse(const SceneDev &S, f3 o, f3 d, Hit &best, Stack<WG, SHORT> &stk, uint32_t &nvis,
                                 uint32_t &ntest) {
-  Trav<WG, SHORT, ANY, STATS> tr;
+  Trav<WG, SHORT, ANY, STATS, NW> tr;
   tr.init(S, o, d, best.t, stk);
   tr.step(stk, 0x7FFFFFFF, nvis, ntest);
   best = tr.best;
@@ -537,11 +561,11 @@ __device__ inline void traverse(const SceneDev &S, f3 o, f3 d, Hit &best, Stack<
 // load(k, off, S, o, d, tmax, tag) reads one entry; store(tag, hit) writes a
 // finished ray's result.
 static_assert(kShards <= 64, "the sub-queue scan gives one lane of a wave to each shard");
-template <int WG, int SHORT, bool ANY, bool STATS, class Count, class Load, class Store>
+template <int WG, int SHORT, bool ANY, bool STATS, int NW, class Count, class Load, class Store>
 __device__ inline uint32_t dynamic_fetch(Sharded<uint32_t> &heads, Count count, Stack<WG, SHORT> &stk, int budget,
                                          uint32_t fetch_min, Load load, Store store, uint32_t &nvis,
                                          uint32_t &ntest) {
-  Trav<WG, SHORT, ANY, STATS> tr;
+  Trav<WG, SHORT, ANY, STATS, NW> tr;
   bool has = false, exhausted = false;
   uint32_t tag = 0, done = 0, shard = wave_shard();
   const uint32_t active = (uint32_t)__popcll(__ballot(true));
@@ -946,7 +970,7 @@ __global__ __launch_bounds__(256) void k_fill_fresh(KernelArgs A, uint32_t n0) {
   }
 }
 
-template <int WG, int SHORT, bool STATS, bool DF, bool SOLO>
+template <int WG, int SHORT, int NW, bool STATS, bool DF, bool SOLO>
 __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
   __shared__ uint2 lds[SHORT * WG];
   __shared__ uint32_t s_pre[kShards + 1];
@@ -984,7 +1008,7 @@ __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
     auto store = [&](uint32_t slot, const Hit &h) {
       reinterpret_cast<float4 *>(A.p.hit)[slot] = make_float4(h.t, __int_as_float(h.prim), h.u, h.v);
     };
-    segs = dynamic_fetch<WG, SHORT, false, STATS>(A.ctrl->qhead[parity], count, stk, (int)A.p.df_budget,
+    segs = dynamic_fetch<WG, SHORT, false, STATS, NW>(A.ctrl->qhead[parity], count, stk, (int)A.p.df_budget,
                                                  A.p.df_fetch_min, load, store, nv, nt);
   }
   for (uint32_t base = blockIdx.x * WG; !DF && base < n; base += stride) {
@@ -998,7 +1022,7 @@ __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
       h.t = kInf;
       h.u = h.v = 0.0f;
       h.prim = -1;
-      traverse<WG, SHORT, false, STATS>(S, xyz(ro), xyz(rd), h, stk, nv, nt);
+      traverse<WG, SHORT, false, STATS, NW>(S, xyz(ro), xyz(rd), h, stk, nv, nt);
       reinterpret_cast<float4 *>(A.p.hit)[slot] = make_float4(h.t, __int_as_float(h.prim), h.u, h.v);
       segs++;
     }
@@ -1363,7 +1387,7 @@ __device__ inline void add_shadow(const KernelArgs &A, uint32_t r, const JobDev 
   deposit(J.recs, J.accum, __float_as_uint(c.w), __float_as_uint(b.w), fix(c.x), fix(c.y), fix(c.z));
 }
 
-template <int WG, int SHORT, bool DF, bool SOLO, bool TAG = false>
+template <int WG, int SHORT, int NW, bool DF, bool SOLO, bool TAG = false>
 __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
   __shared__ uint2 lds[SHORT * WG];
   __shared__ uint32_t s_pre[kShards + 1];
@@ -1399,7 +1423,7 @@ __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
       if (h.prim < 0) add_shadow<SOLO>(A, r, Js);
     };
     uint32_t nv = 0, nt = 0;
-    cnt = dynamic_fetch<WG, SHORT, true, false>(A.ctrl->shead, count, stk, (int)A.p.df_budget, A.p.df_fetch_min,
+    cnt = dynamic_fetch<WG, SHORT, true, false, NW>(A.ctrl->shead, count, stk, (int)A.p.df_budget, A.p.df_fetch_min,
                                                load, store, nv, nt);
   }
   for (uint32_t base = blockIdx.x * WG; !DF && base < n; base += stride) {
@@ -1416,7 +1440,7 @@ __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
       h.u = h.v = 0.0f;
       h.prim = -1;
       uint32_t nv = 0, nt = 0;
-      traverse<WG, SHORT, true, false>(*S, o, d, h, stk, nv, nt);
+      traverse<WG, SHORT, true, false, NW>(*S, o, d, h, stk, nv, nt);
       if (h.prim < 0) add_shadow<SOLO>(A, r, Js);
       cnt++;
     }
@@ -1434,8 +1458,10 @@ __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
 // Held to 128 VGPRs (four waves per SIMD; nine registers spill, which costs
 // less than the fourth wave gains: 6.8 -> 5.9 ms on the Cornell + icosphere
 // render).
-template <int WG, int SHORT, bool STATS, bool SOLO>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_drain(KernelArgs A, int parity) {
+// Four waves per SIMD (128 VGPRs) with 4-wide nodes; the 8-wide node visit
+// holds twice the sort keys and spilled 38-75 VGPRs there, so it runs three.
+template <int WG, int SHORT, int NW, bool STATS, bool SOLO>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 3 : 4))) void k_drain(KernelArgs A, int parity) {
   __shared__ uint2 lds[SHORT * WG];
   __shared__ uint32_t s_pre[kShards + 1];
   const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
@@ -1463,7 +1489,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
   // wave shades once half of its paths are ready or none is tracing
   bool tracing = false, ready = false;
   const int budget = A.p.df_budget > 0 ? (int)A.p.df_budget : 8;
-  Trav<WG, SHORT, false, STATS> tr;
+  Trav<WG, SHORT, false, STATS, NW> tr;
   uint32_t slot = 0;
   PathState st;
   st.js = 0;
@@ -1540,7 +1566,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
       hs.u = hs.v = 0.0f;
       hs.prim = -1;
       uint32_t snv = 0, snt = 0;
-      traverse<WG, SHORT, true, false>(S, sh.o, sh.d, hs, stk, snv, snt);
+      traverse<WG, SHORT, true, false, NW>(S, sh.o, sh.d, hs, stk, snv, snt);
       shadows++;
       if (hs.prim < 0) deposit(J.recs, J.accum, st.sidx, sh.px, fix(sh.c.x), fix(sh.c.y), fix(sh.c.z));
     }
@@ -1680,7 +1706,7 @@ __global__ void k_resolve(const unsigned long long *accum, float *out, uint32_t 
   if (i < n) out[i] = (float)((double)(long long)accum[i] * (1.0 / 16777216.0) / n_samples);
 }
 
-template <int WG, int SHORT, bool STATS, bool DF>
+template <int WG, int SHORT, int NW, bool STATS, bool DF>
 __global__ __launch_bounds__(WG) void k_trace_rays(const SceneDev *scene, const float4 *rays, uint32_t n, int32_t *prim,
                                                    float *tout, unsigned long long *counters, uint2 *gstack,
                                                    uint32_t gcap, Sharded<uint32_t> *head, int budget, uint32_t fetch_min) {
@@ -1709,7 +1735,7 @@ __global__ __launch_bounds__(WG) void k_trace_rays(const SceneDev *scene, const 
       prim[i] = h.prim;
       tout[i] = h.prim >= 0 ? h.t : kInf;
     };
-    dynamic_fetch<WG, SHORT, false, STATS>(*head, count, stk, budget, fetch_min, load, store, nv, nt);
+    dynamic_fetch<WG, SHORT, false, STATS, NW>(*head, count, stk, budget, fetch_min, load, store, nv, nt);
   }
   for (uint32_t base = blockIdx.x * WG; !DF && base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
@@ -1719,7 +1745,7 @@ __global__ __launch_bounds__(WG) void k_trace_rays(const SceneDev *scene, const 
       h.t = a.w;
       h.u = h.v = 0.0f;
       h.prim = -1;
-      traverse<WG, SHORT, false, STATS>(S, xyz(a), xyz(b), h, stk, nv, nt);
+      traverse<WG, SHORT, false, STATS, NW>(S, xyz(a), xyz(b), h, stk, nv, nt);
       prim[i] = h.prim;
       tout[i] = h.prim >= 0 ? h.t : kInf;
     }
@@ -1847,19 +1873,30 @@ __global__ void k_env_guides(const unsigned long long *row, const unsigned long 
   }
 }
 
-template <int WG, int SHORT>
-hipError_t launch_extend_ws(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
+// The launchers pick the template instance of the run's launch choices:
+// group size, LDS stack depth (a.p.stack_short), node width (a.p.node_width),
+// fetch mode, statistics, and the single-job form.
+template <int WG, int SHORT, int NW>
+hipError_t launch_extend_n(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
   bool df = a.p.df_budget > 0;
   if (WG == 64 && df && a.solo != kNoSolo) {  // single-job runs at the default group size and fetch mode
-    if (stats) hipLaunchKernelGGL((k_extend<WG, SHORT, true, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
-    else hipLaunchKernelGGL((k_extend<WG, SHORT, false, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+    if (stats) hipLaunchKernelGGL((k_extend<WG, SHORT, NW, true, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+    else hipLaunchKernelGGL((k_extend<WG, SHORT, NW, false, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
     return hipGetLastError();
   }
-  if (stats && df) hipLaunchKernelGGL((k_extend<WG, SHORT, true, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else if (stats) hipLaunchKernelGGL((k_extend<WG, SHORT, true, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else if (df) hipLaunchKernelGGL((k_extend<WG, SHORT, false, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else hipLaunchKernelGGL((k_extend<WG, SHORT, false, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  if (stats && df)
+    hipLaunchKernelGGL((k_extend<WG, SHORT, NW, true, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else if (stats)
+    hipLaunchKernelGGL((k_extend<WG, SHORT, NW, true, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else if (df)
+    hipLaunchKernelGGL((k_extend<WG, SHORT, NW, false, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else hipLaunchKernelGGL((k_extend<WG, SHORT, NW, false, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
   return hipGetLastError();
+}
+template <int WG, int SHORT>
+hipError_t launch_extend_ws(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
+  if (a.p.node_width == 8) return launch_extend_n<WG, SHORT, 8>(a, grid, parity, stats, st);
+  return launch_extend_n<WG, SHORT, 4>(a, grid, parity, stats, st);
 }
 template <int WG>
 hipError_t launch_extend_wg(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
@@ -1867,61 +1904,69 @@ hipError_t launch_extend_wg(const KernelArgs &a, uint32_t grid, int parity, bool
   return launch_extend_ws<WG, (int)kShortStackMax>(a, grid, parity, stats, st);
 }
 
-template <int WG, int SHORT>
-hipError_t launch_shadow_ws(const KernelArgs &a, uint32_t grid, bool tag, hipStream_t st) {
+template <int WG, int SHORT, int NW>
+hipError_t launch_shadow_n(const KernelArgs &a, uint32_t grid, bool tag, hipStream_t st) {
   if (WG == 64 && a.p.df_budget > 0 && a.solo != kNoSolo && tag)  // profiled single-job step
-    hipLaunchKernelGGL((k_shadow<WG, SHORT, true, true, true>), dim3(grid), dim3(WG), 0, st, a);
+    hipLaunchKernelGGL((k_shadow<WG, SHORT, NW, true, true, true>), dim3(grid), dim3(WG), 0, st, a);
   else if (WG == 64 && a.p.df_budget > 0 && a.solo != kNoSolo)  // single-job runs, default group size and fetch mode
-    hipLaunchKernelGGL((k_shadow<WG, SHORT, true, true>), dim3(grid), dim3(WG), 0, st, a);
-  else if (a.p.df_budget > 0) hipLaunchKernelGGL((k_shadow<WG, SHORT, true, false>), dim3(grid), dim3(WG), 0, st, a);
-  else hipLaunchKernelGGL((k_shadow<WG, SHORT, false, false>), dim3(grid), dim3(WG), 0, st, a);
+    hipLaunchKernelGGL((k_shadow<WG, SHORT, NW, true, true>), dim3(grid), dim3(WG), 0, st, a);
+  else if (a.p.df_budget > 0) hipLaunchKernelGGL((k_shadow<WG, SHORT, NW, true, false>), dim3(grid), dim3(WG), 0, st, a);
+  else hipLaunchKernelGGL((k_shadow<WG, SHORT, NW, false, false>), dim3(grid), dim3(WG), 0, st, a);
   return hipGetLastError();
 }
 template <int WG>
 hipError_t launch_shadow_wg(const KernelArgs &a, uint32_t grid, bool tag, hipStream_t st) {
-  if (a.p.stack_short == kShortStackMin) return launch_shadow_ws<WG, (int)kShortStackMin>(a, grid, tag, st);
-  return launch_shadow_ws<WG, (int)kShortStackMax>(a, grid, tag, st);
+  const bool s8 = a.p.stack_short == kShortStackMin, w8 = a.p.node_width == 8;
+  constexpr int S8 = (int)kShortStackMin, S16 = (int)kShortStackMax;
+  if (s8) return w8 ? launch_shadow_n<WG, S8, 8>(a, grid, tag, st) : launch_shadow_n<WG, S8, 4>(a, grid, tag, st);
+  return w8 ? launch_shadow_n<WG, S16, 8>(a, grid, tag, st) : launch_shadow_n<WG, S16, 4>(a, grid, tag, st);
 }
 
-template <int WG>
-hipError_t launch_drain_wg(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
-  const bool s8 = a.p.stack_short == kShortStackMin;
-  constexpr int S8 = (int)kShortStackMin, S16 = (int)kShortStackMax;
+template <int WG, int SHORT, int NW>
+hipError_t launch_drain_n(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
   if (WG == 64 && a.solo != kNoSolo) {  // single-job runs at the default group size
-    if (stats && s8) hipLaunchKernelGGL((k_drain<WG, S8, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
-    else if (stats) hipLaunchKernelGGL((k_drain<WG, S16, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
-    else if (s8) hipLaunchKernelGGL((k_drain<WG, S8, false, true>), dim3(grid), dim3(WG), 0, st, a, parity);
-    else hipLaunchKernelGGL((k_drain<WG, S16, false, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+    if (stats) hipLaunchKernelGGL((k_drain<WG, SHORT, NW, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+    else hipLaunchKernelGGL((k_drain<WG, SHORT, NW, false, true>), dim3(grid), dim3(WG), 0, st, a, parity);
     return hipGetLastError();
   }
-  if (stats && s8) hipLaunchKernelGGL((k_drain<WG, S8, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else if (stats) hipLaunchKernelGGL((k_drain<WG, S16, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else if (s8) hipLaunchKernelGGL((k_drain<WG, S8, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
-  else hipLaunchKernelGGL((k_drain<WG, S16, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  if (stats) hipLaunchKernelGGL((k_drain<WG, SHORT, NW, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+  else hipLaunchKernelGGL((k_drain<WG, SHORT, NW, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
   return hipGetLastError();
 }
+template <int WG>
+hipError_t launch_drain_wg(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
+  const bool s8 = a.p.stack_short == kShortStackMin, w8 = a.p.node_width == 8;
+  constexpr int S8 = (int)kShortStackMin, S16 = (int)kShortStackMax;
+  if (s8)
+    return w8 ? launch_drain_n<WG, S8, 8>(a, grid, parity, stats, st) : launch_drain_n<WG, S8, 4>(a, grid, parity, stats, st);
+  return w8 ? launch_drain_n<WG, S16, 8>(a, grid, parity, stats, st)
+            : launch_drain_n<WG, S16, 4>(a, grid, parity, stats, st);
+}
 
-template <int WG, int SHORT, bool STATS>
-hipError_t launch_trace_ws(const SceneDev *scene, const float4 *r4, uint32_t n, int32_t *prim, float *t,
-                           unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t grid,
-                           Sharded<uint32_t> *head, int budget, uint32_t fetch_min, hipStream_t st) {
+template <int WG, int SHORT, int NW, bool STATS>
+hipError_t launch_trace_n(const SceneDev *scene, const float4 *r4, uint32_t n, int32_t *prim, float *t,
+                          unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t grid,
+                          Sharded<uint32_t> *head, int budget, uint32_t fetch_min, hipStream_t st) {
   if (budget > 0)
-    hipLaunchKernelGGL((k_trace_rays<WG, SHORT, STATS, true>), dim3(grid), dim3(WG), 0, st, scene, r4, n, prim, t,
+    hipLaunchKernelGGL((k_trace_rays<WG, SHORT, NW, STATS, true>), dim3(grid), dim3(WG), 0, st, scene, r4, n, prim, t,
                        counters, gstack, gcap, head, budget, fetch_min);
   else
-    hipLaunchKernelGGL((k_trace_rays<WG, SHORT, STATS, false>), dim3(grid), dim3(WG), 0, st, scene, r4, n, prim, t,
-                       counters, gstack, gcap, head, budget, fetch_min);
+    hipLaunchKernelGGL((k_trace_rays<WG, SHORT, NW, STATS, false>), dim3(grid), dim3(WG), 0, st, scene, r4, n, prim,
+                       t, counters, gstack, gcap, head, budget, fetch_min);
   return hipGetLastError();
 }
 template <int WG, bool STATS>
 hipError_t launch_trace_wg(const SceneDev *scene, const float4 *r4, uint32_t n, int32_t *prim, float *t,
                            unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t stack_short,
-                           uint32_t grid, Sharded<uint32_t> *head, int budget, uint32_t fetch_min, hipStream_t st) {
-  if (stack_short == kShortStackMin)
-    return launch_trace_ws<WG, (int)kShortStackMin, STATS>(scene, r4, n, prim, t, counters, gstack, gcap, grid, head,
-                                                            budget, fetch_min, st);
-  return launch_trace_ws<WG, (int)kShortStackMax, STATS>(scene, r4, n, prim, t, counters, gstack, gcap, grid, head,
-                                                          budget, fetch_min, st);
+                           uint32_t node_width, uint32_t grid, Sharded<uint32_t> *head, int budget, uint32_t fetch_min,
+                           hipStream_t st) {
+  constexpr int S8 = (int)kShortStackMin, S16 = (int)kShortStackMax;
+  const bool s8 = stack_short == kShortStackMin, w8 = node_width == 8;
+#define OMR_TRACE(S, W) \
+  launch_trace_n<WG, S, W, STATS>(scene, r4, n, prim, t, counters, gstack, gcap, grid, head, budget, fetch_min, st)
+  if (s8) return w8 ? OMR_TRACE(S8, 8) : OMR_TRACE(S8, 4);
+  return w8 ? OMR_TRACE(S16, 8) : OMR_TRACE(S16, 4);
+#undef OMR_TRACE
 }
 
 __global__ void k_zero_words(uint32_t *p, uint32_t n) {
@@ -2039,21 +2084,23 @@ hipError_t resolve(const unsigned long long *accum, float *out, uint32_t n_px, d
 }
 
 hipError_t trace_rays(const SceneDev *scene, const float *rays, uint32_t n, int32_t *prim, float *t,
-                      unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t stack_short, uint32_t grid,
-                      uint32_t wg, Sharded<uint32_t> *head, int budget, uint32_t fetch_min, hipStream_t st) {
+                      unsigned long long *counters, uint2 *gstack, uint32_t gcap, uint32_t stack_short,
+                      uint32_t node_width, uint32_t grid, uint32_t wg, Sharded<uint32_t> *head, int budget,
+                      uint32_t fetch_min, hipStream_t st) {
   const float4 *r4 = reinterpret_cast<const float4 *>(rays);
+  const uint32_t ss = stack_short, nw = node_width;
   if (counters)
-    return launch_trace_wg<128, true>(scene, r4, n, prim, t, counters, gstack, gcap, stack_short, grid, head, budget,
+    return launch_trace_wg<128, true>(scene, r4, n, prim, t, counters, gstack, gcap, ss, nw, grid, head, budget,
                                       fetch_min, st);
   switch (wg) {
     case 64:
-      return launch_trace_wg<64, false>(scene, r4, n, prim, t, nullptr, gstack, gcap, stack_short, grid, head, budget,
+      return launch_trace_wg<64, false>(scene, r4, n, prim, t, nullptr, gstack, gcap, ss, nw, grid, head, budget,
                                         fetch_min, st);
     case 256:
-      return launch_trace_wg<256, false>(scene, r4, n, prim, t, nullptr, gstack, gcap, stack_short, grid, head, budget,
+      return launch_trace_wg<256, false>(scene, r4, n, prim, t, nullptr, gstack, gcap, ss, nw, grid, head, budget,
                                          fetch_min, st);
     default:
-      return launch_trace_wg<128, false>(scene, r4, n, prim, t, nullptr, gstack, gcap, stack_short, grid, head, budget,
+      return launch_trace_wg<128, false>(scene, r4, n, prim, t, nullptr, gstack, gcap, ss, nw, grid, head, budget,
                                          fetch_min, st);
   }
 }

@@ -34,6 +34,16 @@ struct alignas(128) WNode4 {
 };
 static_assert(sizeof(WNode4) == 128, "node must be one cache line");
 
+// 8-wide BVH node, 256 bytes = two cache lines, the same component-major
+// layout over eight children (a bound of children 0-3, then of 4-7, is one
+// float4 each).
+struct alignas(256) WNode8 {
+  float lox[8], hix[8], loy[8], hiy[8], loz[8], hiz[8];
+  int32_t child[8];
+  int32_t meta[8];  // meta[0] = depth of this node (diagnostics)
+};
+static_assert(sizeof(WNode8) == 256, "8-wide node is two cache lines");
+
 OMR_HD inline bool child_is_leaf(int32_t c) { return ((uint32_t)c & 0xC0000000u) == 0x80000000u; }
 OMR_HD inline uint32_t leaf_first(int32_t c) { return (uint32_t)c & 0x07FFFFFFu; }
 OMR_HD inline uint32_t leaf_count(int32_t c) { return (((uint32_t)c >> 27) & 7u) + 1u; }
@@ -75,6 +85,7 @@ struct alignas(8) uint2_t {
 // Device-side view of one resident scene.
 struct SceneDev {
   const WNode4 *nodes;
+  const WNode8 *nodes8;        // the same tree collapsed to 8-wide nodes (same leaves)
   const PrimRec *prims;        // leaf order
   const float *tri_n;          // per original triangle: 12 floats (n0,_, dn1,_, dn2,_)
   const float *sph;            // per original sphere: 4 floats (c, r)
@@ -100,7 +111,9 @@ struct SceneDev {
   uint32_t n_tex;
   uint32_t env_w, env_h;
   uint32_t n_nodes;
-  uint32_t stack_bound;        // max traversal stack entries
+  uint32_t stack_bound;        // max traversal stack entries (4-wide nodes)
+  uint32_t n_nodes8;
+  uint32_t stack_bound8;       // max traversal stack entries (8-wide nodes)
 };
 
 // One render job in the device job ring.

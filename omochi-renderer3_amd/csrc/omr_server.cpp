@@ -453,7 +453,7 @@ void Engine::server_loop() {
       uint32_t need_stack = 1;
       bool okd = true;
       for (auto &r : admit) {
-        need_stack = std::max(need_stack, r->sh->bvh.stack_bound);
+        need_stack = std::max(need_stack, stack_need(r->sh));
         JobRun &j = r->run;
         j.scene = (uint32_t)r->scene;
         j.cam = r->cam;
@@ -483,7 +483,7 @@ void Engine::server_loop() {
         D.max_depth = j.params.max_depth;
         D.rr_depth = j.params.rr_depth;
         D.flags = (j.params.env_is ? 1u : 0u) | (j.params.nee ? 2u : 0u);
-        D.scene = j.scene;
+        D.scene = scene_slot((omr_scene_id)j.scene);  // the device scene table is indexed by slot
         D.seed = j.params.seed;
         for (int c = 0; c < 3; c++) {
           D.cam_pos[c] = j.cam.position[c];

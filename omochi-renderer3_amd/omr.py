@@ -72,7 +72,8 @@ class RenderStats(C.Structure):
 
 class SceneInfo(C.Structure):
     _fields_ = [("n_nodes", C.c_uint32), ("depth", C.c_uint32), ("stack_bound", C.c_uint32), ("n_prims", C.c_uint32),
-                ("device_bytes", C.c_uint64), ("build_ms", C.c_double)]
+                ("device_bytes", C.c_uint64), ("build_ms", C.c_double), ("n_nodes8", C.c_uint32),
+                ("depth8", C.c_uint32), ("stack_bound8", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class Tuning(C.Structure):
@@ -127,7 +128,7 @@ EXPORTS = [
     "omr_get_tuning", "omr_set_tuning", "omr_server_start", "omr_server_stop", "omr_job_submit", "omr_job_poll",
     "omr_job_wait", "omr_job_error", "omr_job_latency", "omr_job_cancel", "omr_server_stats_get",
     "omr_serve_simulate", "omr_procedural_scene", "omr_procedural_free",
-    "omr_camera_from_lookat", "omr_bvh_build", "omr_bvh_data", "omr_bvh_free", "omr_set_profiling", "omr_autotune",
+    "omr_camera_from_lookat", "omr_bvh_build", "omr_bvh_data", "omr_bvh_data8", "omr_bvh_free", "omr_set_profiling", "omr_autotune",
     "omr_procedural_camera", "omr_io_last_error", "omr_free", "omr_hdr_read", "omr_hdr_write", "omr_scene_file_load",
     "omr_obj_load", "omr_loaded_scene_free", "omr_image_read", "omr_texture_read",
 ]
@@ -189,6 +190,7 @@ def load():
     lib.omr_bvh_build.argtypes = [C.POINTER(SceneDesc), C.POINTER(vp)]
     lib.omr_bvh_data.argtypes = [vp, C.POINTER(vp), C.POINTER(sz), C.POINTER(vp), C.POINTER(sz),
                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    lib.omr_bvh_data8.argtypes = [vp, C.POINTER(vp), C.POINTER(sz), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     lib.omr_bvh_free.argtypes = [vp]
     lib.omr_bvh_free.restype = None
     lib.omr_io_last_error.argtypes = []
@@ -408,8 +410,11 @@ def procedural(kind, seed=0x6F6D7233, width=256, height=256):
         lib.omr_procedural_free(C.byref(p))
 
 
-def build_bvh(scene):
-    """Host-only build of the wide BVH: (node bytes, leaf order, depth, stack bound)."""
+def build_bvh(scene, width=4):
+    """Host-only build of the wide BVH: (node bytes, leaf order, depth, stack
+    bound) of its 4-wide (128-byte) or 8-wide (256-byte) nodes."""
+    if width not in (4, 8):
+        raise ValueError("width must be 4 or 8")
     lib = load()
     h = C.c_void_p()
     d = scene.desc()
@@ -420,7 +425,9 @@ def build_bvh(scene):
         nodes, nn, order, no = C.c_void_p(), C.c_size_t(), C.c_void_p(), C.c_size_t()
         depth, sb = C.c_uint32(), C.c_uint32()
         lib.omr_bvh_data(h, C.byref(nodes), C.byref(nn), C.byref(order), C.byref(no), C.byref(depth), C.byref(sb))
-        nb = np.frombuffer(C.string_at(nodes, nn.value * 128), dtype=np.uint8).copy()
+        if width == 8:
+            lib.omr_bvh_data8(h, C.byref(nodes), C.byref(nn), C.byref(depth), C.byref(sb))
+        nb = np.frombuffer(C.string_at(nodes, nn.value * 32 * width), dtype=np.uint8).copy()
         ob = np.frombuffer(C.string_at(order, no.value * 4), dtype=np.uint32).copy()
         return nb, ob, depth.value, sb.value
     finally:

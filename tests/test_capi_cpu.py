@@ -87,20 +87,23 @@ def test_camera_matches_reference_constructor():
     assert np.allclose(cam.screen_x, X, atol=1e-4) and np.allclose(cam.screen_y, Y, atol=1e-4)
 
 
-def _nodes(nb):
-    rec = np.dtype([("b", "<f4", (6, 4)), ("child", "<i4", 4), ("meta", "<i4", 4)])
+def _nodes(nb, width=4):
+    rec = np.dtype([("b", "<f4", (6, width)), ("child", "<i4", width), ("meta", "<i4", width)])
     return np.frombuffer(nb.tobytes(), dtype=rec)
 
 
+@pytest.mark.parametrize("width", [4, 8])
 @pytest.mark.parametrize("kind", [0, 1])
-def test_bvh_invariants(kind):
+def test_bvh_invariants(kind, width):
     """Every primitive in exactly one leaf; every primitive inside its leaf's
-    box; child boxes inside the parent's slot box; leaves of 1..8."""
+    box; child boxes inside the parent's slot box; leaves of 1..8 (4-wide
+    128-byte and 8-wide 256-byte nodes)."""
     import omr
 
     scene = omr.procedural(kind)
-    nb, order, depth, stack = omr.build_bvh(scene)
-    nodes = _nodes(nb)
+    nb, order, depth, stack = omr.build_bvh(scene, width)
+    assert nb.size % (32 * width) == 0
+    nodes = _nodes(nb, width)
     n = scene.n_prims
     assert np.array_equal(np.sort(order), np.arange(n, dtype=np.uint32))
     lo, hi = helpers.scene_bounds(scene)
@@ -118,7 +121,7 @@ def test_bvh_invariants(kind):
     for ni, nd in enumerate(nodes):
         b = nd["b"]
         plo, phi = parent_box[ni]
-        for k in range(4):
+        for k in range(width):
             c = int(nd["child"][k])
             if c == -1:
                 continue
@@ -138,6 +141,37 @@ def test_bvh_invariants(kind):
                 parent_box[c] = (clo, chi)
     assert (seen == 1).all()
     assert stack >= depth
+
+
+@pytest.mark.parametrize("kind", [1, 3])
+def test_bvh_widths_share_leaves(kind):
+    """The 8-wide nodes collapse the same tree as the 4-wide ones: the same
+    set of leaf codes (so both read the same primitive records), fewer nodes
+    and levels, and stack bounds that hold the deepest root-to-leaf sequence
+    of pushes (at most width - 1 per level)."""
+    import omr
+
+    scene = omr.procedural(kind, width=64, height=64)
+    leaves, counts = {}, {}
+    for w in (4, 8):
+        nb, order, depth, stack = omr.build_bvh(scene, w)
+        nodes = _nodes(nb, w)
+        codes = nodes["child"].ravel()
+        leaves[w] = np.sort(codes[(codes.astype(np.int64) & 0xC0000000) == 0x80000000])
+        counts[w] = (len(nodes), depth, stack)
+        # deepest push sequence: children - 1 per node on the path, plus the root entry
+        need = [0] * len(nodes)
+        worst = 0
+        for ni in range(len(nodes)):
+            ch = nodes["child"][ni]
+            here = need[ni] + int((ch != -1).sum()) - 1
+            worst = max(worst, here)
+            for c in ch:
+                if c >= 0:
+                    need[int(c)] = here
+        assert stack == worst + 1
+    assert np.array_equal(leaves[4], leaves[8])
+    assert counts[8][0] < counts[4][0] and counts[8][1] < counts[4][1]
 
 
 def test_bvh_rejects_bad_input():

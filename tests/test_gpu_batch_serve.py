@@ -61,6 +61,32 @@ def test_batch_equals_alone(ctx, record_mb):
         ctx.release(s)
 
 
+def test_batch_with_8wide_nodes_equals_4wide(ctx):
+    """The batched (per-lane job) kernels over 8-wide nodes give the images
+    of the 4-wide nodes, bit for bit."""
+    import omr
+
+    saved = ctx.get_tuning()
+    scenes, spec = _jobs(omr)
+    sids = [ctx.upload(s) for s in scenes]
+    cams, prms, ids = [], [], []
+    for si, w, h, ss, b, e, d, rr, seed in spec:
+        cams.append(_cam(omr, scenes[si], w, h))
+        prms.append(omr.params(ss=ss, spp_begin=b, spp_end=e, max_depth=d, rr_depth=rr, seed=seed))
+        ids.append(sids[si])
+    print("batch of", len(spec), "at node widths 4 and 8", flush=True)
+    try:
+        b4 = ctx.render_batch(ids, cams, prms)
+        ctx.set_tuning(node_width=8)
+        b8 = ctx.render_batch(ids, cams, prms)
+    finally:
+        ctx.set_tuning(**saved)
+    for i in range(len(spec)):
+        assert np.array_equal(b4[i], b8[i]), f"job {i} differs"
+    for s in sids:
+        ctx.release(s)
+
+
 def test_progressive_batches_equal_one_shot(ctx):
     """PathTracer's progressive loop (samples min..max step) gives the same
     fixed-point sums as one call over the whole range."""

@@ -64,11 +64,12 @@ def test_c0_full_render_bit_exact(ctx):
     ctx.release(sid)
 
 
-def test_c1_hit_ids_1m_rays(ctx):
-    """Config 1's hit-id oracle: 1,000,000 seeded rays through the mesh's BVH,
-    ids and distances identical to the restatement traversing the same wide
-    BVH and to its own binary BVH (a different tree: the tie rule makes the
-    result independent of the tree)."""
+@pytest.mark.parametrize("node_width", [4, 8])
+def test_c1_hit_ids_1m_rays(ctx, node_width):
+    """Config 1's hit-id oracle: 1,000,000 seeded rays through the mesh's BVH
+    (4-wide and 8-wide nodes), ids and distances identical to the restatement
+    traversing the 4-wide BVH and its own binary BVH (different trees: the
+    tie rule makes the result independent of the tree)."""
     import omr
     import pyoracle
 
@@ -77,8 +78,13 @@ def test_c1_hit_ids_1m_rays(ctx):
     pts = np.concatenate([mesh[:, 0:3], mesh[:, 0:3] + mesh[:, 3:6], mesh[:, 0:3] + mesh[:, 6:9]])
     rays = helpers.hit_rays(scene, 1_000_000, seed=0x6F6D7233, focus=(pts.min(0), pts.max(0)))
     sid = ctx.upload(scene)
-    print(f"c1 trace n={rays.shape[0]}", flush=True)
-    gp, gt = ctx.trace(sid, rays)
+    print(f"c1 trace n={rays.shape[0]} node_width={node_width}", flush=True)
+    saved = ctx.get_tuning()
+    try:
+        ctx.set_tuning(node_width=node_width)
+        gp, gt = ctx.trace(sid, rays)
+    finally:
+        ctx.set_tuning(**saved)
     o = _oracle(scene, ctx, sid)
     cp, ct = o.trace(rays, pyoracle.WIDE_BVH)
     _same_hits(gp, gt, cp, ct)
@@ -147,22 +153,21 @@ def test_c2_hit_ids_1m_rays(ctx, c2):
     assert (gp >= scene.tri.shape[0]).sum() > 100  # sphere hits
 
 
-@pytest.mark.parametrize("record_mb", [None, 0])
-def test_c2_render_env_is_bit_exact(ctx, c2, record_mb):
+@pytest.mark.parametrize("tune", [{}, dict(record_mb=0), dict(node_width=8)])
+def test_c2_render_env_is_bit_exact(ctx, c2, tune):
     """Config 2's path (terrain + spheres, depth 12, environment importance
     sampling with shadow rays and MIS, bucket-directory CDF inversion on the
     2048x1024 tables) on a 160x90 image, 2x2 sub-pixels x 2 samples; with
-    per-sample records and with pixel atomics."""
+    per-sample records, with pixel atomics, and with 8-wide nodes."""
     import omr
     import pyoracle
 
     scene, sid = c2
     saved = ctx.get_tuning()
     try:
-        if record_mb is not None:
-            ctx.set_tuning(record_mb=record_mb)
+        ctx.set_tuning(**tune)
         prm = omr.params(ss=2, spp_end=2, max_depth=12, rr_depth=3, env_is=1, seed=0x6F6D7233)
-        print(f"c2 160x90 env_is record_mb={record_mb}", flush=True)
+        print(f"c2 160x90 env_is {tune}", flush=True)
         g = ctx.render_sums(sid, scene.camera, prm)
         st = ctx.last_stats()
     finally:

@@ -72,3 +72,34 @@ def test_invalid_arguments_fail_cleanly(ctx):
     ctx.release(sid)
     with pytest.raises(omr.OmrError):
         ctx.render(sid, scene.camera, good)
+
+
+def test_scene_slots_are_reused_under_new_ids(ctx):
+    """More uploads than the 64 resident-scene slots, each released before
+    the next: every id is new, a released id stays unknown after its slot
+    holds another scene, and the reused slot renders its new scene."""
+    import omr
+
+    a = omr.procedural(0, width=8, height=8)
+    b = omr.procedural(1, width=8, height=8)
+    prm = omr.params(ss=1, spp_end=2, max_depth=4, rr_depth=3, seed=5)
+    sa = ctx.upload(a)
+    ref_a = ctx.render(sa, a.camera, prm)
+    ctx.release(sa)
+    sb = ctx.upload(b)
+    ref_b = ctx.render(sb, b.camera, prm)
+    ctx.release(sb)
+    ids = set()
+    print("cycling 80 uploads", flush=True)
+    for i in range(80):
+        s = a if i % 2 == 0 else b
+        sid = ctx.upload(s)
+        assert sid not in ids
+        ids.add(sid)
+        out = ctx.render(sid, s.camera, prm)
+        assert np.array_equal(out, ref_a if i % 2 == 0 else ref_b), f"upload {i}"
+        ctx.release(sid)
+        with pytest.raises(omr.OmrError):
+            ctx.render(sid, s.camera, prm)
+    with pytest.raises(omr.OmrError):
+        ctx.release(min(ids))

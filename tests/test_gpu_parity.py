@@ -50,7 +50,7 @@ def test_env_tables_bit_exact(ctx):
 
 @pytest.mark.parametrize("env_is,tune", [(0, {}), (1, {}), (1, dict(stack_short=8, fetch_budget=8)),
                                          (1, dict(drain_below=1 << 20)), (1, dict(record_mb=0)),
-                                         (1, dict(record_mb=0, drain_below=1 << 20))])
+                                         (1, dict(record_mb=0, drain_below=1 << 20)), (1, dict(node_width=8))])
 def test_render_env_bit_exact(ctx, env_is, tune):
     """Torus knot under an HDR sky: lookup on escape, and with importance
     sampling + shadow rays + MIS (also with the 8-entry LDS stack and
@@ -131,13 +131,19 @@ def test_render_bit_exact(ctx, kind, ss, spp, depth, rr):
                                                             blocks_per_cu=16),
                                   dict(drain_rounds=0, drain_below=1 << 20),
                                   dict(drain_rounds=0, drain_below=2000, pool_paths=8192),
-                                  dict(record_mb=0), dict(drain_rounds=0, record_mb=0, refill_every=2)])
+                                  dict(record_mb=0), dict(drain_rounds=0, record_mb=0, refill_every=2),
+                                  dict(node_width=8),
+                                  dict(node_width=8, drain_rounds=0, stack_short=8, wg_size=128),
+                                  dict(node_width=8, drain_rounds=0, fetch_budget=0, use_graph=0, wg_size=256,
+                                       drain_below=1 << 20)])
 def test_tuning_modes_bit_exact(ctx, tune):
     """Every launch configuration the autotuner can pick gives the same bits:
     drain rounds (one or many per render, small pools) and bounce steps with
     dynamic-fetch traversal, bulk refill cadence, graph off, small pools, the
     drain launch that finishes the last paths in one go; the 8-entry LDS
-    stack (more entries spill to global memory)."""
+    stack (more entries spill to global memory); 8-wide nodes (the oracle
+    traverses the 4-wide tree: the tie rule makes the hit independent of the
+    tree)."""
     import omr
     import pyoracle
 
