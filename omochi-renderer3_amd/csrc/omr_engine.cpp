@@ -138,18 +138,20 @@ int Engine::set_tuning(const omr_tuning &t) {
 }
 
 // Sub-queue capacity.  A producer kernel running GW waves in a grid-stride
-// loop over at most P entries appends at most 64 * ceil(P / (64 GW)) entries
-// per wave, and a shard (ceil(GW / kShards) waves) receives at most
-// ceil(GW/kShards) * 64 * ceil(P / (64 GW)).  Shade (its own grid) and refill
-// (the persistent grid) may both append to one queue in one step.
-static uint64_t shard_appends(uint32_t P, uint64_t gw) {
+// loop over at most P entries, C consecutive entries per wave and trip,
+// appends at most C * ceil(P / (C GW)) entries per wave, and a shard
+// (ceil(GW / kShards) waves) receives at most ceil(GW/kShards) times that.
+// Shade (its own grid, C = 64) and refill (the persistent grid, C =
+// kRefillSlots) may both append to one queue in one step.
+static uint64_t shard_appends(uint32_t P, uint64_t gw, uint64_t chunk) {
   gw = std::max<uint64_t>(1, gw);
-  uint64_t per_wave = 64ull * ((P + 64ull * gw - 1) / (64ull * gw));
+  uint64_t per_wave = chunk * ((P + chunk * gw - 1) / (chunk * gw));
   return (gw + kShards - 1) / kShards * per_wave;
 }
 static uint32_t sub_queue_cap(uint32_t P, uint64_t grid_waves, uint64_t shade_waves, uint64_t solo_shade_waves) {
   return (uint32_t)std::min<uint64_t>(
-      P, shard_appends(P, grid_waves) + std::max(shard_appends(P, shade_waves), shard_appends(P, solo_shade_waves)));
+      P, shard_appends(P, grid_waves, kRefillSlots) +
+             std::max(shard_appends(P, shade_waves, 64), shard_appends(P, solo_shade_waves, 64)));
 }
 // Workgroups of the persistent kernels and of shade for a pool of P paths:
 // shade runs five waves per SIMD (20 per CU), its single-job form four

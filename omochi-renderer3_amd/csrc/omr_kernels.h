@@ -12,6 +12,8 @@ namespace omr {
 // deeper entries spill to the per-thread global area (PoolPtrs::gstack, gcap
 // entries per thread).
 constexpr uint32_t kShortStackMin = 8, kShortStackMax = 16;
+// Slots one wave of the refill kernel covers per trip (4 per lane).
+constexpr uint32_t kRefillSlots = 256;
 
 // Device pointers of the wavefront path pool (capacity `pool` slots).
 struct PoolPtrs {

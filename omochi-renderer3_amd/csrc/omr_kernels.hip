@@ -701,7 +701,11 @@ __device__ inline uint32_t search_guided(const unsigned long long *a, uint32_t n
   }
   return lo;
 }
-__device__ inline bool env_sample4(const SceneDev &S, float u1, float u2, float u3, float u4, f3 &dir) {
+// A direction from the tables with four uniforms in [0,1]: a texel by the
+// marginal and row CDFs (its index into `texel`), then a point of it uniform
+// in (phi, theta).
+__device__ inline bool env_sample4(const SceneDev &S, float u1, float u2, float u3, float u4, f3 &dir,
+                                   size_t &texel) {
   uint32_t W = S.env_w, H = S.env_h;
   unsigned long long tot = gld(S.env_marg + H);
   if (tot == 0ull) return false;
@@ -721,6 +725,7 @@ __device__ inline bool env_sample4(const SceneDev &S, float u1, float u2, float 
   sincos_turns(fx, sp, cp);
   sincos_turns(fy, sth, cth);
   dir = mk(sth * cp, cth, sth * sp);
+  texel = (size_t)iy * W + ix;
   return true;
 }
 
@@ -936,23 +941,76 @@ __device__ inline void start_sample(const KernelArgs &A, uint32_t slot, unsigned
 }
 
 // ------------------------------------------------------------ kernels
-// refill: every inactive slot takes a new sample while samples remain.
+// refill: every inactive slot takes a new sample while samples remain.  A
+// wave covers kRefillSlots consecutive slots per trip, as groups of 64 (lane
+// l holds slot l of each group: coalesced accesses), and takes samples for
+// all its idle ones with one cursor atomic and one queue append (one atomic
+// per 64 slots serialized on the cursor lines and held a 2^26-slot refill to
+// 1.5 ms).  Idle slots are numbered in slot order, so consecutive slots take
+// consecutive samples and enter the queue in slot order, as with one group.
+constexpr uint32_t kRefillPerLane = kRefillSlots / 64;
 template <int WG>
 __global__ __launch_bounds__(WG) void k_refill(KernelArgs A, int parity) {
   const uint32_t P = A.p.pool;
-  const uint32_t stride = gridDim.x * WG;
+  const uint32_t waves = gridDim.x * (WG / 64);
+  const uint32_t wave = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+  const uint32_t lane = lane_id();
   SampleCursor sc = sample_cursor(A.ctrl);
-  for (uint32_t base = blockIdx.x * WG; base < P; base += stride) {
-    uint32_t slot = base + threadIdx.x;
-    bool valid = slot < P;
-    bool idle = valid && ((reinterpret_cast<const uint4 *>(A.p.misc)[slot].y >> 19) & 1u) == 0u;
-    if (__ballot(idle) == 0ull) continue;
-    unsigned long long g = grab_sample(A.ctrl, sc, idle);
-    bool got = g != ~0ull;
-    if (got) start_sample(A, slot, g);
+  for (uint32_t base = wave * kRefillSlots; base < P; base += waves * kRefillSlots) {
+    bool idle[kRefillPerLane];
+    uint32_t req[kRefillPerLane];  // the slot's position among the wave's idle slots
+    uint32_t want = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < kRefillPerLane; u++) {
+      const uint32_t slot = base + u * 64u + lane;
+      idle[u] = slot < P && ((reinterpret_cast<const uint4 *>(A.p.misc)[slot].y >> 19) & 1u) == 0u;
+      const unsigned long long m = __ballot(idle[u]);
+      req[u] = want + rank_in(m);
+      want += (uint32_t)__popcll(m);
+    }
+    if (want == 0u) continue;
+    // positions [done, done + take) come from cursor shard k starting at `old`
+    unsigned long long got[kRefillPerLane];
+#pragma unroll
+    for (uint32_t u = 0; u < kRefillPerLane; u++) got[u] = ~0ull;
+    uint32_t done = 0;
+    while (done < want && sc.open) {
+      const uint32_t k = sc.shard, ask = want - done;
+      unsigned long long old = 0ull, take = 0ull;
+      if (lane == 0u) {
+        const unsigned long long lim = shard_limit(sc.total, k);
+        old = atomicAdd(&A.ctrl->next_sample[k], (unsigned long long)ask);
+        const unsigned long long avail = lim > old ? lim - old : 0ull;
+        take = avail < ask ? avail : (unsigned long long)ask;
+      }
+      old = __shfl(old, 0);
+      take = __shfl(take, 0);
+#pragma unroll
+      for (uint32_t u = 0; u < kRefillPerLane; u++)
+        if (idle[u] && req[u] >= done && req[u] < done + (uint32_t)take) got[u] = shard_sample(old + (req[u] - done), k);
+      done += (uint32_t)take;
+      if (take < ask && !next_sample_shard(A.ctrl, sc)) break;  // shard k is used up
+    }
+    // the started slots enter sub-queue k in slot order
+    uint32_t qpos[kRefillPerLane], n_all = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < kRefillPerLane; u++) {
+      const unsigned long long m = __ballot(got[u] != ~0ull);
+      qpos[u] = n_all + rank_in(m);
+      n_all += (uint32_t)__popcll(m);
+    }
+    if (n_all == 0u) continue;
     const uint32_t k = wave_shard();
-    uint32_t pos = wave_append(&A.ctrl->qcount[parity][k], got);
-    if (got) A.p.queue[parity][k * A.p.qcap + pos] = slot;
+    uint32_t qbase = 0;
+    if (lane == 0u) qbase = atomicAdd(&A.ctrl->qcount[parity][k], n_all);
+    qbase = __shfl(qbase, 0);
+#pragma unroll
+    for (uint32_t u = 0; u < kRefillPerLane; u++) {
+      if (got[u] == ~0ull) continue;
+      const uint32_t slot = base + u * 64u + lane;
+      start_sample(A, slot, got[u]);
+      A.p.queue[parity][k * A.p.qcap + qbase + qpos[u]] = slot;
+    }
   }
 }
 
@@ -1152,9 +1210,10 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
           if (nee && can_continue) {
             float e1 = rng_float(rng), e2 = rng_float(rng), e3 = rng_float(rng), e4 = rng_float(rng);
             f3 ld;
-            if (env_sample4(S, e1, e2, e3, e4, ld)) {
+            size_t ei;
+            if (env_sample4(S, e1, e2, e3, e4, ld, ei)) {
+              // the sampled texel's pdf and radiance
               float c = dot(nl, ld);
-              const size_t ei = env_index(S, ld);
               float pe = env_pdf_at(S, ei, ld);
               if (c > 0.0f && pe > 0.0f) {
                 float pb = c * kInvPi;

@@ -683,8 +683,21 @@ int orc_env_cdf(const orc_handle *h, float *marginal, float *conditional) {
   return 0;
 }
 
-/* solid-angle pdf of the sampler for a direction: texel probability times
- * W*H / (2 pi^2 sin(theta)), evaluated in double, returned as float. */
+/* solid-angle pdf of the sampler at direction d inside texel i: the texel's
+ * probability times W*H / (2 pi^2 sin(theta)), sin(theta) = |d.xz|,
+ * evaluated in double, returned as float. */
+static float env_pdf_texel(const orc_handle *h, size_t i, float dx, float dz) {
+  uint32_t W = h->s.env_w, H = h->s.env_h;
+  uint64_t tot = h->env_marg[H];
+  if (tot == 0) return 0.0f;
+  uint64_t q = h->env_q[i];
+  float st = sqrtf(dx * dx + dz * dz);
+  if (q == 0 || !(st > 0.0f)) return 0.0f;
+  double p = (double)q / (double)tot * ((double)W * (double)H) / (19.739208802178716 * (double)st);
+  return (float)p;
+}
+
+/* the same for any direction: its texel by the latlong mapping */
 float orc_env_pdf(const orc_handle *h, float dx, float dy, float dz) {
   if (!h->env_q) return 0.0f;
   uint32_t W = h->s.env_w, H = h->s.env_h;
@@ -709,8 +722,10 @@ static uint32_t search_le(const uint64_t *a, uint32_t n, uint64_t key) {
   return lo;
 }
 
-/* Draw one direction from the tables with four uniforms in [0,1]. */
-static int env_sample4(const orc_handle *h, float u1, float u2, float u3, float u4, v3 *dir) {
+/* Draw one direction from the tables with four uniforms in [0,1]: a texel
+ * by the marginal and row CDFs (its index into *texel), then a point of it
+ * uniform in (phi, theta). */
+static int env_sample4(const orc_handle *h, float u1, float u2, float u3, float u4, v3 *dir, size_t *texel) {
   uint32_t W = h->s.env_w, H = h->s.env_h;
   uint64_t tot = h->env_marg[H];
   if (tot == 0) return 0;
@@ -730,13 +745,15 @@ static int env_sample4(const orc_handle *h, float u1, float u2, float u3, float 
   orc_sincos_turns(fx, &sp, &cp);
   orc_sincos_turns(fy, &sth, &cth);
   *dir = mk(sth * cp, cth, sth * sp);
+  *texel = (size_t)iy * W + ix;
   return 1;
 }
 
 void orc_env_sample(const orc_handle *h, float u1, float u2, float dir[3], float *pdf, uint32_t *ix, uint32_t *iy) {
   /* u3 = u4 = 0.5: the texel centre */
   v3 d;
-  if (!h->env_q || !env_sample4(h, u1, u2, 0.5f, 0.5f, &d)) { dir[0] = dir[1] = dir[2] = 0; *pdf = 0; return; }
+  size_t ti;
+  if (!h->env_q || !env_sample4(h, u1, u2, 0.5f, 0.5f, &d, &ti)) { dir[0] = dir[1] = dir[2] = 0; *pdf = 0; return; }
   dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
   *pdf = orc_env_pdf(h, d.x, d.y, d.z);
   orc_env_texel(d.x, d.y, d.z, h->s.env_w, h->s.env_h, ix, iy);
@@ -909,14 +926,17 @@ static void trace_sample(const render_job *J, uint32_t pixel, uint32_t subpx, ui
             if (nee && can_continue) {
               float e1 = orc_rng_float(rng), e2 = orc_rng_float(rng), e3 = orc_rng_float(rng), e4 = orc_rng_float(rng);
               v3 ld;
-              if (env_sample4(h, e1, e2, e3, e4, &ld)) {
+              size_t ti;
+              if (env_sample4(h, e1, e2, e3, e4, &ld, &ti)) {
+                /* the sampled texel's pdf and radiance */
                 float c = dot(nl, ld);
-                float pe = orc_env_pdf(h, ld.x, ld.y, ld.z);
+                float pe = env_pdf_texel(h, ti, ld.x, ld.z);
                 if (c > 0.0f && pe > 0.0f) {
                   float pb = c * ORC_INV_PI;
                   float w = (pe * pe) / (pe * pe + pb * pb);
                   float s = ((pb / pe) * w) * k;
-                  v3 L = env_rgb(h, ld);
+                  const float *lp = h->s.env + 3 * ti;
+                  v3 L = mk(lp[0], lp[1], lp[2]);
                   v3 bc = mk(beta.x * col.x, beta.y * col.y, beta.z * col.z);
                   float sray[8] = {x.x, x.y, x.z, ORC_INF, ld.x, ld.y, ld.z, 0.0f};
                   (*nshadow)++;
