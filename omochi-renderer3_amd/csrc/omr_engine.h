@@ -25,7 +25,7 @@ namespace omr {
 // Resident scenes per context.  A scene id is use * kMaxScenes + slot: a
 // released slot takes the next upload, under a new id, and the device scene
 // table is indexed by the slot.
-constexpr uint32_t kMaxScenes = 64;
+constexpr uint32_t kMaxScenes = kMaxSceneSlots;
 inline uint32_t scene_slot(omr_scene_id id) { return (uint32_t)id % kMaxScenes; }
 
 // Summaries of the sharded control block as read back to the host.

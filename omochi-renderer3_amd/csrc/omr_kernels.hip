@@ -925,8 +925,18 @@ __device__ inline void wave_count(Sharded<unsigned long long> &shards, unsigned 
   if (lane_id() == 0 && v) atomicAdd(&shards[wave_shard()], v);
 }
 
-// Writes a fresh camera path for global sample g into slot.
-__device__ inline void start_sample(const KernelArgs &A, uint32_t slot, unsigned long long g) {
+// A ray-queue entry: the path's slot (pool <= 2^26 slots) and, in the top
+// six bits, its scene's slot in the scene table, so a traversal can read the
+// scene record without first loading the path's job.
+constexpr uint32_t kEntrySlotBits = 26;
+static_assert(kMaxSceneSlots <= (1u << (32 - kEntrySlotBits)), "scene slot must fit the entry's top bits");
+__device__ inline uint32_t q_entry(uint32_t slot, uint32_t scene) { return slot | (scene << kEntrySlotBits); }
+__device__ inline uint32_t q_slot(uint32_t e) { return e & ((1u << kEntrySlotBits) - 1u); }
+__device__ inline uint32_t q_scene(uint32_t e) { return e >> kEntrySlotBits; }
+
+// Writes a fresh camera path for global sample g into slot; returns the
+// queue entry for it.
+__device__ inline uint32_t start_sample(const KernelArgs &A, uint32_t slot, unsigned long long g) {
   uint32_t js = find_job(A.ctrl, A.jobs, g);
   const JobDev &J = A.jobs[js];
   uint32_t s = (uint32_t)(g - J.sample_base);
@@ -938,6 +948,7 @@ __device__ inline void start_sample(const KernelArgs &A, uint32_t slot, unsigned
   reinterpret_cast<float4 *>(A.p.ray_d)[slot] = make_float4(d.x, d.y, d.z, 1.0f);
   reinterpret_cast<uint4 *>(A.p.misc)[slot] = make_uint4(__float_as_uint(1.0f), ps_pack(0, 0, 0, 0), s, js);
   reinterpret_cast<uint4 *>(A.p.rng)[slot] = rng;
+  return q_entry(slot, J.scene);
 }
 
 // ------------------------------------------------------------ kernels
@@ -1008,8 +1019,7 @@ __global__ __launch_bounds__(WG) void k_refill(KernelArgs A, int parity) {
     for (uint32_t u = 0; u < kRefillPerLane; u++) {
       if (got[u] == ~0ull) continue;
       const uint32_t slot = base + u * 64u + lane;
-      start_sample(A, slot, got[u]);
-      A.p.queue[parity][k * A.p.qcap + qbase + qpos[u]] = slot;
+      A.p.queue[parity][k * A.p.qcap + qbase + qpos[u]] = start_sample(A, slot, got[u]);
     }
   }
 }
@@ -1022,9 +1032,9 @@ __global__ __launch_bounds__(WG) void k_refill(KernelArgs A, int parity) {
 __global__ __launch_bounds__(256) void k_fill_fresh(KernelArgs A, uint32_t n0) {
   const uint32_t stride = gridDim.x * 256u;
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n0; i += stride) {
-    start_sample(A, i, (unsigned long long)i);
+    const uint32_t e = start_sample(A, i, (unsigned long long)i);
     const uint32_t r = i / 64u;
-    A.p.queue[0][(r % kShards) * A.p.qcap + (r / kShards) * 64u + i % 64u] = i;
+    A.p.queue[0][(r % kShards) * A.p.qcap + (r / kShards) * 64u + i % 64u] = e;
   }
 }
 
@@ -1053,11 +1063,11 @@ __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
   if (DF) {
     auto count = [&](uint32_t k) { return s_pre[k + 1] - s_pre[k]; };
     auto load = [&](uint32_t k, uint32_t off, const SceneDev *&S, f3 &o, f3 &d, float &tmax, uint32_t &tag) {
-      uint32_t slot = queue[k * qcap + off];
+      const uint32_t e = queue[k * qcap + off], slot = q_slot(e);
       float4 ro = reinterpret_cast<const float4 *>(A.p.ray_o)[slot];
       float4 rd = reinterpret_cast<const float4 *>(A.p.ray_d)[slot];
       if (SOLO) S = &Ss;
-      else S = &A.scenes[A.jobs[A.p.misc[4 * (size_t)slot + 3]].scene];
+      else S = &A.scenes[q_scene(e)];
       o = xyz(ro);
       d = xyz(rd);
       tmax = kInf;
@@ -1072,10 +1082,10 @@ __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
   for (uint32_t base = blockIdx.x * WG; !DF && base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     if (i < n) {
-      uint32_t slot = queue[flat_slot(s_pre, i, qcap)];
+      const uint32_t e = queue[flat_slot(s_pre, i, qcap)], slot = q_slot(e);
       float4 ro = reinterpret_cast<const float4 *>(A.p.ray_o)[slot];
       float4 rd = reinterpret_cast<const float4 *>(A.p.ray_d)[slot];
-      const SceneDev &S = SOLO ? Ss : A.scenes[A.jobs[A.p.misc[4 * (size_t)slot + 3]].scene];
+      const SceneDev &S = SOLO ? Ss : A.scenes[q_scene(e)];
       Hit h;
       h.t = kInf;
       h.u = h.v = 0.0f;
@@ -1374,11 +1384,11 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO ? 4 : 5
     Ss = cload(A.scenes + Js.scene);
   }
   const uint32_t i0 = blockIdx.x * WG + threadIdx.x;
-  uint32_t next = i0 < n ? A.p.queue[parity][flat_slot(s_pre, i0, qcap)] : 0u;
+  uint32_t next = i0 < n ? A.p.queue[parity][flat_slot(s_pre, i0, qcap)] : 0u;  // a queue entry
   for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     bool valid = i < n;
-    const uint32_t slot = next;
+    const uint32_t slot = q_slot(next);
     const uint32_t i2 = i + stride;
     next = i2 < n ? A.p.queue[parity][flat_slot(s_pre, i2, qcap)] : 0u;
     bool cont = false, fin = false, shadow = false;
@@ -1409,8 +1419,12 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO ? 4 : 5
     // regenerate finished slots while samples remain
     unsigned long long g = grab_sample(A.ctrl, sc, fin && regen);
     bool fresh = g != ~0ull;
+    uint32_t entry = 0;
     if (fresh) {
-      start_sample(A, slot, g);
+      entry = start_sample(A, slot, g);
+    } else if (cont) {
+      // a continuing path keeps its scene (read again rather than kept live)
+      entry = q_entry(slot, SOLO ? Js.scene : A.jobs[st.js].scene);
     } else if (fin) {
       misc[slot].y = 0u;  // inactive
     }
@@ -1424,7 +1438,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO ? 4 : 5
     bool enq = valid && (cont || fresh);
     const uint32_t k = wave_shard();
     uint32_t pos = wave_append(&A.ctrl->qcount[parity ^ 1][k], enq);
-    if (enq) A.p.queue[parity ^ 1][k * qcap + pos] = slot;
+    if (enq) A.p.queue[parity ^ 1][k * qcap + pos] = entry;
     uint32_t spos = wave_append(&A.ctrl->scount[k], shadow);
     if (shadow) {
       float4 *sq = reinterpret_cast<float4 *>(A.p.shadow) + ((size_t)k * qcap + spos) * 3;
@@ -1570,7 +1584,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 3 
       uint32_t cnt = count(shard);
       uint32_t r = rank_in(m);
       if (!have && base + r < cnt) {
-        slot = A.p.queue[parity][shard * qcap + base + r];
+        const uint32_t e = A.p.queue[parity][shard * qcap + base + r];
+        slot = q_slot(e);
         float4 ro = ray_o[slot], rd = ray_d[slot];
         uint4 ms = misc[slot];
         st.rng = rngs[slot];
@@ -1584,7 +1599,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 3 
         have = true;
         tracing = true;
         taken++;
-        tr.init(SOLO ? Ss : A.scenes[A.jobs[st.js].scene], st.o, st.d, kInf, stk);
+        tr.init(SOLO ? Ss : A.scenes[q_scene(e)], st.o, st.d, kInf, stk);
       }
       if (base + idle >= cnt) {  // this sub-queue is used up: find one with entries left
         const uint32_t lane = lane_id();

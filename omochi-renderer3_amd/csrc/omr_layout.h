@@ -193,6 +193,8 @@ struct Ctrl {
 };
 
 constexpr uint32_t kJobRing = 1024;
+// Resident scenes per context (slots of the device scene table).
+constexpr uint32_t kMaxSceneSlots = 64;
 // Completed-sample counters: kShards x kJobRing, shard-major, so the waves of
 // one shard that finish samples of one job update one line.
 OMR_HD inline size_t done_index(uint32_t shard, uint32_t ring) { return (size_t)shard * kJobRing + ring; }
