@@ -583,6 +583,44 @@ int Engine::launch_round(bool first, bool with_shadow) {
   return OMR_OK;
 }
 
+void Engine::plan_round(const std::vector<uint64_t> &first, const std::vector<uint64_t> &count,
+                        const std::vector<uint32_t> &ring, uint64_t base, uint64_t total, uint32_t *n0,
+                        uint32_t *solo_job) const {
+  const uint64_t P = pool_cap_;
+  uint64_t n = std::min<uint64_t>(P, total - base);
+  *solo_job = kNoSolo;
+  // the job holding sample `base` (ranges ascend)
+  auto it = std::upper_bound(first.begin(), first.end(), base);
+  const size_t i = (size_t)(it - first.begin()) - 1;
+  const uint64_t rem = first[i] + count[i] - base;
+  if (rem >= n) {
+    *solo_job = ring[i];  // the round lies inside one job
+  } else if (2 * rem >= P) {
+    // at least half a pool of this job is left: it ends in a round of its own
+    // (single-job kernels) rather than sharing one with the next job
+    n = rem;
+    *solo_job = ring[i];
+  }
+  *n0 = (uint32_t)n;
+}
+
+int Engine::launch_placed_round(bool with_shadow, uint64_t base, uint32_t n0, uint32_t solo_job) {
+  const uint32_t saved = solo_;
+  solo_ = solo_job == kNoSolo ? kNoSolo : 0u;  // a flag: the kernels read the slot from Ctrl
+  KernelArgs a = args();
+  const LaunchShape sh = shape();
+  hipError_t e = k::round_setup(d_ctrl_, n0, solo_job == kNoSolo ? 0u : solo_job, stream_);
+  if (e == hipSuccess) e = k::fill_fresh(a, base, n0, stream_);
+  int rc = e == hipSuccess ? OMR_OK : hipfail(e, "round fill");
+  if (!rc && tuning.round_steps) rc = launch_steps(tuning.round_steps, with_shadow, true);
+  // an even number of steps leaves the live paths in queue 0
+  if (!rc && profile && hipEventRecord(ev_c_, stream_) != hipSuccess) rc = hipfail(hipGetLastError(), "event");
+  if (!rc && (e = k::drain(a, sh, 0, profile, stream_)) != hipSuccess) rc = hipfail(e, "drain");
+  if (!rc && profile && hipEventRecord(ev_d_, stream_) != hipSuccess) rc = hipfail(hipGetLastError(), "event");
+  solo_ = saved;
+  return rc;
+}
+
 int Engine::collect_profile() {
   for (size_t i = 0; i < prof_kind_.size(); i++) {
     float ms = 0.0f;
@@ -655,9 +693,9 @@ int Engine::launch_steps(uint32_t steps, bool with_shadow, bool round) {
     if (e != hipSuccess) return hipfail(e, "bounce step launch");
     return OMR_OK;
   }
-  int gi = with_shadow ? 1 : 0;
-  if (!graph_[gi] || graph_steps_ != steps || graph_solo_ != a.solo || graph_round_ != round) {
-    if (graph_steps_ != steps || graph_solo_ != a.solo || graph_round_ != round) drop_graphs();
+  int gi = (with_shadow ? 1 : 0) + (a.solo != kNoSolo ? 2 : 0);
+  if (!graph_[gi] || graph_steps_ != steps || graph_round_ != round) {
+    if (graph_steps_ != steps || graph_round_ != round) drop_graphs();
     hipGraph_t g = nullptr;
     OMR_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     hipError_t e = enqueue();
@@ -670,7 +708,6 @@ int Engine::launch_steps(uint32_t steps, bool with_shadow, bool round) {
     if (e != hipSuccess) return hipfail(e, "hipGraphInstantiate");
     graph_[gi] = ex;
     graph_steps_ = steps;
-    graph_solo_ = a.solo;
     graph_round_ = round;
   }
   OMR_HIP(hipGraphLaunch(graph_[gi], stream_));
@@ -770,8 +807,13 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
   }
   OMR_HIP(hipEventRecord(ev_a_, stream_));
   KernelArgs a = args();
-  hipError_t e = fresh ? k::fill_fresh(a, n0, stream_) : k::refill(a, sh, 0, stream_);
-  if (e != hipSuccess) return hipfail(e, "refill");
+  // drain rounds placed by the host: every round fills the pool itself
+  const bool placed = tuning.drain_rounds && fresh;
+  hipError_t e = hipSuccess;
+  if (!placed) {
+    e = fresh ? k::fill_fresh(a, 0, n0, stream_) : k::refill(a, sh, 0, stream_);
+    if (e != hipSuccess) return hipfail(e, "refill");
+  }
   uint64_t steps_done = 0;
   const uint32_t chunk = tuning.steps_per_sync;
   uint64_t drained = 0, drain_launches = 0;
@@ -780,13 +822,31 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
   prof_extend_launches_ = 0;
   if (tuning.drain_rounds) {
     // Rounds: every slot holds a sample after the fill, the drain runs them
-    // all to their end (and marks the slots idle), the refill gives every
-    // idle slot the next sample.  Each round hands out min(pool, remaining)
-    // samples, so the number of rounds is known and nothing is read back
-    // until the end.
+    // all to their end (and marks the slots idle), the next round fills the
+    // pool again.  The host places each round on a range of the global
+    // sample order (plan_round), so nothing is read back until the end and a
+    // round that lies inside one job runs the single-job kernels; with a pool
+    // whose full fill does not fit the sub-queues the refill hands out
+    // min(pool, remaining) samples per round instead.
     const uint64_t rounds = (total + pool_cap_ - 1) / pool_cap_;
-    for (uint64_t r = 0; r < rounds; r++) {
-      rc = launch_round(r == 0, any_env_is);
+    std::vector<uint64_t> jfirst, jcount;
+    std::vector<uint32_t> jring;
+    for (size_t i = 0, b = 0; i < jobs.size(); b += jobs[i].n_samples, i++) {
+      jfirst.push_back(b);
+      jcount.push_back(jobs[i].n_samples);
+      jring.push_back(jobs[i].ring);
+    }
+    // placed rounds: the host gives each round a range of the global sample
+    // order; a round inside one job runs the single-job kernels
+    for (uint64_t r = 0, base = 0; placed ? base < total : r < rounds; r++) {
+      if (placed) {
+        uint32_t n_r = 0, solo_job = kNoSolo;
+        plan_round(jfirst, jcount, jring, base, total, &n_r, &solo_job);
+        rc = launch_placed_round(any_env_is, base, n_r, solo_job);
+        base += n_r;
+      } else {
+        rc = launch_round(r == 0, any_env_is);
+      }
       if (rc) return rc;
       steps_done += tuning.round_steps;
       drain_launches++;

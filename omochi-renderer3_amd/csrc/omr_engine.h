@@ -75,6 +75,7 @@ struct JobRun {
   float *d_out = nullptr;                 // W*H*3 resolved means (optional)
   uint32_t ring = 0;                      // ring slot while live
   uint64_t logical = 0;                   // logical ring index
+  unsigned long long first = 0;           // its first sample in the run's global sample order
   bool done = false;
 };
 
@@ -177,12 +178,11 @@ class Engine {
   uint32_t gcap_ = 0;
   size_t gstack_threads_ = 0;
 
-  // graphs: [with shadow kernel]
-  hipGraphExec_t graph_[2] = {nullptr, nullptr};
+  // graphs: [with shadow kernel + 2 x single-job kernels]
+  hipGraphExec_t graph_[4] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t graph_steps_ = 0;
-  uint32_t graph_solo_ = kNoSolo;  // KernelArgs::solo the graphs were captured with
   bool graph_round_ = false;       // the graphs hold drain-round steps (no regeneration, no refill)
-  uint32_t solo_ = kNoSolo;        // ring slot of the current run's only job, or kNoSolo
+  uint32_t solo_ = kNoSolo;        // the current launches are single-job (Ctrl::solo_job names it), or kNoSolo
 
   int ensure_pool();
   int ensure_gstack(uint32_t need_entries, size_t threads);
@@ -193,6 +193,16 @@ class Engine {
   // One drain round on queue 0: (unless first) every idle slot takes a new
   // sample, tuning.round_steps bounce steps, then one drain launch.
   int launch_round(bool first, bool with_shadow);
+  // A round the host places: samples [base, base + n0) fill the idle pool
+  // without atomics, round_steps steps, then the drain; solo_job = the ring
+  // slot of the round's only job (single-job kernels), or kNoSolo.
+  int launch_placed_round(bool with_shadow, uint64_t base, uint32_t n0, uint32_t solo_job);
+  // The next placed round at global sample `base` of `total`, over jobs whose
+  // sample ranges [first[i], first[i] + count[i]) follow one another: its
+  // sample count and its only job's ring slot (or kNoSolo).
+  void plan_round(const std::vector<uint64_t> &first, const std::vector<uint64_t> &count,
+                  const std::vector<uint32_t> &ring, uint64_t base, uint64_t total, uint32_t *n0,
+                  uint32_t *solo_job) const;
   // profiling: event pairs around each launch of the last chunk
   std::vector<hipEvent_t> prof_ev_;
   std::vector<int> prof_kind_;  // 0 extend, 1 shade, 2 shadow

@@ -45,7 +45,7 @@ struct KernelArgs {
   unsigned long long *done;  // completed-sample counters, done_index(shard, ring)
   const SceneDev *scenes;
   PoolPtrs p;
-  uint32_t solo;         // ring slot of the run's only job, or kNoSolo
+  uint32_t solo;         // kNoSolo, or single-job kernels (the job's ring slot is Ctrl::solo_job)
 };
 
 struct LaunchShape {
@@ -68,10 +68,13 @@ namespace k {
 hipError_t noalloc_scratch(void **ptr);
 // p[0..n) = 0 (a kernel, not a memset: callers may capture it in a graph).
 hipError_t zero_words(uint32_t *p, uint32_t n, hipStream_t st);
-// First fill of a cleared pool (queue 0): slot i takes sample i for i < n0,
-// without atomics; the caller has set the cursors and queue-0 counts of
-// shard k to shard_limit(n0, k).
-hipError_t fill_fresh(const KernelArgs &a, uint32_t n0, hipStream_t st);
+// Fill of an idle pool (queue 0): slot i takes sample base + i for i < n0,
+// without atomics; the queue-0 counts of shard k are shard_limit(n0, k)
+// (set by the host or by round_setup).
+hipError_t fill_fresh(const KernelArgs &a, unsigned long long base, uint32_t n0, hipStream_t st);
+// Before a host-placed round of n0 samples: queue-0 counts for fill_fresh,
+// zero fetch heads, and the single-job launches' job slot.
+hipError_t round_setup(Ctrl *ctrl, uint32_t n0, uint32_t solo_job, hipStream_t st);
 hipError_t refill(const KernelArgs &a, const LaunchShape &s, int parity, hipStream_t st);
 // stats: count node visits / primitive tests into Ctrl (profiling runs)
 hipError_t extend(const KernelArgs &a, const LaunchShape &s, int parity, bool stats, hipStream_t st);

@@ -1024,15 +1024,28 @@ __global__ __launch_bounds__(WG) void k_refill(KernelArgs A, int parity) {
   }
 }
 
-// First fill of a cleared pool: slot i < n0 takes sample i.  Sample i sits
+// A round of samples [base, base + n0) placed by the host: queue 0 holds
+// the n0 entries k_fill_fresh writes (sub-queue k: shard_limit(n0, k)), its
+// fetch heads start at zero, and solo_job names the round's only job when
+// it has one.  One block of kShards threads.
+__global__ void k_round_setup(Ctrl *ctrl, uint32_t n0, uint32_t solo_job) {
+  const uint32_t k = threadIdx.x;
+  if (k < kShards) {
+    ctrl->qcount[0][k] = (uint32_t)shard_limit(n0, k);
+    ctrl->qhead[0][k] = 0u;
+  }
+  if (k == 0) ctrl->solo_job = solo_job;
+}
+
+// First fill of a cleared pool: slot i < n0 takes sample base + i.  Sample i sits
 // at cursor position (r / kShards) * 64 + i % 64 of shard r % kShards, r =
 // i / 64, so run r goes to that sub-queue at the same entries, and the host
 // sets the cursors and counts to match: the same state the cursor grabs
 // would leave, without an atomic.
-__global__ __launch_bounds__(256) void k_fill_fresh(KernelArgs A, uint32_t n0) {
+__global__ __launch_bounds__(256) void k_fill_fresh(KernelArgs A, unsigned long long base, uint32_t n0) {
   const uint32_t stride = gridDim.x * 256u;
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n0; i += stride) {
-    const uint32_t e = start_sample(A, i, (unsigned long long)i);
+    const uint32_t e = start_sample(A, i, base + i);
     const uint32_t r = i / 64u;
     A.p.queue[0][(r % kShards) * A.p.qcap + (r / kShards) * 64u + i % 64u] = e;
   }
@@ -1059,7 +1072,7 @@ __global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
   stk.g = A.p.gstack + (size_t)gtid * A.p.gcap;
   uint32_t segs = 0, nv = 0, nt = 0;
   SceneDev Ss{};  // SOLO: the only job's scene, in scalar registers
-  if (SOLO) Ss = cload(A.scenes + cload(A.jobs + A.solo).scene);
+  if (SOLO) Ss = cload(A.scenes + cload(A.jobs + cload(&A.ctrl->solo_job)).scene);
   if (DF) {
     auto count = [&](uint32_t k) { return s_pre[k + 1] - s_pre[k]; };
     auto load = [&](uint32_t k, uint32_t off, const SceneDev *&S, f3 &o, f3 &d, float &tmax, uint32_t &tag) {
@@ -1380,7 +1393,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO ? 4 : 5
   JobDev Js{};
   SceneDev Ss{};
   if (SOLO) {
-    Js = cload(A.jobs + A.solo);
+    Js = cload(A.jobs + cload(&A.ctrl->solo_job));
     Ss = cload(A.scenes + Js.scene);
   }
   const uint32_t i0 = blockIdx.x * WG + threadIdx.x;
@@ -1476,7 +1489,7 @@ __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
   JobDev Js{};  // SOLO: the only job and its scene, in scalar registers
   SceneDev Ss{};
   if (SOLO) {
-    Js = cload(A.jobs + A.solo);
+    Js = cload(A.jobs + cload(&A.ctrl->solo_job));
     Ss = cload(A.scenes + Js.scene);
   }
   auto load = [&](uint32_t k, uint32_t off, const SceneDev *&S, f3 &o, f3 &d, float &tmax, uint32_t &tag) {
@@ -1570,7 +1583,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 3 
   JobDev Js{};
   SceneDev Ss{};
   if (SOLO) {
-    Js = cload(A.jobs + A.solo);
+    Js = cload(A.jobs + cload(&A.ctrl->solo_job));
     Ss = cload(A.scenes + Js.scene);
   }
   for (;;) {
@@ -2065,10 +2078,15 @@ hipError_t zero_words(uint32_t *p, uint32_t n, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t fill_fresh(const KernelArgs &a, uint32_t n0, hipStream_t st) {
+hipError_t fill_fresh(const KernelArgs &a, unsigned long long base, uint32_t n0, hipStream_t st) {
   if (n0 == 0) return hipSuccess;
   const uint32_t need = (n0 + 255u) / 256u;
-  hipLaunchKernelGGL(k_fill_fresh, dim3(need < 16384u ? need : 16384u), dim3(256), 0, st, a, n0);
+  hipLaunchKernelGGL(k_fill_fresh, dim3(need < 16384u ? need : 16384u), dim3(256), 0, st, a, base, n0);
+  return hipGetLastError();
+}
+
+hipError_t round_setup(Ctrl *ctrl, uint32_t n0, uint32_t solo_job, hipStream_t st) {
+  hipLaunchKernelGGL(k_round_setup, dim3(1), dim3(kShards), 0, st, ctrl, n0, solo_job);
   return hipGetLastError();
 }
 

@@ -178,6 +178,7 @@ struct Ctrl {
   alignas(kLine) unsigned long long total_samples;        // samples admitted so far (host-written)
   alignas(kLine) uint32_t job_lo;                         // live job window [job_lo, job_hi) (logical ring
   uint32_t job_hi;                                        // indices); the pair is copied as 8 bytes
+  uint32_t solo_job;                                      // ring slot of the only job of single-job launches
   Sharded<unsigned long long> segments;     // closest-hit queries traced
   Sharded<unsigned long long> shadow_rays;  // environment shadow rays traced
   Sharded<unsigned long long> node_visits;  // stats kernels only

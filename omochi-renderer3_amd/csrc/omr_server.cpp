@@ -259,6 +259,12 @@ void Engine::server_loop() {
   std::deque<std::shared_ptr<Req>> live;  // admitted, by logical ring index
   uint64_t lo = 0, hi = 0;                // logical ring window [lo, hi)
   unsigned long long total = 0;           // samples admitted
+  unsigned long long placed = 0;          // samples placed in rounds (drain rounds)
+  // rounds are placed by the host when a full pool's fill fits the sub-queues
+  // (the pool is fixed while serving); otherwise the refill hands out samples
+  bool place_ok = true;
+  for (uint32_t k = 0; k < kShards; k++)
+    if (shard_limit(pool_cap_, k) > pool_.qcap) place_ok = false;
   JobDev *h_jobs = nullptr;               // pinned staging for job records
   // Status slots: two chunks' statuses and a drain's can be in flight at
   // once; a slot is reused only kStatusSlots enqueues later.
@@ -474,6 +480,7 @@ void Engine::server_loop() {
         JobDev &D = h_jobs[j.ring];
         std::memset(&D, 0, sizeof(D));
         D.sample_base = total;
+        j.first = total;
         D.n_samples = j.n_samples;
         D.width = j.cam.width;
         D.height = j.cam.height;
@@ -528,18 +535,37 @@ void Engine::server_loop() {
       // until the window passes them; their scene hold is already released)
       for (auto &r : live)
         if (!r->run.done && r->params.env_is && r->sh && r->sh->has_env) env_is = true;
-      bool okc;
+      bool okc = true, launched = true;
       if (tuning.drain_rounds) {
-        okc = launch_round(false, env_is) == OMR_OK;
+        // the next round: a range of the admitted samples in admission order;
+        // a range inside one request runs the single-job kernels
+        if (!place_ok) {
+          okc = launch_round(false, env_is) == OMR_OK;  // the refill hands out samples
+        } else if (placed < total) {
+          std::vector<uint64_t> jfirst, jcount;
+          std::vector<uint32_t> jring;
+          for (auto &r : live) {
+            if (r->run.first + r->run.n_samples <= placed) continue;
+            jfirst.push_back(r->run.first);
+            jcount.push_back(r->run.n_samples);
+            jring.push_back(r->run.ring);
+          }
+          uint32_t n0 = 0, solo_job = kNoSolo;
+          plan_round(jfirst, jcount, jring, placed, total, &n0, &solo_job);
+          okc = launch_placed_round(env_is, placed, n0, solo_job) == OMR_OK;
+          placed += n0;
+        } else {
+          launched = false;  // every admitted sample is placed: the statuses show the rest
+        }
       } else {
         okc = launch_steps(tuning.steps_per_sync, env_is) == OMR_OK;
       }
-      if (!okc || !enqueue_status()) {
+      if (!okc || ((launched || pend.empty()) && !enqueue_status())) {
         dev.unlock();
         abort_with(OMR_ERR_HIP, "bounce step launch failed: " + get_err());
         return;
       }
-      if (pend.size() < 2) continue;  // fill the pipeline before waiting
+      if (launched && pend.size() < 2) continue;  // fill the pipeline before waiting
     }
     if (pend.empty()) continue;
     // ---- 3. the oldest status
