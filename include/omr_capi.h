@@ -184,6 +184,9 @@ typedef struct {
   uint32_t round_steps;       /* drain rounds: extend/shade bounce steps (even, 0..16) before each
                                  round's drain -- the first bounces of fresh camera paths run as
                                  wide, coherent steps */
+  uint32_t solo_round_min;    /* drain rounds: a job with at least this many samples left ends in
+                                 a round of its own, run by the single-job kernels, instead of
+                                 sharing one with the next job (0 = half the pool) */
 } omr_tuning;
 
 typedef struct {

@@ -41,6 +41,7 @@ Engine::Engine(int device) : device_(device) {
   tuning.record_mb = 32768;
   tuning.drain_rounds = 1;
   tuning.round_steps = 2;
+  tuning.solo_round_min = 1u << 22;  // 4M samples (swept: half a pool, 16M, 4M, 1M)
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
     set_err("no HIP device " + std::to_string(device));
@@ -595,9 +596,9 @@ void Engine::plan_round(const std::vector<uint64_t> &first, const std::vector<ui
   const uint64_t rem = first[i] + count[i] - base;
   if (rem >= n) {
     *solo_job = ring[i];  // the round lies inside one job
-  } else if (2 * rem >= P) {
-    // at least half a pool of this job is left: it ends in a round of its own
-    // (single-job kernels) rather than sharing one with the next job
+  } else if (rem >= (tuning.solo_round_min ? (uint64_t)tuning.solo_round_min : (P + 1) / 2)) {
+    // enough of this job is left: it ends in a round of its own (single-job
+    // kernels) rather than sharing one with the next job
     n = rem;
     *solo_job = ring[i];
   }
