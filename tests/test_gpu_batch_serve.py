@@ -61,6 +61,39 @@ def test_batch_equals_alone(ctx, record_mb):
         ctx.release(s)
 
 
+@pytest.mark.parametrize("tune", [dict(pool_paths=4096, solo_round_min=1024), dict(pool_paths=4096, solo_round_min=0),
+                                  dict(pool_paths=8192, solo_round_min=1 << 30)])
+def test_batch_placed_rounds_equal_alone(ctx, tune):
+    """Drain rounds placed by the host over a small pool: rounds inside one
+    job run the single-job kernels, rounds across jobs the per-lane ones,
+    and a job with enough samples left ends in a round of its own; every
+    image equals the job alone."""
+    import omr
+
+    saved = ctx.get_tuning()
+    scenes, spec = _jobs(omr)
+    sids = [ctx.upload(s) for s in scenes]
+    cams, prms, ids = [], [], []
+    for si, w, h, ss, b, e, d, rr, seed in spec:
+        cams.append(_cam(omr, scenes[si], w, h))
+        prms.append(omr.params(ss=ss, spp_begin=b, spp_end=e, max_depth=d, rr_depth=rr, seed=seed))
+        ids.append(sids[si])
+    alone = [ctx.render(ids[i], cams[i], prms[i]) for i in range(len(spec))]
+    print("placed rounds", tune, flush=True)
+    try:
+        ctx.set_tuning(**tune)
+        batch = ctx.render_batch(ids, cams, prms)
+        st = ctx.last_stats()
+    finally:
+        ctx.set_tuning(**saved)
+    for i in range(len(spec)):
+        assert np.array_equal(alone[i], batch[i]), f"job {i} differs"
+    assert st["drain_launches"] >= -(-sum(
+        w * h * ss * ss * (e - b) for _, w, h, ss, b, e, _, _, _ in spec) // tune["pool_paths"])
+    for s in sids:
+        ctx.release(s)
+
+
 def test_batch_with_8wide_nodes_equals_4wide(ctx):
     """The batched (per-lane job) kernels over 8-wide nodes give the images
     of the 4-wide nodes, bit for bit."""
@@ -101,13 +134,17 @@ def test_progressive_batches_equal_one_shot(ctx):
     ctx.release(sid)
 
 
-@pytest.mark.parametrize("record_mb", [None, 0])
-def test_served_equals_alone(record_mb):
+@pytest.mark.parametrize("record_mb,solo_min", [(None, None), (0, None), (None, 1024)])
+def test_served_equals_alone(record_mb, solo_min):
+    """Requests submitted while others run; with solo_min, requests with
+    1024 or more samples left end in placed rounds of their own."""
     import omr
 
     ctx = omr.Context(0)
     if record_mb is not None:
         ctx.set_tuning(record_mb=record_mb)
+    if solo_min is not None:
+        ctx.set_tuning(solo_round_min=solo_min)
     scenes, spec = _jobs(omr)
     sids = [ctx.upload(s) for s in scenes]
     jobs = []
