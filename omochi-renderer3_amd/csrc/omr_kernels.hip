@@ -1218,6 +1218,15 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
     uint32_t dmax = J.max_depth == 0u ? 255u : J.max_depth;
     bool can_continue = !killed && depth + 1u < dmax;
     if (!killed) {
+      // Each material branch only chooses the continuation: the unnormalized
+      // direction `raw` (normalized once after the branches, where the lanes
+      // of every material run it together, instead of once per material
+      // branch) or the refracted direction, and the throughput factor
+      // num / den on the colour col.  Operations and their order per lane are
+      // those of PathTracer.cpp:445-591.
+      bool go = false, use_refr = false, lam = false;
+      f3 raw = mk(0.0f, 0.0f, 0.0f);
+      float num = 1.0f, den = prr;
       if (m.type == 0u) {
         // Radiance_Lambert (PathTracer.cpp:445-500), unnormalized basis
         if (dot(em, em) == 0.0f) {
@@ -1228,114 +1237,95 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
           float s1, c1;
           sincos_turns(u1, s1, c1);
           float r2 = sqrtf(u2), r3 = sqrtf(1.0f - u2);
-          f3 nd = normalize(add(add(scl(scl(u, r3), c1), scl(scl(vv, r3), s1)), scl(w, r2)));
-          float k = 1.0f / prr;
-          if (nee && can_continue) {
-            float e1 = rng_float(rng), e2 = rng_float(rng), e3 = rng_float(rng), e4 = rng_float(rng);
-            f3 ld;
-            size_t ei;
-            if (env_sample4(S, e1, e2, e3, e4, ld, ei)) {
-              // the sampled texel's pdf and radiance
-              float c = dot(nl, ld);
-              float pe = env_pdf_at(S, ei, ld);
-              if (c > 0.0f && pe > 0.0f) {
-                float pb = c * kInvPi;
-                float wl = (pe * pe) / (pe * pe + pb * pb);
-                float s = ((pb / pe) * wl) * k;
-                f3 L = env_at(S, ei);
-                f3 bc = mk(beta.x * col.x, beta.y * col.y, beta.z * col.z);
-                sh.c = mk(bc.x * L.x * s, bc.y * L.y * s, bc.z * L.z * s);
-                sh.o = x;
-                sh.d = ld;
-                sh.px = pixel;
-                shadow = true;
-              }
-            }
-          }
-          if (can_continue) {
-            beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
-            if (nee) {
-              mis_next = 1u;
-              pdf_next = dot(nl, nd) * kInvPi;
-            }
-            o = x;
-            d = nd;
-            depth++;
-            cont = true;
-          }
-        }
-      } else if (m.type == 1u) {
-        // Radiance_Specular (PathTracer.cpp:503-520)
-        f3 r = normalize(sub(d, scl(scl(nl, 2.0f), dot(d, nl))));
-        if (can_continue) {
-          float k = 1.0f / prr;
-          beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
-          o = x;
-          d = r;
-          depth++;
-          cont = true;
+          raw = add(add(scl(scl(u, r3), c1), scl(scl(vv, r3), s1)), scl(w, r2));
+          lam = true;
+          go = can_continue;
         }
       } else {
-        // Radiance_Refraction (PathTracer.cpp:523-591)
-        bool into = dot(nrm, nl) > 0.0f;
-        f3 refl = normalize(sub(d, scl(scl(nl, 2.0f), dot(d, nl))));
-        float nr = into ? 1.0f / m.ior : m.ior / 1.0f;
-        float dd = dot(d, nl);
-        float cos2t = 1.0f - nr * nr * (1.0f - dd * dd);
-        if (cos2t < 0.0f) {
-          // total internal reflection weighs by the untextured colour (PathTracer.cpp:543-548)
-          if (can_continue) {
-            float k = 1.0f / prr;
-            beta = mk(beta.x * (col0.x * k), beta.y * (col0.y * k), beta.z * (col0.z * k));
-            o = x;
-            d = refl;
-            depth++;
-            cont = true;
-          }
+        // Radiance_Specular (PathTracer.cpp:503-520) and the reflected ray of
+        // Radiance_Refraction (:523-591) are the same mirror direction
+        raw = sub(d, scl(scl(nl, 2.0f), dot(d, nl)));
+        if (m.type == 1u) {
+          go = can_continue;
         } else {
-          f3 refr = normalize(sub(scl(d, nr), scl(scl(nrm, into ? 1.0f : -1.0f), dd * nr + sqrtf(cos2t))));
-          float F0 = ((m.ior - 1.0f) * (m.ior - 1.0f)) / ((m.ior + 1.0f) * (m.ior + 1.0f));
-          float c = 1.0f - (into ? -dd : -dot(refr, nl));
-          float c5 = c * c * c * c * c;
-          float Fr = F0 + (1.0f - F0) * c5;
-          float Tr = (1.0f - Fr) * (nr * nr);
-          if (depth > 2u) {
-            float prob = 0.1f + 0.8f * Fr;
-            if (rng_float(rng) < prob) {
-              if (can_continue) {
-                float k = Fr / (prr * prob);
-                beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
-                o = x;
-                d = refl;
-                depth++;
-                cont = true;
+          bool into = dot(nrm, nl) > 0.0f;
+          float nr = into ? 1.0f / m.ior : m.ior / 1.0f;
+          float dd = dot(d, nl);
+          float cos2t = 1.0f - nr * nr * (1.0f - dd * dd);
+          if (cos2t < 0.0f) {
+            // total internal reflection weighs by the untextured colour (PathTracer.cpp:543-548)
+            col = col0;
+            go = can_continue;
+          } else {
+            const f3 refr = normalize(sub(scl(d, nr), scl(scl(nrm, into ? 1.0f : -1.0f), dd * nr + sqrtf(cos2t))));
+            float F0 = ((m.ior - 1.0f) * (m.ior - 1.0f)) / ((m.ior + 1.0f) * (m.ior + 1.0f));
+            float c = 1.0f - (into ? -dd : -dot(refr, nl));
+            float c5 = c * c * c * c * c;
+            float Fr = F0 + (1.0f - F0) * c5;
+            float Tr = (1.0f - Fr) * (nr * nr);
+            if (depth > 2u) {
+              float prob = 0.1f + 0.8f * Fr;
+              if (rng_float(rng) < prob) {
+                num = Fr;
+                den = prr * prob;
+              } else {
+                num = Tr;
+                den = prr * (1.0f - prob);
+                raw = refr;  // already normalized
+                use_refr = true;
               }
+              go = can_continue;
             } else if (can_continue) {
-              float k = Tr / (prr * (1.0f - prob));
-              beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
-              o = x;
-              d = refr;
-              depth++;
-              cont = true;
+              // depth <= 2: both branches; the refracted one waits in the
+              // slot's pending stack, the reflected one continues now.
+              float kt = Tr / prr;
+              f3 bt = mk(beta.x * (col.x * kt), beta.y * (col.y * kt), beta.z * (col.z * kt));
+              float4 *pe = pend + ((size_t)npend * P + slot) * 3;
+              pe[0] = make_float4(x.x, x.y, x.z, bt.x);
+              pe[1] = make_float4(refr.x, refr.y, refr.z, bt.y);
+              pe[2] = make_float4(bt.z, __uint_as_float(depth + 1u), __uint_as_float(branch | (1u << depth)), 0.0f);
+              npend++;
+              num = Fr;
+              go = true;
             }
-          } else if (can_continue) {
-            // depth <= 2: both branches; the refracted one waits in the
-            // slot's pending stack, the reflected one continues now.
-            float kt = Tr / prr;
-            f3 bt = mk(beta.x * (col.x * kt), beta.y * (col.y * kt), beta.z * (col.z * kt));
-            float4 *pe = pend + ((size_t)npend * P + slot) * 3;
-            pe[0] = make_float4(x.x, x.y, x.z, bt.x);
-            pe[1] = make_float4(refr.x, refr.y, refr.z, bt.y);
-            pe[2] = make_float4(bt.z, __uint_as_float(depth + 1u), __uint_as_float(branch | (1u << depth)), 0.0f);
-            npend++;
-            float kr = Fr / prr;
-            beta = mk(beta.x * (col.x * kr), beta.y * (col.y * kr), beta.z * (col.z * kr));
-            o = x;
-            d = refl;
-            depth++;
-            cont = true;
           }
         }
+      }
+      const float k = num / den;
+      const f3 nd = normalize(raw);
+      if (lam && nee && can_continue) {
+        // one environment light sample, weighed by the power heuristic
+        float e1 = rng_float(rng), e2 = rng_float(rng), e3 = rng_float(rng), e4 = rng_float(rng);
+        f3 ld;
+        size_t ei;
+        if (env_sample4(S, e1, e2, e3, e4, ld, ei)) {
+          // the sampled texel's pdf and radiance
+          float c = dot(nl, ld);
+          float pe = env_pdf_at(S, ei, ld);
+          if (c > 0.0f && pe > 0.0f) {
+            float pb = c * kInvPi;
+            float wl = (pe * pe) / (pe * pe + pb * pb);
+            float s = ((pb / pe) * wl) * k;
+            f3 L = env_at(S, ei);
+            f3 bc = mk(beta.x * col.x, beta.y * col.y, beta.z * col.z);
+            sh.c = mk(bc.x * L.x * s, bc.y * L.y * s, bc.z * L.z * s);
+            sh.o = x;
+            sh.d = ld;
+            sh.px = pixel;
+            shadow = true;
+          }
+        }
+      }
+      if (go) {
+        beta = mk(beta.x * (col.x * k), beta.y * (col.y * k), beta.z * (col.z * k));
+        if (lam && nee) {
+          mis_next = 1u;
+          pdf_next = dot(nl, nd) * kInvPi;
+        }
+        o = x;
+        d = mk(use_refr ? raw.x : nd.x, use_refr ? raw.y : nd.y, use_refr ? raw.z : nd.z);
+        depth++;
+        cont = true;
       }
     }
   }
@@ -1645,6 +1635,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 3 
     segs++;
     bool shadow = false;
     ShadowRay sh;
+    // the segment's origin and direction live in the traversal only, so one
+    // copy of them stays in registers across the loop
+    st.o = tr.o;
+    st.d = tr.d;
     bool cont = shade_segment(A, slot, st, make_float4(tr.best.t, __int_as_float(tr.best.prim), tr.best.u, tr.best.v),
                               nullptr, shadow, sh, J, S);
     if (shadow) {
