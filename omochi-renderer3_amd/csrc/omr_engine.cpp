@@ -427,7 +427,7 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
         (rc = dup(nullptr, (size_t)w * h * 8, &p_q)) || (rc = dup(nullptr, (size_t)h * (w + 1) * 8, &p_row)) ||
         (rc = dup(nullptr, (size_t)(h + 1) * 8, &p_marg)) || (rc = dup(nullptr, (size_t)w * h * 8, &p_rg)) ||
         (rc = dup(nullptr, (size_t)h * 8, &p_mg)) || (rc = dup(nullptr, (size_t)h * 8, &p_rs)) ||
-        (rc = dup(nullptr, 8, &p_ms))) {
+        (rc = dup(nullptr, 16, &p_ms))) {
       free_scene_mem(S);
       return rc;
     }

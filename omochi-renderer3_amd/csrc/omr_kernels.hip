@@ -35,9 +35,13 @@ __device__ inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b
 __device__ inline f3 cross(f3 a, f3 b) {
   return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
+// normalize: one correctly rounded reciprocal of the length and three
+// products (the reference divides each component by the length; the two
+// differ by at most an ulp per component, and the CPU restatement uses the
+// same form)
 __device__ inline f3 normalize(f3 a) {
-  float len = sqrtf(dot(a, a));
-  return mk(a.x / len, a.y / len, a.z / len);
+  float inv = 1.0f / sqrtf(dot(a, a));
+  return mk(a.x * inv, a.y * inv, a.z * inv);
 }
 __device__ inline f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
 
@@ -626,21 +630,21 @@ __device__ inline void env_texel(f3 d, uint32_t w, uint32_t h, uint32_t &ix, uin
   iy = (uint32_t)((int)(v * (float)h) % (int)h);
 }
 // Texel index of direction d and the importance-sampling pdf there:
-// pdf = q / total * (w h) / (2 pi^2 sin(theta)) for the texel's quantized
-// weight q, sin(theta) = |d.xz| (0 without weights); env_at(i) is its radiance.
+// pdf = q (w h) / (2 pi^2 total) / sin(theta) for the texel's quantized
+// weight q, sin(theta) = |d.xz|: the product q times the scene's pdf factor in
+// binary64, rounded to binary32, then one binary32 division (0 without
+// weights); env_at(i) is its radiance.
 __device__ inline size_t env_index(const SceneDev &S, f3 d) {
   uint32_t ix, iy;
   env_texel(d, S.env_w, S.env_h, ix, iy);
   return (size_t)iy * S.env_w + ix;
 }
 __device__ inline float env_pdf_at(const SceneDev &S, size_t i, f3 d) {
-  unsigned long long tot = gld(S.env_marg + S.env_h);
-  if (tot == 0ull) return 0.0f;
+  const double kf = gld(S.env_mscale + 1);
   unsigned long long q = gld(S.env_q + i);
   float st = sqrtf(d.x * d.x + d.z * d.z);
-  if (q == 0ull || !(st > 0.0f)) return 0.0f;
-  double p = (double)q / (double)tot * ((double)S.env_w * (double)S.env_h) / (19.739208802178716 * (double)st);
-  return (float)p;
+  if (kf == 0.0 || q == 0ull || !(st > 0.0f)) return 0.0f;
+  return (float)((double)q * kf) / st;
 }
 __device__ inline f3 env_at(const SceneDev &S, size_t i) {
   const float *p = S.env + 3 * i;
@@ -1386,23 +1390,58 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO ? 4 : 5
     Js = cload(A.jobs + cload(&A.ctrl->solo_job));
     Ss = cload(A.scenes + Js.scene);
   }
+  const float4 *hits = reinterpret_cast<const float4 *>(A.p.hit);
+  // Read-ahead: while a lane shades item i, the queue entry of item
+  // i + stride is in flight, and in the single-job form (four waves per SIMD,
+  // registers to spare) also the path state of item i + stride and the entry
+  // of item i + 2 stride, so none of these loads sits on the chain of
+  // dependent loads of the next item.  The per-lane form (five waves) reads
+  // the path state when it reaches the item.
+  constexpr bool AHEAD = SOLO;
+  auto entry_at = [&](uint32_t j) { return j < n ? A.p.queue[parity][flat_slot(s_pre, j, qcap)] : 0u; };
+  struct State {
+    float4 ro, rd, hr;
+    uint4 ms, rng;
+  } nx{};
+  auto state_of = [&](uint32_t j, uint32_t e, State &to) {
+    if (j < n) {
+      const uint32_t sl = q_slot(e);
+      to.ro = ray_o[sl];
+      to.rd = ray_d[sl];
+      to.ms = misc[sl];
+      to.hr = hits[sl];
+      to.rng = rngs[sl];
+    }
+  };
   const uint32_t i0 = blockIdx.x * WG + threadIdx.x;
-  uint32_t next = i0 < n ? A.p.queue[parity][flat_slot(s_pre, i0, qcap)] : 0u;  // a queue entry
+  uint32_t next = entry_at(i0);  // entry of the lane's next item
+  uint32_t next2 = 0u;           // AHEAD: and of the item after it
+  if (AHEAD) {
+    state_of(i0, next, nx);
+    next2 = entry_at(i0 + stride);
+  }
   for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     bool valid = i < n;
     const uint32_t slot = q_slot(next);
-    const uint32_t i2 = i + stride;
-    next = i2 < n ? A.p.queue[parity][flat_slot(s_pre, i2, qcap)] : 0u;
+    State cur{};
+    if (AHEAD) {
+      cur = nx;
+      next = next2;
+      state_of(i + stride, next, nx);
+      next2 = entry_at(i + 2u * stride);
+    } else {
+      next = entry_at(i + stride);
+      state_of(i, q_entry(slot, 0u), cur);
+    }
     bool cont = false, fin = false, shadow = false;
     PathState st;
     st.js = 0;
     ShadowRay sh;
     if (valid) {
-      float4 ro = ray_o[slot], rd = ray_d[slot];
-      uint4 ms = misc[slot];
-      float4 hr = reinterpret_cast<const float4 *>(A.p.hit)[slot];
-      st.rng = rngs[slot];
+      const float4 ro = cur.ro, rd = cur.rd, hr = cur.hr;
+      const uint4 ms = cur.ms;
+      st.rng = cur.rng;
       st.o = xyz(ro);
       st.d = xyz(rd);
       st.beta = mk(ro.w, rd.w, __uint_as_float(ms.x));
@@ -1940,8 +1979,12 @@ __global__ void k_env_guides(const unsigned long long *row, const unsigned long 
   uint2_t *g = m ? mguide : rguide + (size_t)t * w;
   const unsigned long long total = a[n];
   const double scale = total > 0ull ? (double)n / (double)total : 0.0;
-  if (m) mscale[0] = scale;
-  else rscale[t] = scale;
+  if (m) {
+    mscale[0] = scale;
+    mscale[1] = total > 0ull ? ((double)w * (double)h) / (19.739208802178716 * (double)total) : 0.0;
+  } else {
+    rscale[t] = scale;
+  }
   for (uint32_t b = 0; b < n; b++) g[b] = uint2_t{0xFFFFFFFFu, 0u};
   if (total == 0ull) return;
   for (uint32_t i = 0; i < n; i++) {

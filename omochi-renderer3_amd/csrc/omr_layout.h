@@ -101,7 +101,8 @@ struct SceneDev {
   const uint2_t *env_rguide;           // env_h x env_w
   const uint2_t *env_mguide;           // env_h
   const double *env_rscale;            // env_h: env_w / row total (0 for an empty row)
-  const double *env_mscale;            // 1: env_h / total
+  // 2: env_h / total, and the pdf factor (w h) / (2 pi^2 total) (0 without weights)
+  const double *env_mscale;
   uint32_t n_tri, n_sph;
   const float *planes;         // infinite floors y = planes[i]; ids n_tri + n_sph + i
   uint32_t n_planes;

@@ -58,10 +58,13 @@ static inline float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; 
 static inline v3 cross(v3 a, v3 b) {
   return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-/* Vector3::normalize (src/tools/Vector.h:83-86): divide each component by length */
+/* Vector3::normalize (src/tools/Vector.h:83-86) divides each component by the
+ * length; here one correctly rounded reciprocal of the length and three
+ * products (at most an ulp apart per component; the GPU path uses the same
+ * form, so the two stay bit-identical) */
 static inline v3 normalize(v3 a) {
-  float len = sqrtf(dot(a, a));
-  return mk(a.x / len, a.y / len, a.z / len);
+  float inv = 1.0f / sqrtf(dot(a, a));
+  return mk(a.x * inv, a.y * inv, a.z * inv);
 }
 static inline v3 ld3(const float *p) { return mk(p[0], p[1], p[2]); }
 
@@ -683,9 +686,15 @@ int orc_env_cdf(const orc_handle *h, float *marginal, float *conditional) {
   return 0;
 }
 
+/* W*H / (2 pi^2 total) in binary64: the texel weight's factor in the pdf */
+static double env_pdf_factor(uint32_t W, uint32_t H, uint64_t tot) {
+  return ((double)W * (double)H) / (19.739208802178716 * (double)tot);
+}
+
 /* solid-angle pdf of the sampler at direction d inside texel i: the texel's
- * probability times W*H / (2 pi^2 sin(theta)), sin(theta) = |d.xz|,
- * evaluated in double, returned as float. */
+ * probability times W*H / (2 pi^2 sin(theta)), sin(theta) = |d.xz|, as
+ * q * (W*H / (2 pi^2 total)) in binary64, rounded to binary32, divided by
+ * sin(theta) in binary32. */
 static float env_pdf_texel(const orc_handle *h, size_t i, float dx, float dz) {
   uint32_t W = h->s.env_w, H = h->s.env_h;
   uint64_t tot = h->env_marg[H];
@@ -693,8 +702,7 @@ static float env_pdf_texel(const orc_handle *h, size_t i, float dx, float dz) {
   uint64_t q = h->env_q[i];
   float st = sqrtf(dx * dx + dz * dz);
   if (q == 0 || !(st > 0.0f)) return 0.0f;
-  double p = (double)q / (double)tot * ((double)W * (double)H) / (19.739208802178716 * (double)st);
-  return (float)p;
+  return (float)((double)q * env_pdf_factor(W, H, tot)) / st;
 }
 
 /* the same for any direction: its texel by the latlong mapping */
@@ -708,8 +716,7 @@ float orc_env_pdf(const orc_handle *h, float dx, float dy, float dz) {
   uint64_t q = h->env_q[(size_t)iy * W + ix];
   float st = sqrtf(dx * dx + dz * dz);
   if (q == 0 || !(st > 0.0f)) return 0.0f;
-  double p = (double)q / (double)tot * ((double)W * (double)H) / (19.739208802178716 * (double)st);
-  return (float)p;
+  return (float)((double)q * env_pdf_factor(W, H, tot)) / st;
 }
 
 /* largest i in [0, n) with a[i] <= key (a nondecreasing, a[0] <= key) */
