@@ -746,26 +746,28 @@ __device__ inline unsigned long long fix(float v) {
 // lanes ever hold the same sample (its paths and pending branches run one at
 // a time in one slot, and a shadow record exists only for the step that made
 // it).
-__device__ inline void deposit(unsigned long long *recs, unsigned long long *accum, uint32_t sidx, uint32_t pixel,
-                               unsigned long long a, unsigned long long b, unsigned long long c) {
+// Without records the pixel of the sample is computed here, where it is
+// needed (pixel = sample / (ss^2 spp): samples are stored pixel-major).
+__device__ inline void deposit(const JobDev &J, uint32_t sidx, unsigned long long a, unsigned long long b,
+                               unsigned long long c) {
   if ((a | b | c) == 0ull) return;
-  if (recs) {
+  if (J.recs) {
     __attribute__((address_space(1))) unsigned long long *r =
-        (__attribute__((address_space(1))) unsigned long long *)recs + 3ull * sidx;
+        (__attribute__((address_space(1))) unsigned long long *)J.recs + 3ull * sidx;
     const unsigned long long ra = r[0], rb = r[1], rc = r[2];
     r[0] = ra + a;
     r[1] = rb + b;
     r[2] = rc + c;
     return;
   }
-  unsigned long long *acc = accum + 3ull * pixel;
+  const uint32_t pixel = sidx / (J.ss * J.ss * J.nspp);
+  unsigned long long *acc = J.accum + 3ull * pixel;
   if (a) gadd(acc + 0, a);
   if (b) gadd(acc + 1, b);
   if (c) gadd(acc + 2, c);
 }
-__device__ inline void add_px(unsigned long long *recs, unsigned long long *accum, uint32_t sidx, uint32_t pixel,
-                              f3 beta, f3 L, float s) {
-  deposit(recs, accum, sidx, pixel, fix(beta.x * L.x * s), fix(beta.y * L.y * s), fix(beta.z * L.z * s));
+__device__ inline void add_px(const JobDev &J, uint32_t sidx, f3 beta, f3 L, float s) {
+  deposit(J, sidx, fix(beta.x * L.x * s), fix(beta.y * L.y * s), fix(beta.z * L.z * s));
 }
 
 // ------------------------------------------------------------ job lookup
@@ -1131,7 +1133,6 @@ struct PathState {
 // An environment light sample waiting for its visibility test.
 struct ShadowRay {
   f3 o, d, c;
-  uint32_t px;
 };
 
 // Shades the segment of slot `slot` that ended at hr (t, primitive id bits,
@@ -1155,7 +1156,6 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
   uint32_t depth = ps_depth(word), branch = ps_branch(word), npend = ps_pending(word);
   const bool env_on = S.env != nullptr;
   bool nee = env_on && (J.flags & 1u) && gld(S.env_marg + S.env_h) > 0ull;
-  const uint32_t pixel = sidx / (J.ss * J.ss * J.nspp);
   int32_t prim = __float_as_int(hr.y);
   uint32_t mis_next = 0;
   if (prim < 0) {
@@ -1168,7 +1168,7 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
         float pe = env_pdf_at(S, ei, d);
         w = (pp * pp) / (pp * pp + pe * pe);
       }
-      add_px(J.recs, J.accum, sidx, pixel, beta, env_at(S, ei), w);
+      add_px(J, sidx, beta, env_at(S, ei), w);
     }
   } else {
     uint32_t pid = (uint32_t)prim;
@@ -1218,7 +1218,9 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
     // with the reference's next-event estimation (flags bit 1) emission counts
     // only where the camera ray hits (PathTracer.cpp:406-439); its direct
     // term is zero (:312-314)
-    if (!(J.flags & 2u) || depth == 0u) add_px(J.recs, J.accum, sidx, pixel, beta, em, 1.0f);
+    // (a surface that emits nothing adds zero: skipped)
+    if ((!(J.flags & 2u) || depth == 0u) && (em.x != 0.0f || em.y != 0.0f || em.z != 0.0f))
+      add_px(J, sidx, beta, em, 1.0f);
     uint32_t dmax = J.max_depth == 0u ? 255u : J.max_depth;
     bool can_continue = !killed && depth + 1u < dmax;
     if (!killed) {
@@ -1315,7 +1317,6 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
             sh.c = mk(bc.x * L.x * s, bc.y * L.y * s, bc.z * L.z * s);
             sh.o = x;
             sh.d = ld;
-            sh.px = pixel;
             shadow = true;
           }
         }
@@ -1485,7 +1486,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO ? 4 : 5
     if (shadow) {
       float4 *sq = reinterpret_cast<float4 *>(A.p.shadow) + ((size_t)k * qcap + spos) * 3;
       sq[0] = make_float4(sh.o.x, sh.o.y, sh.o.z, __uint_as_float(st.js));
-      sq[1] = make_float4(sh.d.x, sh.d.y, sh.d.z, __uint_as_float(sh.px));
+      sq[1] = make_float4(sh.d.x, sh.d.y, sh.d.z, 0.0f);
       sq[2] = make_float4(sh.c.x, sh.c.y, sh.c.z, __uint_as_float(st.sidx));
     }
   }
@@ -1499,7 +1500,7 @@ __device__ inline void add_shadow(const KernelArgs &A, uint32_t r, const JobDev 
   const float4 *sq = reinterpret_cast<const float4 *>(A.p.shadow) + (size_t)r * 3;
   float4 a = sq[0], b = sq[1], c = sq[2];
   const JobDev &J = SOLO ? Js : A.jobs[__float_as_uint(a.w)];
-  deposit(J.recs, J.accum, __float_as_uint(c.w), __float_as_uint(b.w), fix(c.x), fix(c.y), fix(c.z));
+  deposit(J, __float_as_uint(c.w), fix(c.x), fix(c.y), fix(c.z));
 }
 
 template <int WG, int SHORT, int NW, bool DF, bool SOLO, bool TAG = false>
@@ -1688,7 +1689,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 3 
       uint32_t snv = 0, snt = 0;
       traverse<WG, SHORT, true, false, NW>(S, sh.o, sh.d, hs, stk, snv, snt);
       shadows++;
-      if (hs.prim < 0) deposit(J.recs, J.accum, st.sidx, sh.px, fix(sh.c.x), fix(sh.c.y), fix(sh.c.z));
+      if (hs.prim < 0) deposit(J, st.sidx, fix(sh.c.x), fix(sh.c.y), fix(sh.c.z));
     }
     if (cont) {
       tracing = true;
