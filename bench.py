@@ -194,7 +194,8 @@ def profiled_pass(ctx, args_list):
     """Renders with per-launch HIP events on the engine's stream (profiling
     on: every bounce-step kernel launched directly, node visits and primitive
     tests counted by the extend and drain kernels' counting variants)."""
-    keys = ("segments", "shadow_rays", "extend_ms", "shade_ms", "shadow_ms", "extend_launches", "node_visits",
+    keys = ("segments", "shadow_rays", "extend_ms", "shade_ms", "shadow_ms", "extend_launches", "shadow_launches",
+            "node_visits",
             "prim_tests", "drain_segments", "drained_paths", "drain_ms", "drain_node_visits", "drain_prim_tests")
     acc = dict.fromkeys(keys, 0)
     drains = 0
@@ -220,7 +221,7 @@ def roofline_from(acc, workload, node_width=4):
         "k_extend": (B_SEG_TRACE * ext_seg + bn * acc["node_visits"] + B_PRIM * acc["prim_tests"],
                      acc["extend_ms"], acc["extend_launches"], f"k_extend<64, 16, {w}, true, true, true>"),
         "k_shade": (B_SEG_SHADE * ext_seg, acc["shade_ms"], acc["extend_launches"], "k_shade<64, true, true>"),
-        "k_shadow": (B_SHADOW * acc["shadow_rays"], acc["shadow_ms"], acc["extend_launches"],
+        "k_shadow": (B_SHADOW * acc["shadow_rays"], acc["shadow_ms"], acc["shadow_launches"],
                      f"k_shadow<64, 16, {w}, true, true, true>"),
         "k_drain": ((B_SEG_TRACE + B_SEG_SHADE) * acc["drain_segments"] + bn * acc["drain_node_visits"]
                     + B_PRIM * acc["drain_prim_tests"], acc["drain_ms"], acc["drain_launches"],
@@ -405,6 +406,13 @@ def run_served(ctx, omr, args, dev):
     # warm-up: a few requests alone and one short served stream (allocations,
     # graph captures)
     render_alone(ctx, alist[:max(1, args.warmup)])
+    if args.roofline_only:
+        # only the profiled pass below (one host thread: for hardware-counter
+        # and kernel-trace runs of the roofline kernel)
+        nroof = min(len(reqs), args.roofline_requests)
+        log(f"c4: profiled pass over {nroof} requests")
+        roof = roofline_from(profiled_pass(ctx, alist[:nroof]), "c4", ctx.get_tuning()["node_width"])
+        return {"workload": "c4", "roofline_only": True, "roofline": roof}, []
     rng = np.random.default_rng(11)
     # plain: each request alone, back to back; also the service rate
     log(f"c4: {len(reqs)} requests alone")
@@ -511,6 +519,8 @@ def main():
     ap.add_argument("--load", type=float, default=2.0, help="c4: offered load of the timed streams (x plain capacity)")
     ap.add_argument("--roofline-requests", type=int, default=32, help="c4: requests of the profiled pass")
     ap.add_argument("--no-c3", action="store_true", help="c4: leave out the batched configuration")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="c4: only the profiled pass of the roofline (no served streams, no batched run)")
     ap.add_argument("--batch-reps", type=int, default=3, help="c3: timed batches")
     ap.add_argument("--tune-ms", type=int, default=10000, help="c3: autotuner budget (0 = default tuning)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -541,7 +551,7 @@ def main():
         items = []
     elif args.workload == "c4":
         out, items = run_served(ctx, omr, args, dev)
-        if not args.no_c3:
+        if not args.no_c3 and not args.roofline_only:
             out["batched"] = run_batched(ctx, omr, args)
     else:
         out, items = run_plain(ctx, omr, args, dev)

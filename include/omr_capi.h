@@ -138,6 +138,7 @@ typedef struct {
   double drain_ms;
   uint64_t drain_node_visits, drain_prim_tests;  /* drain launch, when profiling */
   uint64_t drain_launches;
+  uint64_t shadow_launches;  /* shadow-ray launches, when profiling (scenes with an environment sampler) */
 } omr_render_stats;
 
 typedef struct {

@@ -628,6 +628,7 @@ int Engine::collect_profile() {
     OMR_HIP(hipEventElapsedTime(&ms, prof_ev_[2 * i], prof_ev_[2 * i + 1]));
     prof_ms_[prof_kind_[i]] += ms;
     if (prof_kind_[i] == 0) prof_extend_launches_++;
+    if (prof_kind_[i] == 2) prof_shadow_launches_++;
   }
   prof_kind_.clear();
   return OMR_OK;
@@ -820,7 +821,7 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
   uint64_t drained = 0, drain_launches = 0;
   double drain_ms = 0.0;
   prof_ms_[0] = prof_ms_[1] = prof_ms_[2] = 0.0;
-  prof_extend_launches_ = 0;
+  prof_extend_launches_ = prof_shadow_launches_ = 0;
   if (tuning.drain_rounds) {
     // Rounds: every slot holds a sample after the fill, the drain runs them
     // all to their end (and marks the slots idle), the next round fills the
@@ -929,6 +930,7 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     st->shade_ms = prof_ms_[1];
     st->shadow_ms = prof_ms_[2];
     st->extend_launches = prof_extend_launches_;
+    st->shadow_launches = prof_shadow_launches_;
     st->node_visits = sum_shards(h_ctrl_->node_visits);
     st->prim_tests = sum_shards(h_ctrl_->prim_tests);
     st->drained_paths = drained;

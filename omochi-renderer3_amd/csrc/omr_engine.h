@@ -207,7 +207,7 @@ class Engine {
   std::vector<hipEvent_t> prof_ev_;
   std::vector<int> prof_kind_;  // 0 extend, 1 shade, 2 shadow
   double prof_ms_[3] = {0, 0, 0};
-  uint64_t prof_extend_launches_ = 0;
+  uint64_t prof_extend_launches_ = 0, prof_shadow_launches_ = 0;
   int collect_profile();
   LaunchShape shape() const;
   KernelArgs args() const;

@@ -67,7 +67,8 @@ class RenderStats(C.Structure):
                 ("shade_ms", C.c_double), ("shadow_ms", C.c_double), ("extend_launches", C.c_uint64),
                 ("node_visits", C.c_uint64), ("prim_tests", C.c_uint64), ("drained_paths", C.c_uint64),
                 ("drain_segments", C.c_uint64), ("drain_ms", C.c_double), ("drain_node_visits", C.c_uint64),
-                ("drain_prim_tests", C.c_uint64), ("drain_launches", C.c_uint64)]
+                ("drain_prim_tests", C.c_uint64), ("drain_launches", C.c_uint64),
+                ("shadow_launches", C.c_uint64)]
 
 
 class SceneInfo(C.Structure):
