@@ -206,6 +206,7 @@ void collapse(const std::vector<BNode> &bn, float pad, Leaf leaf, std::vector<ty
     }
     Node w;
     std::memset(&w, 0, sizeof(w));
+    // empty slots: code -1 and an inverted box
     for (int k = 0; k < W; k++) {
       w.child[k] = -1;
       w.lox[k] = w.loy[k] = w.loz[k] = kInf;

@@ -1142,6 +1142,15 @@ struct ShadowRay {
 // stored (null: st.pdf).  Returns true while the path, or one of its pending
 // glass branches, continues; `shadow` is set when an environment sample needs
 // a visibility ray.
+// SPEC: the environment light sample of a Lambert vertex is drawn at the
+// start of the hit branch, before the material is known, so its chain of
+// dependent table loads overlaps the normal and material loads instead of
+// following them (the sample is used only where the vertex turns out to be a
+// non-emissive Lambert one that samples the environment; its uniforms are the
+// same draws: one for the material, one for Russian roulette past rr_depth,
+// two for the Lambert direction come first).  Costs registers, so only the
+// shading kernel uses it.
+template <bool SPEC = false>
 __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathState &st, float4 hr,
                                      const float *pdf_mem, bool &shadow, ShadowRay &sh, const JobDev &J,
                                      const SceneDev &S) {
@@ -1172,6 +1181,23 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
     }
   } else {
     uint32_t pid = (uint32_t)prim;
+    bool s_ok = false;  // SPEC: the environment sample, its texel's pdf and radiance
+    f3 s_ld = mk(0.0f, 0.0f, 0.0f), s_L = s_ld;
+    float s_pe = 0.0f;
+    if (SPEC && nee) {
+      uint4 r2 = rng;
+      (void)rng_next(r2);                             // material roulette
+      if (depth > J.rr_depth) (void)rng_next(r2);     // Russian roulette
+      (void)rng_next(r2);                             // Lambert direction (two draws)
+      (void)rng_next(r2);
+      float e1 = rng_float(r2), e2 = rng_float(r2), e3 = rng_float(r2), e4 = rng_float(r2);
+      size_t ei;
+      s_ok = env_sample4(S, e1, e2, e3, e4, s_ld, ei);
+      if (s_ok) {
+        s_pe = env_pdf_at(S, ei, s_ld);
+        s_L = env_at(S, ei);
+      }
+    }
     f3 x = add(o, scl(d, hr.x));
     f3 nrm;
     if (pid < S.n_tri) {
@@ -1302,17 +1328,22 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
       if (lam && nee && can_continue) {
         // one environment light sample, weighed by the power heuristic
         float e1 = rng_float(rng), e2 = rng_float(rng), e3 = rng_float(rng), e4 = rng_float(rng);
-        f3 ld;
-        size_t ei;
-        if (env_sample4(S, e1, e2, e3, e4, ld, ei)) {
+        f3 ld = s_ld;
+        float pe = s_pe;
+        bool ok = s_ok;
+        size_t ei = 0;
+        if (!SPEC) {
+          ok = env_sample4(S, e1, e2, e3, e4, ld, ei);
+          if (ok) pe = env_pdf_at(S, ei, ld);
+        }
+        if (ok) {
           // the sampled texel's pdf and radiance
           float c = dot(nl, ld);
-          float pe = env_pdf_at(S, ei, ld);
           if (c > 0.0f && pe > 0.0f) {
             float pb = c * kInvPi;
             float wl = (pe * pe) / (pe * pe + pb * pb);
             float s = ((pb / pe) * wl) * k;
-            f3 L = env_at(S, ei);
+            const f3 L = SPEC ? s_L : env_at(S, ei);
             f3 bc = mk(beta.x * col.x, beta.y * col.y, beta.z * col.z);
             sh.c = mk(bc.x * L.x * s, bc.y * L.y * s, bc.z * L.z * s);
             sh.o = x;
@@ -1451,7 +1482,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO ? 4 : 5
       st.js = ms.w;
       st.pdf = 0.0f;
       if (SOLO) {
-        cont = shade_segment(A, slot, st, hr, A.p.pdf, shadow, sh, Js, Ss);
+        cont = shade_segment<true>(A, slot, st, hr, A.p.pdf, shadow, sh, Js, Ss);
       } else {
         const JobDev &J = A.jobs[st.js];
         cont = shade_segment(A, slot, st, hr, A.p.pdf, shadow, sh, J, A.scenes[J.scene]);

@@ -87,6 +87,9 @@ def test_camera_matches_reference_constructor():
     assert np.allclose(cam.screen_x, X, atol=1e-4) and np.allclose(cam.screen_y, Y, atol=1e-4)
 
 
+KINF = np.float32(np.ldexp(np.float32(0x1FFFFFE), 102))  # (FLT_MAX - 1) / 2, as the builder writes it
+
+
 def _nodes(nb, width=4):
     rec = np.dtype([("b", "<f4", (6, width)), ("child", "<i4", width), ("meta", "<i4", width)])
     return np.frombuffer(nb.tobytes(), dtype=rec)
@@ -97,7 +100,8 @@ def _nodes(nb, width=4):
 def test_bvh_invariants(kind, width):
     """Every primitive in exactly one leaf; every primitive inside its leaf's
     box; child boxes inside the parent's slot box; leaves of 1..8 (4-wide
-    128-byte and 8-wide 256-byte nodes)."""
+    128-byte and 8-wide 256-byte nodes); an empty child slot (code -1) has
+    the inverted box lo = +kInf, hi = -kInf on every axis."""
     import omr
 
     scene = omr.procedural(kind)
@@ -124,6 +128,7 @@ def test_bvh_invariants(kind, width):
         for k in range(width):
             c = int(nd["child"][k])
             if c == -1:
+                assert all(b[2 * ax][k] == KINF and b[2 * ax + 1][k] == -KINF for ax in range(3))
                 continue
             clo = np.array([b[0][k], b[2][k], b[4][k]])
             chi = np.array([b[1][k], b[3][k], b[5][k]])
