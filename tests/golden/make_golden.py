@@ -3,7 +3,8 @@
 Runs only in a container that has the reference checkout: it builds
 oracle/_ref/libref.so from the reference sources (oracle/Makefile) and calls
 the reference's Random, Scene::CheckIntersection (Polygon / Sphere, FP64),
-IBL::Sample, PathTracer::RenderScene, InfiniteFloor, HDRImage's RGBE reader
+IBL::Sample, PathTracer::RenderScene (procedural scenes, its own CornellBoxScene and
+loaded .scene files), InfiniteFloor, HDRImage's RGBE reader
 and writer and SceneFromExternalFile (with Model's OBJ/MTL reader) through
 oracle/ref_harness.cpp.
 Inputs are regenerated from seeds by the tests; the fixtures hold the
@@ -207,6 +208,25 @@ def cornell_ref():
     print("cornell hits", int((res["obj"] >= 0).sum()), "of", len(rays), "render mean", float(mean.mean()))
 
 
+def textured_render():
+    """textured.scene (loaded by the reference's SceneFromExternalFile: its
+    image library decodes the PNG/JPEG textures, its IBL the sky) rendered
+    by the reference's PathTracer (1 thread, reference mode)."""
+    c = gi.TEXTURED_RENDER
+    cwd = os.getcwd()
+    os.chdir(SCENES)
+    try:
+        r = pyref.RefFileScene("textured.scene")
+        mean, var = r.render(c["w"], c["h"], c["pos"], c["dir"], (0, 1, 0), c["screen_h"], c["dist"], c["ss"],
+                             c["batches"])
+        r.close()
+    finally:
+        os.chdir(cwd)
+    np.savez_compressed(os.path.join(HERE, "ref_render_textured.npz"), mean=mean.astype(np.float32),
+                        var=var.astype(np.float32), **{k: np.asarray(v) for k, v in c.items()})
+    print("textured render mean", float(mean.mean()))
+
+
 def ibl_sphere():
     """A Lambert sphere lit only by the reference's Barce HDR: the reference's
     PathTracer render.  The HDR's texels (as the reference decodes them) are
@@ -233,6 +253,7 @@ if __name__ == "__main__":
         sys.exit(0)
     cornell_ref()
     ibl_sphere()
+    textured_render()
     textures()
     texture_lookup()
     hdr_files()

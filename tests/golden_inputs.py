@@ -55,6 +55,12 @@ IBL_RENDER = dict(w=32, h=32, pos=(0.0, 0.0, 90.0), dir=(0.0, 0.0, -1.0), screen
                   batches=256)
 
 
+# tests/golden/scenes/textured.scene (textured spheres, a textured OBJ mesh
+# and floor, a sphere light, an infinite floor, sky.hdr) rendered from above.
+TEXTURED_RENDER = dict(w=48, h=32, pos=(0.0, 30.0, 70.0), dir=(0.0, -25.0, -70.0), screen_h=30.0, dist=40.0,
+                       ss=2, batches=256)
+
+
 def cornell_rays(scene):
     n, seed = CORNELL_RAYS
     lo, hi = (np.asarray(v, np.float32) for v in CORNELL_BOX)
