@@ -261,6 +261,11 @@ def roofline_from(acc, workload, node_width=4):
         roof["hbm_measured_frac"] = round(hbm / HBM_PEAK_GBS, 4)
     if valu:
         roof["valu_issue"] = valu
+        # what actually binds the kernel: the node and primitive bytes come from
+        # L2 and the MALL (measured HBM traffic well under the algorithmic
+        # bytes), and the VALU issue slots are mostly full
+        roof["binding"] = {"limit": "VALU issue", "issue_frac": valu["issue_frac"],
+                           "hbm_measured_frac": roof.get("hbm_measured_frac")}
     roof["kernels"] = table
     seg = max(1, acc["segments"])
     roof["nodes_per_segment"] = round((acc["node_visits"] + acc["drain_node_visits"]) / seg, 3)
