@@ -290,13 +290,17 @@ void Engine::server_loop() {
            hipMemset(pool_.misc, 0, (size_t)pool_cap_ * 16) == hipSuccess;
     }
   }
-  // Marks every live and queued request failed (the server stops serving).
+  // Marks every unfinished live request and every queued one failed (the
+  // server stops serving).  A live request whose samples are all done is the
+  // copier's (queued for its copy, or already completed and waiting in the
+  // window behind an unfinished one): its outcome is the copy's.
   auto fail_all = [&](int code, const std::string &why) {
     set_err(why);
     std::lock_guard<std::mutex> g(smu);
     srv_err_ = why;
     server_failed_ = true;
     auto fail_one = [&](const std::shared_ptr<Req> &r) {
+      if (r->run.done) return;
       r->state = 4;
       r->code = code;
       r->msg = why;
