@@ -226,7 +226,12 @@ int Engine::ensure_gstack(uint32_t need_entries, size_t threads) {
   drop_graphs();
   cap = std::max(cap, gcap_);
   threads = std::max(threads, gstack_threads_);
-  if (d_gstack_) hipFree(d_gstack_);
+  // kernels already on the stream (a served chunk in flight, when a trace
+  // call grows the stacks) still use the old stacks
+  if (d_gstack_) {
+    hipStreamSynchronize(stream_);
+    hipFree(d_gstack_);
+  }
   d_gstack_ = nullptr;
   OMR_HIP(hipMalloc(&d_gstack_, threads * cap * sizeof(uint2)));
   gcap_ = cap;
