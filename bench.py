@@ -220,12 +220,14 @@ def roofline_from(acc, workload, node_width=4):
         # name: (algorithmic bytes, summed event ms, launches, rocprof symbol of the counting variant)
         "k_extend": (B_SEG_TRACE * ext_seg + bn * acc["node_visits"] + B_PRIM * acc["prim_tests"],
                      acc["extend_ms"], acc["extend_launches"], f"k_extend<64, 16, {w}, true, true, true>"),
-        "k_shade": (B_SEG_SHADE * ext_seg, acc["shade_ms"], acc["extend_launches"], "k_shade<64, true, true>"),
+        # (single-job shading and drain kernels come in two forms, for scenes
+        # with and without an environment map: the last template argument)
+        "k_shade": (B_SEG_SHADE * ext_seg, acc["shade_ms"], acc["extend_launches"], "k_shade<64, true, true, *>"),
         "k_shadow": (B_SHADOW * acc["shadow_rays"], acc["shadow_ms"], acc["shadow_launches"],
                      f"k_shadow<64, 16, {w}, true, true, true>"),
         "k_drain": ((B_SEG_TRACE + B_SEG_SHADE) * acc["drain_segments"] + bn * acc["drain_node_visits"]
                     + B_PRIM * acc["drain_prim_tests"], acc["drain_ms"], acc["drain_launches"],
-                    f"k_drain<64, 16, {w}, true, true>"),
+                    f"k_drain<64, 16, {w}, true, true, *>"),
     }
     table = {}
     for name, (b, ms, n, sym) in kernels.items():

@@ -186,6 +186,7 @@ KernelArgs Engine::args() const {
   a.p.stack_short = tuning.stack_short;
   a.p.node_width = tuning.node_width;
   a.solo = solo_;
+  a.solo_env = solo_env_ ? 1u : 0u;
   return a;
 }
 
@@ -612,7 +613,9 @@ void Engine::plan_round(const std::vector<uint64_t> &first, const std::vector<ui
 
 int Engine::launch_placed_round(bool with_shadow, uint64_t base, uint32_t n0, uint32_t solo_job) {
   const uint32_t saved = solo_;
+  const bool saved_env = solo_env_;
   solo_ = solo_job == kNoSolo ? kNoSolo : 0u;  // a flag: the kernels read the slot from Ctrl
+  solo_env_ = solo_job == kNoSolo || ring_env_[solo_job] != 0;
   KernelArgs a = args();
   const LaunchShape sh = shape();
   hipError_t e = k::round_setup(d_ctrl_, n0, solo_job == kNoSolo ? 0u : solo_job, stream_);
@@ -624,6 +627,7 @@ int Engine::launch_placed_round(bool with_shadow, uint64_t base, uint32_t n0, ui
   if (!rc && (e = k::drain(a, sh, 0, profile, stream_)) != hipSuccess) rc = hipfail(e, "drain");
   if (!rc && profile && hipEventRecord(ev_d_, stream_) != hipSuccess) rc = hipfail(hipGetLastError(), "event");
   solo_ = saved;
+  solo_env_ = saved_env;
   return rc;
 }
 
@@ -700,7 +704,7 @@ int Engine::launch_steps(uint32_t steps, bool with_shadow, bool round) {
     if (e != hipSuccess) return hipfail(e, "bounce step launch");
     return OMR_OK;
   }
-  int gi = (with_shadow ? 1 : 0) + (a.solo != kNoSolo ? 2 : 0);
+  int gi = (with_shadow ? 1 : 0) + (a.solo != kNoSolo ? 2 : 0) + (a.solo != kNoSolo && !a.solo_env ? 4 : 0);
   if (!graph_[gi] || graph_steps_ != steps || graph_round_ != round) {
     if (graph_steps_ != steps || graph_round_ != round) drop_graphs();
     hipGraph_t g = nullptr;
@@ -776,8 +780,10 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     D.accum = j.d_accum;
     D.recs = j.d_recs;
     j.ring = (uint32_t)i;
+    ring_env_[j.ring] = scene((omr_scene_id)j.scene)->has_env ? 1u : 0u;
     total += j.n_samples;
   }
+  solo_env_ = jobs.size() != 1 || ring_env_[jobs[0].ring] != 0;
   Ctrl c;
   std::memset(&c, 0, sizeof(c));
   c.total_samples = total;

@@ -179,10 +179,13 @@ class Engine {
   size_t gstack_threads_ = 0;
 
   // graphs: [with shadow kernel + 2 x single-job kernels]
-  hipGraphExec_t graph_[4] = {nullptr, nullptr, nullptr, nullptr};
+  // bounce-step graphs by (shadow rays, single-job, single-job scene without environment map)
+  hipGraphExec_t graph_[8] = {};
   uint32_t graph_steps_ = 0;
   bool graph_round_ = false;       // the graphs hold drain-round steps (no regeneration, no refill)
   uint32_t solo_ = kNoSolo;        // the current launches are single-job (Ctrl::solo_job names it), or kNoSolo
+  bool solo_env_ = true;           // ... and that job's scene has an environment map
+  uint8_t ring_env_[kJobRing] = {};  // per job-ring slot: the job's scene has an environment map
 
   int ensure_pool();
   int ensure_gstack(uint32_t need_entries, size_t threads);

@@ -1149,8 +1149,12 @@ struct ShadowRay {
 // non-emissive Lambert one that samples the environment; its uniforms are the
 // same draws: one for the material, one for Russian roulette past rr_depth,
 // two for the Lambert direction come first).  Costs registers, so only the
-// shading kernel uses it.
-template <bool SPEC = false>
+// shading kernel uses it.  ENV = false: the scene has no environment map, so
+// the code for misses into it, its sampling and MIS is left out (it is the
+// peak of register pressure: the single-job kernels of such a scene keep
+// every register and the shading kernel runs five waves per SIMD instead of
+// four).
+template <bool SPEC = false, bool ENV = true>
 __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathState &st, float4 hr,
                                      const float *pdf_mem, bool &shadow, ShadowRay &sh, const JobDev &J,
                                      const SceneDev &S) {
@@ -1163,7 +1167,7 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
   const uint32_t sidx = st.sidx;
   float pdf_next = 0.0f;
   uint32_t depth = ps_depth(word), branch = ps_branch(word), npend = ps_pending(word);
-  const bool env_on = S.env != nullptr;
+  const bool env_on = ENV && S.env != nullptr;
   bool nee = env_on && (J.flags & 1u) && gld(S.env_marg + S.env_h) > 0ull;
   int32_t prim = __float_as_int(hr.y);
   uint32_t mis_next = 0;
@@ -1402,8 +1406,8 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
 // registers, instead of per lane and segment (two dependent loads ahead of
 // every other load of the segment).  TAG: the launch of a profiled step (the
 // same code under its own name, so a profiler can tell those launches apart).
-template <int WG, bool SOLO, bool TAG = false>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO ? 4 : 5))) void k_shade(KernelArgs A, int parity,
+template <int WG, bool SOLO, bool TAG = false, bool ENV = true>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO && ENV ? 4 : 5))) void k_shade(KernelArgs A, int parity,
                                                                                       int regen) {
   __shared__ uint32_t s_pre[kShards + 1];
   load_prefix(s_pre, A.ctrl->qcount[parity]);
@@ -1482,7 +1486,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO ? 4 : 5
       st.js = ms.w;
       st.pdf = 0.0f;
       if (SOLO) {
-        cont = shade_segment<true>(A, slot, st, hr, A.p.pdf, shadow, sh, Js, Ss);
+        cont = shade_segment<ENV, ENV>(A, slot, st, hr, A.p.pdf, shadow, sh, Js, Ss);
       } else {
         const JobDev &J = A.jobs[st.js];
         cont = shade_segment(A, slot, st, hr, A.p.pdf, shadow, sh, J, A.scenes[J.scene]);
@@ -1607,7 +1611,7 @@ __global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
 // render).
 // Four waves per SIMD (128 VGPRs) with 4-wide nodes; the 8-wide node visit
 // holds twice the sort keys and spilled 38-75 VGPRs there, so it runs three.
-template <int WG, int SHORT, int NW, bool STATS, bool SOLO>
+template <int WG, int SHORT, int NW, bool STATS, bool SOLO, bool ENV = true>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 3 : 4))) void k_drain(KernelArgs A, int parity) {
   __shared__ uint2 lds[SHORT * WG];
   __shared__ uint32_t s_pre[kShards + 1];
@@ -1710,7 +1714,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 3 
     // copy of them stays in registers across the loop
     st.o = tr.o;
     st.d = tr.d;
-    bool cont = shade_segment(A, slot, st, make_float4(tr.best.t, __int_as_float(tr.best.prim), tr.best.u, tr.best.v),
+    bool cont = shade_segment<false, ENV>(A, slot, st, make_float4(tr.best.t, __int_as_float(tr.best.prim), tr.best.u, tr.best.v),
                               nullptr, shadow, sh, J, S);
     if (shadow) {
       Hit hs;
@@ -2081,8 +2085,13 @@ hipError_t launch_shadow_wg(const KernelArgs &a, uint32_t grid, bool tag, hipStr
 template <int WG, int SHORT, int NW>
 hipError_t launch_drain_n(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
   if (WG == 64 && a.solo != kNoSolo) {  // single-job runs at the default group size
-    if (stats) hipLaunchKernelGGL((k_drain<WG, SHORT, NW, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
-    else hipLaunchKernelGGL((k_drain<WG, SHORT, NW, false, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+    if (a.solo_env) {
+      if (stats) hipLaunchKernelGGL((k_drain<WG, SHORT, NW, true, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+      else hipLaunchKernelGGL((k_drain<WG, SHORT, NW, false, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
+    } else {
+      if (stats) hipLaunchKernelGGL((k_drain<WG, SHORT, NW, true, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+      else hipLaunchKernelGGL((k_drain<WG, SHORT, NW, false, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+    }
     return hipGetLastError();
   }
   if (stats) hipLaunchKernelGGL((k_drain<WG, SHORT, NW, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
@@ -2182,9 +2191,16 @@ hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool reg
   const bool solo = a.solo != kNoSolo;
   switch (s.wg) {
     case 64:
-      if (solo && tag)
+      // the single-job form without an environment map runs five waves per
+      // SIMD, on the grid of the per-lane form
+      if (solo && a.solo_env && tag)
         hipLaunchKernelGGL((k_shade<64, true, true>), dim3(s.solo_shade_grid), dim3(64), 0, st, a, parity, r);
-      else if (solo) hipLaunchKernelGGL((k_shade<64, true>), dim3(s.solo_shade_grid), dim3(64), 0, st, a, parity, r);
+      else if (solo && a.solo_env)
+        hipLaunchKernelGGL((k_shade<64, true>), dim3(s.solo_shade_grid), dim3(64), 0, st, a, parity, r);
+      else if (solo && tag)
+        hipLaunchKernelGGL((k_shade<64, true, true, false>), dim3(s.shade_grid), dim3(64), 0, st, a, parity, r);
+      else if (solo)
+        hipLaunchKernelGGL((k_shade<64, true, false, false>), dim3(s.shade_grid), dim3(64), 0, st, a, parity, r);
       else hipLaunchKernelGGL((k_shade<64, false>), dim3(s.shade_grid), dim3(64), 0, st, a, parity, r);
       break;
     case 128:

@@ -255,6 +255,7 @@ void Engine::server_loop() {
   {
     std::lock_guard<std::mutex> g(mu);
     solo_ = kNoSolo;  // requests share the run
+    solo_env_ = true;
   }
   std::deque<std::shared_ptr<Req>> live;  // admitted, by logical ring index
   uint64_t lo = 0, hi = 0;                // logical ring window [lo, hi)
@@ -471,6 +472,7 @@ void Engine::server_loop() {
         j.n_samples = (uint32_t)samples_of(r->cam, r->params);
         j.logical = hi++;
         j.ring = (uint32_t)(j.logical % kJobRing);
+        ring_env_[j.ring] = r->sh->has_env ? 1u : 0u;
         const size_t px = (size_t)r->cam.width * r->cam.height;
         j.d_recs = nullptr;
         live.push_back(r);
