@@ -53,7 +53,6 @@ struct LaunchShape {
   uint32_t grid;        // workgroups of the persistent kernels
   uint32_t wg;          // threads per workgroup: 64, 128 or 256
   uint32_t shade_grid;       // workgroups of the shade kernel: five waves per SIMD
-  uint32_t solo_shade_grid;  // ... of its single-job form: four waves per SIMD
 };
 
 // Working memory of the allocation-free closest-hit entry point: a fixed
