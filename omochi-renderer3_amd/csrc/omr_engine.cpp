@@ -149,21 +149,26 @@ static uint64_t shard_appends(uint32_t P, uint64_t gw, uint64_t chunk) {
   uint64_t per_wave = chunk * ((P + chunk * gw - 1) / (chunk * gw));
   return (gw + kShards - 1) / kShards * per_wave;
 }
-static uint32_t sub_queue_cap(uint32_t P, uint64_t grid_waves, uint64_t shade_waves) {
-  return (uint32_t)std::min<uint64_t>(P, shard_appends(P, grid_waves, kRefillSlots) + shard_appends(P, shade_waves, 64));
+static uint32_t sub_queue_cap(uint32_t P, uint64_t grid_waves, uint64_t shade_waves, uint64_t shade6_waves) {
+  return (uint32_t)std::min<uint64_t>(
+      P, shard_appends(P, grid_waves, kRefillSlots) +
+             std::max(shard_appends(P, shade_waves, 64), shard_appends(P, shade6_waves, 64)));
 }
 // Workgroups of the persistent kernels and of shade for a pool of P paths:
-// shade runs five waves per SIMD (20 per CU), all resident at once.
-static void grids(uint32_t n_cu, const omr_tuning &t, uint32_t P, uint32_t &grid, uint32_t &shade_grid) {
+// shade runs five waves per SIMD (20 per CU), its single-job form for a
+// scene without an environment map six (24 per CU), all resident at once.
+static void grids(uint32_t n_cu, const omr_tuning &t, uint32_t P, uint32_t &grid, uint32_t &shade_grid,
+                  uint32_t &shade6_grid) {
   uint32_t need = std::max(1u, (P + t.wg_size - 1) / t.wg_size);
   grid = std::min(need, n_cu * t.blocks_per_cu);
   shade_grid = std::min(need, n_cu * (20u * 64u / t.wg_size));
+  shade6_grid = std::min(need, n_cu * (24u * 64u / t.wg_size));
 }
 
 LaunchShape Engine::shape() const {
   LaunchShape s;
   s.wg = tuning.wg_size;
-  grids((uint32_t)n_cu_, tuning, pool_cap_, s.grid, s.shade_grid);
+  grids((uint32_t)n_cu_, tuning, pool_cap_, s.grid, s.shade_grid, s.shade6_grid);
   return s;
 }
 
@@ -187,9 +192,10 @@ KernelArgs Engine::args() const {
 
 int Engine::ensure_pool() {
   uint32_t P = tuning.pool_paths;
-  uint32_t grid, sgrid;
-  grids((uint32_t)n_cu_, tuning, P, grid, sgrid);
-  uint32_t qcap = sub_queue_cap(P, (uint64_t)grid * tuning.wg_size / 64, (uint64_t)sgrid * tuning.wg_size / 64);
+  uint32_t grid, sgrid, s6grid;
+  grids((uint32_t)n_cu_, tuning, P, grid, sgrid, s6grid);
+  uint32_t qcap = sub_queue_cap(P, (uint64_t)grid * tuning.wg_size / 64, (uint64_t)sgrid * tuning.wg_size / 64,
+                                (uint64_t)s6grid * tuning.wg_size / 64);
   if (pool_cap_ == P && pool_.ray_o && pool_.qcap == qcap) return OMR_OK;
   drop_graphs();
   PoolPtrs &p = pool_;

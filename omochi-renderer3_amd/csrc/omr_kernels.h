@@ -53,6 +53,7 @@ struct LaunchShape {
   uint32_t grid;        // workgroups of the persistent kernels
   uint32_t wg;          // threads per workgroup: 64, 128 or 256
   uint32_t shade_grid;       // workgroups of the shade kernel: five waves per SIMD
+  uint32_t shade6_grid;  // ... of its single-job form for a scene without an environment map: six
 };
 
 // Working memory of the allocation-free closest-hit entry point: a fixed

@@ -1397,9 +1397,10 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
 // One shading step over the ray queue `parity`: shade each hit, then
 // regenerate finished slots (regen) and compact the continuing paths into the
 // other queue.
-// Five waves per SIMD (96 VGPRs, no spills) in every form: shade waits on
-// memory most of the time, and a fifth wave hides more of it than reading
-// further ahead with the registers of a fourth (c2: 5 % faster).  Each lane
+// Five waves per SIMD (96 VGPRs, no spills), six in the single-job form for a
+// scene without an environment map (78 VGPRs): shade waits on memory most of
+// the time, and another wave hides more of it than reading further ahead
+// with the registers (c2: 5 % faster, c1 4 %).  Each lane
 // reads the queue entry of its next item one item ahead, so that load is off
 // the front of the next item's chain of dependent loads.  SOLO: the run has
 // one job (A.solo), whose job and scene records are read once, into scalar
@@ -1407,7 +1408,7 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
 // every other load of the segment).  TAG: the launch of a profiled step (the
 // same code under its own name, so a profiler can tell those launches apart).
 template <int WG, bool SOLO, bool TAG = false, bool ENV = true>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5))) void k_shade(KernelArgs A, int parity,
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO && !ENV ? 6 : 5))) void k_shade(KernelArgs A, int parity,
                                                                                       int regen) {
   __shared__ uint32_t s_pre[kShards + 1];
   load_prefix(s_pre, A.ctrl->qcount[parity]);
@@ -1428,12 +1429,11 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5))) void k_
   }
   const float4 *hits = reinterpret_cast<const float4 *>(A.p.hit);
   // Read-ahead: while a lane shades item i, the queue entry of item
-  // i + stride is in flight, and in the single-job form for a scene without
-  // an environment map (registers to spare at five waves) also the path state
-  // of item i + stride and the entry of item i + 2 stride, so none of these
-  // loads sits on the chain of dependent loads of the next item.  The other
-  // forms read the path state when they reach the item.
-  constexpr bool AHEAD = SOLO && !ENV;
+  // i + stride is in flight.  (AHEAD: also the path state of item i + stride
+  // and the entry of item i + 2 stride, 20 more VGPRs; every form now spends
+  // its registers on a fifth or sixth wave instead, which hides more of the
+  // memory latency.)
+  constexpr bool AHEAD = false;
   auto entry_at = [&](uint32_t j) { return j < n ? A.p.queue[parity][flat_slot(s_pre, j, qcap)] : 0u; };
   struct State {
     float4 ro, rd, hr;
@@ -2196,9 +2196,9 @@ hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool reg
       else if (solo && a.solo_env)
         hipLaunchKernelGGL((k_shade<64, true>), dim3(s.shade_grid), dim3(64), 0, st, a, parity, r);
       else if (solo && tag)
-        hipLaunchKernelGGL((k_shade<64, true, true, false>), dim3(s.shade_grid), dim3(64), 0, st, a, parity, r);
+        hipLaunchKernelGGL((k_shade<64, true, true, false>), dim3(s.shade6_grid), dim3(64), 0, st, a, parity, r);
       else if (solo)
-        hipLaunchKernelGGL((k_shade<64, true, false, false>), dim3(s.shade_grid), dim3(64), 0, st, a, parity, r);
+        hipLaunchKernelGGL((k_shade<64, true, false, false>), dim3(s.shade6_grid), dim3(64), 0, st, a, parity, r);
       else hipLaunchKernelGGL((k_shade<64, false>), dim3(s.shade_grid), dim3(64), 0, st, a, parity, r);
       break;
     case 128:
