@@ -1429,16 +1429,14 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO && !ENV
   }
   const float4 *hits = reinterpret_cast<const float4 *>(A.p.hit);
   // Read-ahead: while a lane shades item i, the queue entry of item
-  // i + stride is in flight.  (AHEAD: also the path state of item i + stride
-  // and the entry of item i + 2 stride, 20 more VGPRs; every form now spends
-  // its registers on a fifth or sixth wave instead, which hides more of the
-  // memory latency.)
-  constexpr bool AHEAD = false;
+  // i + stride is in flight.  (Reading the next item's path state ahead as
+  // well costs 20 VGPRs; every form spends its registers on a fifth or sixth
+  // wave instead, which hides more of the memory latency.)
   auto entry_at = [&](uint32_t j) { return j < n ? A.p.queue[parity][flat_slot(s_pre, j, qcap)] : 0u; };
   struct State {
     float4 ro, rd, hr;
     uint4 ms, rng;
-  } nx{};
+  };
   auto state_of = [&](uint32_t j, uint32_t e, State &to) {
     if (j < n) {
       const uint32_t sl = q_slot(e);
@@ -1449,27 +1447,15 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO && !ENV
       to.rng = rngs[sl];
     }
   };
-  const uint32_t i0 = blockIdx.x * WG + threadIdx.x;
-  uint32_t next = entry_at(i0);  // entry of the lane's next item
-  uint32_t next2 = 0u;           // AHEAD: and of the item after it
-  if (AHEAD) {
-    state_of(i0, next, nx);
-    next2 = entry_at(i0 + stride);
-  }
+  uint32_t next = entry_at(blockIdx.x * WG + threadIdx.x);  // entry of the lane's next item
   for (uint32_t base = blockIdx.x * WG; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     bool valid = i < n;
     const uint32_t slot = q_slot(next);
     State cur{};
-    if (AHEAD) {
-      cur = nx;
-      next = next2;
-      state_of(i + stride, next, nx);
-      next2 = entry_at(i + 2u * stride);
-    } else {
-      next = entry_at(i + stride);
-      state_of(i, q_entry(slot, 0u), cur);
-    }
+    next = entry_at(i + stride);
+    state_of(i, q_entry(slot, 0u), cur);  // (in a block of its own: loading it in the shading
+                                          // block below spilled three VGPRs)
     bool cont = false, fin = false, shadow = false;
     PathState st;
     st.js = 0;
