@@ -248,9 +248,11 @@ int omr_trace_closest_device(omr_ctx *ctx, omr_scene_id scene, const float *d_ra
  * for callers that cannot allocate device memory -- code under HIP graph
  * capture, serving loops and harnesses that forbid hipMalloc.  Its working
  * memory (fetch heads and traversal overflow stacks) is one fixed 32 MiB
- * device array per GPU, held until the process ends and shared by every call
- * on that GPU: calls from one process take turns, calls from several
- * processes on one GPU race.  The traversal grid shrinks to what the array
+ * device array per GPU (each process has its own), held until the process
+ * ends and shared by every call on that GPU: calls take turns under a lock
+ * per GPU and each returns only when its work is done, so they never overlap.
+ * The array keeps the last call's contents; each call clears the
+ * part that must start at zero.  The traversal grid shrinks to what the array
  * holds; a scene whose stack bound leaves no room for one workgroup is
  * refused with OMR_ERR_INVALID.  Same results as omr_trace_closest_device. */
 int omr_trace_closest_device_noalloc(omr_ctx *ctx, omr_scene_id scene, const float *d_rays, size_t n,
