@@ -11,9 +11,13 @@
  * Pinning of each function (what checks it besides the GPU comparison):
  *   orc_rng_*            reference Random (src/tools/Random.h:9-29) compiled
  *                        from the reference sources -> tests/golden/rng_kat.json
- *   orc_path_seed        project-defined seeding (no reference counterpart;
- *                        determinism + distinctness tests) -- parity unpinned
- *                        against the reference, which seeds per row.
+ *   orc_path_seed        project-defined per-path seeding (the reference seeds
+ *                        one stream per row, PathTracer.cpp:151, so no
+ *                        reference vector exists); pinned by what the hash
+ *                        must satisfy: every argument changes the seed, and
+ *                        the streams of neighbouring pixels, sub-pixels,
+ *                        samples, branches and the lens stream are distinct
+ *                        and decorrelated (tests/test_oracle_math.py).
  *   orc_sincos_turns     closed form: sin^2+cos^2, libm comparison < 2e-7
  *   orc_acos             closed form: libm comparison < 4e-7
  *   orc_intersect_*      compiled reference Polygon/Sphere (FP64) golden hits
@@ -26,7 +30,10 @@
  *   orc_env_texel        compiled reference IBL::Sample texel golden vectors
  *   orc_env_cdf/sample   chi^2 histogram + pdf integrates to 1 (closed form)
  *   orc_render           white furnace (closed form), compiled reference
- *                        PathTracer statistical comparison (z-score)
+ *                        PathTracer statistical comparison (z-score) on the
+ *                        config-0 box, the reference's CornellBoxScene (with
+ *                        and without next-event estimation), a Lambert sphere
+ *                        under the reference's HDR and textured.scene
  */
 #include "omr_oracle.h"
 
