@@ -806,22 +806,23 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
   OMR_HIP(hipMemcpyAsync(d_jobs_, jd.data(), jd.size() * sizeof(JobDev), hipMemcpyHostToDevice, stream_));
   OMR_HIP(hipMemcpyAsync(d_ctrl_, &c, sizeof(Ctrl), hipMemcpyHostToDevice, stream_));
   OMR_HIP(hipMemsetAsync(d_done_, 0, kJobRing * kShards * sizeof(unsigned long long), stream_));
+  // drain rounds placed by the host: every round fills the pool itself
+  const bool placed = tuning.drain_rounds && fresh;
   if (fresh) {
-    // slots past n0 are read only by the bulk refill (with regeneration they
-    // never enter a queue)
-    if (n0 < pool_cap_ && (tuning.refill_every != 0 || tuning.drain_rounds))
+    // slots past n0 are read only by the bulk refill (with regeneration
+    // they never enter a queue; placed rounds fill the slots they use)
+    if (n0 < pool_cap_ && !tuning.drain_rounds && tuning.refill_every != 0)
       OMR_HIP(hipMemsetAsync(pool_.misc + 4ull * n0, 0, (size_t)(pool_cap_ - n0) * 16, stream_));
   } else {
     OMR_HIP(hipMemsetAsync(pool_.misc, 0, (size_t)pool_cap_ * 16, stream_));
   }
   for (auto &j : jobs) {
-    OMR_HIP(hipMemsetAsync(j.d_accum, 0, (size_t)j.cam.width * j.cam.height * 3 * 8, stream_));
+    // with per-sample records the gather writes every pixel's sums
     if (j.d_recs) OMR_HIP(hipMemsetAsync(j.d_recs, 0, (size_t)j.n_samples * 24, stream_));
+    else OMR_HIP(hipMemsetAsync(j.d_accum, 0, (size_t)j.cam.width * j.cam.height * 3 * 8, stream_));
   }
   OMR_HIP(hipEventRecord(ev_a_, stream_));
   KernelArgs a = args();
-  // drain rounds placed by the host: every round fills the pool itself
-  const bool placed = tuning.drain_rounds && fresh;
   hipError_t e = hipSuccess;
   if (!placed) {
     e = fresh ? k::fill_fresh(a, 0, n0, stream_) : k::refill(a, sh, 0, stream_);

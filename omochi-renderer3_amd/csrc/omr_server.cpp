@@ -480,8 +480,10 @@ void Engine::server_loop() {
         j.d_accum = reinterpret_cast<unsigned long long *>(blk);
         j.d_out = reinterpret_cast<float *>(blk + align256(px * 3 * 8));
         if (r->place.records) j.d_recs = reinterpret_cast<unsigned long long *>(blk + r->place.image);
-        okd = okd && hipMemsetAsync(j.d_accum, 0, px * 3 * 8, stream_) == hipSuccess &&
-              (!j.d_recs || hipMemsetAsync(j.d_recs, 0, (size_t)j.n_samples * 24, stream_) == hipSuccess) &&
+        // with per-sample records the gather writes every pixel's sums
+        okd = okd &&
+              (j.d_recs ? hipMemsetAsync(j.d_recs, 0, (size_t)j.n_samples * 24, stream_)
+                        : hipMemsetAsync(j.d_accum, 0, px * 3 * 8, stream_)) == hipSuccess &&
               k::clear_done(d_done_, j.ring, stream_) == hipSuccess;
         JobDev &D = h_jobs[j.ring];
         std::memset(&D, 0, sizeof(D));
