@@ -425,6 +425,7 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
   D.stack_bound = S.bvh.stack_bound;
   D.n_nodes8 = (uint32_t)S.bvh.nodes8.size();
   D.stack_bound8 = S.bvh.stack_bound8;
+  D.o_far = S.bvh.pad * 2097152.0f;  // pad / 8 = o_far * 2^-24
   if (d->env_rgb) {
     uint32_t w = d->env_width, h = d->env_height;
     void *p_env, *p_w, *p_max, *p_q, *p_row, *p_marg, *p_rg, *p_mg, *p_rs, *p_ms;

@@ -324,12 +324,17 @@ __device__ inline void cswap(float &ka, int32_t &ca, float &kb, int32_t &cb) {
 // to the smaller primitive id so the result does not depend on the order.
 // The traversal is resumable: init() sets up a ray, step(k) performs at most
 // k node or leaf visits and reports whether the ray is finished.  traverse()
-// runs one ray to the end.
-template <int WG, int SHORT, bool ANY, bool STATS, int NW>
+// runs one ray to the end.  FMA: slab distances as one fused operation per
+// plane (see init); without it as (bound - o) * inverse, for kernels whose
+// registers cannot hold the three more values per ray that the fused form
+// keeps.
+template <int WG, int SHORT, bool ANY, bool STATS, int NW, bool FMA = true>
 struct Trav {
   const float4 *nodes, *prims;
   f3 o, d;
   float ix, iy, iz;
+  float oix, oiy, oiz;  // o * inverse direction: a slab distance is fma(bound, ix, -oix)
+  float slk;            // absolute slack of a box's far distance (see init)
   uint32_t nx, ny, nz;  // 1 when the ray runs towards -axis: the near slab is the upper bound
   int32_t cur;   // current entry: inner node (>= 0), leaf code or kNoEntry
   int32_t pend;  // postponed leaf or kNoEntry
@@ -357,13 +362,42 @@ struct Trav {
   __device__ inline void init(const SceneDev &S, f3 o_, f3 d_, float tmax, Stack<WG, SHORT> &stk) {
     o = o_;
     d = d_;
-    // QBVH.cpp:56-60: zero direction component -> INF
-    ix = (d.x == 0.0f) ? kInf : 1.0f / d.x;
-    iy = (d.y == 0.0f) ? kInf : 1.0f / d.y;
-    iz = (d.z == 0.0f) ? kInf : 1.0f / d.z;
     nx = !(d.x >= 0.0f) ? 1u : 0u;
     ny = !(d.y >= 0.0f) ? 1u : 0u;
     nz = !(d.z >= 0.0f) ? 1u : 0u;
+    if (!FMA) {
+      // QBVH.cpp:56-60: zero direction component -> INF
+      ix = (d.x == 0.0f) ? kInf : 1.0f / d.x;
+      iy = (d.y == 0.0f) ? kInf : 1.0f / d.y;
+      iz = (d.z == 0.0f) ? kInf : 1.0f / d.z;
+      oix = oiy = oiz = slk = 0.0f;
+    } else {
+    // Slab distances are fma(bound, ix, -o.x * ix): one operation where
+    // (bound - o.x) * ix takes two, and one more rounding, that of o.x * ix:
+    // the distances of an origin moved by at most 2^-24 |o.x| along x.  The
+    // builder pads every child box by 2^-17 of the scene's largest
+    // coordinate magnitude, which covers that move eight times over while
+    // every |o| <= o_far (16 times that magnitude): then a box the ray enters
+    // is never culled and the hit does not change (the tie rule decides it,
+    // whatever boxes are visited).  A farther origin adds the absolute slack
+    // slk = 2^-21 max |o ix| to the far distances and to best.t where keys
+    // are compared with it.  QBVH.cpp:56-60 takes INF for a zero direction
+    // component; here a component below 2^-64 takes +-2^100 with the sign of
+    // its slab choice: (bound - o.x) * 2^100 with one rounding, o.x * 2^100
+    // exact, and no product overflows.
+    const bool tx = fabsf(d.x) < 0x1p-64f, ty = fabsf(d.y) < 0x1p-64f, tz = fabsf(d.z) < 0x1p-64f;
+    ix = tx ? (nx ? -0x1p100f : 0x1p100f) : 1.0f / d.x;
+    iy = ty ? (ny ? -0x1p100f : 0x1p100f) : 1.0f / d.y;
+    iz = tz ? (nz ? -0x1p100f : 0x1p100f) : 1.0f / d.z;
+    oix = o.x * ix;
+    oiy = o.y * iy;
+    oiz = o.z * iz;
+    const float om = __builtin_fmaxf(__builtin_fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z));
+    slk = om <= S.o_far ? 0.0f
+                        : __builtin_fmaxf(__builtin_fmaxf(tx ? 0.0f : fabsf(oix), ty ? 0.0f : fabsf(oiy)),
+                                          tz ? 0.0f : fabsf(oiz)) *
+                              0x1p-21f;
+    }
     nodes = NW == 8 ? reinterpret_cast<const float4 *>(S.nodes8) : reinterpret_cast<const float4 *>(S.nodes);
     prims = reinterpret_cast<const float4 *>(S.prims);
     planes = S.planes;
@@ -394,15 +428,18 @@ struct Trav {
 constexpr uint32_t kLeafBatch = 32;
 constexpr int32_t kNoEntry = -1;
 
-template <int WG, int SHORT, bool ANY, bool STATS, int NW>
-__device__ inline bool Trav<WG, SHORT, ANY, STATS, NW>::step(Stack<WG, SHORT> &stk, int budget, uint32_t &nvis,
+template <int WG, int SHORT, bool ANY, bool STATS, int NW, bool FMA>
+__device__ inline bool Trav<WG, SHORT, ANY, STATS, NW, FMA>::step(Stack<WG, SHORT> &stk, int budget, uint32_t &nvis,
                                                           uint32_t &ntest) {
   const float ox = o.x, oy = o.y, oz = o.z;
-  // next stack entry that can still hold a closer hit, or kNoEntry
+  // next stack entry that can still hold a closer hit, or kNoEntry (an
+  // entry's key is its near distance; in the fused form best.t gets the
+  // slack of a far origin)
   auto pop_next = [&]() -> int32_t {
+    const float lim = FMA ? best.t + slk : best.t;
     while (stk.sp > 0) {
       uint2 e = stk.pop();
-      if (__uint_as_float(e.y) > best.t) continue;
+      if (__uint_as_float(e.y) > lim) continue;
       return (int32_t)e.x;
     }
     return kNoEntry;
@@ -447,19 +484,23 @@ __device__ inline bool Trav<WG, SHORT, ANY, STATS, NW>::step(Stack<WG, SHORT> &s
 #pragma unroll
           for (int k = 0; k < 4; k++) {
             float tmin = 0.0f, tmax = best.t;
-            float n = (near_x[k] - ox) * ix;
-            float f = (far_x[k] - ox) * ix;
+            auto dist = [&](float b, float i, float o_, float oi) {
+              return FMA ? __builtin_fmaf(b, i, -oi) : (b - o_) * i;
+            };
+            float n = dist(near_x[k], ix, ox, oix);
+            float f = dist(far_x[k], ix, ox, oix);
             tmin = __builtin_fmaxf(n, tmin);
             tmax = __builtin_fminf(f, tmax);
-            n = (near_y[k] - oy) * iy;
-            f = (far_y[k] - oy) * iy;
+            n = dist(near_y[k], iy, oy, oiy);
+            f = dist(far_y[k], iy, oy, oiy);
             tmin = __builtin_fmaxf(n, tmin);
             tmax = __builtin_fminf(f, tmax);
-            n = (near_z[k] - oz) * iz;
-            f = (far_z[k] - oz) * iz;
+            n = dist(near_z[k], iz, oz, oiz);
+            f = dist(far_z[k], iz, oz, oiz);
             tmin = __builtin_fmaxf(n, tmin);
             tmax = __builtin_fminf(f, tmax);
-            bool hit = (c4[k] != -1) && (tmin <= tmax * kSlabInflate);
+            const float lim = FMA ? __builtin_fmaf(tmax, kSlabInflate, slk) : tmax * kSlabInflate;
+            bool hit = (c4[k] != -1) && (tmin <= lim);
             key[4 * h + k] = hit ? tmin : __builtin_huge_valf();
             code[4 * h + k] = c4[k];
           }
@@ -491,7 +532,7 @@ __device__ inline bool Trav<WG, SHORT, ANY, STATS, NW>::step(Stack<WG, SHORT> &s
 #pragma unroll
         for (int k = NW - 1; k >= 1; k--)
           if (key[k] != kmiss) stk.push(make_uint2((uint32_t)code[k], __float_as_uint(key[k])));
-        cur = (key[0] != kmiss && !(key[0] > best.t)) ? code[0] : pop_next();
+        cur = (key[0] != kmiss && !(key[0] > (FMA ? best.t + slk : best.t))) ? code[0] : pop_next();
         postpone();
       }
     } else if (pend != kNoEntry) {
@@ -544,10 +585,10 @@ __device__ inline bool Trav<WG, SHORT, ANY, STATS, NW>::step(Stack<WG, SHORT> &s
   return false;
 }
 
-template <int WG, int SHORT, bool ANY, bool STATS, int NW>
+template <int WG, int SHORT, bool ANY, bool STATS, int NW, bool FMA = true>
 __device__ inline void traverse(const SceneDev &S, f3 o, f3 d, Hit &best, Stack<WG, SHORT> &stk, uint32_t &nvis,
                                 uint32_t &ntest) {
-  Trav<WG, SHORT, ANY, STATS, NW> tr;
+  Trav<WG, SHORT, ANY, STATS, NW, FMA> tr;
   tr.init(S, o, d, best.t, stk);
   tr.step(stk, 0x7FFFFFFF, nvis, ntest);
   best = tr.best;
@@ -1057,8 +1098,12 @@ __global__ __launch_bounds__(256) void k_fill_fresh(KernelArgs A, unsigned long 
   }
 }
 
+// Five waves per SIMD with 4-wide nodes and 64-lane groups (96 VGPRs; the
+// fused slab test's three more values per ray would otherwise take the
+// kernel to 98-102 and four waves, which measured 5 % slower on c2).
 template <int WG, int SHORT, int NW, bool STATS, bool DF, bool SOLO>
-__global__ __launch_bounds__(WG) void k_extend(KernelArgs A, int parity) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NW == 4 && WG == 64 ? 5 : 4))) void k_extend(
+    KernelArgs A, int parity) {
   __shared__ uint2 lds[SHORT * WG];
   __shared__ uint32_t s_pre[kShards + 1];
   const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
@@ -1524,8 +1569,10 @@ __device__ inline void add_shadow(const KernelArgs &A, uint32_t r, const JobDev 
   deposit(J, __float_as_uint(c.w), fix(c.x), fix(c.y), fix(c.z));
 }
 
+// Five waves per SIMD with 4-wide nodes and 64-lane groups, as k_extend.
 template <int WG, int SHORT, int NW, bool DF, bool SOLO, bool TAG = false>
-__global__ __launch_bounds__(WG) void k_shadow(KernelArgs A) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NW == 4 && WG == 64 ? 5 : 4))) void k_shadow(
+    KernelArgs A) {
   __shared__ uint2 lds[SHORT * WG];
   __shared__ uint32_t s_pre[kShards + 1];
   const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
@@ -1626,7 +1673,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 3 
   // wave shades once half of its paths are ready or none is tracing
   bool tracing = false, ready = false;
   const int budget = A.p.df_budget > 0 ? (int)A.p.df_budget : 8;
-  Trav<WG, SHORT, false, STATS, NW> tr;
+  // fused slab form only where its registers fit (single job, no sky)
+  constexpr bool FMA = SOLO && !ENV;
+  Trav<WG, SHORT, false, STATS, NW, FMA> tr;
   uint32_t slot = 0;
   PathState st;
   st.js = 0;
@@ -1708,7 +1757,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 3 
       hs.u = hs.v = 0.0f;
       hs.prim = -1;
       uint32_t snv = 0, snt = 0;
-      traverse<WG, SHORT, true, false, NW>(S, sh.o, sh.d, hs, stk, snv, snt);
+      traverse<WG, SHORT, true, false, NW, FMA>(S, sh.o, sh.d, hs, stk, snv, snt);
       shadows++;
       if (hs.prim < 0) deposit(J, st.sidx, fix(sh.c.x), fix(sh.c.y), fix(sh.c.z));
     }

@@ -115,6 +115,9 @@ struct SceneDev {
   uint32_t stack_bound;        // max traversal stack entries (4-wide nodes)
   uint32_t n_nodes8;
   uint32_t stack_bound8;       // max traversal stack entries (8-wide nodes)
+  // largest ray-origin coordinate magnitude whose rounding in a fused slab
+  // distance (2^-24 of it) the child boxes' padding covers eight times over
+  float o_far;
 };
 
 // One render job in the device job ring.

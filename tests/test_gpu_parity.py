@@ -32,6 +32,66 @@ def test_trace_bit_exact(ctx, kind):
     ctx.release(sid)
 
 
+def test_trace_far_origins_and_axis_directions(ctx):
+    """The slab test's edge cases: origins 100 times the scene's extent away
+    (beyond what the box padding covers for the fused slab distances, so the
+    far-origin slack is used; much farther, binary32 hit distances tie
+    within their rounding and no box test reproduces the brute force's
+    order, the reference's included), directions with zero,
+    denormal and tiny components (the +-2^100 inverse), and origins on the
+    bounding planes; ids and distances equal the CPU restatement's brute
+    force (every primitive tested, no boxes)."""
+    import omr
+    import pyoracle
+
+    scene = omr.procedural(1, width=64, height=64)
+    sid = ctx.upload(scene)
+    rng = np.random.default_rng(23)
+    lo, hi = helpers.scene_bounds(scene)
+    c, e = (lo + hi) / 2, (hi - lo) / 2
+    n = 5_000
+    ti = rng.integers(0, scene.tri.shape[0], n)
+    a, b = rng.random(n), rng.random(n)
+    flip = a + b > 1
+    a[flip], b[flip] = 1 - a[flip], 1 - b[flip]
+    tgt = scene.tri[ti, 0:3] + scene.tri[ti, 3:6] * a[:, None] + scene.tri[ti, 6:9] * b[:, None]
+    parts_o, parts_d = [], []
+    for far in (30.0, 100.0):
+        o = (c + e.max() * far * helpers.unit_dirs(rng, n)).astype(np.float32)
+        d = tgt - o
+        parts_o.append(o)
+        parts_d.append((d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32))
+    # aimed rays with one component forced to 0, -0, a denormal or 2^-70
+    # (the origin's coordinate on that axis is the target's)
+    o = (c + e * 1.2 * helpers.unit_dirs(rng, n)).astype(np.float32)
+    d = (tgt - o).astype(np.float32)
+    ax = rng.integers(0, 3, n)
+    small = np.array([0.0, -0.0, 1e-40, -1e-40, 2.0 ** -70, -(2.0 ** -70)], np.float32)
+    d[np.arange(n), ax] = small[rng.integers(0, small.size, n)]
+    o[np.arange(n), ax] = tgt[np.arange(n), ax]
+    parts_o.append(o)
+    parts_d.append((d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32))
+    # axis-aligned rays from inside the bounds and from the bounding planes
+    o = (c + e * (2 * rng.random((n, 3)) - 1)).astype(np.float32)
+    d = np.zeros((n, 3), np.float32)
+    d[np.arange(n), rng.integers(0, 3, n)] = rng.choice(np.array([1.0, -1.0], np.float32), n)
+    plane = rng.integers(0, 3, n)
+    on = rng.random(n) < 0.5
+    o[on, plane[on]] = np.where(rng.random(on.sum()) < 0.5, lo[plane[on]], hi[plane[on]])
+    parts_o.append(o)
+    parts_d.append(d)
+    rays = helpers.make_rays(np.concatenate(parts_o), np.concatenate(parts_d))
+    print(f"trace edge cases n={rays.shape[0]}", flush=True)
+    gp, gt = ctx.trace(sid, rays)
+    cp, ct = _oracle(scene, ctx, sid).trace(rays, pyoracle.BRUTE)
+    bad = np.nonzero(gp != cp)[0]
+    assert bad.size == 0, f"{bad.size} hit-id mismatches, by part {np.bincount(bad // n, minlength=4)}"
+    hit = cp >= 0
+    assert np.array_equal(gt[hit].view(np.uint32), ct[hit].view(np.uint32))
+    assert hit[: 2 * n].mean() > 0.5 and hit[2 * n: 3 * n].mean() > 0.3
+    ctx.release(sid)
+
+
 def test_env_tables_bit_exact(ctx):
     """The device-scanned importance-sampling tables equal the sequential
     CPU tables exactly (integer prefix sums)."""
