@@ -188,6 +188,11 @@ typedef struct {
   uint32_t solo_round_min;    /* drain rounds: a job with at least this many samples left ends in
                                  a round of its own, run by the single-job kernels, instead of
                                  sharing one with the next job (0 = half the pool) */
+  uint32_t lanes;             /* drain rounds: 1 or 2 path pools of pool_paths slots each; with 2,
+                                 the consecutive rounds of a single job alternate between them on
+                                 two streams, so one round's last, longest paths run beside the
+                                 next round's first steps (twice the pool memory; batches keep
+                                 one lane) */
 } omr_tuning;
 
 typedef struct {

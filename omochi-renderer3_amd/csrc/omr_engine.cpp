@@ -42,6 +42,7 @@ Engine::Engine(int device) : device_(device) {
   tuning.drain_rounds = 1;
   tuning.round_steps = 2;
   tuning.solo_round_min = 1u << 22;  // 4M samples (swept: half a pool, 16M, 4M, 1M)
+  tuning.lanes = 2;  // a job's next round starts beside the last paths of the one before
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
     set_err("no HIP device " + std::to_string(device));
@@ -63,6 +64,10 @@ Engine::Engine(int device) : device_(device) {
   if (hipMemset(d_scenes_, 0, kMaxScenes * sizeof(SceneDev)) != hipSuccess) { set_err("scene table"); return; }
   if (hipMalloc(&d_ctrl_, sizeof(Ctrl)) != hipSuccess) { set_err("ctrl"); return; }
   if (hipHostMalloc(&h_ctrl_, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess) { set_err("ctrl pinned"); return; }
+  if (hipStreamCreateWithFlags(&alt_.stream, hipStreamNonBlocking) != hipSuccess) { set_err("stream"); return; }
+  if (hipMalloc(&alt_.d_ctrl, sizeof(Ctrl)) != hipSuccess) { set_err("ctrl"); return; }
+  if (hipHostMalloc(&alt_.h_ctrl, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess) { set_err("ctrl pinned"); return; }
+  std::memset(alt_.h_ctrl, 0, sizeof(Ctrl));
   if (hipMalloc(&d_jobs_, kJobRing * sizeof(JobDev)) != hipSuccess) { set_err("jobs"); return; }
   if (hipMalloc(&d_done_, kJobRing * kShards * sizeof(unsigned long long)) != hipSuccess) { set_err("done"); return; }
   if (hipHostMalloc(&h_done_, kJobRing * kShards * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
@@ -86,14 +91,19 @@ Engine::~Engine() {
   if (!init_ok_) return;
   hipSetDevice(device_);
   hipStreamSynchronize(stream_);
-  drop_graphs();
+  if (alt_.stream) hipStreamSynchronize(alt_.stream);
   for (auto &s : scenes_)
     if (s) free_scene_mem(*s);
-  PoolPtrs &p = pool_;
-  void *bufs[] = {p.ray_o, p.ray_d, p.misc, p.rng, p.hit, p.pdf, p.pend, p.queue[0], p.queue[1], p.shadow};
-  for (void *b : bufs)
-    if (b) hipFree(b);
-  if (d_gstack_) hipFree(d_gstack_);
+  for (int l = 0; l < 2; l++) {
+    drop_graphs();
+    free_lane_pool();
+    if (d_gstack_) hipFree(d_gstack_);
+    d_gstack_ = nullptr;
+    swap_lane();
+  }
+  hipFree(alt_.d_ctrl);
+  hipHostFree(alt_.h_ctrl);
+  if (alt_.stream) hipStreamDestroy(alt_.stream);
   hipFree(d_scenes_);
   hipFree(d_ctrl_);
   hipFree(d_jobs_);
@@ -129,10 +139,23 @@ int Engine::set_tuning(const omr_tuning &t) {
   if ((uint64_t)t.record_mb << 20 > hbm_) return fail(OMR_ERR_INVALID, "record_mb exceeds the device memory");
   if (t.drain_rounds > 1) return fail(OMR_ERR_INVALID, "drain_rounds must be 0 or 1");
   if (t.round_steps > 16 || (t.round_steps & 1u)) return fail(OMR_ERR_INVALID, "round_steps must be even, 0..16");
+  if (t.lanes != 1 && t.lanes != 2) return fail(OMR_ERR_INVALID, "lanes must be 1 or 2");
   std::lock_guard<std::mutex> g(mu);
   hipSetDevice(device_);
   hipStreamSynchronize(stream_);
   drop_graphs();
+  swap_lane();
+  hipStreamSynchronize(stream_);
+  drop_graphs();
+  if (t.lanes == 1) {
+    // the second pool is not kept when runs no longer use it
+    free_lane_pool();
+    if (d_gstack_) hipFree(d_gstack_);
+    d_gstack_ = nullptr;
+    gcap_ = 0;
+    gstack_threads_ = 0;
+  }
+  swap_lane();
   tuning = t;
   tuning.steps_per_sync = (t.steps_per_sync + 1u) & ~1u;
   return OMR_OK;
@@ -190,6 +213,30 @@ KernelArgs Engine::args() const {
   return a;
 }
 
+void Engine::swap_lane() {
+  std::swap(pool_, alt_.pool);
+  std::swap(pool_cap_, alt_.pool_cap);
+  std::swap(d_ctrl_, alt_.d_ctrl);
+  std::swap(h_ctrl_, alt_.h_ctrl);
+  std::swap(d_gstack_, alt_.d_gstack);
+  std::swap(gcap_, alt_.gcap);
+  std::swap(gstack_threads_, alt_.gstack_threads);
+  for (int i = 0; i < 8; i++) std::swap(graph_[i], alt_.graph[i]);
+  std::swap(graph_steps_, alt_.graph_steps);
+  std::swap(graph_round_, alt_.graph_round);
+  std::swap(stream_, alt_.stream);
+  on_alt_ = !on_alt_;
+}
+
+void Engine::free_lane_pool() {
+  PoolPtrs &p = pool_;
+  void *bufs[] = {p.ray_o, p.ray_d, p.misc, p.rng, p.hit, p.pdf, p.pend, p.queue[0], p.queue[1], p.shadow};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  p = PoolPtrs();
+  pool_cap_ = 0;
+}
+
 int Engine::ensure_pool() {
   uint32_t P = tuning.pool_paths;
   uint32_t grid, sgrid, s6grid;
@@ -198,12 +245,8 @@ int Engine::ensure_pool() {
                                 (uint64_t)s6grid * tuning.wg_size / 64);
   if (pool_cap_ == P && pool_.ray_o && pool_.qcap == qcap) return OMR_OK;
   drop_graphs();
+  free_lane_pool();
   PoolPtrs &p = pool_;
-  void *bufs[] = {p.ray_o, p.ray_d, p.misc, p.rng, p.hit, p.pdf, p.pend, p.queue[0], p.queue[1], p.shadow};
-  for (void *b : bufs)
-    if (b) hipFree(b);
-  p = PoolPtrs();
-  pool_cap_ = 0;
   OMR_HIP(hipMalloc(&p.ray_o, (size_t)P * 16));
   OMR_HIP(hipMalloc(&p.ray_d, (size_t)P * 16));
   OMR_HIP(hipMalloc(&p.misc, (size_t)P * 16));
@@ -261,11 +304,13 @@ uint64_t Engine::scene_bytes() {
 }
 
 uint64_t Engine::pool_bytes() const {
-  const uint64_t P = pool_cap_;
-  // ray_o, ray_d, misc, rng, hit (16 B each), pdf (4 B), pending branches (144 B)
-  const uint64_t slots = P * (5 * 16 + 4 + 3 * 48);
-  const uint64_t queues = (uint64_t)kShards * pool_.qcap * (4 + 4 + 48);
-  return slots + queues + (uint64_t)gstack_threads_ * gcap_ * sizeof(uint2);
+  // ray_o, ray_d, misc, rng, hit (16 B each), pdf (4 B), pending branches
+  // (144 B) per slot; two queues and the shadow records per queue entry
+  auto lane = [](uint64_t P, uint64_t qcap, uint64_t threads, uint64_t gcap) {
+    return P * (5 * 16 + 4 + 3 * 48) + (uint64_t)kShards * qcap * (4 + 4 + 48) + threads * gcap * sizeof(uint2);
+  };
+  return lane(pool_cap_, pool_.qcap, gstack_threads_, gcap_) +
+         lane(alt_.pool_cap, alt_.pool.qcap, alt_.gstack_threads, alt_.gcap);
 }
 
 int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
@@ -730,8 +775,24 @@ static uint64_t job_samples(const omr_camera &c, const omr_render_params &p) {
   return (uint64_t)c.width * c.height * p.supersamples * p.supersamples * (uint64_t)(p.spp_end - p.spp_begin);
 }
 
+// Adds the traced-ray counters of a second lane's control block to `c`.
+static void add_counters(Ctrl &c, const Ctrl &o) {
+  Sharded<unsigned long long> Ctrl::*f[] = {&Ctrl::segments,       &Ctrl::shadow_rays,    &Ctrl::node_visits,
+                                            &Ctrl::prim_tests,     &Ctrl::drain_segments, &Ctrl::drain_paths,
+                                            &Ctrl::drain_node_visits, &Ctrl::drain_prim_tests};
+  for (auto m : f)
+    for (uint32_t k = 0; k < kShards; k++) (c.*m)[k] += (o.*m)[k];
+}
+
 int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
   auto t0 = std::chrono::steady_clock::now();
+  // whatever the way out, lane 0 is current afterwards
+  struct LaneBack {
+    Engine *e;
+    ~LaneBack() {
+      if (e->on_alt_) e->swap_lane();
+    }
+  } lane_back{this};
   if (jobs.size() > kJobRing) return fail(OMR_ERR_INVALID, "too many jobs in one batch (max 1024)");
   hipSetDevice(device_);
   int rc = ensure_pool();
@@ -822,6 +883,24 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     if (j.d_recs) OMR_HIP(hipMemsetAsync(j.d_recs, 0, (size_t)j.n_samples * 24, stream_));
     else OMR_HIP(hipMemsetAsync(j.d_accum, 0, (size_t)j.cam.width * j.cam.height * 3 * 8, stream_));
   }
+  // two lanes: placed rounds alternate between the pools on two streams, for
+  // a single job of more than one round (c2: +2-5 %; a batch's rounds side by
+  // side measured 4 % slower on c3, rounds of different scenes sharing the
+  // caches; not in profiled runs, which time each launch on one stream)
+  const bool dual = placed && tuning.lanes == 2 && !profile && total > pool_cap_ && jobs.size() == 1;
+  if (dual) {
+    // lane 1: its pool, stacks and control block (the same start as lane
+    // 0's, copied from pinned memory on lane 0's stream so the host does not
+    // wait), behind lane 0's set-up
+    swap_lane();
+    if ((rc = ensure_pool()) || (rc = ensure_gstack(need_stack, (size_t)sh.grid * sh.wg))) return rc;
+    swap_lane();
+    if (pool_.qcap != alt_.pool.qcap) return fail(OMR_ERR_STATE, "the two lanes' pools differ");
+    std::memcpy(alt_.h_ctrl, &c, sizeof(Ctrl));
+    OMR_HIP(hipMemcpyAsync(alt_.d_ctrl, alt_.h_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, stream_));
+    OMR_HIP(hipEventRecord(ev_c_, stream_));
+    OMR_HIP(hipStreamWaitEvent(alt_.stream, ev_c_, 0));
+  }
   OMR_HIP(hipEventRecord(ev_a_, stream_));
   KernelArgs a = args();
   hipError_t e = hipSuccess;
@@ -855,6 +934,7 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     // order; a round inside one job runs the single-job kernels
     for (uint64_t r = 0, base = 0; placed ? base < total : r < rounds; r++) {
       if (placed) {
+        if (dual && r > 0) swap_lane();
         uint32_t n_r = 0, solo_job = kNoSolo;
         plan_round(jfirst, jcount, jring, base, total, &n_r, &solo_job);
         rc = launch_placed_round(any_env_is, base, n_r, solo_job);
@@ -873,8 +953,16 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
         drain_ms += ms;
       }
     }
+    if (dual) {
+      // lane 1's rounds end before anything after them on lane 0's stream
+      if (on_alt_) swap_lane();
+      OMR_HIP(hipMemcpyAsync(alt_.h_ctrl, alt_.d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, alt_.stream));
+      OMR_HIP(hipEventRecord(ev_d_, alt_.stream));
+      OMR_HIP(hipStreamWaitEvent(stream_, ev_d_, 0));
+    }
     OMR_HIP(hipMemcpyAsync(h_ctrl_, d_ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_));
     OMR_HIP(hipStreamSynchronize(stream_));
+    if (dual) add_counters(*h_ctrl_, *alt_.h_ctrl);
     drained = sum_shards(h_ctrl_->drain_paths);
   }
   for (uint64_t it = 0; !tuning.drain_rounds; it++) {

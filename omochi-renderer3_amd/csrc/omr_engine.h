@@ -165,6 +165,30 @@ class Engine {
   uint32_t slot_uses_[kMaxScenes] = {};  // uploads each slot has held (the id's high part)
   SceneDev *d_scenes_ = nullptr;
 
+  // The second path pool ("lane") of a run with tuning.lanes = 2: its own
+  // pool, control block, overflow stacks, graphs and stream.  swap_lane()
+  // exchanges it with the current one, so every launcher works on whichever
+  // lane is current; placed drain rounds alternate between the two, and one
+  // round's drain tail overlaps the next round's fill and steps on the other
+  // stream.  Lane 0 is current outside run_jobs.
+  struct Lane {
+    PoolPtrs pool{};
+    uint32_t pool_cap = 0;
+    Ctrl *d_ctrl = nullptr;
+    Ctrl *h_ctrl = nullptr;
+    uint2 *d_gstack = nullptr;
+    uint32_t gcap = 0;
+    size_t gstack_threads = 0;
+    hipGraphExec_t graph[8] = {};
+    uint32_t graph_steps = 0;
+    bool graph_round = false;
+    hipStream_t stream = nullptr;
+  };
+  Lane alt_;
+  bool on_alt_ = false;  // the members below hold lane 1
+  void swap_lane();
+  void free_lane_pool();  // frees the current lane's pool buffers
+
   // pool
   PoolPtrs pool_{};
   uint32_t pool_cap_ = 0;

@@ -1564,7 +1564,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO && !ENV
 template <bool SOLO>
 __device__ inline void add_shadow(const KernelArgs &A, uint32_t r, const JobDev &Js) {
   const float4 *sq = reinterpret_cast<const float4 *>(A.p.shadow) + (size_t)r * 3;
-  float4 a = sq[0], b = sq[1], c = sq[2];
+  const float4 a = sq[0], c = sq[2];
   const JobDev &J = SOLO ? Js : A.jobs[__float_as_uint(a.w)];
   deposit(J, __float_as_uint(c.w), fix(c.x), fix(c.y), fix(c.z));
 }

@@ -55,6 +55,18 @@ int Engine::server_start(const omr_server_config &cfg) {
       drop_graphs();
       tuning.pool_paths = (uint32_t)c.pool_paths;
     }
+    // the served loop runs one lane: the second pool of two-lane runs is
+    // not kept while serving
+    hipStreamSynchronize(stream_);
+    swap_lane();
+    hipStreamSynchronize(stream_);
+    drop_graphs();
+    free_lane_pool();
+    if (d_gstack_) hipFree(d_gstack_);
+    d_gstack_ = nullptr;
+    gcap_ = 0;
+    gstack_threads_ = 0;
+    swap_lane();
     // the pool is part of the budget: allocate it now so its size is known
     int rc = ensure_pool();
     if (rc) {

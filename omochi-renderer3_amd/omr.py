@@ -82,7 +82,8 @@ class Tuning(C.Structure):
                 ("pool_paths", C.c_uint32), ("steps_per_sync", C.c_uint32), ("use_graph", C.c_uint32),
                 ("refill_every", C.c_uint32), ("fetch_budget", C.c_uint32), ("fetch_min", C.c_uint32),
                 ("stack_short", C.c_uint32), ("drain_below", C.c_uint32), ("record_mb", C.c_uint32),
-                ("drain_rounds", C.c_uint32), ("round_steps", C.c_uint32), ("solo_round_min", C.c_uint32)]
+                ("drain_rounds", C.c_uint32), ("round_steps", C.c_uint32), ("solo_round_min", C.c_uint32),
+                ("lanes", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):

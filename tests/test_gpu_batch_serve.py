@@ -62,12 +62,13 @@ def test_batch_equals_alone(ctx, record_mb):
 
 
 @pytest.mark.parametrize("tune", [dict(pool_paths=4096, solo_round_min=1024), dict(pool_paths=4096, solo_round_min=0),
-                                  dict(pool_paths=8192, solo_round_min=1 << 30)])
+                                  dict(pool_paths=8192, solo_round_min=1 << 30),
+                                  dict(pool_paths=4096, solo_round_min=1024, lanes=1)])
 def test_batch_placed_rounds_equal_alone(ctx, tune):
     """Drain rounds placed by the host over a small pool: rounds inside one
     job run the single-job kernels, rounds across jobs the per-lane ones,
-    and a job with enough samples left ends in a round of its own; every
-    image equals the job alone."""
+    and a job with enough samples left ends in a round of its own (with one
+    lane or two: batches keep one); every image equals the job alone."""
     import omr
 
     saved = ctx.get_tuning()

@@ -110,7 +110,8 @@ def test_env_tables_bit_exact(ctx):
 
 @pytest.mark.parametrize("env_is,tune", [(0, {}), (1, {}), (1, dict(stack_short=8, fetch_budget=8)),
                                          (1, dict(drain_below=1 << 20)), (1, dict(record_mb=0)),
-                                         (1, dict(record_mb=0, drain_below=1 << 20)), (1, dict(node_width=8))])
+                                         (1, dict(record_mb=0, drain_below=1 << 20)), (1, dict(node_width=8)),
+                                         (1, dict(pool_paths=512, lanes=2))])
 def test_render_env_bit_exact(ctx, env_is, tune):
     """Torus knot under an HDR sky: lookup on escape, and with importance
     sampling + shadow rays + MIS (also with the 8-entry LDS stack and
@@ -186,6 +187,7 @@ def test_render_bit_exact(ctx, kind, ss, spp, depth, rr):
                                        wg_size=256),
                                   dict(drain_rounds=0, wg_size=64, blocks_per_cu=2, pool_paths=4096, steps_per_sync=4),
                                   dict(wg_size=64, blocks_per_cu=2, pool_paths=4096),
+                                  dict(pool_paths=4096, lanes=1), dict(pool_paths=4096, lanes=2),
                                   dict(pool_paths=1000, wg_size=128, blocks_per_cu=1),
                                   dict(stack_short=8), dict(stack_short=8, fetch_budget=8, wg_size=64,
                                                             blocks_per_cu=16),
@@ -198,7 +200,8 @@ def test_render_bit_exact(ctx, kind, ss, spp, depth, rr):
                                        drain_below=1 << 20)])
 def test_tuning_modes_bit_exact(ctx, tune):
     """Every launch configuration the autotuner can pick gives the same bits:
-    drain rounds (one or many per render, small pools) and bounce steps with
+    drain rounds (one or many per render, small pools, on one lane or
+    alternating between two pools on two streams) and bounce steps with
     dynamic-fetch traversal, bulk refill cadence, graph off, small pools, the
     drain launch that finishes the last paths in one go; the 8-entry LDS
     stack (more entries spill to global memory); 8-wide nodes (the oracle
