@@ -209,7 +209,7 @@ KernelArgs Engine::args() const {
   a.p.stack_short = tuning.stack_short;
   a.p.node_width = tuning.node_width;
   a.solo = solo_;
-  a.solo_env = solo_env_ ? 1u : 0u;
+  a.any_env = any_env_ ? 1u : 0u;
   return a;
 }
 
@@ -638,7 +638,7 @@ int Engine::launch_round(bool first, bool with_shadow) {
 
 void Engine::plan_round(const std::vector<uint64_t> &first, const std::vector<uint64_t> &count,
                         const std::vector<uint32_t> &ring, uint64_t base, uint64_t total, uint32_t *n0,
-                        uint32_t *solo_job) const {
+                        uint32_t *solo_job, bool *env) const {
   const uint64_t P = pool_cap_;
   uint64_t n = std::min<uint64_t>(P, total - base);
   *solo_job = kNoSolo;
@@ -655,13 +655,16 @@ void Engine::plan_round(const std::vector<uint64_t> &first, const std::vector<ui
     *solo_job = ring[i];
   }
   *n0 = (uint32_t)n;
+  *env = false;
+  for (size_t k = i; k < first.size() && first[k] < base + n; k++)
+    if (ring_env_[ring[k]]) *env = true;
 }
 
-int Engine::launch_placed_round(bool with_shadow, uint64_t base, uint32_t n0, uint32_t solo_job) {
+int Engine::launch_placed_round(bool with_shadow, uint64_t base, uint32_t n0, uint32_t solo_job, bool env) {
   const uint32_t saved = solo_;
-  const bool saved_env = solo_env_;
+  const bool saved_env = any_env_;
   solo_ = solo_job == kNoSolo ? kNoSolo : 0u;  // a flag: the kernels read the slot from Ctrl
-  solo_env_ = solo_job == kNoSolo || ring_env_[solo_job] != 0;
+  any_env_ = env;
   KernelArgs a = args();
   const LaunchShape sh = shape();
   hipError_t e = k::round_setup(d_ctrl_, n0, solo_job == kNoSolo ? 0u : solo_job, stream_);
@@ -673,7 +676,7 @@ int Engine::launch_placed_round(bool with_shadow, uint64_t base, uint32_t n0, ui
   if (!rc && (e = k::drain(a, sh, 0, profile, stream_)) != hipSuccess) rc = hipfail(e, "drain");
   if (!rc && profile && hipEventRecord(ev_d_, stream_) != hipSuccess) rc = hipfail(hipGetLastError(), "event");
   solo_ = saved;
-  solo_env_ = saved_env;
+  any_env_ = saved_env;
   return rc;
 }
 
@@ -750,7 +753,7 @@ int Engine::launch_steps(uint32_t steps, bool with_shadow, bool round) {
     if (e != hipSuccess) return hipfail(e, "bounce step launch");
     return OMR_OK;
   }
-  int gi = (with_shadow ? 1 : 0) + (a.solo != kNoSolo ? 2 : 0) + (a.solo != kNoSolo && !a.solo_env ? 4 : 0);
+  int gi = (with_shadow ? 1 : 0) + (a.solo != kNoSolo ? 2 : 0) + (!a.any_env ? 4 : 0);
   if (!graph_[gi] || graph_steps_ != steps || graph_round_ != round) {
     if (graph_steps_ != steps || graph_round_ != round) drop_graphs();
     hipGraph_t g = nullptr;
@@ -845,7 +848,9 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
     ring_env_[j.ring] = scene((omr_scene_id)j.scene)->has_env ? 1u : 0u;
     total += j.n_samples;
   }
-  solo_env_ = jobs.size() != 1 || ring_env_[jobs[0].ring] != 0;
+  any_env_ = false;
+  for (auto &j : jobs)
+    if (ring_env_[j.ring]) any_env_ = true;
   Ctrl c;
   std::memset(&c, 0, sizeof(c));
   c.total_samples = total;
@@ -936,8 +941,9 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
       if (placed) {
         if (dual && r > 0) swap_lane();
         uint32_t n_r = 0, solo_job = kNoSolo;
-        plan_round(jfirst, jcount, jring, base, total, &n_r, &solo_job);
-        rc = launch_placed_round(any_env_is, base, n_r, solo_job);
+        bool env = true;
+        plan_round(jfirst, jcount, jring, base, total, &n_r, &solo_job, &env);
+        rc = launch_placed_round(any_env_is, base, n_r, solo_job, env);
         base += n_r;
       } else {
         rc = launch_round(r == 0, any_env_is);

@@ -208,7 +208,7 @@ class Engine {
   uint32_t graph_steps_ = 0;
   bool graph_round_ = false;       // the graphs hold drain-round steps (no regeneration, no refill)
   uint32_t solo_ = kNoSolo;        // the current launches are single-job (Ctrl::solo_job names it), or kNoSolo
-  bool solo_env_ = true;           // ... and that job's scene has an environment map
+  bool any_env_ = true;            // a job of the current launches has a scene with an environment map
   uint8_t ring_env_[kJobRing] = {};  // per job-ring slot: the job's scene has an environment map
 
   int ensure_pool();
@@ -223,13 +223,14 @@ class Engine {
   // A round the host places: samples [base, base + n0) fill the idle pool
   // without atomics, round_steps steps, then the drain; solo_job = the ring
   // slot of the round's only job (single-job kernels), or kNoSolo.
-  int launch_placed_round(bool with_shadow, uint64_t base, uint32_t n0, uint32_t solo_job);
+  int launch_placed_round(bool with_shadow, uint64_t base, uint32_t n0, uint32_t solo_job, bool env);
   // The next placed round at global sample `base` of `total`, over jobs whose
   // sample ranges [first[i], first[i] + count[i]) follow one another: its
-  // sample count and its only job's ring slot (or kNoSolo).
+  // sample count, its only job's ring slot (or kNoSolo) and whether a job in
+  // it has a scene with an environment map.
   void plan_round(const std::vector<uint64_t> &first, const std::vector<uint64_t> &count,
                   const std::vector<uint32_t> &ring, uint64_t base, uint64_t total, uint32_t *n0,
-                  uint32_t *solo_job) const;
+                  uint32_t *solo_job, bool *env) const;
   // profiling: event pairs around each launch of the last chunk
   std::vector<hipEvent_t> prof_ev_;
   std::vector<int> prof_kind_;  // 0 extend, 1 shade, 2 shadow

@@ -46,7 +46,7 @@ struct KernelArgs {
   const SceneDev *scenes;
   PoolPtrs p;
   uint32_t solo;         // kNoSolo, or single-job kernels (the job's ring slot is Ctrl::solo_job)
-  uint32_t solo_env;     // single-job kernels: 0 when the job's scene has no environment map
+  uint32_t any_env;      // 0 when no job of the launch has a scene with an environment map
 };
 
 struct LaunchShape {

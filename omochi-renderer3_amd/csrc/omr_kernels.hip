@@ -1442,8 +1442,8 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
 // One shading step over the ray queue `parity`: shade each hit, then
 // regenerate finished slots (regen) and compact the continuing paths into the
 // other queue.
-// Five waves per SIMD (96 VGPRs, no spills), six in the single-job form for a
-// scene without an environment map (78 VGPRs): shade waits on memory most of
+// Five waves per SIMD (96 VGPRs, no spills), six in the forms for launches
+// whose jobs have no environment map (76-78 VGPRs): shade waits on memory most of
 // the time, and another wave hides more of it than reading further ahead
 // with the registers (c2: 5 % faster, c1 4 %).  Each lane
 // reads the queue entry of its next item one item ahead, so that load is off
@@ -1453,7 +1453,7 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
 // every other load of the segment).  TAG: the launch of a profiled step (the
 // same code under its own name, so a profiler can tell those launches apart).
 template <int WG, bool SOLO, bool TAG = false, bool ENV = true>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO && !ENV ? 6 : 5))) void k_shade(KernelArgs A, int parity,
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(!ENV ? 6 : 5))) void k_shade(KernelArgs A, int parity,
                                                                                       int regen) {
   __shared__ uint32_t s_pre[kShards + 1];
   load_prefix(s_pre, A.ctrl->qcount[parity]);
@@ -1520,7 +1520,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(SOLO && !ENV
         cont = shade_segment<ENV, ENV>(A, slot, st, hr, A.p.pdf, shadow, sh, Js, Ss);
       } else {
         const JobDev &J = A.jobs[st.js];
-        cont = shade_segment(A, slot, st, hr, A.p.pdf, shadow, sh, J, A.scenes[J.scene]);
+        cont = shade_segment<false, ENV>(A, slot, st, hr, A.p.pdf, shadow, sh, J, A.scenes[J.scene]);
       }
       fin = !cont;
     }
@@ -1673,8 +1673,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 3 
   // wave shades once half of its paths are ready or none is tracing
   bool tracing = false, ready = false;
   const int budget = A.p.df_budget > 0 ? (int)A.p.df_budget : 8;
-  // fused slab form only where its registers fit (single job, no sky)
-  constexpr bool FMA = SOLO && !ENV;
+  // fused slab form only where its registers fit (no sky)
+  constexpr bool FMA = !ENV;
   Trav<WG, SHORT, false, STATS, NW, FMA> tr;
   uint32_t slot = 0;
   PathState st;
@@ -2120,13 +2120,18 @@ hipError_t launch_shadow_wg(const KernelArgs &a, uint32_t grid, bool tag, hipStr
 template <int WG, int SHORT, int NW>
 hipError_t launch_drain_n(const KernelArgs &a, uint32_t grid, int parity, bool stats, hipStream_t st) {
   if (WG == 64 && a.solo != kNoSolo) {  // single-job runs at the default group size
-    if (a.solo_env) {
+    if (a.any_env) {
       if (stats) hipLaunchKernelGGL((k_drain<WG, SHORT, NW, true, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
       else hipLaunchKernelGGL((k_drain<WG, SHORT, NW, false, true, true>), dim3(grid), dim3(WG), 0, st, a, parity);
     } else {
       if (stats) hipLaunchKernelGGL((k_drain<WG, SHORT, NW, true, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
       else hipLaunchKernelGGL((k_drain<WG, SHORT, NW, false, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
     }
+    return hipGetLastError();
+  }
+  if (WG == 64 && !a.any_env) {  // no job of the launch has an environment map
+    if (stats) hipLaunchKernelGGL((k_drain<WG, SHORT, NW, true, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
+    else hipLaunchKernelGGL((k_drain<WG, SHORT, NW, false, false, false>), dim3(grid), dim3(WG), 0, st, a, parity);
     return hipGetLastError();
   }
   if (stats) hipLaunchKernelGGL((k_drain<WG, SHORT, NW, true, false>), dim3(grid), dim3(WG), 0, st, a, parity);
@@ -2226,14 +2231,16 @@ hipError_t shade(const KernelArgs &a, const LaunchShape &s, int parity, bool reg
   const bool solo = a.solo != kNoSolo;
   switch (s.wg) {
     case 64:
-      if (solo && a.solo_env && tag)
+      if (solo && a.any_env && tag)
         hipLaunchKernelGGL((k_shade<64, true, true>), dim3(s.shade_grid), dim3(64), 0, st, a, parity, r);
-      else if (solo && a.solo_env)
+      else if (solo && a.any_env)
         hipLaunchKernelGGL((k_shade<64, true>), dim3(s.shade_grid), dim3(64), 0, st, a, parity, r);
       else if (solo && tag)
         hipLaunchKernelGGL((k_shade<64, true, true, false>), dim3(s.shade6_grid), dim3(64), 0, st, a, parity, r);
       else if (solo)
         hipLaunchKernelGGL((k_shade<64, true, false, false>), dim3(s.shade6_grid), dim3(64), 0, st, a, parity, r);
+      else if (!a.any_env)
+        hipLaunchKernelGGL((k_shade<64, false, false, false>), dim3(s.shade6_grid), dim3(64), 0, st, a, parity, r);
       else hipLaunchKernelGGL((k_shade<64, false>), dim3(s.shade_grid), dim3(64), 0, st, a, parity, r);
       break;
     case 128:

@@ -267,7 +267,7 @@ void Engine::server_loop() {
   {
     std::lock_guard<std::mutex> g(mu);
     solo_ = kNoSolo;  // requests share the run
-    solo_env_ = true;
+    any_env_ = true;
   }
   std::deque<std::shared_ptr<Req>> live;  // admitted, by logical ring index
   uint64_t lo = 0, hi = 0;                // logical ring window [lo, hi)
@@ -571,8 +571,9 @@ void Engine::server_loop() {
             jring.push_back(r->run.ring);
           }
           uint32_t n0 = 0, solo_job = kNoSolo;
-          plan_round(jfirst, jcount, jring, placed, total, &n0, &solo_job);
-          okc = launch_placed_round(env_is, placed, n0, solo_job) == OMR_OK;
+          bool env = true;
+          plan_round(jfirst, jcount, jring, placed, total, &n0, &solo_job, &env);
+          okc = launch_placed_round(env_is, placed, n0, solo_job, env) == OMR_OK;
           placed += n0;
         } else {
           launched = false;  // every admitted sample is placed: the statuses show the rest

@@ -95,6 +95,37 @@ def test_batch_placed_rounds_equal_alone(ctx, tune):
         ctx.release(s)
 
 
+def test_batch_rounds_with_and_without_sky_equal_alone(ctx):
+    """Placed rounds over jobs of a scene with an HDR sky (importance
+    sampling on) and of scenes without one: rounds whose jobs have no sky run
+    the kernels built without the environment code, the others the full
+    ones; every image equals the job alone."""
+    import omr
+
+    saved = ctx.get_tuning()
+    scenes = (omr.procedural(1, width=32, height=32), omr.procedural(3, width=32, height=32),
+              omr.procedural(0, width=32, height=32))
+    sids = [ctx.upload(s) for s in scenes]
+    spec = [(0, 40, 30, 2, 8, 11), (1, 36, 28, 1, 6, 12), (2, 40, 30, 2, 8, 13), (0, 24, 20, 3, 16, 14),
+            (1, 20, 16, 2, 8, 15), (2, 48, 36, 1, 8, 16)]  # (scene, w, h, spp, depth, seed)
+    cams = [_cam(omr, scenes[si], w, h) for si, w, h, _, _, _ in spec]
+    prms = [omr.params(ss=1, spp_end=spp, max_depth=d, rr_depth=3, env_is=1, seed=seed)
+            for _, _, _, spp, d, seed in spec]
+    ids = [sids[si] for si, *_ in spec]
+    alone = [ctx.render(ids[i], cams[i], prms[i]) for i in range(len(spec))]
+    print("rounds with and without sky", flush=True)
+    try:
+        ctx.set_tuning(pool_paths=1024, solo_round_min=1 << 30)
+        batch = ctx.render_batch(ids, cams, prms)
+    finally:
+        ctx.set_tuning(**saved)
+    for i in range(len(spec)):
+        assert np.array_equal(alone[i], batch[i]), f"job {i} differs"
+        assert alone[i].mean() > 0
+    for s in sids:
+        ctx.release(s)
+
+
 def test_batch_with_8wide_nodes_equals_4wide(ctx):
     """The batched (per-lane job) kernels over 8-wide nodes give the images
     of the 4-wide nodes, bit for bit."""
