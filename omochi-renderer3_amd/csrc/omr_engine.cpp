@@ -891,8 +891,10 @@ int Engine::run_jobs(std::vector<JobRun> &jobs, omr_render_stats *st) {
   // two lanes: placed rounds alternate between the pools on two streams, for
   // a single job of more than one round (c2: +2-5 %; a batch's rounds side by
   // side measured 4 % slower on c3, rounds of different scenes sharing the
-  // caches; not in profiled runs, which time each launch on one stream)
-  const bool dual = placed && tuning.lanes == 2 && !profile && total > pool_cap_ && jobs.size() == 1;
+  // caches; not in profiled runs, which time each launch on one stream, nor
+  // while serving, whose HBM budget holds one pool)
+  const bool dual =
+      placed && tuning.lanes == 2 && !profile && total > pool_cap_ && jobs.size() == 1 && !server_on_.load();
   if (dual) {
     // lane 1: its pool, stacks and control block (the same start as lane
     // 0's, copied from pinned memory on lane 0's stream so the host does not
