@@ -473,13 +473,13 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
   D.o_far = S.bvh.pad * 2097152.0f;  // pad / 8 = o_far * 2^-24
   if (d->env_rgb) {
     uint32_t w = d->env_width, h = d->env_height;
-    void *p_env, *p_w, *p_max, *p_q, *p_row, *p_marg, *p_rg, *p_mg, *p_rs, *p_ms;
+    void *p_env, *p_w, *p_max, *p_q, *p_row, *p_marg, *p_rg, *p_mg, *p_rs, *p_ms, *p_e4;
     if ((rc = dup(d->env_rgb, (size_t)w * h * 3 * sizeof(float), &p_env)) ||
         (rc = dup(nullptr, (size_t)w * h * sizeof(float), &p_w)) || (rc = dup(nullptr, 16, &p_max)) ||
         (rc = dup(nullptr, (size_t)w * h * 8, &p_q)) || (rc = dup(nullptr, (size_t)h * (w + 1) * 8, &p_row)) ||
         (rc = dup(nullptr, (size_t)(h + 1) * 8, &p_marg)) || (rc = dup(nullptr, (size_t)w * h * 8, &p_rg)) ||
         (rc = dup(nullptr, (size_t)h * 8, &p_mg)) || (rc = dup(nullptr, (size_t)h * 8, &p_rs)) ||
-        (rc = dup(nullptr, 16, &p_ms))) {
+        (rc = dup(nullptr, 16, &p_ms)) || (rc = dup(nullptr, (size_t)w * h * 16, &p_e4))) {
       free_scene_mem(S);
       return rc;
     }
@@ -490,10 +490,13 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
     if (e == hipSuccess)
       e = k::env_guides((const unsigned long long *)p_row, (const unsigned long long *)p_marg, w, h, (uint2_t *)p_rg,
                         (uint2_t *)p_mg, (double *)p_rs, (double *)p_ms, stream_);
+    if (e == hipSuccess)
+      e = k::env_pack((const float *)p_env, (const unsigned long long *)p_q, w * h, (float4 *)p_e4, stream_);
     if (e != hipSuccess) { free_scene_mem(S); return hipfail(e, "env tables"); }
     OMR_HIP(hipStreamSynchronize(stream_));
     D.env = (const float *)p_env;
     D.env_q = (const unsigned long long *)p_q;
+    D.env4 = (const float *)p_e4;
     D.env_row = (const unsigned long long *)p_row;
     D.env_marg = (const unsigned long long *)p_marg;
     D.env_rguide = (const uint2_t *)p_rg;

@@ -118,5 +118,8 @@ hipError_t env_tables(const float *env, uint32_t w, uint32_t h, float *wbuf, uns
 // Bucket directories of the row and marginal inversions (after env_tables).
 hipError_t env_guides(const unsigned long long *row, const unsigned long long *marg, uint32_t w, uint32_t h,
                       uint2_t *rguide, uint2_t *mguide, double *rscale, double *mscale, hipStream_t st);
+// Texel records for shading (after env_tables): radiance and quantized weight
+// of n texels, 16 bytes each.
+hipError_t env_pack(const float *env, const unsigned long long *q, uint32_t n, float4 *env4, hipStream_t st);
 }  // namespace k
 }  // namespace omr

@@ -680,16 +680,17 @@ __device__ inline size_t env_index(const SceneDev &S, f3 d) {
   env_texel(d, S.env_w, S.env_h, ix, iy);
   return (size_t)iy * S.env_w + ix;
 }
+// Both read the texel's 16-byte record, so a pdf and a radiance lookup of one
+// texel are one load.
 __device__ inline float env_pdf_at(const SceneDev &S, size_t i, f3 d) {
   const double kf = gld(S.env_mscale + 1);
-  unsigned long long q = gld(S.env_q + i);
+  const uint32_t q = __float_as_uint(ld4(reinterpret_cast<const float4 *>(S.env4) + i).w);
   float st = sqrtf(d.x * d.x + d.z * d.z);
-  if (kf == 0.0 || q == 0ull || !(st > 0.0f)) return 0.0f;
+  if (kf == 0.0 || q == 0u || !(st > 0.0f)) return 0.0f;
   return (float)((double)q * kf) / st;
 }
 __device__ inline f3 env_at(const SceneDev &S, size_t i) {
-  const float *p = S.env + 3 * i;
-  return mk(gld(p), gld(p + 1), gld(p + 2));
+  return xyz(ld4(reinterpret_cast<const float4 *>(S.env4) + i));
 }
 // ------------------------------------------------------------ textures
 // Surface uv of a hit: interpolated vertex uv on triangles (Polygon.h:86-96,
@@ -2328,6 +2329,18 @@ hipError_t env_tables(const float *env, uint32_t w, uint32_t h, float *wbuf, uns
   hipLaunchKernelGGL(k_env_weight, dim3((n + 255u) / 256u), dim3(256), 0, st, env, w, h, wbuf, wmax_bits);
   hipLaunchKernelGGL(k_env_rowscan, dim3(h), dim3(256), 0, st, wbuf, w, wmax_bits, q, row);
   hipLaunchKernelGGL(k_env_margscan, dim3(1), dim3(256), 0, st, row, w, h, marg);
+  return hipGetLastError();
+}
+
+__global__ void k_env_pack(const float *env, const unsigned long long *q, uint32_t n, float4 *env4) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const float *p = env + 3 * (size_t)i;
+    env4[i] = make_float4(p[0], p[1], p[2], __uint_as_float((uint32_t)q[i]));  // q < 2^32 (k_env_rowscan)
+  }
+}
+hipError_t env_pack(const float *env, const unsigned long long *q, uint32_t n, float4 *env4, hipStream_t st) {
+  hipLaunchKernelGGL(k_env_pack, dim3((n + 255u) / 256u), dim3(256), 0, st, env, q, n, env4);
   return hipGetLastError();
 }
 

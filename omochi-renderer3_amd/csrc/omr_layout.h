@@ -93,6 +93,10 @@ struct SceneDev {
   const MatRec *mats;
   const float *env;            // env_h x env_w x 3 or null
   const unsigned long long *env_q;     // env_h x env_w
+  // env_h x env_w texel records for shading, 4 words each: radiance r, g, b
+  // and the quantized weight q (< 2^32) as the fourth word, so the radiance
+  // and the pdf of a texel come with one 16-byte load
+  const float *env4;
   const unsigned long long *env_row;   // env_h x (env_w+1)
   const unsigned long long *env_marg;  // env_h+1
   // bucket directories of the inversions: for key k of a table with total T
