@@ -370,10 +370,13 @@ int Engine::upload_scene(const omr_scene_desc *d, omr_scene_id *id) {
   std::string berr;
   if (!build_wide_bvh(bi, sh->bvh, berr)) return fail(OMR_ERR_INVALID, berr);
 
-  // per-triangle shading normals: n0, n1-n0, n2-n0 padded to 12 floats
+  // per-triangle shading normals: n0, n1-n0, n2-n0 padded to 12 floats; the
+  // first pad word holds the triangle's material word (prim_mat), so shading
+  // a triangle hit needs no separate load for it
   std::vector<float> tn(nt * 12, 0.0f);
   for (size_t i = 0; i < nt; i++) {
     float *o = &tn[i * 12];
+    std::memcpy(&o[3], &pm[i], 4);
     if (d->tri_normals) {
       const float *s = d->tri_normals + 9 * i;
       for (int a = 0; a < 3; a++) { o[a] = s[a]; o[4 + a] = s[3 + a]; o[8 + a] = s[6 + a]; }

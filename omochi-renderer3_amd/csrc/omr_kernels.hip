@@ -1250,19 +1250,22 @@ __device__ inline bool shade_segment(const KernelArgs &A, uint32_t slot, PathSta
     }
     f3 x = add(o, scl(d, hr.x));
     f3 nrm;
+    uint32_t pm;  // material word: in the normal record of a triangle
     if (pid < S.n_tri) {
       const float4 *tn = reinterpret_cast<const float4 *>(S.tri_n) + (size_t)pid * 3;
       float4 n0 = ld4(tn), n1 = ld4(tn + 1), n2 = ld4(tn + 2);
       nrm = normalize(add(add(scl(xyz(n1), hr.z), scl(xyz(n2), hr.w)), xyz(n0)));
+      pm = __float_as_uint(n0.w);
     } else if (pid < S.n_tri + S.n_sph) {
       float4 sc = ld4(reinterpret_cast<const float4 *>(S.sph) + (pid - S.n_tri));
       nrm = normalize(sub(x, xyz(sc)));
+      pm = gld(S.prim_mat + pid);
     } else {
       nrm = mk(0.0f, d.y < 0.0f ? 1.0f : -1.0f, 0.0f);  // InfiniteFloor.h:34-36: facing the ray
+      pm = gld(S.prim_mat + pid);
     }
     f3 nl = dot(nrm, d) < 0.0f ? nrm : scl(nrm, -1.0f);
     // material roulette (PathTracer.cpp:368-386)
-    uint32_t pm = gld(S.prim_mat + pid);
     uint32_t mfirst = pm & 0x00FFFFFFu, mcount = (pm >> 24) + 1u;
     float value = rng_float(rng);
     uint32_t sel = mcount - 1u;

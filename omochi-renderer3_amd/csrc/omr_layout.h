@@ -87,7 +87,7 @@ struct SceneDev {
   const WNode4 *nodes;
   const WNode8 *nodes8;        // the same tree collapsed to 8-wide nodes (same leaves)
   const PrimRec *prims;        // leaf order
-  const float *tri_n;          // per original triangle: 12 floats (n0,_, dn1,_, dn2,_)
+  const float *tri_n;          // per original triangle: 12 floats (n0,pm, dn1,_, dn2,_), pm = its prim_mat word
   const float *sph;            // per original sphere: 4 floats (c, r)
   const uint32_t *prim_mat;    // per original primitive: first | (count-1) << 24
   const MatRec *mats;
