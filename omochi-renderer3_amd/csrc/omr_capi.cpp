@@ -399,6 +399,7 @@ int omr_autotune(omr_ctx *ctx, size_t n, const omr_scene_id *scenes, const omr_c
       {&omr_tuning::drain_rounds, {0, 1}},
       {&omr_tuning::round_steps, {0, 2, 4}},
       {&omr_tuning::solo_round_min, {0, 1u << 20, 1u << 22, 1u << 24}},
+      {&omr_tuning::lanes, {1, 2}},
   };
   for (int pass = 0; pass < 2; pass++) {
     for (auto &d : dims) {
