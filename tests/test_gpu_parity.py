@@ -181,6 +181,21 @@ def test_render_bit_exact(ctx, kind, ss, spp, depth, rr):
     ctx.release(sid)
 
 
+def test_tuning_rejects_invalid_values(ctx):
+    """omr_set_tuning refuses values outside what the kernels are built for
+    (node width, lanes, odd round steps, group size, stack depth) with
+    OMR_ERR_INVALID and leaves the tuning as it was."""
+    import omr
+
+    saved = ctx.get_tuning()
+    for bad in (dict(node_width=5), dict(lanes=3), dict(lanes=0), dict(round_steps=3), dict(wg_size=96),
+                dict(stack_short=12), dict(pool_paths=(1 << 26) + 1)):
+        with pytest.raises(omr.OmrError) as ei:
+            ctx.set_tuning(**bad)
+        assert ei.value.code == omr.ERR_INVALID, bad
+        assert ctx.get_tuning() == saved, bad
+
+
 @pytest.mark.parametrize("tune", [dict(fetch_budget=8, fetch_min=16), dict(drain_rounds=0),
                                   dict(drain_rounds=0, refill_every=2),
                                   dict(drain_rounds=0, fetch_budget=4, fetch_min=32, refill_every=1, use_graph=0,
